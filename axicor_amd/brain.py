@@ -1,0 +1,207 @@
+"""Seeded synthetic brains in the genesis-baker binary layouts.
+
+The generator itself is native (csrc/brain_gen.cpp). This module holds the
+variant tables and the named workloads:
+
+* ``c1`` — 4,096 integer GLIF neurons (80% excitatory / 20% inhibitory), 64
+  dendrite slots each, 16x16x16 grid, axons 1-32 segments, |w| in [64,512],
+  256 inputs at p=0.05 per tick.
+* ``c2`` — 1.5M neurons of 4 types (excitatory/inhibitory x fast/slow leak),
+  100 slots each (150M synapses), 128x128x64 grid, axons 1-255 segments,
+  |w| in [64,512], 5% of neurons as inputs (75,000) at p=0.02 per tick.
+
+Parameter values follow the reference template (_template/blueprints.toml:13-37)
+and the blueprint defaults (genesis-core/src/config/blueprints.rs:131-152).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from ._ffi import VARIANT_DTYPE, GenSpec, gen_lib, ptr
+from .layout import StateView, calculate_state_blob_size
+
+# genesis-core/src/seed.rs:29,33-43: seed_from_str("GENESIS").
+def seed_from_str(s: str) -> int:
+    h = 0xcbf29ce484222325
+    for b in s.encode():
+        h ^= b
+        h = (h * 0x00000100000001B3) & 0xFFFFFFFFFFFFFFFF
+    return h
+
+
+DEFAULT_MASTER_SEED = seed_from_str("GENESIS")
+DEFAULT_INERTIA = [128, 120, 112, 104, 96, 88, 80, 72, 64, 56, 48, 40, 32, 24, 16, 8]  # blueprints.rs:150-152
+
+
+def template_variant(inhibitory: bool = False, **over) -> np.ndarray:
+    """One VariantParameters record with the _template values."""
+    v = np.zeros(1, dtype=VARIANT_DTYPE)
+    v["threshold"] = 30000
+    v["rest_potential"] = 10000
+    v["leak_rate"] = 1500
+    v["homeostasis_penalty"] = 10000
+    v["spontaneous_firing_period_ticks"] = 400
+    v["initial_synapse_weight"] = 74
+    v["gsop_potentiation"] = 150
+    v["gsop_depression"] = 60
+    v["homeostasis_decay"] = 20
+    v["refractory_period"] = 15
+    v["synapse_refractory_period"] = 15
+    v["signal_propagation_length"] = 20
+    v["is_inhibitory"] = 1 if inhibitory else 0
+    v["inertia_curve"] = DEFAULT_INERTIA
+    v["adaptive_leak_max"] = 0
+    v["adaptive_leak_gain"] = 0
+    v["adaptive_mode"] = 0
+    v["d1_affinity"] = 128
+    v["d2_affinity"] = 128
+    for k, val in over.items():
+        v[k] = val
+    return v[0]
+
+
+def lut_from(variants: list) -> np.ndarray:
+    """16-entry table; unused entries are zero (as the reference uploads)."""
+    lut = np.zeros(16, dtype=VARIANT_DTYPE)
+    for i, v in enumerate(variants):
+        lut[i] = v
+    return lut
+
+
+def lut_c1() -> np.ndarray:
+    return lut_from([template_variant(False), template_variant(True)])
+
+
+def lut_c2() -> np.ndarray:
+    # Fast leak: template. Slow leak: half the leak with the adaptive mode on
+    # (leak grows with the homeostatic offset up to 3000).
+    return lut_from([
+        template_variant(False),
+        template_variant(False, leak_rate=750, adaptive_mode=1, adaptive_leak_gain=64, adaptive_leak_max=3000),
+        template_variant(True),
+        template_variant(True, leak_rate=750, adaptive_mode=1, adaptive_leak_gain=64, adaptive_leak_max=3000),
+    ])
+
+
+@dataclass
+class BrainSpec:
+    n_neurons: int
+    n_inputs: int
+    grid: tuple[int, int, int]
+    slots_filled: int
+    axon_len: tuple[int, int]
+    w_range: tuple[int, int] = (64, 512)
+    v_seg: int = 1
+    type_ppm: list[int] = field(default_factory=lambda: [800_000, 200_000])
+    mean_dist: float = 3.0
+    seed: int = DEFAULT_MASTER_SEED
+    input_ppm: int = 50_000
+    lut: np.ndarray | None = None
+
+    def to_c(self) -> GenSpec:
+        g = GenSpec()
+        g.n_neurons = self.n_neurons
+        g.n_inputs = self.n_inputs
+        g.gx, g.gy, g.gz = self.grid
+        g.slots_filled = self.slots_filled
+        g.axon_len_min, g.axon_len_max = self.axon_len
+        g.w_min, g.w_max = self.w_range
+        g.v_seg = self.v_seg
+        g.n_types = len(self.type_ppm)
+        acc = 0
+        for i, p in enumerate(self.type_ppm):
+            acc += p
+            g.type_ppm_cdf[i] = acc
+        g.mean_dist_milli = int(round(self.mean_dist * 1000))
+        g.master_seed = self.seed & 0xFFFFFFFFFFFFFFFF
+        return g
+
+
+def spec_c1(seed: int = DEFAULT_MASTER_SEED) -> BrainSpec:
+    return BrainSpec(4096, 256, (16, 16, 16), 64, (1, 32), seed=seed, input_ppm=50_000, lut=lut_c1())
+
+
+def spec_c2(seed: int = DEFAULT_MASTER_SEED) -> BrainSpec:
+    return BrainSpec(1_500_000, 75_000, (128, 128, 64), 100, (1, 255),
+                     type_ppm=[400_000, 400_000, 100_000, 100_000], seed=seed, input_ppm=20_000, lut=lut_c2())
+
+
+def spec_small(n: int, slots: int, n_inputs: int, seed: int, axon_len=(1, 32), input_ppm=50_000,
+               grid=None, lut=None, type_ppm=None) -> BrainSpec:
+    """A small brain for parity tests; grid of about one neuron per voxel."""
+    if grid is None:
+        side = max(1, int(round(n ** (1 / 3))))
+        grid = (side, side, max(1, (n + side * side - 1) // (side * side)))
+    return BrainSpec(n, n_inputs, grid, slots, axon_len, seed=seed, input_ppm=input_ppm,
+                     lut=lut if lut is not None else lut_c1(),
+                     type_ppm=type_ppm if type_ppm is not None else [800_000, 200_000])
+
+
+@dataclass
+class Brain:
+    spec: BrainSpec
+    padded_n: int
+    total_axons: int
+    state: np.ndarray     # uint8[910*padded_n]
+    heads: np.ndarray     # uint32[total_axons*8]
+    lut: np.ndarray       # VARIANT_DTYPE[16]
+    axon_len: np.ndarray  # uint8[total_axons]
+    soma_pos: np.ndarray  # uint32[padded_n]
+
+    @property
+    def virtual_offset(self) -> int:
+        """memory.rs:128-130 with zero ghosts: inputs follow the local axons."""
+        return self.padded_n
+
+    @property
+    def n_inputs(self) -> int:
+        return self.spec.n_inputs
+
+    @property
+    def input_words(self) -> int:
+        return (self.spec.n_inputs + 31) // 32
+
+    def view(self) -> StateView:
+        return StateView(self.state, self.padded_n)
+
+    def copy(self) -> "Brain":
+        return Brain(self.spec, self.padded_n, self.total_axons, self.state.copy(), self.heads.copy(),
+                     self.lut.copy(), self.axon_len.copy(), self.soma_pos.copy())
+
+    def inputs(self, tick0: int, ticks: int) -> np.ndarray:
+        """uint32[ticks, words] Bernoulli input bitmasks for ticks tick0.."""
+        return gen_inputs(self.spec.seed, self.spec.n_inputs, self.spec.input_ppm, tick0, ticks)
+
+
+def sizes(spec: BrainSpec) -> tuple[int, int]:
+    g = spec.to_c()
+    pn, a = C.c_uint32(), C.c_uint32()
+    gen_lib().axc_gen_sizes(C.byref(g), C.byref(pn), C.byref(a))
+    return pn.value, a.value
+
+
+def generate(spec: BrainSpec) -> Brain:
+    g = spec.to_c()
+    pn, a = sizes(spec)
+    _, nbytes = calculate_state_blob_size(pn)
+    state = np.empty(nbytes, dtype=np.uint8)
+    heads = np.empty(a * 8, dtype=np.uint32)
+    alen = np.empty(a, dtype=np.uint8)
+    pos = np.empty(pn, dtype=np.uint32)
+    lut = spec.lut if spec.lut is not None else lut_c1()
+    lut = np.ascontiguousarray(lut, dtype=VARIANT_DTYPE)
+    rc = gen_lib().axc_gen_brain(C.byref(g), ptr(lut), ptr(state), ptr(heads), ptr(alen), ptr(pos))
+    if rc != 0:
+        raise ValueError(f"bad brain spec: {spec}")
+    return Brain(spec, pn, a, state, heads, lut, alen, pos)
+
+
+def gen_inputs(seed: int, n_inputs: int, ppm: int, tick0: int, ticks: int) -> np.ndarray:
+    words = (n_inputs + 31) // 32
+    out = np.zeros((ticks, max(words, 1)), dtype=np.uint32)
+    if n_inputs > 0 and ticks > 0:
+        gen_lib().axc_gen_inputs(seed & 0xFFFFFFFFFFFFFFFF, n_inputs, ppm, tick0, ticks, ptr(out))
+    return out[:, :words] if words else out[:, :0]

@@ -1,0 +1,52 @@
+// day_common.h — shared definitions of the Day-Phase kernels and host code.
+#pragma once
+#include <stdint.h>
+#include <stddef.h>
+#include "../../include/axicor_day.h"
+
+namespace axc {
+
+constexpr uint32_t kMaxDendrites = AXC_MAX_DENDRITES;
+constexpr uint32_t kSentinel = AXC_AXON_SENTINEL;
+constexpr uint32_t kNoAxon = AXC_NO_AXON;
+
+// Per-tick list counters (device, reset at the start of every tick).
+constexpr uint32_t kCntSpikes = 0;  // entries of spk_list
+constexpr uint32_t kCntOwners = 1;  // entries of own_list
+constexpr uint32_t kNumCounters = 4;
+
+// Per-workgroup partial counters (device, accumulated across ticks).
+constexpr uint32_t kStatsPerBlock = 4;
+constexpr uint32_t kStIntegrated = 0;
+constexpr uint32_t kStScanned = 1;
+constexpr uint32_t kStTargets = 2;
+constexpr uint32_t kStHits = 3;
+
+// Workgroups of the list-driven kernels (gsop, push_apply): fixed so that a
+// HIP graph of a batch does not depend on the number of spikes.
+constexpr uint32_t kListBlocks = 2048;
+
+// Per-batch values of a graph-replayed batch, written by a one-lane kernel
+// before each replay: kernels add their tick offset t to tick_base and index
+// the per-tick input/output slices at base + t*stride. A null TickParams
+// pointer means "take the kernel arguments as they are" (the per-tick entry
+// point).
+struct TickParams {
+  uint32_t tick_base;
+  int32_t dopamine;
+  const uint32_t* bitmask;  // [T][words] input bitmasks, or null
+  const uint32_t* spikes;   // [T][max_spikes] spike schedule, or null
+  uint8_t* out;             // [T][n_out] output history, or null
+};
+
+// Engine-private per-shard device buffers (no counterpart in the reference).
+struct ShardScratch {
+  uint32_t* spk_list;    // [N] somas that spiked this tick
+  uint32_t* own_list;    // [A] axons receiving pushes this tick (first owner)
+  uint32_t* push_cnt;    // [A] pending pushes per axon, zero between uses
+  uint32_t* counters;    // [kNumCounters]
+  uint64_t* stats_part;  // [ceil(N/256)][kStatsPerBlock] neuron_tick partials
+  uint64_t* stats_part2; // [kListBlocks][kStatsPerBlock] gsop partials: spikes, slots, targets, ticks
+};
+
+}  // namespace axc

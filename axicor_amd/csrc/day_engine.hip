@@ -1,0 +1,830 @@
+// day_engine.hip — host side of libaxicor_day.so: the C-ABI of
+// include/axicor_day.h over HIP.
+//
+// Mirrors genesis-compute/src/cuda/bindings.cu (memory, streams, I/O, Night,
+// multi-zone helpers) and physics.cu:441-506 (tick orchestration, parameter
+// table), with the engine's own additions: a per-shard workspace (spike
+// lists, push counters, work counters), the batch entry point that captures a
+// whole BSP batch in a HIP graph, and the host sort&prune.
+#include <hip/hip_runtime.h>
+#include <cstdio>
+#include <cstring>
+#include <cstdlib>
+#include <mutex>
+#include <unordered_map>
+#include <vector>
+#include <map>
+#include <tuple>
+#include <thread>
+#include <algorithm>
+#include "day_common.h"
+#include "day_kernels.hip"
+
+using namespace axc;
+
+static_assert(sizeof(AxcVariantParameters) == 64, "VariantParameters is 64 B (layout.rs:47)");
+static_assert(alignof(AxcVariantParameters) == 64, "VariantParameters align 64 (layout.rs:48)");
+static_assert(offsetof(AxcVariantParameters, spontaneous_firing_period_ticks) == 16, "@16");
+static_assert(offsetof(AxcVariantParameters, initial_synapse_weight) == 20, "@20");
+static_assert(offsetof(AxcVariantParameters, refractory_period) == 28, "@28");
+static_assert(offsetof(AxcVariantParameters, inertia_curve) == 32, "@32");
+static_assert(offsetof(AxcVariantParameters, adaptive_leak_max) == 48, "@48");
+static_assert(offsetof(AxcVariantParameters, adaptive_leak_gain) == 52, "@52");
+static_assert(offsetof(AxcVariantParameters, adaptive_mode) == 54, "@54");
+static_assert(offsetof(AxcVariantParameters, d1_affinity) == 58, "@58");
+static_assert(offsetof(AxcVariantParameters, d2_affinity) == 59, "@59");
+static_assert(sizeof(AxcBurstHeads8) == 32 && alignof(AxcBurstHeads8) == 32, "BurstHeads8 (layout.rs:73-74)");
+static_assert(sizeof(ShardVramPtrs) == 9 * sizeof(void*), "ShardVramPtrs is 9 pointers (ffi.rs:25-42)");
+
+namespace {
+
+constexpr uint32_t kThreads = 256;
+
+inline uint32_t blocks_for(uint64_t n, uint32_t t = kThreads) { return (uint32_t)((n + t - 1) / t); }
+
+// ---- per-device variant table (the reference's __constant__ VARIANT_LUT[16],
+// physics.cu:82; process-global per device like constant memory) -----------
+struct DeviceLut {
+  AxcVariantParameters* dev = nullptr;
+};
+std::mutex g_lut_mu;
+std::map<int, DeviceLut> g_luts;
+
+AxcVariantParameters* device_lut() {
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  std::lock_guard<std::mutex> lk(g_lut_mu);
+  DeviceLut& d = g_luts[dev];
+  if (!d.dev) {
+    if (hipMalloc((void**)&d.dev, 16 * sizeof(AxcVariantParameters)) != hipSuccess) return nullptr;
+    (void)hipMemset(d.dev, 0, 16 * sizeof(AxcVariantParameters));
+    (void)hipDeviceSynchronize();
+  }
+  return d.dev;
+}
+
+// ---- per-shard workspace, keyed by the state base pointer ------------------
+struct GraphKey {
+  uint32_t T, words, virt_off, n_virt, max_spk, n_out;
+  bool has_bitmask, has_spikes, has_out;
+  const void* mapped;
+  uint32_t v_seg;
+  std::vector<uint32_t> spike_counts;
+  bool operator==(const GraphKey& o) const {
+    return T == o.T && words == o.words && virt_off == o.virt_off && n_virt == o.n_virt &&
+           max_spk == o.max_spk && n_out == o.n_out && has_bitmask == o.has_bitmask &&
+           has_spikes == o.has_spikes && has_out == o.has_out && mapped == o.mapped && v_seg == o.v_seg &&
+           spike_counts == o.spike_counts;
+  }
+};
+
+struct Shard {
+  uint32_t N = 0, A = 0;
+  int device = 0;
+  void* state_base = nullptr;
+  ShardScratch sc{};
+  uint32_t stats_blocks = 0;
+  TickParams* prm = nullptr;  // device
+  bool owned_by_library = false;
+  // One cached graph per shard (the common case: the same batch shape every
+  // BSP step); replaced when the shape changes.
+  bool has_graph = false;
+  GraphKey gkey{};
+  hipGraphExec_t gexec = nullptr;
+  hipGraph_t graph = nullptr;
+};
+
+std::mutex g_mu;
+std::unordered_map<const void*, Shard*> g_shards;
+
+// Optional per-kernel timing of the per-tick path: HIP events recorded on the
+// caller's stream around each launch, summed by axc_kernel_timing_read.
+enum TimedKernel { kKAxon = 0, kKNeuron, kKPush, kKGsop, kKReadout, kKSpikes, kNumTimed };
+struct TimingState {
+  bool enabled = false;
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending;
+  double ms[kNumTimed] = {0};
+  uint64_t launches[kNumTimed] = {0};
+};
+std::mutex g_tmu;
+TimingState g_timing;
+
+struct TimedScope {
+  hipEvent_t a = nullptr, b = nullptr;
+  hipStream_t st;
+  int which;
+  bool on;
+  TimedScope(int w, hipStream_t s, bool active) : st(s), which(w), on(active) {
+    if (!on) return;
+    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) { on = false; return; }
+    (void)hipEventRecord(a, st);
+  }
+  ~TimedScope() {
+    if (!on) return;
+    (void)hipEventRecord(b, st);
+    std::lock_guard<std::mutex> lk(g_tmu);
+    g_timing.pending.push_back({which, {a, b}});
+  }
+};
+
+void free_scratch(Shard* s) {
+  if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
+  if (s->graph) (void)hipGraphDestroy(s->graph);
+  s->gexec = nullptr;
+  s->graph = nullptr;
+  s->has_graph = false;
+  (void)hipFree(s->sc.spk_list);
+  (void)hipFree(s->sc.own_list);
+  (void)hipFree(s->sc.push_cnt);
+  (void)hipFree(s->sc.counters);
+  (void)hipFree(s->sc.stats_part);
+  (void)hipFree(s->sc.stats_part2);
+  (void)hipFree(s->prm);
+  s->sc = ShardScratch{};
+  s->prm = nullptr;
+}
+
+hipError_t alloc_scratch(Shard* s) {
+  const size_t N = s->N ? s->N : 1, A = s->A ? s->A : 1;
+  s->stats_blocks = blocks_for(N);
+  hipError_t e;
+  if ((e = hipMalloc((void**)&s->sc.spk_list, N * 4)) != hipSuccess) return e;
+  if ((e = hipMalloc((void**)&s->sc.own_list, A * 4)) != hipSuccess) return e;
+  if ((e = hipMalloc((void**)&s->sc.push_cnt, A * 4)) != hipSuccess) return e;
+  if ((e = hipMalloc((void**)&s->sc.counters, kNumCounters * 4)) != hipSuccess) return e;
+  if ((e = hipMalloc((void**)&s->sc.stats_part, (size_t)s->stats_blocks * kStatsPerBlock * 8)) != hipSuccess) return e;
+  if ((e = hipMalloc((void**)&s->sc.stats_part2, (size_t)kListBlocks * kStatsPerBlock * 8)) != hipSuccess) return e;
+  if ((e = hipMalloc((void**)&s->prm, sizeof(TickParams))) != hipSuccess) return e;
+  if ((e = hipMemset(s->sc.push_cnt, 0, A * 4)) != hipSuccess) return e;
+  if ((e = hipMemset(s->sc.counters, 0, kNumCounters * 4)) != hipSuccess) return e;
+  if ((e = hipMemset(s->sc.stats_part, 0, (size_t)s->stats_blocks * kStatsPerBlock * 8)) != hipSuccess) return e;
+  if ((e = hipMemset(s->sc.stats_part2, 0, (size_t)kListBlocks * kStatsPerBlock * 8)) != hipSuccess) return e;
+  if ((e = hipMemset(s->prm, 0, sizeof(TickParams))) != hipSuccess) return e;
+  return hipDeviceSynchronize();
+}
+
+// Finds the workspace of a shard; for pointer sets the library did not
+// allocate (a caller's own hipMalloc), creates one sized for (N, A).
+Shard* shard_for(const ShardVramPtrs* v, uint32_t N, uint32_t A, hipError_t* err) {
+  *err = hipSuccess;
+  std::lock_guard<std::mutex> lk(g_mu);
+  auto it = g_shards.find(v->soma_voltage);
+  if (it != g_shards.end()) {
+    Shard* s = it->second;
+    if (s->N == N && s->A == A) return s;
+    if (s->owned_by_library) {  // the arrays were sized for (s->N, s->A)
+      *err = hipErrorInvalidValue;
+      return nullptr;
+    }
+    free_scratch(s);
+    s->N = N;
+    s->A = A;
+    if ((*err = alloc_scratch(s)) != hipSuccess) return nullptr;
+    return s;
+  }
+  Shard* s = new Shard();
+  s->N = N;
+  s->A = A;
+  (void)hipGetDevice(&s->device);
+  s->state_base = v->soma_voltage;
+  if ((*err = alloc_scratch(s)) != hipSuccess) {
+    free_scratch(s);
+    delete s;
+    return nullptr;
+  }
+  g_shards[v->soma_voltage] = s;
+  return s;
+}
+
+// Enqueues one tick. prm == nullptr: tick, dopamine and the per-tick
+// pointers are taken from the arguments; otherwise kernels read them from prm
+// (tick = tick_base + t_off, slices at base + t_off*stride) and the pointer
+// arguments only say whether inputs/spikes/outputs are present.
+hipError_t enqueue_tick(const ShardVramPtrs& v, Shard* s, uint32_t v_seg, uint32_t tick,
+                        const uint32_t* bitmask, uint32_t words, uint32_t virt_off, uint32_t n_virt,
+                        const uint32_t* spikes, uint32_t max_spk, uint32_t n_spk, const uint32_t* mapped,
+                        uint8_t* out, uint32_t n_out, int32_t dopamine, const TickParams* prm, hipStream_t st,
+                        const AxcVariantParameters* lut) {
+  const uint32_t N = s->N, A = s->A;
+  bool timed;
+  {
+    std::lock_guard<std::mutex> lk(g_tmu);
+    timed = g_timing.enabled && prm == nullptr;  // never inside a captured graph
+  }
+  const bool k3 = n_spk > 0 && spikes != nullptr;
+  if (k3) {
+    TimedScope ts(kKSpikes, st, timed);
+    spike_count_kernel<<<blocks_for(n_spk), kThreads, 0, st>>>(spikes, n_spk, A, s->sc.push_cnt, prm, tick,
+                                                                max_spk);
+  }
+  const bool k2 = n_virt > 0 && bitmask != nullptr;
+  {
+    TimedScope ts(kKAxon, st, timed);
+    axon_tick_kernel<<<blocks_for(A > kNumCounters ? A : kNumCounters), kThreads, 0, st>>>(
+        v.axon_heads, A, v_seg, k2 ? bitmask : nullptr, virt_off, k2 ? n_virt : 0u, s->sc.push_cnt, k3 ? 1 : 0,
+        s->sc.counters, prm, tick, words);
+  }
+  if (N > 0) {
+    TimedScope ts(kKNeuron, st, timed);
+    neuron_tick_kernel<<<blocks_for(N), kThreads, 0, st>>>(v, N, A, lut, prm, tick, s->sc);
+  }
+  {
+    TimedScope ts(kKPush, st, timed);
+    push_apply_kernel<<<kListBlocks / 4, kThreads, 0, st>>>(v.axon_heads, s->sc);
+  }
+  {
+    TimedScope ts(kKGsop, st, timed);
+    gsop_kernel<<<kListBlocks, kThreads, 0, st>>>(v, N, A, lut, prm, dopamine, s->sc);
+  }
+  if (n_out > 0 && mapped != nullptr && out != nullptr) {
+    TimedScope ts(kKReadout, st, timed);
+    readout_kernel<<<blocks_for(n_out), kThreads, 0, st>>>(v.soma_flags, N, mapped, out, n_out, prm, tick);
+  }
+  return hipGetLastError();
+}
+
+}  // namespace
+
+extern "C" {
+
+// ===========================================================================
+// Shard memory (bindings.cu:571-691)
+// ===========================================================================
+int32_t cu_allocate_shard(uint32_t padded_n, uint32_t total_axons, ShardVramPtrs* out) {
+  if (!out) return (int32_t)hipErrorInvalidValue;
+  const size_t N = padded_n;
+  const size_t total = N * AXC_STATE_BYTES_PER_NEURON;
+  void* base = nullptr;
+  hipError_t e = hipMalloc(&base, total ? total : 64);
+  if (e != hipSuccess) {
+    fprintf(stderr, "[cu_allocate_shard] hipMalloc state failed: %s\n", hipGetErrorString(e));
+    return (int32_t)e;
+  }
+  if ((e = hipMemset(base, 0, total ? total : 64)) != hipSuccess) {
+    (void)hipFree(base);
+    return (int32_t)e;
+  }
+  char* b = (char*)base;
+  size_t off = 0;
+  out->soma_voltage = (int32_t*)(b + off); off += 4 * N;
+  out->soma_flags = (uint8_t*)(b + off); off += 1 * N;
+  out->threshold_offset = (int32_t*)(b + off); off += 4 * N;
+  out->timers = (uint8_t*)(b + off); off += 1 * N;
+  out->soma_to_axon = (uint32_t*)(b + off); off += 4 * N;
+  out->dendrite_targets = (uint32_t*)(b + off); off += 512 * N;
+  out->dendrite_weights = (int16_t*)(b + off); off += 256 * N;
+  out->dendrite_timers = (uint8_t*)(b + off);
+  e = hipMalloc((void**)&out->axon_heads, (size_t)(total_axons ? total_axons : 1) * sizeof(AxcBurstHeads8));
+  if (e != hipSuccess) {
+    fprintf(stderr, "[cu_allocate_shard] hipMalloc axon_heads failed: %s\n", hipGetErrorString(e));
+    (void)hipFree(base);
+    return (int32_t)e;
+  }
+  if (total_axons) init_sentinels_kernel<<<blocks_for(total_axons), kThreads>>>(out->axon_heads, total_axons);
+  if ((e = hipDeviceSynchronize()) != hipSuccess) return (int32_t)e;
+  hipError_t se;
+  Shard* s = shard_for(out, padded_n, total_axons, &se);
+  if (!s) return (int32_t)se;
+  s->owned_by_library = true;
+  return 0;
+}
+
+int32_t cu_upload_state_blob(const ShardVramPtrs* vram, const void* blob, size_t size) {
+  if (!vram || (!blob && size)) return (int32_t)hipErrorInvalidValue;
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = g_shards.find(vram->soma_voltage);
+    if (it != g_shards.end() && size > (size_t)it->second->N * AXC_STATE_BYTES_PER_NEURON)
+      return (int32_t)hipErrorInvalidValue;
+  }
+  hipError_t e = hipMemcpy(vram->soma_voltage, blob, size, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    fprintf(stderr, "[cu_upload_state_blob] hipMemcpy failed: %s\n", hipGetErrorString(e));
+    return (int32_t)e;
+  }
+  return 0;
+}
+
+int32_t cu_upload_axons_blob(const ShardVramPtrs* vram, const void* blob, size_t size) {
+  if (!vram || (!blob && size)) return (int32_t)hipErrorInvalidValue;
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = g_shards.find(vram->soma_voltage);
+    if (it != g_shards.end() && size > (size_t)it->second->A * sizeof(AxcBurstHeads8))
+      return (int32_t)hipErrorInvalidValue;
+  }
+  hipError_t e = hipMemcpy(vram->axon_heads, blob, size, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    fprintf(stderr, "[cu_upload_axons_blob] hipMemcpy failed: %s\n", hipGetErrorString(e));
+    return (int32_t)e;
+  }
+  return 0;
+}
+
+void cu_free_shard(ShardVramPtrs* vram) {
+  if (!vram) return;
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = g_shards.find(vram->soma_voltage);
+    if (it != g_shards.end()) {
+      (void)hipDeviceSynchronize();
+      free_scratch(it->second);
+      delete it->second;
+      g_shards.erase(it);
+    }
+  }
+  if (vram->soma_voltage) {
+    (void)hipFree(vram->soma_voltage);
+    vram->soma_voltage = nullptr;
+    vram->soma_flags = nullptr;
+    vram->threshold_offset = nullptr;
+    vram->timers = nullptr;
+    vram->soma_to_axon = nullptr;
+    vram->dendrite_targets = nullptr;
+    vram->dendrite_weights = nullptr;
+    vram->dendrite_timers = nullptr;
+  }
+  if (vram->axon_heads) {
+    (void)hipFree(vram->axon_heads);
+    vram->axon_heads = nullptr;
+  }
+}
+
+// ===========================================================================
+// Day phase (physics.cu:441-506)
+// ===========================================================================
+void cu_reset_burst_counters(const ShardVramPtrs* ptrs, uint32_t padded_n, void* stream) {
+  if (!ptrs || padded_n == 0) return;
+  reset_burst_kernel<<<blocks_for(padded_n), kThreads, 0, (hipStream_t)stream>>>(ptrs->soma_flags, padded_n);
+}
+
+int32_t cu_step_day_phase(const ShardVramPtrs* vram, uint32_t padded_n, uint32_t total_axons, uint32_t v_seg,
+                          uint32_t current_tick, const uint32_t* input_bitmask, uint32_t virtual_offset,
+                          uint32_t num_virtual_axons, const uint32_t* incoming_spikes,
+                          uint32_t num_incoming_spikes, const uint32_t* mapped_soma_ids,
+                          uint8_t* output_history, uint32_t num_outputs, int16_t dopamine, void* stream) {
+  if (!vram) return (int32_t)hipErrorInvalidValue;
+  if (num_virtual_axons > 0 && input_bitmask &&
+      (uint64_t)virtual_offset + num_virtual_axons > (uint64_t)total_axons)
+    return (int32_t)hipErrorInvalidValue;
+  hipError_t e;
+  Shard* s = shard_for(vram, padded_n, total_axons, &e);
+  if (!s) return (int32_t)e;
+  const AxcVariantParameters* lut = device_lut();
+  if (!lut) return (int32_t)hipErrorOutOfMemory;
+  e = enqueue_tick(*vram, s, v_seg, current_tick, input_bitmask, 0, virtual_offset, num_virtual_axons,
+                   incoming_spikes, 0, num_incoming_spikes, mapped_soma_ids, output_history, num_outputs,
+                   (int32_t)dopamine, nullptr, (hipStream_t)stream, lut);
+  return (int32_t)e;
+}
+
+int32_t cu_upload_constant_memory(const AxcVariantParameters* lut) {
+  if (!lut) return (int32_t)hipErrorInvalidValue;
+  AxcVariantParameters* d = device_lut();
+  if (!d) return (int32_t)hipErrorOutOfMemory;
+  // The table is read by in-flight ticks on any stream of this device: wait
+  // for them, then overwrite (the reference's symbol copy has the same effect
+  // on a device without concurrent streams).
+  hipError_t e = hipDeviceSynchronize();
+  if (e != hipSuccess) return (int32_t)e;
+  return (int32_t)hipMemcpy(d, lut, 16 * sizeof(AxcVariantParameters), hipMemcpyHostToDevice);
+}
+
+// ===========================================================================
+// Runtime helpers (bindings.cu:210-291)
+// ===========================================================================
+void* gpu_malloc(size_t size) {
+  void* p = nullptr;
+  if (hipMalloc(&p, size) != hipSuccess) return nullptr;
+  return p;
+}
+void gpu_free(void* p) {
+  if (p) (void)hipFree(p);
+}
+void* gpu_host_alloc(size_t size) {
+  void* p = nullptr;
+  if (hipHostMalloc(&p, size, hipHostMallocDefault) != hipSuccess) return nullptr;
+  return p;
+}
+void gpu_host_free(void* p) {
+  if (p) (void)hipHostFree(p);
+}
+void gpu_memcpy_host_to_device_async(void* dst, const void* src, size_t size, void* stream) {
+  (void)hipMemcpyAsync(dst, src, size, hipMemcpyHostToDevice, (hipStream_t)stream);
+}
+void gpu_memcpy_device_to_host_async(void* dst, const void* src, size_t size, void* stream) {
+  (void)hipMemcpyAsync(dst, src, size, hipMemcpyDeviceToHost, (hipStream_t)stream);
+}
+bool gpu_memcpy_host_to_device(void* dst, const void* src, size_t size) {
+  hipError_t e = hipMemcpy(dst, src, size, hipMemcpyHostToDevice);
+  if (e != hipSuccess) fprintf(stderr, "[gpu_memcpy_host_to_device] %s (size %zu)\n", hipGetErrorString(e), size);
+  return e == hipSuccess;
+}
+bool gpu_memcpy_device_to_host(void* dst, const void* src, size_t size) {
+  hipError_t e = hipMemcpy(dst, src, size, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) fprintf(stderr, "[gpu_memcpy_device_to_host] %s (src=%p, size=%zu)\n", hipGetErrorString(e), src, size);
+  return e == hipSuccess;
+}
+int32_t gpu_stream_create(void** out_stream) {
+  if (!out_stream) return (int32_t)hipErrorInvalidValue;
+  return (int32_t)hipStreamCreateWithFlags((hipStream_t*)out_stream, hipStreamNonBlocking);
+}
+int32_t gpu_stream_destroy(void* stream) { return (int32_t)hipStreamDestroy((hipStream_t)stream); }
+void gpu_stream_synchronize(void* stream) { (void)hipStreamSynchronize((hipStream_t)stream); }
+void gpu_set_device(int32_t id) {
+  hipError_t e = hipSetDevice(id);
+  if (e != hipSuccess) fprintf(stderr, "FATAL HIP ERROR in gpu_set_device: %s\n", hipGetErrorString(e));
+}
+void gpu_device_synchronize(void) { (void)hipDeviceSynchronize(); }
+void gpu_synchronize(void) { (void)hipDeviceSynchronize(); }
+void gpu_load_constants(const void* host_ptr) {
+  (void)cu_upload_constant_memory((const AxcVariantParameters*)host_ptr);
+}
+void update_constant_memory_hot_reload(const AxcVariantParameters* nv, void* stream) {
+  AxcVariantParameters* d = device_lut();
+  if (!d || !nv) return;
+  (void)hipMemcpyAsync(d, nv, 16 * sizeof(AxcVariantParameters), hipMemcpyHostToDevice, (hipStream_t)stream);
+}
+
+// ===========================================================================
+// Night (bindings.cu:450-515, 699-710)
+// ===========================================================================
+void launch_sort_and_prune(const ShardVramPtrs* ptrs, uint32_t padded_n, int16_t prune_threshold) {
+  if (!ptrs || padded_n == 0) return;
+  sort_prune_kernel<<<blocks_for(padded_n, 64), kThreads>>>(ptrs->soma_flags, ptrs->dendrite_targets,
+                                                             ptrs->dendrite_weights, ptrs->dendrite_timers,
+                                                             padded_n, (int32_t)prune_threshold);
+}
+
+// ===========================================================================
+// Multi-zone helpers (bindings.cu:341-436)
+// ===========================================================================
+void launch_extract_outgoing_spikes(const AxcBurstHeads8* heads, const uint32_t* src, const uint32_t* dst_ghost,
+                                    uint32_t count, uint32_t T, uint32_t v_seg, void* out_events,
+                                    uint32_t* out_count, void* stream) {
+  hipStream_t st = (hipStream_t)stream;
+  if (!out_count) return;
+  zero_u32_kernel<<<1, 1, 0, st>>>(out_count);
+  if (count == 0 || v_seg == 0) return;
+  extract_spikes_kernel<<<blocks_for(count), kThreads, 0, st>>>(heads, src, dst_ghost, count, T, v_seg,
+                                                                 (uint32_t*)out_events, out_count);
+}
+
+void launch_ghost_sync(const AxcBurstHeads8* src, AxcBurstHeads8* dst, const uint32_t* si, const uint32_t* di,
+                       uint32_t count, void* stream) {
+  if (count == 0) return;
+  ghost_sync_kernel<<<blocks_for(count), kThreads, 0, (hipStream_t)stream>>>(src, dst, si, di, count);
+}
+
+// ===========================================================================
+// Telemetry (physics.cu:400-434, bindings.cu:780-796)
+// ===========================================================================
+void gpu_reset_telemetry_count(const ShardVramPtrs* ptrs, void* stream) {
+  (void)ptrs;
+  (void)stream;
+}
+
+void launch_extract_telemetry(const ShardVramPtrs* ptrs, uint32_t padded_n, uint32_t* out_ids,
+                              uint32_t* out_count, void* stream) {
+  if (!ptrs || !out_ids || !out_count || padded_n == 0) return;
+  hipError_t e;
+  Shard* s = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = g_shards.find(ptrs->soma_voltage);
+    if (it != g_shards.end()) s = it->second;
+  }
+  if (!s) {
+    s = shard_for(ptrs, padded_n, 1, &e);
+    if (!s) return;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  const uint32_t nb = blocks_for(padded_n);
+  // The spike list buffer (N entries) holds the per-block counts here; it is
+  // rewritten by the next tick before use.
+  telem_count_kernel<<<nb, kThreads, 0, st>>>(ptrs->soma_flags, padded_n, s->sc.spk_list);
+  telem_scan_kernel<<<1, 1024, 0, st>>>(s->sc.spk_list, nb, out_count);
+  telem_write_kernel<<<nb, kThreads, 0, st>>>(ptrs->soma_flags, padded_n, s->sc.spk_list, out_ids);
+}
+
+// ===========================================================================
+// I/O buffers (bindings.cu:717-778)
+// ===========================================================================
+int32_t cu_allocate_io_buffers(uint32_t input_words, uint32_t schedule_capacity, uint32_t output_capacity,
+                               uint32_t** d_input_bitmask, uint32_t** d_incoming_spikes,
+                               uint8_t** d_output_history) {
+  if (!d_input_bitmask || !d_incoming_spikes || !d_output_history) return -1;
+  *d_input_bitmask = nullptr;
+  *d_incoming_spikes = nullptr;
+  *d_output_history = nullptr;
+  if (input_words > 0 && hipMalloc((void**)d_input_bitmask, (size_t)input_words * 4) != hipSuccess) return -1;
+  if (schedule_capacity > 0 && hipMalloc((void**)d_incoming_spikes, (size_t)schedule_capacity * 4) != hipSuccess)
+    return -1;
+  if (output_capacity > 0 && hipMalloc((void**)d_output_history, (size_t)output_capacity) != hipSuccess) return -1;
+  return 0;
+}
+
+int32_t cu_dma_h2d_io(uint32_t* d_in, const uint32_t* h_in, uint32_t words, uint32_t* d_spk, const uint32_t* h_spk,
+                      uint32_t cap, void* stream) {
+  hipStream_t st = (hipStream_t)stream;
+  if (words > 0 && d_in && h_in) (void)hipMemcpyAsync(d_in, h_in, (size_t)words * 4, hipMemcpyHostToDevice, st);
+  if (cap > 0 && d_spk && h_spk) (void)hipMemcpyAsync(d_spk, h_spk, (size_t)cap * 4, hipMemcpyHostToDevice, st);
+  return 0;
+}
+
+int32_t cu_dma_d2h_io(uint8_t* h_out, const uint8_t* d_out, uint32_t cap, void* stream) {
+  if (cap > 0 && d_out && h_out)
+    (void)hipMemcpyAsync(h_out, d_out, cap, hipMemcpyDeviceToHost, (hipStream_t)stream);
+  return 0;
+}
+
+// ===========================================================================
+// Legacy launchers (ffi.rs:193-233): routed to the current kernels on the
+// null stream. The reference's launch_apply_spike_batch passes a host pointer
+// as the count (bindings.cu:316-317); here the count argument is used.
+// ===========================================================================
+static ShardVramPtrs legacy_ptrs(const AxcLegacyVramState& v) {
+  ShardVramPtrs p;
+  p.soma_voltage = v.voltage;
+  p.soma_flags = v.flags;
+  p.threshold_offset = v.threshold_offset;
+  p.timers = v.refractory_timer;
+  p.soma_to_axon = v.soma_to_axon;
+  p.dendrite_targets = v.dendrite_targets;
+  p.dendrite_weights = v.dendrite_weights;
+  p.dendrite_timers = v.dendrite_timers;
+  p.axon_heads = v.axon_heads;
+  return p;
+}
+
+void launch_inject_inputs(AxcLegacyVramState v, const uint32_t* bitmask, uint32_t current_tick,
+                          uint32_t total_virtual_axons) {
+  (void)current_tick;
+  if (!bitmask || total_virtual_axons == 0 || v.total_axons == 0) return;
+  // Inputs at the start of the axon array, stride 1 (bindings.cu:295-302),
+  // without propagation: a push-only pass.
+  hipError_t e;
+  ShardVramPtrs p = legacy_ptrs(v);
+  Shard* s = shard_for(&p, v.padded_n, v.total_axons, &e);
+  if (!s) return;
+  uint32_t n = total_virtual_axons < v.total_axons ? total_virtual_axons : v.total_axons;
+  axon_tick_kernel<<<blocks_for(v.total_axons), kThreads>>>(v.axon_heads, n, 0u, bitmask, 0u, n, s->sc.push_cnt, 0,
+                                                            s->sc.counters, nullptr, 0u, 0u);
+}
+void launch_apply_spike_batch(AxcLegacyVramState v, const void* schedule, uint32_t count) {
+  if (!schedule || count == 0 || v.total_axons == 0) return;
+  hipError_t e;
+  ShardVramPtrs p = legacy_ptrs(v);
+  Shard* s = shard_for(&p, v.padded_n, v.total_axons, &e);
+  if (!s) return;
+  spike_count_kernel<<<blocks_for(count), kThreads>>>((const uint32_t*)schedule, count, v.total_axons, s->sc.push_cnt,
+                                                      nullptr, 0u, 0u);
+  axon_tick_kernel<<<blocks_for(v.total_axons), kThreads>>>(v.axon_heads, v.total_axons, 0u, nullptr, 0u, 0u,
+                                                            s->sc.push_cnt, 1, s->sc.counters, nullptr, 0u, 0u);
+}
+void launch_propagate_axons(AxcLegacyVramState v, uint32_t v_seg) {
+  if (v.total_axons == 0) return;
+  hipError_t e;
+  ShardVramPtrs p = legacy_ptrs(v);
+  Shard* s = shard_for(&p, v.padded_n, v.total_axons, &e);
+  if (!s) return;
+  axon_tick_kernel<<<blocks_for(v.total_axons), kThreads>>>(v.axon_heads, v.total_axons, v_seg, nullptr, 0u, 0u,
+                                                            s->sc.push_cnt, 0, s->sc.counters, nullptr, 0u, 0u);
+}
+void launch_update_neurons(AxcLegacyVramState v, const void* constants_ptr, uint32_t current_tick) {
+  (void)constants_ptr;
+  if (v.padded_n == 0) return;
+  hipError_t e;
+  ShardVramPtrs p = legacy_ptrs(v);
+  Shard* s = shard_for(&p, v.padded_n, v.total_axons, &e);
+  const AxcVariantParameters* lut = device_lut();
+  if (!s || !lut) return;
+  neuron_tick_kernel<<<blocks_for(v.padded_n), kThreads>>>(p, v.padded_n, v.total_axons, lut, nullptr, current_tick,
+                                                           s->sc);
+  push_apply_kernel<<<kListBlocks / 4, kThreads>>>(p.axon_heads, s->sc);
+}
+void launch_apply_gsop(AxcLegacyVramState v) {
+  if (v.padded_n == 0) return;
+  hipError_t e;
+  ShardVramPtrs p = legacy_ptrs(v);
+  Shard* s = shard_for(&p, v.padded_n, v.total_axons, &e);
+  const AxcVariantParameters* lut = device_lut();
+  if (!s || !lut) return;
+  gsop_kernel<<<kListBlocks, kThreads>>>(p, v.padded_n, v.total_axons, lut, nullptr, 0, s->sc);
+}
+void launch_record_readout(AxcLegacyVramState v, const uint32_t* mapped, uint8_t* out, uint32_t current_tick,
+                           uint32_t total_pixels) {
+  if (!mapped || !out || total_pixels == 0) return;
+  readout_kernel<<<blocks_for(total_pixels), kThreads>>>(v.flags, v.padded_n, mapped,
+                                                         out + (size_t)current_tick * total_pixels, total_pixels,
+                                                         nullptr, 0u);
+}
+
+// ===========================================================================
+// Engine extensions
+// ===========================================================================
+int32_t axc_shard_stats_reset(const ShardVramPtrs* vram, void* stream) {
+  if (!vram) return (int32_t)hipErrorInvalidValue;
+  Shard* s = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = g_shards.find(vram->soma_voltage);
+    if (it != g_shards.end()) s = it->second;
+  }
+  if (!s) return (int32_t)hipErrorInvalidValue;
+  hipStream_t st = (hipStream_t)stream;
+  hipError_t e = hipMemsetAsync(s->sc.stats_part, 0, (size_t)s->stats_blocks * kStatsPerBlock * 8, st);
+  if (e == hipSuccess) e = hipMemsetAsync(s->sc.stats_part2, 0, (size_t)kListBlocks * kStatsPerBlock * 8, st);
+  return (int32_t)e;
+}
+
+int32_t axc_shard_stats_read(const ShardVramPtrs* vram, AxcShardStats* out) {
+  if (!vram || !out) return (int32_t)hipErrorInvalidValue;
+  Shard* s = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = g_shards.find(vram->soma_voltage);
+    if (it != g_shards.end()) s = it->second;
+  }
+  if (!s) return (int32_t)hipErrorInvalidValue;
+  std::vector<uint64_t> a((size_t)s->stats_blocks * kStatsPerBlock), b((size_t)kListBlocks * kStatsPerBlock);
+  hipError_t e = hipMemcpy(a.data(), s->sc.stats_part, a.size() * 8, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(b.data(), s->sc.stats_part2, b.size() * 8, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return (int32_t)e;
+  AxcShardStats r{};
+  for (size_t k = 0; k < s->stats_blocks; k++) {
+    r.integrated += a[k * kStatsPerBlock + kStIntegrated];
+    r.slots_scanned += a[k * kStatsPerBlock + kStScanned];
+    r.targets_read += a[k * kStatsPerBlock + kStTargets];
+    r.hits += a[k * kStatsPerBlock + kStHits];
+  }
+  for (size_t k = 0; k < kListBlocks; k++) {
+    r.spikes += b[k * kStatsPerBlock + 0];
+    r.gsop_slots += b[k * kStatsPerBlock + 1];
+    r.gsop_targets += b[k * kStatsPerBlock + 2];
+    r.ticks += b[k * kStatsPerBlock + 3];
+  }
+  *out = r;
+  return 0;
+}
+
+int32_t axc_step_day_phase_batch(const ShardVramPtrs* vram, uint32_t padded_n, uint32_t total_axons, uint32_t v_seg,
+                                 uint32_t tick_base, uint32_t T, const uint32_t* d_bitmask, uint32_t words,
+                                 uint32_t virt_off, uint32_t n_virt, const uint32_t* d_spikes, uint32_t max_spk,
+                                 const uint32_t* h_counts, const uint32_t* d_mapped, uint8_t* d_out, uint32_t n_out,
+                                 int16_t dopamine, void* stream) {
+  if (!vram) return (int32_t)hipErrorInvalidValue;
+  if (n_virt > 0 && d_bitmask && (uint64_t)virt_off + n_virt > (uint64_t)total_axons)
+    return (int32_t)hipErrorInvalidValue;
+  if (T == 0) return 0;
+  hipError_t e;
+  Shard* s = shard_for(vram, padded_n, total_axons, &e);
+  if (!s) return (int32_t)e;
+  const AxcVariantParameters* lut = device_lut();
+  if (!lut) return (int32_t)hipErrorOutOfMemory;
+  hipStream_t st = (hipStream_t)stream;
+
+  GraphKey key;
+  key.T = T;
+  key.words = words;
+  key.virt_off = virt_off;
+  key.n_virt = n_virt;
+  key.max_spk = max_spk;
+  key.n_out = n_out;
+  key.has_bitmask = d_bitmask != nullptr;
+  key.has_spikes = d_spikes != nullptr;
+  key.has_out = d_out != nullptr && d_mapped != nullptr && n_out > 0;
+  key.mapped = d_mapped;
+  key.v_seg = v_seg;
+  key.spike_counts.assign(T, 0);
+  if (d_spikes && h_counts)
+    for (uint32_t t = 0; t < T; t++) key.spike_counts[t] = h_counts[t];
+
+  // The per-batch burst reset of shard.rs:71 (idempotent with the per-tick one).
+  reset_burst_kernel<<<blocks_for(padded_n ? padded_n : 1), kThreads, 0, st>>>(vram->soma_flags, padded_n);
+  TickParams pv;
+  pv.tick_base = tick_base;
+  pv.dopamine = (int32_t)dopamine;
+  pv.bitmask = d_bitmask;
+  pv.spikes = d_spikes;
+  pv.out = d_out;
+  set_params_kernel<<<1, 1, 0, st>>>(s->prm, pv);
+
+  if (!(s->has_graph && s->gkey == key)) {
+    if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
+    if (s->graph) (void)hipGraphDestroy(s->graph);
+    s->gexec = nullptr;
+    s->graph = nullptr;
+    s->has_graph = false;
+    hipStream_t cap;
+    if ((e = hipStreamCreateWithFlags(&cap, hipStreamNonBlocking)) != hipSuccess) return (int32_t)e;
+    e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+      for (uint32_t t = 0; t < T && e == hipSuccess; t++) {
+        e = enqueue_tick(*vram, s, v_seg, t, d_bitmask, words, virt_off, n_virt, d_spikes, max_spk,
+                         key.spike_counts[t], d_mapped, d_out, n_out, 0, s->prm, cap, lut);
+      }
+      hipGraph_t g = nullptr;
+      hipError_t e2 = hipStreamEndCapture(cap, &g);
+      if (e == hipSuccess) e = e2;
+      if (e == hipSuccess) {
+        e = hipGraphInstantiate(&s->gexec, g, nullptr, nullptr, 0);
+        s->graph = g;
+      } else if (g) {
+        (void)hipGraphDestroy(g);
+      }
+    }
+    (void)hipStreamDestroy(cap);
+    if (e != hipSuccess) return (int32_t)e;
+    s->gkey = key;
+    s->has_graph = true;
+  }
+  e = hipGraphLaunch(s->gexec, st);
+  return (int32_t)e;
+}
+
+void axc_host_sort_and_prune(uint8_t* flags, uint32_t* targets, int16_t* weights, uint8_t* timers, uint32_t padded_n,
+                             int16_t prune) {
+  const size_t N = padded_n;
+  auto work = [&](size_t lo, size_t hi) {
+    uint32_t key[kMaxDendrites], t[kMaxDendrites], idx[kMaxDendrites];
+    int16_t w[kMaxDendrites];
+    uint8_t m[kMaxDendrites];
+    for (size_t i = lo; i < hi; i++) {
+      flags[i] &= 0xF1;
+      uint32_t nf = 0, ne = 0;
+      uint32_t empties[kMaxDendrites];
+      for (uint32_t s = 0; s < kMaxDendrites; s++) {
+        size_t col = (size_t)s * N + i;
+        t[s] = targets[col];
+        w[s] = weights[col];
+        m[s] = timers[col];
+        int32_t a = w[s] < 0 ? -(int32_t)w[s] : w[s];
+        key[s] = (uint32_t)a;
+        if (t[s]) idx[nf++] = s; else empties[ne++] = s;
+      }
+      // Filled slots by descending |w|, ties in slot order; empty slots after
+      // them in slot order: the outcome of the reference insertion sort.
+      std::stable_sort(idx, idx + nf, [&](uint32_t x, uint32_t y) { return key[x] > key[y]; });
+      for (uint32_t e = 0; e < ne; e++) idx[nf + e] = empties[e];
+      for (uint32_t r = 0; r < kMaxDendrites; r++) {
+        uint32_t s = idx[r];
+        bool pr = t[s] != 0 && (int32_t)key[s] < (int32_t)prune;
+        size_t col = (size_t)r * N + i;
+        targets[col] = pr ? 0u : t[s];
+        weights[col] = pr ? (int16_t)0 : w[s];
+        timers[col] = pr ? (uint8_t)0 : m[s];
+      }
+    }
+  };
+  unsigned nt = std::thread::hardware_concurrency();
+  nt = nt == 0 ? 1 : (nt > 16 ? 16 : nt);
+  if (N < 4096) nt = 1;
+  std::vector<std::thread> th;
+  size_t chunk = (N + nt - 1) / nt;
+  for (unsigned k = 0; k < nt; k++) {
+    size_t lo = k * chunk, hi = lo + chunk < N ? lo + chunk : N;
+    if (lo >= hi) break;
+    th.emplace_back(work, lo, hi);
+  }
+  for (auto& x : th) x.join();
+}
+
+void axc_kernel_timing_enable(int enable) {
+  std::lock_guard<std::mutex> lk(g_tmu);
+  g_timing.enabled = enable != 0;
+}
+
+int32_t axc_kernel_timing_read(double* ms_out, uint64_t* launches_out, int reset) {
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pend;
+  {
+    std::lock_guard<std::mutex> lk(g_tmu);
+    pend.swap(g_timing.pending);
+  }
+  hipError_t err = hipSuccess;
+  for (auto& p : pend) {
+    hipError_t e = hipEventSynchronize(p.second.second);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, p.second.first, p.second.second);
+    if (e != hipSuccess) err = e;
+    std::lock_guard<std::mutex> lk(g_tmu);
+    g_timing.ms[p.first] += ms;
+    g_timing.launches[p.first] += 1;
+    (void)hipEventDestroy(p.second.first);
+    (void)hipEventDestroy(p.second.second);
+  }
+  std::lock_guard<std::mutex> lk(g_tmu);
+  for (int k = 0; k < kNumTimed; k++) {
+    if (ms_out) ms_out[k] = g_timing.ms[k];
+    if (launches_out) launches_out[k] = g_timing.launches[k];
+    if (reset) {
+      g_timing.ms[k] = 0;
+      g_timing.launches[k] = 0;
+    }
+  }
+  return (int32_t)err;
+}
+
+const char* axc_version(void) { return "axicor_amd day-phase engine 0.1 (gfx950)"; }
+
+}  // extern "C"

@@ -1,0 +1,606 @@
+// day_kernels.hip — gfx950 kernels of the Day Phase and the Night sort&prune.
+//
+// Integer rules restated from genesis-compute/src/cuda/physics.cu (K1-K7) and
+// bindings.cu (K9-K12); the per-kernel comments cite the lines. Data stay in
+// the reference's SoA layout (columnar dendrites, BurstHeads8 heads), so the
+// device pointers handed out by cu_allocate_shard keep their meaning.
+//
+// Work split of one tick (all on the caller's stream):
+//   axon_tick      K2 + K3 counts + K4, one lane per axon, 32 B RMW
+//   neuron_tick    K1 + K5 integrate; spikers compacted into a list with a
+//                  wave ballot and one atomic per workgroup; the push of the
+//                  own axon is counted per axon, not applied
+//   push_apply     applies the counted pushes (snapshot rule: every dendrite
+//                  read of neuron_tick saw the post-K4 heads)
+//   gsop           K6 over the spiker list: one wave per spiker, one lane
+//                  per dendrite slot (two rounds of 64)
+//   readout        K7
+// Everything is integer VALU work; nothing here is a dense contraction.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "day_common.h"
+
+namespace axc {
+
+__device__ __forceinline__ int32_t wadd(int32_t a, int32_t b) { return (int32_t)((uint32_t)a + (uint32_t)b); }
+__device__ __forceinline__ int32_t wsub(int32_t a, int32_t b) { return (int32_t)((uint32_t)a - (uint32_t)b); }
+__device__ __forceinline__ int32_t wmul(int32_t a, int32_t b) { return (int32_t)((uint32_t)a * (uint32_t)b); }
+
+struct Heads8 { uint4 lo, hi; };
+
+__device__ __forceinline__ Heads8 load_heads(const AxcBurstHeads8* __restrict__ heads, uint32_t a) {
+  const uint4* p = reinterpret_cast<const uint4*>(heads + a);
+  Heads8 h;
+  h.lo = p[0];
+  h.hi = p[1];
+  return h;
+}
+
+__device__ __forceinline__ void store_heads(AxcBurstHeads8* __restrict__ heads, uint32_t a, const Heads8& h) {
+  uint4* p = reinterpret_cast<uint4*>(heads + a);
+  p[0] = h.lo;
+  p[1] = h.hi;
+}
+
+// k pushes of push_burst_head (physics.cu:32-41): h[j] <- h[j-k], h[0..k) <- 0.
+__device__ __forceinline__ void push_heads(Heads8& h, uint32_t k) {
+  if (k == 0) return;
+  uint32_t v[8] = {h.lo.x, h.lo.y, h.lo.z, h.lo.w, h.hi.x, h.hi.y, h.hi.z, h.hi.w};
+  uint32_t o[8];
+#pragma unroll
+  for (int j = 0; j < 8; j++) o[j] = (j >= (int)k) ? v[(j - (int)k) & 7] : 0u;
+  h.lo = make_uint4(o[0], o[1], o[2], o[3]);
+  h.hi = make_uint4(o[4], o[5], o[6], o[7]);
+}
+
+// Any head in the active tail of `seg` (physics.cu:198-201; signal.rs:22-25).
+__device__ __forceinline__ bool any_hit(const Heads8& h, uint32_t seg, uint32_t prop) {
+  return ((h.lo.x - seg) < prop) | ((h.lo.y - seg) < prop) | ((h.lo.z - seg) < prop) |
+         ((h.lo.w - seg) < prop) | ((h.hi.x - seg) < prop) | ((h.hi.y - seg) < prop) |
+         ((h.hi.z - seg) < prop) | ((h.hi.w - seg) < prop);
+}
+
+__device__ __forceinline__ uint32_t dist_or_max(uint32_t head, uint32_t seg, uint32_t len) {
+  uint32_t d = head - seg;
+  return d < len ? d : 0xFFFFFFFFu;
+}
+
+// Heads of a packed target; an axon index outside [0, A) reads as all
+// sentinels (the reference would read out of bounds).
+__device__ __forceinline__ Heads8 target_heads(const AxcBurstHeads8* __restrict__ heads, uint32_t A, uint32_t t) {
+  uint32_t a = (t & 0x00FFFFFFu) - 1u;
+  if (a < A) return load_heads(heads, a);
+  Heads8 s;
+  s.lo = make_uint4(kSentinel, kSentinel, kSentinel, kSentinel);
+  s.hi = s.lo;
+  return s;
+}
+
+__device__ __forceinline__ uint32_t tick_of(const TickParams* prm, uint32_t tick_arg) {
+  return prm ? prm->tick_base + tick_arg : tick_arg;
+}
+__device__ __forceinline__ int32_t dopamine_of(const TickParams* prm, int32_t dop_arg) {
+  return prm ? (int32_t)prm->dopamine : dop_arg;
+}
+
+// ---------------------------------------------------------------------------
+// K2 + K3 + K4: pushes from the input bitmask and the spike-schedule counts,
+// then propagation (physics.cu:88-154). Pushes commute with each other, so the
+// k pushes an axon receives this tick are applied at once, before propagation.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void axon_tick_kernel(AxcBurstHeads8* __restrict__ heads, uint32_t A,
+                                                        uint32_t v_seg, const uint32_t* __restrict__ bitmask_arg,
+                                                        uint32_t virt_off, uint32_t n_virt,
+                                                        uint32_t* __restrict__ push_cnt, int use_cnt,
+                                                        uint32_t* __restrict__ counters,
+                                                        const TickParams* __restrict__ prm, uint32_t t_off,
+                                                        uint32_t words) {
+  uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t* __restrict__ bitmask =
+      prm ? (prm->bitmask ? prm->bitmask + (size_t)t_off * words : nullptr) : bitmask_arg;
+  // The list counters of the previous tick were consumed by its gsop and
+  // push_apply launches; this tick's neuron_tick fills them again.
+  if (a < kNumCounters) counters[a] = 0;
+  if (a >= A) return;
+  Heads8 h = load_heads(heads, a);
+  uint32_t k = 0;
+  if (bitmask) {
+    uint32_t i = a - virt_off;  // wraps for a < virt_off
+    if (i < n_virt) k += (bitmask[i >> 5] >> (i & 31)) & 1u;
+  }
+  if (use_cnt) {
+    uint32_t c = push_cnt[a];
+    if (c) {
+      k += c;
+      push_cnt[a] = 0;
+    }
+  }
+  push_heads(h, k > 8 ? 8 : k);
+  h.lo.x += (h.lo.x != kSentinel) ? v_seg : 0u;
+  h.lo.y += (h.lo.y != kSentinel) ? v_seg : 0u;
+  h.lo.z += (h.lo.z != kSentinel) ? v_seg : 0u;
+  h.lo.w += (h.lo.w != kSentinel) ? v_seg : 0u;
+  h.hi.x += (h.hi.x != kSentinel) ? v_seg : 0u;
+  h.hi.y += (h.hi.y != kSentinel) ? v_seg : 0u;
+  h.hi.z += (h.hi.z != kSentinel) ? v_seg : 0u;
+  h.hi.w += (h.hi.w != kSentinel) ? v_seg : 0u;
+  store_heads(heads, a, h);
+}
+
+// K3 (physics.cu:114-132): one count per occurrence of an id < A.
+__global__ __launch_bounds__(256) void spike_count_kernel(const uint32_t* __restrict__ ids_arg, uint32_t n,
+                                                          uint32_t A, uint32_t* __restrict__ push_cnt,
+                                                          const TickParams* __restrict__ prm, uint32_t t_off,
+                                                          uint32_t stride) {
+  uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const uint32_t* __restrict__ ids = prm ? prm->spikes + (size_t)t_off * stride : ids_arg;
+  uint32_t a = ids[j];
+  if (a < A) atomicAdd(&push_cnt[a], 1u);
+}
+
+// ---------------------------------------------------------------------------
+// K1 + K5 (physics.cu:25-30, 159-276), one lane per soma.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void neuron_tick_kernel(ShardVramPtrs vram, uint32_t N, uint32_t A,
+                                                          const AxcVariantParameters* __restrict__ lut_g,
+                                                          const TickParams* __restrict__ prm, uint32_t tick_arg,
+                                                          ShardScratch sc) {
+  __shared__ AxcVariantParameters lut[16];
+  __shared__ uint32_t wave_cnt[2][4];
+  __shared__ uint32_t blk_base[2];
+  __shared__ uint64_t red[4][4];
+
+  const uint32_t tid = threadIdx.x;
+  if (tid < 16 * 64 / 16) {
+    reinterpret_cast<uint4*>(lut)[tid] = reinterpret_cast<const uint4*>(lut_g)[tid];
+  }
+  __syncthreads();
+
+  const uint32_t i = blockIdx.x * blockDim.x + tid;
+  const uint32_t tick = tick_of(prm, tick_arg);
+  bool fin = false;
+  bool push_own = false;
+  uint32_t my_axon = 0;
+  uint32_t n_int = 0, n_scan = 0, n_tread = 0, n_hits = 0;
+
+  if (i < N) {
+    uint8_t flags = vram.soma_flags[i] & 0xF1;  // K1
+    uint8_t timer = vram.timers[i];
+    flags &= (uint8_t)~0x01;
+    if (timer > 0) {
+      vram.timers[i] = (uint8_t)(timer - 1);
+      vram.soma_flags[i] = flags;
+    } else {
+      n_int = 1;
+      const AxcVariantParameters& p = lut[(flags >> 4) & 0x0F];
+      const uint32_t prop = p.signal_propagation_length;
+      int32_t i_in = 0;
+      const uint32_t* __restrict__ tcol = vram.dendrite_targets + i;
+      const int16_t* __restrict__ wcol = vram.dendrite_weights + i;
+      uint32_t s = 0;
+      bool alive = true;
+      // Four slots per step: the four target loads and the four head gathers
+      // are independent, which keeps several requests in flight per lane.
+      for (; s < kMaxDendrites && alive; s += 4) {
+        uint32_t t[4];
+        int16_t w[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) t[u] = tcol[(size_t)(s + u) * N];
+#pragma unroll
+        for (int u = 0; u < 4; u++) w[u] = wcol[(size_t)(s + u) * N];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          alive = alive && (t[u] != 0);
+          if (alive) {
+            Heads8 h = target_heads(vram.axon_heads, A, t[u]);
+            bool hit = any_hit(h, t[u] >> 24, prop);
+            n_hits += hit;
+            i_in = wadd(i_in, hit ? (int32_t)w[u] : 0);
+            n_scan++;
+          }
+        }
+      }
+      n_tread = n_scan < kMaxDendrites ? n_scan + 1 : kMaxDendrites;
+
+      int32_t thr = vram.threshold_offset[i];
+      int32_t decayed = wsub(thr, (int32_t)p.homeostasis_decay);
+      thr = decayed & ~(decayed >> 31);
+      int32_t leak = p.leak_rate;
+      if (p.adaptive_mode == 1) {
+        int32_t add = wmul(thr, (int32_t)p.adaptive_leak_gain) >> 8;
+        leak = wadd(leak, add);
+        int32_t over = wsub(leak, p.adaptive_leak_max);
+        leak = wsub(leak, over & ~(over >> 31));
+      }
+      int32_t v = wadd(vram.soma_voltage[i], i_in);
+      int32_t diff = wsub(v, p.rest_potential);
+      int32_t sign = (diff > 0) - (diff < 0);
+      int32_t abs_diff = wmul(diff, sign);
+      int32_t leaked = wsub(abs_diff, leak);
+      leaked = leaked & ~(leaked >> 31);
+      v = wadd(p.rest_potential, wmul(sign, leaked));
+      int32_t eff = wadd(p.threshold, thr);
+      bool glif = v >= eff;
+      uint32_t period = p.spontaneous_firing_period_ticks;
+      bool hb = period > 0 && ((tick + i * 104729u) % period) == 0;
+      fin = glif || hb;
+      if (glif) {
+        v = p.rest_potential;
+        thr = wadd(thr, p.homeostasis_penalty);
+      }
+      uint8_t new_timer = glif ? p.refractory_period : (uint8_t)0;
+      if (fin) {
+        flags = (uint8_t)((flags & 0xFC) | 0x03);
+        my_axon = vram.soma_to_axon[i];
+        push_own = (my_axon != kNoAxon) && (my_axon < A);
+      } else {
+        flags = (uint8_t)(flags & ~0x01);
+      }
+      uint8_t bc = (flags >> 1) & 0x07;
+      bc = (uint8_t)(bc + ((fin && bc < 7) ? 1 : 0));
+      vram.soma_voltage[i] = v;
+      vram.soma_flags[i] = (uint8_t)((flags & 0xF0) | (bc << 1) | (fin ? 1 : 0));
+      vram.threshold_offset[i] = thr;
+      vram.timers[i] = new_timer;
+    }
+  }
+
+  // Push counts: one atomic per spiking soma with an axon, on distinct
+  // addresses unless somas share an axon. The first count of an axon lists it.
+  bool own_first = false;
+  if (push_own) own_first = atomicAdd(&sc.push_cnt[my_axon], 1u) == 0u;
+
+  // Compaction of spikers and first-owners: ballot per wave, one atomic per
+  // workgroup and list.
+  const uint32_t lane = tid & 63u, wv = tid >> 6;
+  const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  uint64_t bs = __ballot(fin);
+  uint64_t bo = __ballot(own_first);
+  if (lane == 0) {
+    wave_cnt[0][wv] = (uint32_t)__popcll(bs);
+    wave_cnt[1][wv] = (uint32_t)__popcll(bo);
+  }
+  // Block partial sums of the counters.
+  uint32_t r0 = n_int, r1 = n_scan, r2 = n_tread, r3 = n_hits;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    r0 += __shfl_xor(r0, off, 64);
+    r1 += __shfl_xor(r1, off, 64);
+    r2 += __shfl_xor(r2, off, 64);
+    r3 += __shfl_xor(r3, off, 64);
+  }
+  if (lane == 0) {
+    red[wv][0] = r0;
+    red[wv][1] = r1;
+    red[wv][2] = r2;
+    red[wv][3] = r3;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    uint32_t cs = wave_cnt[0][0] + wave_cnt[0][1] + wave_cnt[0][2] + wave_cnt[0][3];
+    uint32_t co = wave_cnt[1][0] + wave_cnt[1][1] + wave_cnt[1][2] + wave_cnt[1][3];
+    blk_base[0] = cs ? atomicAdd(&sc.counters[kCntSpikes], cs) : 0u;
+    blk_base[1] = co ? atomicAdd(&sc.counters[kCntOwners], co) : 0u;
+    uint64_t* part = sc.stats_part + (size_t)blockIdx.x * kStatsPerBlock;
+    part[kStIntegrated] += red[0][0] + red[1][0] + red[2][0] + red[3][0];
+    part[kStScanned] += red[0][1] + red[1][1] + red[2][1] + red[3][1];
+    part[kStTargets] += red[0][2] + red[1][2] + red[2][2] + red[3][2];
+    part[kStHits] += red[0][3] + red[1][3] + red[2][3] + red[3][3];
+  }
+  __syncthreads();
+  uint32_t pre_s = blk_base[0], pre_o = blk_base[1];
+  for (uint32_t w = 0; w < wv; w++) {
+    pre_s += wave_cnt[0][w];
+    pre_o += wave_cnt[1][w];
+  }
+  if (fin) sc.spk_list[pre_s + (uint32_t)__popcll(bs & below)] = i;
+  if (own_first) sc.own_list[pre_o + (uint32_t)__popcll(bo & below)] = my_axon;
+}
+
+// Applies the counted pushes of this tick's spikes (physics.cu:253-262 with
+// the snapshot rule). One lane per listed axon; clears the count.
+__global__ __launch_bounds__(256) void push_apply_kernel(AxcBurstHeads8* __restrict__ heads, ShardScratch sc) {
+  const uint32_t n = sc.counters[kCntOwners];
+  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+    uint32_t a = sc.own_list[j];
+    uint32_t c = sc.push_cnt[a];
+    sc.push_cnt[a] = 0;
+    Heads8 h = load_heads(heads, a);
+    push_heads(h, c > 8 ? 8 : c);
+    store_heads(heads, a, h);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// K6 (physics.cu:281-373): one wave per spiking soma, lane = dendrite slot.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void gsop_kernel(ShardVramPtrs vram, uint32_t N, uint32_t A,
+                                                   const AxcVariantParameters* __restrict__ lut_g,
+                                                   const TickParams* __restrict__ prm, int32_t dop_arg,
+                                                   ShardScratch sc) {
+  __shared__ AxcVariantParameters lut[16];
+  __shared__ uint64_t red[4][3];
+  const uint32_t tid = threadIdx.x;
+  if (tid < 64) reinterpret_cast<uint4*>(lut)[tid] = reinterpret_cast<const uint4*>(lut_g)[tid];
+  __syncthreads();
+
+  const uint32_t lane = tid & 63u, wv = tid >> 6;
+  const uint32_t n_spk = sc.counters[kCntSpikes];
+  const int32_t dop = dopamine_of(prm, dop_arg);
+  const uint32_t waves = gridDim.x * 4;
+  uint32_t cnt_spk = 0, cnt_slots = 0, cnt_tgt = 0;
+
+  for (uint32_t k = blockIdx.x * 4 + wv; k < n_spk; k += waves) {
+    const uint32_t i = sc.spk_list[k];
+    const uint8_t flags = vram.soma_flags[i];
+    const uint8_t bc = (flags >> 1) & 0x07;
+    const int32_t bm = bc > 0 ? bc : 1;
+    const AxcVariantParameters& p = lut[(flags >> 4) & 0x0F];
+    const uint32_t len = p.signal_propagation_length;
+    const int32_t pot_mod = wmul(dop, (int32_t)p.d1_affinity) >> 7;
+    const int32_t dep_mod = wmul(dop, (int32_t)p.d2_affinity) >> 7;
+    const int32_t raw_pot = wadd((int32_t)p.gsop_potentiation, pot_mod);
+    const int32_t raw_dep = wsub((int32_t)p.gsop_depression, dep_mod);
+    const int32_t final_dep = raw_dep & ~(raw_dep >> 31);
+    uint32_t filled = 0;
+    bool done = false;
+    for (uint32_t base = 0; base < kMaxDendrites && !done; base += 64) {
+      const uint32_t s = base + lane;
+      const size_t col = (size_t)s * N + i;
+      const uint32_t t = vram.dendrite_targets[col];
+      const uint64_t empty = __ballot(t == 0);
+      const uint32_t first_empty = empty ? (uint32_t)__ffsll((long long)empty) - 1u : 64u;
+      if (lane < first_empty) {
+        Heads8 h = target_heads(vram.axon_heads, A, t);
+        const uint32_t seg = t >> 24;
+        uint32_t md = dist_or_max(h.lo.x, seg, len);
+        md = min(md, dist_or_max(h.lo.y, seg, len));
+        md = min(md, dist_or_max(h.lo.z, seg, len));
+        md = min(md, dist_or_max(h.lo.w, seg, len));
+        md = min(md, dist_or_max(h.hi.x, seg, len));
+        md = min(md, dist_or_max(h.hi.y, seg, len));
+        md = min(md, dist_or_max(h.hi.z, seg, len));
+        md = min(md, dist_or_max(h.hi.w, seg, len));
+        const bool act = md != 0xFFFFFFFFu;
+        const int16_t w = vram.dendrite_weights[col];
+        const int32_t sign = (w >= 0) ? 1 : -1;
+        int32_t abs_w = (int32_t)w;
+        abs_w = abs_w < 0 ? -abs_w : abs_w;
+        uint32_t rank = (uint32_t)abs_w >> 11;
+        rank = rank > 15 ? 15 : rank;
+        const int32_t inertia = p.inertia_curve[rank];
+        const int32_t dpot = wmul(wmul(raw_pot, inertia), bm) >> 7;
+        const int32_t ddep = wmul(wmul(final_dep, inertia), bm) >> 7;
+        const uint32_t cs = act ? (md >> 4) : 0u;
+        int32_t delta = act ? (dpot >> cs) : wsub(0, ddep);
+        delta = wmul(delta, 128) >> (7 + cs);
+        int32_t na = wadd(abs_w, delta);
+        na = na & ~(na >> 31);
+        na = na > 32767 ? 32767 : na;
+        vram.dendrite_weights[col] = (int16_t)wmul(na, sign);
+      }
+      filled += first_empty;
+      done = first_empty < 64;
+    }
+    cnt_spk += 1;
+    cnt_slots += filled;
+    cnt_tgt += filled < kMaxDendrites ? filled + 1 : kMaxDendrites;
+  }
+  if (lane == 0) {
+    red[wv][0] = cnt_spk;
+    red[wv][1] = cnt_slots;
+    red[wv][2] = cnt_tgt;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    uint64_t* part = sc.stats_part2 + (size_t)blockIdx.x * kStatsPerBlock;
+    part[0] += red[0][0] + red[1][0] + red[2][0] + red[3][0];
+    part[1] += red[0][1] + red[1][1] + red[2][1] + red[3][1];
+    part[2] += red[0][2] + red[1][2] + red[2][2] + red[3][2];
+    if (blockIdx.x == 0) part[3] += 1;  // ticks
+  }
+}
+
+// K7 (physics.cu:378-395); ids outside [0, N) read 0.
+__global__ __launch_bounds__(256) void readout_kernel(const uint8_t* __restrict__ flags, uint32_t N,
+                                                      const uint32_t* __restrict__ mapped,
+                                                      uint8_t* __restrict__ out_arg, uint32_t n_out,
+                                                      const TickParams* __restrict__ prm, uint32_t t_off) {
+  uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n_out) return;
+  uint8_t* __restrict__ out = prm ? prm->out + (size_t)t_off * n_out : out_arg;
+  uint32_t id = mapped[j];
+  out[j] = (id != kNoAxon && id < N) ? (flags[id] & 1u) : 0u;
+}
+
+// K1 alone (physics.cu:25-30).
+__global__ __launch_bounds__(256) void reset_burst_kernel(uint8_t* __restrict__ flags, uint32_t N) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < N) flags[i] &= 0xF1;
+}
+
+// Zeroes the per-tick list counters (a fill kernel, so the batch graph holds
+// no memset node).
+__global__ void clear_counters_kernel(uint32_t* __restrict__ counters) {
+  if (threadIdx.x < kNumCounters) counters[threadIdx.x] = 0;
+}
+
+__global__ void set_params_kernel(TickParams* __restrict__ prm, TickParams v) { *prm = v; }
+
+// K10 init_sentinels_kernel (bindings.cu:560-568).
+__global__ __launch_bounds__(256) void init_sentinels_kernel(AxcBurstHeads8* __restrict__ heads, uint32_t A) {
+  uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= A) return;
+  Heads8 h;
+  h.lo = make_uint4(kSentinel, kSentinel, kSentinel, kSentinel);
+  h.hi = h.lo;
+  store_heads(heads, a, h);
+}
+
+// ---------------------------------------------------------------------------
+// K9 sort_and_prune (bindings.cu:450-515). The reference's per-thread
+// insertion sort is a stable sort by the key (filled, |w|) in descending
+// order, empties keeping their relative order after the filled slots. Here a
+// workgroup loads the 128x64 tile of 64 consecutive somas with coalesced row
+// reads into LDS, each wave ranks the 128 slots of one soma at a time
+// (rank = #greater keys + #equal keys before it), scatters, prunes, and the
+// tile is stored back with coalesced rows.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void sort_prune_kernel(uint8_t* __restrict__ flags,
+                                                         uint32_t* __restrict__ targets,
+                                                         int16_t* __restrict__ weights,
+                                                         uint8_t* __restrict__ timers, uint32_t N,
+                                                         int32_t prune) {
+  __shared__ uint32_t sT[kMaxDendrites][64];
+  __shared__ int16_t sW[kMaxDendrites][64];
+  __shared__ uint8_t sM[kMaxDendrites][64];
+  __shared__ uint32_t keys[4][kMaxDendrites];
+  const uint32_t tid = threadIdx.x;
+  const uint32_t n0 = blockIdx.x * 64;
+  const uint32_t lane = tid & 63u, wv = tid >> 6;
+
+  for (uint32_t r = wv; r < kMaxDendrites; r += 4) {
+    uint32_t i = n0 + lane;
+    size_t col = (size_t)r * N + i;
+    bool ok = i < N;
+    sT[r][lane] = ok ? targets[col] : 0u;
+    sW[r][lane] = ok ? weights[col] : (int16_t)0;
+    sM[r][lane] = ok ? timers[col] : (uint8_t)0;
+  }
+  if (tid < 64 && n0 + tid < N) flags[n0 + tid] &= 0xF1;
+  __syncthreads();
+
+  // 16 rounds; in each, wave wv handles column c = 4*round + wv. Every wave
+  // runs every round so the workgroup barriers below are uniform.
+  for (uint32_t rnd = 0; rnd < 16; rnd++) {
+    const uint32_t c = rnd * 4 + wv;
+    uint32_t t0 = sT[lane][c], t1 = sT[lane + 64][c];
+    int16_t w0 = sW[lane][c], w1 = sW[lane + 64][c];
+    uint8_t m0 = sM[lane][c], m1 = sM[lane + 64][c];
+    int32_t a0 = w0 < 0 ? -(int32_t)w0 : w0, a1 = w1 < 0 ? -(int32_t)w1 : w1;
+    uint32_t k0 = t0 ? (0x20000u | (uint32_t)a0) : 0u;
+    uint32_t k1 = t1 ? (0x20000u | (uint32_t)a1) : 0u;
+    keys[wv][lane] = k0;
+    keys[wv][lane + 64] = k1;
+    __syncthreads();
+    uint32_t r0 = 0, r1 = 0;
+    for (uint32_t j = 0; j < kMaxDendrites; j++) {
+      uint32_t kj = keys[wv][j];
+      r0 += (kj > k0) || (kj == k0 && j < lane);
+      r1 += (kj > k1) || (kj == k1 && j < lane + 64);
+    }
+    // Prune (bindings.cu:497-505) and scatter into the soma's column. The
+    // column was read by this wave only, before the barrier above.
+    if (t0 != 0 && a0 < prune) { t0 = 0; w0 = 0; m0 = 0; }
+    if (t1 != 0 && a1 < prune) { t1 = 0; w1 = 0; m1 = 0; }
+    sT[r0][c] = t0; sW[r0][c] = w0; sM[r0][c] = m0;
+    sT[r1][c] = t1; sW[r1][c] = w1; sM[r1][c] = m1;
+    __syncthreads();
+  }
+
+  for (uint32_t r = wv; r < kMaxDendrites; r += 4) {
+    uint32_t i = n0 + lane;
+    if (i >= N) continue;
+    size_t col = (size_t)r * N + i;
+    targets[col] = sT[r][lane];
+    weights[col] = sW[r][lane];
+    timers[col] = sM[r][lane];
+  }
+}
+
+// K11 ghost_sync_kernel (bindings.cu:341-359).
+__global__ __launch_bounds__(256) void ghost_sync_kernel(const AxcBurstHeads8* __restrict__ src,
+                                                         AxcBurstHeads8* __restrict__ dst,
+                                                         const uint32_t* __restrict__ si,
+                                                         const uint32_t* __restrict__ di, uint32_t count) {
+  uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= count) return;
+  uint32_t s = si[j];
+  if (s < 0x80000000u) store_heads(dst, di[j], load_heads(src, s));
+}
+
+// K12 extract_outgoing_spikes_kernel (bindings.cu:378-418).
+__global__ __launch_bounds__(256) void extract_spikes_kernel(const AxcBurstHeads8* __restrict__ heads,
+                                                             const uint32_t* __restrict__ si,
+                                                             const uint32_t* __restrict__ gid, uint32_t count,
+                                                             uint32_t T, uint32_t v_seg,
+                                                             uint32_t* __restrict__ out_events,
+                                                             uint32_t* __restrict__ out_count) {
+  uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= count) return;
+  uint32_t a = si[j];
+  if (a >= 0x80000000u) return;
+  Heads8 h = load_heads(heads, a);
+  uint32_t g = gid[j];
+  uint32_t v[8] = {h.lo.x, h.lo.y, h.lo.z, h.lo.w, h.hi.x, h.hi.y, h.hi.z, h.hi.w};
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    if (v[k] >= 0x70000000u) continue;
+    uint32_t ts = v[k] / v_seg;
+    if (ts < T) {
+      uint32_t o = atomicAdd(out_count, 1u);
+      out_events[2 * o] = g;
+      out_events[2 * o + 1] = T - 1 - ts;
+    }
+  }
+}
+
+// Telemetry (physics.cu:400-434) with ids in ascending order: per-block
+// counts, one-block exclusive scan, then the write.
+__global__ __launch_bounds__(256) void telem_count_kernel(const uint8_t* __restrict__ flags, uint32_t N,
+                                                          uint32_t* __restrict__ blk_cnt) {
+  __shared__ uint32_t wc[4];
+  uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  uint64_t b = __ballot(i < N && (flags[i] & 1u));
+  if ((threadIdx.x & 63) == 0) wc[threadIdx.x >> 6] = (uint32_t)__popcll(b);
+  __syncthreads();
+  if (threadIdx.x == 0) blk_cnt[blockIdx.x] = wc[0] + wc[1] + wc[2] + wc[3];
+}
+
+__global__ __launch_bounds__(1024) void telem_scan_kernel(uint32_t* __restrict__ blk_cnt, uint32_t nb,
+                                                          uint32_t* __restrict__ out_count) {
+  __shared__ uint32_t carry;
+  __shared__ uint32_t tmp[1024];
+  // Appends after what *out_count already holds, like the reference's
+  // atomicAdd on the caller's counter.
+  if (threadIdx.x == 0) carry = *out_count;
+  __syncthreads();
+  for (uint32_t base = 0; base < nb; base += 1024) {
+    uint32_t j = base + threadIdx.x;
+    uint32_t v = j < nb ? blk_cnt[j] : 0;
+    tmp[threadIdx.x] = v;
+    __syncthreads();
+    for (uint32_t off = 1; off < 1024; off <<= 1) {
+      uint32_t x = threadIdx.x >= off ? tmp[threadIdx.x - off] : 0;
+      __syncthreads();
+      tmp[threadIdx.x] += x;
+      __syncthreads();
+    }
+    if (j < nb) blk_cnt[j] = carry + tmp[threadIdx.x] - v;
+    __syncthreads();
+    if (threadIdx.x == 1023) carry += tmp[1023];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out_count = carry;
+}
+
+__global__ void zero_u32_kernel(uint32_t* __restrict__ p) { *p = 0; }
+
+__global__ __launch_bounds__(256) void telem_write_kernel(const uint8_t* __restrict__ flags, uint32_t N,
+                                                          const uint32_t* __restrict__ blk_off,
+                                                          uint32_t* __restrict__ out_ids) {
+  __shared__ uint32_t wc[4];
+  uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  bool s = i < N && (flags[i] & 1u);
+  uint64_t b = __ballot(s);
+  uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0) wc[wv] = (uint32_t)__popcll(b);
+  __syncthreads();
+  uint32_t off = blk_off[blockIdx.x];
+  for (uint32_t w = 0; w < wv; w++) off += wc[w];
+  uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+  if (s) out_ids[off + (uint32_t)__popcll(b & below)] = i;
+}
+
+}  // namespace axc

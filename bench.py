@@ -1,0 +1,277 @@
+#!/usr/bin/env python3
+"""Day-Phase throughput benchmark on one MI355X.
+
+Workload (default ``c2``): one brain of 1,500,000 integer GLIF neurons of 4
+types (excitatory/inhibitory x fast/slow leak), 100 dendrite slots each
+(150M synapses), 128x128x64 voxel grid, axons of 1-255 segments, |w| in
+[64,512], 75,000 input axons driven by Bernoulli p=0.02 per tick, GSOP on.
+A step is one BSP batch of ``--ticks`` ticks (default 50) through the engine's
+batch entry point (one captured HIP graph per batch); inputs for every tick
+are resident in HBM before the timed region starts.
+
+Prints one JSON line (metric, value = synaptic events/s, roofline of the
+dominant kernel, cpu_baseline of the CPU oracle on the same host, parity of
+the first ticks against that oracle). Progress goes to stderr.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+METRIC = "synaptic events/s at 1.5M neurons (% HBM peak); batched vs serial brains/s"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes(st: dict, N: int, A: int, words: int, n_out: int = 0) -> dict:
+    """Compulsory traffic of the reference algorithm over the counted ticks:
+    2N + 18|R| + 2(N-|R|) + 4 sum_R c_i + 2H + 64A + 4 words + 4|F|
+    + sum_F (4 c_i + 4 S_i) (+ 5 n_out) per tick, with |R|, c_i, H, |F|, S_i
+    counted exactly by the engine. Split per kernel: neuron_tick owns the soma
+    and dendrite-scan terms and the soma_to_axon reads of spikers, axon_tick
+    the heads and the bitmask, gsop the GSOP terms."""
+    T = st["ticks"]
+    R = st["integrated"]
+    neuron = 2 * N * T + 18 * R + 2 * (N * T - R) + 4 * st["targets_read"] + 2 * st["hits"] + 4 * st["spikes"]
+    axon = 64 * A * T + 4 * words * T
+    gsop = 4 * st["gsop_targets"] + 4 * st["gsop_slots"]
+    readout = 5 * n_out * T
+    return {"neuron_tick": neuron, "axon_tick": axon, "gsop": gsop, "readout": readout,
+            "total": neuron + axon + gsop + readout}
+
+
+def cpu_baseline(brain, inputs_host: np.ndarray, seconds: float, max_ticks: int):
+    """The CPU oracle on a copy of the initial brain, one tick at a time,
+    until ``seconds`` of work or ``max_ticks``. Returns (state, heads, ticks,
+    elapsed, stats)."""
+    from oracle import oracle as O
+    state = brain.state.copy()
+    heads = brain.heads.copy()
+    ticks = 0
+    tot = None
+    t0 = time.perf_counter()
+    while ticks < max_ticks:
+        _, st = O.step_batch(state, brain.padded_n, heads, brain.total_axons, brain.lut, brain.spec.v_seg, ticks, 1,
+                             inputs_host[ticks:ticks + 1], brain.virtual_offset, brain.n_inputs)
+        tot = st if tot is None else {k: tot[k] + st[k] for k in st}
+        ticks += 1
+        if time.perf_counter() - t0 >= seconds:
+            break
+    return state, heads, ticks, time.perf_counter() - t0, tot
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--ticks", type=int, default=50, help="ticks per step (one BSP batch)")
+    ap.add_argument("--workload", default="c2", choices=["c2", "c1"])
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--cpu-max-ticks", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--timing-ticks", type=int, default=10, help="per-tick launches timed with HIP events")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: every GPU box here has one GPU and the engine runs one brain per GPU. "
+              "Multi-GPU sharding is described in DESIGN.md and not run.", file=sys.stderr)
+        return 2
+
+    from axicor_amd import brain as B
+    from axicor_amd._ffi import TIMED_KERNELS, day_lib
+    from axicor_amd.engine import DeviceBuffer, ShardEngine, VramState, upload_constant_memory
+
+    lib = day_lib()
+    lib.gpu_set_device(0)
+
+    spec = B.spec_c2() if args.workload == "c2" else B.spec_c1()
+    t0 = time.perf_counter()
+    brain = B.generate(spec)
+    N, A = brain.padded_n, brain.total_axons
+    log(f"generated {args.workload}: padded_n={N} axons={A} state={brain.state.nbytes / 1e9:.2f} GB "
+        f"in {time.perf_counter() - t0:.1f}s")
+
+    T = args.ticks
+    total_ticks = (args.warmup + args.steps) * T
+    cpu_ticks_cap = 0 if args.no_cpu else args.cpu_max_ticks
+    n_in_ticks = max(total_ticks, cpu_ticks_cap)
+    t0 = time.perf_counter()
+    inputs = brain.inputs(0, n_in_ticks)
+    words = brain.input_words
+    log(f"inputs {inputs.shape} in {time.perf_counter() - t0:.1f}s")
+
+    # ---- CPU baseline (the oracle on the host cores) -----------------------
+    cpu = None
+    if not args.no_cpu:
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        log(f"cpu oracle: up to {args.cpu_seconds}s / {cpu_ticks_cap} ticks on {threads} threads")
+        c_state, c_heads, c_ticks, c_el, c_st = cpu_baseline(brain, inputs, args.cpu_seconds, cpu_ticks_cap)
+        cpu = {"state": c_state, "heads": c_heads, "ticks": c_ticks, "elapsed": c_el, "stats": c_st,
+               "threads": threads}
+        log(f"cpu oracle: {c_ticks} ticks in {c_el:.2f}s, {c_st['slots_scanned'] / c_el:.3e} events/s")
+
+    # ---- GPU ----------------------------------------------------------------
+    upload_constant_memory(brain.lut)
+    vram = VramState(N, A)
+    vram.upload_state(brain.state)
+    vram.upload_axon_heads(brain.heads)
+    eng = ShardEngine(vram)
+    d_in = DeviceBuffer.from_array(inputs)
+    ptrs = C.byref(vram.ptrs)
+    tick = 0
+
+    def batch(ticks: int):
+        nonlocal tick
+        err = lib.axc_step_day_phase_batch(ptrs, N, A, brain.spec.v_seg, tick, ticks, d_in.ptr + tick * words * 4,
+                                           words, brain.virtual_offset, brain.n_inputs, None, 0, None, None, None,
+                                           0, 0, eng.stream)
+        if err != 0:
+            raise RuntimeError(f"axc_step_day_phase_batch failed: {err}")
+        tick += ticks
+
+    parity = None
+    if cpu is not None:
+        batch(cpu["ticks"])
+        eng.synchronize()
+        g_state, g_heads = vram.download_state(), vram.download_axons()
+        ok = bool(np.array_equal(g_state, cpu["state"]) and np.array_equal(g_heads, cpu["heads"]))
+        parity = {"ticks": cpu["ticks"], "bit_exact": ok}
+        log(f"parity vs CPU oracle after {cpu['ticks']} ticks: {'bit-exact' if ok else 'MISMATCH'}")
+        del g_state, g_heads
+        if not ok:
+            print(json.dumps({"error": "GPU state differs from the CPU oracle", "parity": parity}))
+            return 1
+
+    # Warmup up to a batch boundary so the timed batches replay one graph.
+    rem = (args.warmup * T - tick) if tick < args.warmup * T else 0
+    while rem > 0:
+        k = min(T, rem)
+        batch(k)
+        rem -= k
+    eng.synchronize()
+    log(f"warmup done at tick {tick}")
+
+    eng.stats_reset()
+    start = C.c_void_p()
+    stop = C.c_void_p()
+    hip = C.CDLL("libamdhip64.so")
+    hip.hipEventCreate(C.byref(start))
+    hip.hipEventCreate(C.byref(stop))
+    eng.synchronize()
+    wall0 = time.perf_counter()
+    hip.hipEventRecord(start, C.c_void_p(eng.stream))
+    for s in range(args.steps):
+        batch(T)
+    hip.hipEventRecord(stop, C.c_void_p(eng.stream))
+    eng.synchronize()
+    wall = time.perf_counter() - wall0
+    ms = C.c_float()
+    hip.hipEventElapsedTime(C.byref(ms), start, stop)
+    elapsed = ms.value / 1e3
+    st = eng.stats()
+    log(f"timed {args.steps} steps x {T} ticks: {elapsed * 1e3:.1f} ms (wall {wall * 1e3:.1f} ms)")
+
+    ab = algorithmic_bytes(st, N, A, words)
+    ticks_timed = st["ticks"]
+    events = st["slots_scanned"] / elapsed
+    tick_frac = ab["total"] / elapsed / (HBM_PEAK_GBS * 1e9)
+
+    # ---- per-kernel durations (HIP events on the engine's stream) ----------
+    lib.axc_kernel_timing_read(None, None, 1)
+    lib.axc_kernel_timing_enable(1)
+    eng.stats_reset()
+    for k in range(args.timing_ticks):
+        err = lib.cu_step_day_phase(ptrs, N, A, brain.spec.v_seg, tick, d_in.ptr + (tick % n_in_ticks) * words * 4,
+                                    brain.virtual_offset, brain.n_inputs, None, 0, None, None, 0, 0, eng.stream)
+        if err != 0:
+            raise RuntimeError(f"cu_step_day_phase failed: {err}")
+        tick += 1
+    eng.synchronize()
+    lib.axc_kernel_timing_enable(0)
+    kms = (C.c_double * 6)()
+    kn = (C.c_uint64 * 6)()
+    lib.axc_kernel_timing_read(kms, kn, 1)
+    st_t = eng.stats()
+    ab_t = algorithmic_bytes(st_t, N, A, words)
+    per_kernel = {TIMED_KERNELS[k]: {"avg_us": kms[k] * 1e3 / kn[k], "launches": int(kn[k])}
+                  for k in range(6) if kn[k]}
+    nt_us = per_kernel["neuron_tick"]["avg_us"]
+    nt_bytes = ab_t["neuron_tick"] / st_t["ticks"]
+    achieved = nt_bytes / (nt_us * 1e-6) / 1e9
+    for k in ("axon_tick", "gsop"):
+        if k in per_kernel:
+            b = ab_t[k] / st_t["ticks"]
+            per_kernel[k]["algorithmic_bytes"] = b
+            per_kernel[k]["achieved_GBs"] = b / (per_kernel[k]["avg_us"] * 1e-6) / 1e9
+    per_kernel["neuron_tick"]["algorithmic_bytes"] = nt_bytes
+    per_kernel["neuron_tick"]["achieved_GBs"] = achieved
+    log("per-kernel: " + json.dumps(per_kernel))
+
+    traffic = None
+    tf = os.path.join(REPO, "profiles", "neuron_tick_c2_traffic.json")
+    if os.path.exists(tf):
+        try:
+            traffic = json.load(open(tf)).get("hbm_bytes_per_launch")
+        except Exception:
+            traffic = None
+
+    out = {
+        "metric": METRIC,
+        "value": events,
+        "unit": "synaptic events/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic",
+        "config": {
+            "workload": (f"{args.workload}: {spec.n_neurons} GLIF neurons x {spec.slots_filled} slots, "
+                         f"{spec.n_inputs} inputs at {spec.input_ppm / 1e6:g}/tick, axons "
+                         f"{spec.axon_len[0]}-{spec.axon_len[1]} segments, GSOP on"),
+            "padded_n": N, "total_axons": A, "ticks_per_step": T, "parallelism": "single GPU",
+        },
+        "roofline": {
+            "bound": "hbm", "kernel": "neuron_tick", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+        },
+        "ticks_per_s": ticks_timed / elapsed,
+        "neuron_ticks_per_s": N * ticks_timed / elapsed,
+        "delivered_events_per_s": st["hits"] / elapsed,
+        "gsop_updates_per_s": st["gsop_slots"] / elapsed,
+        "firing_rate": st["spikes"] / (N * ticks_timed),
+        "tick_hbm_frac": tick_frac,
+        "algorithmic_bytes_per_tick": ab["total"] / ticks_timed,
+        "us_per_tick": elapsed * 1e6 / ticks_timed,
+        "per_kernel": per_kernel,
+        "parity": parity,
+    }
+    if cpu is not None:
+        out["cpu_baseline"] = {
+            "value": cpu["stats"]["slots_scanned"] / cpu["elapsed"], "unit": "synaptic events/s",
+            "cores": cpu["threads"], "kind": "port",
+            "sample": f"first {cpu['ticks']} ticks of the same workload, CPU oracle (C, OpenMP over somas)",
+            "ticks_per_s": cpu["ticks"] / cpu["elapsed"],
+        }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

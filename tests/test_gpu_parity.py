@@ -1,0 +1,158 @@
+"""Bit-exact parity of the HIP engine (through the C-ABI) with the CPU oracle.
+
+Every test runs the same seeded brain and inputs on both sides and compares
+all eight `.state` arrays and the axon heads byte for byte (and the output
+raster where outputs are mapped)."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from axicor_amd import brain as B
+from helpers import Pair
+
+pytestmark = pytest.mark.gpu
+
+
+def test_c1_per_tick_then_batches(gpu):
+    """C1 workload: 4,096 neurons x 64 slots, 256 inputs at p=0.05; every tick
+    for the first 20 ticks, then 50-tick batches to tick 1,000 with the Night
+    sort&prune every 250 ticks (prune 15)."""
+    p = Pair(B.generate(B.spec_c1()), n_out=96)
+    for t in range(20):
+        out_ref, _ = p.step(1)
+        np.testing.assert_array_equal(p.outputs(1), out_ref, err_msg=f"outputs tick {t}")
+        p.check(f"tick {t}")
+    while p.tick < 1000:
+        T = min(50, 250 - p.tick % 250)
+        out_ref, _ = p.step(T)
+        np.testing.assert_array_equal(p.outputs(T), out_ref, err_msg=f"outputs batch ending {p.tick}")
+        p.check(f"batch ending tick {p.tick}")
+        if p.tick % 250 == 0:
+            p.night(15)
+            p.check(f"night at tick {p.tick}")
+
+
+def test_graph_batch_equals_oracle(gpu):
+    """The HIP-graph batch path (axc_step_day_phase_batch), replayed several
+    times with the program's own synchronisations in between."""
+    p = Pair(B.generate(B.spec_small(3000, 48, 200, seed=11)), n_out=40)
+    for k in range(6):
+        out_ref, _ = p.step(25, use_graph=True, dopamine=(-300 if k % 2 else 250))
+        np.testing.assert_array_equal(p.outputs(25), out_ref, err_msg=f"graph outputs replay {k}")
+        p.check(f"graph replay {k}")
+
+
+def test_four_types_adaptive_dopamine(gpu):
+    """Four variants (two with the adaptive leak), nonzero dopamine of both
+    signs, padded neurons (n not a multiple of 64), v_seg = 2."""
+    spec = B.spec_small(5000, 100, 300, seed=7, axon_len=(1, 255), lut=B.lut_c2(),
+                        type_ppm=[400_000, 400_000, 100_000, 100_000])
+    spec.v_seg = 2
+    p = Pair(B.generate(spec), n_out=0)
+    for k, dop in enumerate([0, 500, -700, 32767, -32768, 90]):
+        p.step(10, dopamine=dop, v_seg=2)
+        p.check(f"dopamine {dop}")
+
+
+def test_spike_schedule_duplicates_and_bad_ids(gpu):
+    """K3 network spikes: repeated ids push once per occurrence, ids >= A are
+    ignored (physics.cu:127)."""
+    b = B.generate(B.spec_small(2048, 32, 128, seed=5))
+    p = Pair(b, max_spikes=64, max_batch=8)
+    rng = np.random.default_rng(3)
+    T = 8
+    sched = np.zeros((T, 64), np.uint32)
+    counts = rng.integers(0, 64, size=T).astype(np.uint32)
+    for t in range(T):
+        ids = rng.integers(0, b.total_axons, size=64).astype(np.uint32)
+        ids[:6] = ids[6]                # duplicates
+        ids[10] = b.total_axons + 3     # out of range
+        ids[11] = 0xFFFFFFFF
+        sched[t] = ids
+    p.step(T, spikes=sched.reshape(-1), counts=counts)
+    p.check("spike schedule")
+
+
+def test_edge_state_values(gpu):
+    """Hand-placed edge cases: heads close to the u32 wrap and past the
+    sentinel, targets whose axon field is 0 or >= A, two somas sharing one
+    axon, somas without an axon, extreme voltages/thresholds/weights."""
+    b = B.generate(B.spec_small(1024, 64, 64, seed=9))
+    v = b.view()
+    n, A = b.padded_n, b.total_axons
+    h = b.heads.reshape(-1, 8)
+    h[5] = [0xFFFFFFF0, 0x80000001, 0x7FFFFFFF, 0, 3, 0x80000000, 0xFFFFFFFF, 17]
+    h[6] = [1, 2, 3, 4, 5, 6, 7, 8]
+    v.targets[0, 10] = (7 << 24) | 0              # axon field 0 -> index 0xFFFFFFFF
+    v.targets[1, 10] = (3 << 24) | (A + 9)        # axon >= A
+    v.targets[0, 11] = (0 << 24) | (5 + 1)        # seg 0 of the wrapped axon
+    v.targets[5, 12] = 0                          # hole: later slots are ignored
+    v.soma_to_axon[20] = v.soma_to_axon[21]       # shared axon
+    v.soma_to_axon[22] = 0xFFFFFFFF
+    v.soma_to_axon[23] = A + 100                  # out of range: ignored
+    v.voltage[30] = 2**31 - 5
+    v.voltage[31] = -2**31 + 3
+    v.threshold_offset[32] = 2**31 - 10
+    v.weights[:, 40] = -32768
+    v.weights[:, 41] = 32767
+    v.weights[:, 42] = 0
+    v.timers[50] = 255
+    p = Pair(b, n_out=16)
+    for k in range(12):
+        p.step(5, dopamine=(-1000 if k % 3 == 0 else 1000))
+        p.check(f"edge step {k}")
+    p.night(100)
+    p.check("edge night")
+    p.step(5)
+    p.check("edge after night")
+
+
+def test_sort_and_prune_random_columns(gpu):
+    """K9 alone on columns with random fill, holes, ties and garbage in empty
+    slots, at prune thresholds 0, 15, 300 and -5."""
+    b = B.generate(B.spec_small(1500, 128, 0, seed=21))
+    v = b.view()
+    rng = np.random.default_rng(8)
+    v.weights[:] = rng.integers(-600, 600, size=v.weights.shape).astype(np.int16)
+    v.weights[:, ::3] = rng.choice([-20, 20, 15, -15, 0], size=v.weights[:, ::3].shape)
+    holes = rng.random(v.targets.shape) < 0.3
+    v.targets[holes] = 0
+    v.dendrite_timers[:] = rng.integers(0, 255, size=v.dendrite_timers.shape).astype(np.uint8)
+    v.flags[:] = rng.integers(0, 256, size=v.flags.shape).astype(np.uint8)
+    p = Pair(b)
+    for prune in (0, 15, 300, -5):
+        p.night(prune)
+        p.check(f"sort&prune {prune}")
+
+
+def test_telemetry_ascending(gpu):
+    """launch_extract_telemetry lists the spiking somas in ascending order and
+    appends after the caller's count."""
+    from axicor_amd.engine import DeviceBuffer
+    from axicor_amd._ffi import day_lib
+    p = Pair(B.generate(B.spec_small(4096, 32, 256, seed=4)))
+    p.step(3)
+    lib = day_lib()
+    ids = DeviceBuffer(4 * p.b.padded_n)
+    cnt = DeviceBuffer(4)
+    cnt.upload(np.array([0], np.uint32))
+    lib.launch_extract_telemetry(C.byref(p.vram.ptrs), p.b.padded_n, ids.ptr, cnt.ptr, p.eng.stream)
+    p.eng.synchronize()
+    n = int(cnt.download(np.uint32, 1)[0])
+    got = ids.download(np.uint32, n)
+    flags = p.vram.download_state()[4 * p.b.padded_n:5 * p.b.padded_n]
+    want = np.nonzero(flags & 1)[0].astype(np.uint32)
+    np.testing.assert_array_equal(got, want)
+
+
+def test_counters_match_oracle(gpu):
+    """The engine's work counters equal the oracle's exact counts."""
+    p = Pair(B.generate(B.spec_small(3000, 64, 200, seed=2)))
+    p.eng.stats_reset()
+    total = None
+    for k in range(4):
+        _, st = p.step(20)
+        total = st if total is None else {kk: total[kk] + st[kk] for kk in st}
+    got = p.eng.stats()
+    assert got == total, (got, total)
