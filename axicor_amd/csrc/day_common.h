@@ -16,11 +16,24 @@ constexpr uint32_t kCntOwners = 1;  // entries of own_list
 constexpr uint32_t kNumCounters = 4;
 
 // Per-workgroup partial counters (device, accumulated across ticks).
-constexpr uint32_t kStatsPerBlock = 4;
+constexpr uint32_t kStatsPerBlock = 8;
 constexpr uint32_t kStIntegrated = 0;
 constexpr uint32_t kStScanned = 1;
 constexpr uint32_t kStTargets = 2;
 constexpr uint32_t kStHits = 3;
+constexpr uint32_t kStSpikes = 4;
+constexpr uint32_t kStGsopSlots = 5;
+constexpr uint32_t kStGsopTargets = 6;
+constexpr uint32_t kStTicks = 7;
+
+// Compact head snapshot: per axon the eight heads as u16, each min(h, 0xFFFF)
+// (one 16 B load). For every segment s <= 255 and length p <= 255 the tests
+// of the reference give the same answer on the snapshot as on the u32 heads:
+// (h - s) mod 2^32 < p  <=>  (c - s) mod 2^16 < p, and the smallest such
+// distance is the same, because any h >= 510 fails the test either way and a
+// c < s wraps to >= 2^16 - 255 in u16 as h < s does in u32. Entry A (one past
+// the last axon) is all 0xFFFF and stands for an axon with no live head.
+constexpr uint32_t kInertHead = 0xFFFFu;
 
 // Workgroups of the list-driven kernels (gsop, push_apply): fixed so that a
 // HIP graph of a batch does not depend on the number of spikes.
@@ -46,7 +59,8 @@ struct ShardScratch {
   uint32_t* push_cnt;    // [A] pending pushes per axon, zero between uses
   uint32_t* counters;    // [kNumCounters]
   uint64_t* stats_part;  // [ceil(N/256)][kStatsPerBlock] neuron_tick partials
-  uint64_t* stats_part2; // [kListBlocks][kStatsPerBlock] gsop partials: spikes, slots, targets, ticks
+  uint64_t* stats_part2; // [kListBlocks][kStatsPerBlock] gsop partials (slots, targets)
+  uint4* comp[2];        // [A+1] compact head snapshots, used alternately tick by tick
 };
 
 }  // namespace axc

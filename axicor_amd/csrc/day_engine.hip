@@ -84,6 +84,7 @@ struct Shard {
   void* state_base = nullptr;
   ShardScratch sc{};
   uint32_t stats_blocks = 0;
+  uint32_t flip = 0;          // snapshot buffer of the next per-tick call
   TickParams* prm = nullptr;  // device
   bool owned_by_library = false;
   // One cached graph per shard (the common case: the same batch shape every
@@ -139,6 +140,8 @@ void free_scratch(Shard* s) {
   (void)hipFree(s->sc.counters);
   (void)hipFree(s->sc.stats_part);
   (void)hipFree(s->sc.stats_part2);
+  (void)hipFree(s->sc.comp[0]);
+  (void)hipFree(s->sc.comp[1]);
   (void)hipFree(s->prm);
   s->sc = ShardScratch{};
   s->prm = nullptr;
@@ -155,6 +158,11 @@ hipError_t alloc_scratch(Shard* s) {
   if ((e = hipMalloc((void**)&s->sc.stats_part, (size_t)s->stats_blocks * kStatsPerBlock * 8)) != hipSuccess) return e;
   if ((e = hipMalloc((void**)&s->sc.stats_part2, (size_t)kListBlocks * kStatsPerBlock * 8)) != hipSuccess) return e;
   if ((e = hipMalloc((void**)&s->prm, sizeof(TickParams))) != hipSuccess) return e;
+  for (int k = 0; k < 2; k++) {
+    if ((e = hipMalloc((void**)&s->sc.comp[k], (A + 1) * sizeof(uint4))) != hipSuccess) return e;
+    // Entry A is the inert axon; entries [0, A) are rewritten every tick.
+    if ((e = hipMemset(s->sc.comp[k], 0xFF, (A + 1) * sizeof(uint4))) != hipSuccess) return e;
+  }
   if ((e = hipMemset(s->sc.push_cnt, 0, A * 4)) != hipSuccess) return e;
   if ((e = hipMemset(s->sc.counters, 0, kNumCounters * 4)) != hipSuccess) return e;
   if ((e = hipMemset(s->sc.stats_part, 0, (size_t)s->stats_blocks * kStatsPerBlock * 8)) != hipSuccess) return e;
@@ -200,12 +208,18 @@ Shard* shard_for(const ShardVramPtrs* v, uint32_t N, uint32_t A, hipError_t* err
 // pointers are taken from the arguments; otherwise kernels read them from prm
 // (tick = tick_base + t_off, slices at base + t_off*stride) and the pointer
 // arguments only say whether inputs/spikes/outputs are present.
+// snap: compact snapshot buffer of this tick (the other one holds the
+// previous tick's). fuse_prev: apply the previous tick's GSOP inside this
+// tick's neuron pass. gsop_now: launch this tick's GSOP (otherwise the next
+// tick of the batch applies it).
 hipError_t enqueue_tick(const ShardVramPtrs& v, Shard* s, uint32_t v_seg, uint32_t tick,
                         const uint32_t* bitmask, uint32_t words, uint32_t virt_off, uint32_t n_virt,
                         const uint32_t* spikes, uint32_t max_spk, uint32_t n_spk, const uint32_t* mapped,
                         uint8_t* out, uint32_t n_out, int32_t dopamine, const TickParams* prm, hipStream_t st,
-                        const AxcVariantParameters* lut) {
+                        const AxcVariantParameters* lut, uint32_t snap, bool fuse_prev, bool gsop_now) {
   const uint32_t N = s->N, A = s->A;
+  uint4* cur = s->sc.comp[snap & 1];
+  uint4* prev = s->sc.comp[(snap & 1) ^ 1];
   bool timed;
   {
     std::lock_guard<std::mutex> lk(g_tmu);
@@ -222,19 +236,24 @@ hipError_t enqueue_tick(const ShardVramPtrs& v, Shard* s, uint32_t v_seg, uint32
     TimedScope ts(kKAxon, st, timed);
     axon_tick_kernel<<<blocks_for(A > kNumCounters ? A : kNumCounters), kThreads, 0, st>>>(
         v.axon_heads, A, v_seg, k2 ? bitmask : nullptr, virt_off, k2 ? n_virt : 0u, s->sc.push_cnt, k3 ? 1 : 0,
-        s->sc.counters, prm, tick, words);
+        s->sc.counters, prm, tick, words, cur);
   }
   if (N > 0) {
     TimedScope ts(kKNeuron, st, timed);
-    neuron_tick_kernel<<<blocks_for(N), kThreads, 0, st>>>(v, N, A, lut, prm, tick, s->sc);
+    if (fuse_prev)
+      neuron_tick_kernel<true><<<blocks_for(N), kThreads, 0, st>>>(v, N, A, lut, prm, tick, s->sc, cur, prev,
+                                                                   dopamine);
+    else
+      neuron_tick_kernel<false><<<blocks_for(N), kThreads, 0, st>>>(v, N, A, lut, prm, tick, s->sc, cur, prev,
+                                                                    dopamine);
   }
   {
     TimedScope ts(kKPush, st, timed);
-    push_apply_kernel<<<kListBlocks / 4, kThreads, 0, st>>>(v.axon_heads, s->sc);
+    push_apply_kernel<<<kListBlocks / 4, kThreads, 0, st>>>(v.axon_heads, s->sc, cur);
   }
-  {
+  if (gsop_now) {
     TimedScope ts(kKGsop, st, timed);
-    gsop_kernel<<<kListBlocks, kThreads, 0, st>>>(v, N, A, lut, prm, dopamine, s->sc);
+    gsop_kernel<<<kListBlocks, kThreads, 0, st>>>(v, N, A, lut, prm, dopamine, s->sc, cur);
   }
   if (n_out > 0 && mapped != nullptr && out != nullptr) {
     TimedScope ts(kKReadout, st, timed);
@@ -374,7 +393,8 @@ int32_t cu_step_day_phase(const ShardVramPtrs* vram, uint32_t padded_n, uint32_t
   if (!lut) return (int32_t)hipErrorOutOfMemory;
   e = enqueue_tick(*vram, s, v_seg, current_tick, input_bitmask, 0, virtual_offset, num_virtual_axons,
                    incoming_spikes, 0, num_incoming_spikes, mapped_soma_ids, output_history, num_outputs,
-                   (int32_t)dopamine, nullptr, (hipStream_t)stream, lut);
+                   (int32_t)dopamine, nullptr, (hipStream_t)stream, lut, s->flip, false, true);
+  s->flip ^= 1u;
   return (int32_t)e;
 }
 
@@ -569,7 +589,7 @@ void launch_inject_inputs(AxcLegacyVramState v, const uint32_t* bitmask, uint32_
   if (!s) return;
   uint32_t n = total_virtual_axons < v.total_axons ? total_virtual_axons : v.total_axons;
   axon_tick_kernel<<<blocks_for(v.total_axons), kThreads>>>(v.axon_heads, n, 0u, bitmask, 0u, n, s->sc.push_cnt, 0,
-                                                            s->sc.counters, nullptr, 0u, 0u);
+                                                            s->sc.counters, nullptr, 0u, 0u, nullptr);
 }
 void launch_apply_spike_batch(AxcLegacyVramState v, const void* schedule, uint32_t count) {
   if (!schedule || count == 0 || v.total_axons == 0) return;
@@ -580,7 +600,7 @@ void launch_apply_spike_batch(AxcLegacyVramState v, const void* schedule, uint32
   spike_count_kernel<<<blocks_for(count), kThreads>>>((const uint32_t*)schedule, count, v.total_axons, s->sc.push_cnt,
                                                       nullptr, 0u, 0u);
   axon_tick_kernel<<<blocks_for(v.total_axons), kThreads>>>(v.axon_heads, v.total_axons, 0u, nullptr, 0u, 0u,
-                                                            s->sc.push_cnt, 1, s->sc.counters, nullptr, 0u, 0u);
+                                                            s->sc.push_cnt, 1, s->sc.counters, nullptr, 0u, 0u, nullptr);
 }
 void launch_propagate_axons(AxcLegacyVramState v, uint32_t v_seg) {
   if (v.total_axons == 0) return;
@@ -589,7 +609,7 @@ void launch_propagate_axons(AxcLegacyVramState v, uint32_t v_seg) {
   Shard* s = shard_for(&p, v.padded_n, v.total_axons, &e);
   if (!s) return;
   axon_tick_kernel<<<blocks_for(v.total_axons), kThreads>>>(v.axon_heads, v.total_axons, v_seg, nullptr, 0u, 0u,
-                                                            s->sc.push_cnt, 0, s->sc.counters, nullptr, 0u, 0u);
+                                                            s->sc.push_cnt, 0, s->sc.counters, nullptr, 0u, 0u, nullptr);
 }
 void launch_update_neurons(AxcLegacyVramState v, const void* constants_ptr, uint32_t current_tick) {
   (void)constants_ptr;
@@ -599,9 +619,11 @@ void launch_update_neurons(AxcLegacyVramState v, const void* constants_ptr, uint
   Shard* s = shard_for(&p, v.padded_n, v.total_axons, &e);
   const AxcVariantParameters* lut = device_lut();
   if (!s || !lut) return;
-  neuron_tick_kernel<<<blocks_for(v.padded_n), kThreads>>>(p, v.padded_n, v.total_axons, lut, nullptr, current_tick,
-                                                           s->sc);
-  push_apply_kernel<<<kListBlocks / 4, kThreads>>>(p.axon_heads, s->sc);
+  // Needs the snapshot of the current heads: rebuilt from them first.
+  compact_all_kernel<<<blocks_for(v.total_axons + 1), kThreads>>>(p.axon_heads, v.total_axons, s->sc.comp[0]);
+  neuron_tick_kernel<false><<<blocks_for(v.padded_n), kThreads>>>(p, v.padded_n, v.total_axons, lut, nullptr,
+                                                                  current_tick, s->sc, s->sc.comp[0], s->sc.comp[1], 0);
+  push_apply_kernel<<<kListBlocks / 4, kThreads>>>(p.axon_heads, s->sc, s->sc.comp[0]);
 }
 void launch_apply_gsop(AxcLegacyVramState v) {
   if (v.padded_n == 0) return;
@@ -610,7 +632,8 @@ void launch_apply_gsop(AxcLegacyVramState v) {
   Shard* s = shard_for(&p, v.padded_n, v.total_axons, &e);
   const AxcVariantParameters* lut = device_lut();
   if (!s || !lut) return;
-  gsop_kernel<<<kListBlocks, kThreads>>>(p, v.padded_n, v.total_axons, lut, nullptr, 0, s->sc);
+  compact_all_kernel<<<blocks_for(v.total_axons + 1), kThreads>>>(p.axon_heads, v.total_axons, s->sc.comp[0]);
+  gsop_kernel<<<kListBlocks, kThreads>>>(p, v.padded_n, v.total_axons, lut, nullptr, 0, s->sc, s->sc.comp[0]);
 }
 void launch_record_readout(AxcLegacyVramState v, const uint32_t* mapped, uint8_t* out, uint32_t current_tick,
                            uint32_t total_pixels) {
@@ -652,18 +675,21 @@ int32_t axc_shard_stats_read(const ShardVramPtrs* vram, AxcShardStats* out) {
   if (e == hipSuccess) e = hipMemcpy(b.data(), s->sc.stats_part2, b.size() * 8, hipMemcpyDeviceToHost);
   if (e != hipSuccess) return (int32_t)e;
   AxcShardStats r{};
-  for (size_t k = 0; k < s->stats_blocks; k++) {
-    r.integrated += a[k * kStatsPerBlock + kStIntegrated];
-    r.slots_scanned += a[k * kStatsPerBlock + kStScanned];
-    r.targets_read += a[k * kStatsPerBlock + kStTargets];
-    r.hits += a[k * kStatsPerBlock + kStHits];
-  }
-  for (size_t k = 0; k < kListBlocks; k++) {
-    r.spikes += b[k * kStatsPerBlock + 0];
-    r.gsop_slots += b[k * kStatsPerBlock + 1];
-    r.gsop_targets += b[k * kStatsPerBlock + 2];
-    r.ticks += b[k * kStatsPerBlock + 3];
-  }
+  auto add = [&](const std::vector<uint64_t>& v, size_t nb) {
+    for (size_t k = 0; k < nb; k++) {
+      const uint64_t* q = &v[k * kStatsPerBlock];
+      r.integrated += q[kStIntegrated];
+      r.slots_scanned += q[kStScanned];
+      r.targets_read += q[kStTargets];
+      r.hits += q[kStHits];
+      r.spikes += q[kStSpikes];
+      r.gsop_slots += q[kStGsopSlots];
+      r.gsop_targets += q[kStGsopTargets];
+      r.ticks += q[kStTicks];
+    }
+  };
+  add(a, s->stats_blocks);
+  add(b, kListBlocks);
   *out = r;
   return 0;
 }
@@ -721,8 +747,10 @@ int32_t axc_step_day_phase_batch(const ShardVramPtrs* vram, uint32_t padded_n, u
     e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
     if (e == hipSuccess) {
       for (uint32_t t = 0; t < T && e == hipSuccess; t++) {
+        // GSOP of tick t runs inside tick t+1's neuron pass; the last tick's
+        // GSOP is launched on its own so the batch leaves a complete state.
         e = enqueue_tick(*vram, s, v_seg, t, d_bitmask, words, virt_off, n_virt, d_spikes, max_spk,
-                         key.spike_counts[t], d_mapped, d_out, n_out, 0, s->prm, cap, lut);
+                         key.spike_counts[t], d_mapped, d_out, n_out, 0, s->prm, cap, lut, t, t > 0, t + 1 == T);
       }
       hipGraph_t g = nullptr;
       hipError_t e2 = hipStreamEndCapture(cap, &g);

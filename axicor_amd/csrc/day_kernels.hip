@@ -83,10 +83,84 @@ __device__ __forceinline__ int32_t dopamine_of(const TickParams* prm, int32_t do
   return prm ? (int32_t)prm->dopamine : dop_arg;
 }
 
+// Compact form of eight heads (see kInertHead in day_common.h).
+__device__ __forceinline__ uint32_t c16(uint32_t h) { return h < 0xFFFFu ? h : 0xFFFFu; }
+__device__ __forceinline__ uint4 compact_heads(const Heads8& h) {
+  return make_uint4(c16(h.lo.x) | (c16(h.lo.y) << 16), c16(h.lo.z) | (c16(h.lo.w) << 16),
+                    c16(h.hi.x) | (c16(h.hi.y) << 16), c16(h.hi.z) | (c16(h.hi.w) << 16));
+}
+
+// Smallest (c_k - seg) mod 2^16 over the eight compact heads: < prop means
+// "some head in the active tail" (physics.cu:198-201), and for GSOP it is the
+// freshest distance min_dist of physics.cu:311-318 whenever it is < len.
+__device__ __forceinline__ uint32_t min_dist16(const uint4& c, uint32_t seg) {
+  const uint32_t s2 = seg | (seg << 16);
+  // Per-halfword subtraction without borrow across halves.
+  auto sub2 = [&](uint32_t x) -> uint32_t {
+    uint32_t lo = ((x & 0xFFFFu) - seg) & 0xFFFFu;
+    uint32_t hi = ((x >> 16) - seg) & 0xFFFFu;
+    return lo | (hi << 16);
+  };
+  (void)s2;
+  uint32_t d0 = sub2(c.x), d1 = sub2(c.y), d2 = sub2(c.z), d3 = sub2(c.w);
+  uint32_t m0 = min(d0 & 0xFFFFu, d0 >> 16), m1 = min(d1 & 0xFFFFu, d1 >> 16);
+  uint32_t m2 = min(d2 & 0xFFFFu, d2 >> 16), m3 = min(d3 & 0xFFFFu, d3 >> 16);
+  return min(min(m0, m1), min(m2, m3));
+}
+
+// Axon index of a packed target for the snapshot: out-of-range indices (and
+// the empty-slot pattern) go to the inert entry A.
+__device__ __forceinline__ uint32_t snap_index(uint32_t t, uint32_t A) {
+  uint32_t a = (t & 0x00FFFFFFu) - 1u;
+  return a < A ? a : A;
+}
+
+// One GSOP weight update (physics.cu:320-371) from the freshest distance md
+// (>= len: no active head), with the per-soma terms precomputed.
+__device__ __forceinline__ int16_t gsop_weight(int16_t w, uint32_t md, uint32_t len, int32_t raw_pot,
+                                               int32_t final_dep, int32_t bm, const uint8_t* curve) {
+  const bool act = md < len;
+  const int32_t sign = (w >= 0) ? 1 : -1;
+  int32_t abs_w = (int32_t)w;
+  abs_w = abs_w < 0 ? -abs_w : abs_w;
+  uint32_t rank = (uint32_t)abs_w >> 11;
+  rank = rank > 15 ? 15 : rank;
+  const int32_t inertia = curve[rank];
+  const int32_t dpot = wmul(wmul(raw_pot, inertia), bm) >> 7;
+  const int32_t ddep = wmul(wmul(final_dep, inertia), bm) >> 7;
+  const uint32_t cs = act ? (md >> 4) : 0u;
+  int32_t delta = act ? (dpot >> cs) : wsub(0, ddep);
+  delta = wmul(delta, 128) >> (7 + cs);
+  int32_t na = wadd(abs_w, delta);
+  na = na & ~(na >> 31);
+  na = na > 32767 ? 32767 : na;
+  return (int16_t)wmul(na, sign);
+}
+
+struct GsopTerms {
+  int32_t raw_pot, final_dep, bm;
+  uint32_t len;
+};
+
+// Per-soma GSOP terms (physics.cu:289-345) for a soma with flags `flags`.
+__device__ __forceinline__ GsopTerms gsop_terms(uint8_t flags, const AxcVariantParameters& p, int32_t dop) {
+  GsopTerms g;
+  const uint8_t bc = (flags >> 1) & 0x07;
+  g.bm = bc > 0 ? bc : 1;
+  g.len = p.signal_propagation_length;
+  const int32_t pot_mod = wmul(dop, (int32_t)p.d1_affinity) >> 7;
+  const int32_t dep_mod = wmul(dop, (int32_t)p.d2_affinity) >> 7;
+  g.raw_pot = wadd((int32_t)p.gsop_potentiation, pot_mod);
+  const int32_t raw_dep = wsub((int32_t)p.gsop_depression, dep_mod);
+  g.final_dep = raw_dep & ~(raw_dep >> 31);
+  return g;
+}
+
 // ---------------------------------------------------------------------------
 // K2 + K3 + K4: pushes from the input bitmask and the spike-schedule counts,
 // then propagation (physics.cu:88-154). Pushes commute with each other, so the
 // k pushes an axon receives this tick are applied at once, before propagation.
+// Also writes the compact snapshot of the propagated heads.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void axon_tick_kernel(AxcBurstHeads8* __restrict__ heads, uint32_t A,
                                                         uint32_t v_seg, const uint32_t* __restrict__ bitmask_arg,
@@ -94,7 +168,7 @@ __global__ __launch_bounds__(256) void axon_tick_kernel(AxcBurstHeads8* __restri
                                                         uint32_t* __restrict__ push_cnt, int use_cnt,
                                                         uint32_t* __restrict__ counters,
                                                         const TickParams* __restrict__ prm, uint32_t t_off,
-                                                        uint32_t words) {
+                                                        uint32_t words, uint4* __restrict__ comp) {
   uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t* __restrict__ bitmask =
       prm ? (prm->bitmask ? prm->bitmask + (size_t)t_off * words : nullptr) : bitmask_arg;
@@ -125,6 +199,7 @@ __global__ __launch_bounds__(256) void axon_tick_kernel(AxcBurstHeads8* __restri
   h.hi.z += (h.hi.z != kSentinel) ? v_seg : 0u;
   h.hi.w += (h.hi.w != kSentinel) ? v_seg : 0u;
   store_heads(heads, a, h);
+  if (comp) comp[a] = compact_heads(h);
 }
 
 // K3 (physics.cu:114-132): one count per occurrence of an id < A.
@@ -140,21 +215,31 @@ __global__ __launch_bounds__(256) void spike_count_kernel(const uint32_t* __rest
 }
 
 // ---------------------------------------------------------------------------
-// K1 + K5 (physics.cu:25-30, 159-276), one lane per soma.
+// K1 + K5 (physics.cu:25-30, 159-276), one lane per soma, dendrite columns
+// read eight slots at a time (eight independent target loads, then eight
+// independent 16 B snapshot gathers per lane).
+//
+// FUSED: the GSOP of the previous tick (K6, physics.cu:281-373) is applied in
+// the same pass, slot by slot, to the somas that spiked in the previous tick
+// (their spike bit is still set when this tick starts): the weight is updated
+// from the post-push snapshot of the previous tick (prev) before this tick's
+// integration reads it, which is the order of the reference's separate K6
+// launch followed by the next tick's K5. Used inside a batch, where no caller
+// can observe the state between ticks; the batch ends with a plain gsop launch.
 // ---------------------------------------------------------------------------
+template <bool FUSED>
 __global__ __launch_bounds__(256) void neuron_tick_kernel(ShardVramPtrs vram, uint32_t N, uint32_t A,
                                                           const AxcVariantParameters* __restrict__ lut_g,
                                                           const TickParams* __restrict__ prm, uint32_t tick_arg,
-                                                          ShardScratch sc) {
+                                                          ShardScratch sc, const uint4* __restrict__ cur,
+                                                          const uint4* __restrict__ prev, int32_t dop_arg) {
   __shared__ AxcVariantParameters lut[16];
   __shared__ uint32_t wave_cnt[2][4];
   __shared__ uint32_t blk_base[2];
-  __shared__ uint64_t red[4][4];
+  __shared__ uint32_t red[4][8];
 
   const uint32_t tid = threadIdx.x;
-  if (tid < 16 * 64 / 16) {
-    reinterpret_cast<uint4*>(lut)[tid] = reinterpret_cast<const uint4*>(lut_g)[tid];
-  }
+  if (tid < 64) reinterpret_cast<uint4*>(lut)[tid] = reinterpret_cast<const uint4*>(lut_g)[tid];
   __syncthreads();
 
   const uint32_t i = blockIdx.x * blockDim.x + tid;
@@ -162,47 +247,120 @@ __global__ __launch_bounds__(256) void neuron_tick_kernel(ShardVramPtrs vram, ui
   bool fin = false;
   bool push_own = false;
   uint32_t my_axon = 0;
-  uint32_t n_int = 0, n_scan = 0, n_tread = 0, n_hits = 0;
+  uint32_t n_int = 0, n_scan = 0, n_tread = 0, n_hits = 0, n_gslots = 0, n_gtgt = 0;
 
   if (i < N) {
-    uint8_t flags = vram.soma_flags[i] & 0xF1;  // K1
-    uint8_t timer = vram.timers[i];
+    const uint8_t flags_in = vram.soma_flags[i];
+    uint8_t flags = flags_in & 0xF1;  // K1
+    const uint8_t timer = vram.timers[i];
     flags &= (uint8_t)~0x01;
-    if (timer > 0) {
+    const bool refr = timer > 0;
+    const bool pend = FUSED && (flags_in & 1);  // GSOP of the previous tick pending
+    const AxcVariantParameters& p = lut[(flags >> 4) & 0x0F];
+    const uint32_t prop = p.signal_propagation_length;
+    int32_t i_in = 0;
+
+    if (!refr || pend) {
+      GsopTerms g;
+      if (FUSED && pend) g = gsop_terms(flags_in, p, dopamine_of(prm, dop_arg));
+      // Byte offsets below are 32-bit lane offsets from wave-uniform bases,
+      // so loads use a scalar base register plus one VGPR each.
+      const uint32_t toff = i * 4u, woff = i * 2u;
+      const char* __restrict__ cbase = reinterpret_cast<const char*>(cur);
+      const char* __restrict__ pbase = reinterpret_cast<const char*>(prev);
+      uint32_t filled = 0;
+      bool alive = true;
+      // Software pipeline: the targets and weights of the next eight slots
+      // are requested before this group's snapshot gathers, so each lane has
+      // two groups of loads in flight.
+      uint32_t t[8];
+      int16_t w[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        const char* row = reinterpret_cast<const char*>(vram.dendrite_targets + (size_t)u * N);
+        t[u] = __builtin_nontemporal_load(reinterpret_cast<const uint32_t*>(row + toff));
+      }
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        const char* row = reinterpret_cast<const char*>(vram.dendrite_weights + (size_t)u * N);
+        w[u] = *reinterpret_cast<const int16_t*>(row + woff);
+      }
+      for (uint32_t s = 0; s < kMaxDendrites && alive; s += 8) {
+        uint32_t nv = 8;
+#pragma unroll
+        for (int u = 7; u >= 0; u--) nv = (t[u] == 0) ? (uint32_t)u : nv;
+        alive = nv == 8;
+        filled += nv;
+        uint32_t ao[8];  // byte offset of the slot's snapshot entry; slots past the first empty one read the inert entry
+#pragma unroll
+        for (int u = 0; u < 8; u++) ao[u] = (((uint32_t)u < nv) ? snap_index(t[u], A) : A) << 4;
+        uint32_t tn[8];
+        int16_t wn[8];
+        const bool more = alive && (s + 8 < kMaxDendrites);
+        if (more) {
+#pragma unroll
+          for (int u = 0; u < 8; u++) {
+            const char* row = reinterpret_cast<const char*>(vram.dendrite_targets + (size_t)(s + 8 + u) * N);
+            tn[u] = __builtin_nontemporal_load(reinterpret_cast<const uint32_t*>(row + toff));
+          }
+#pragma unroll
+          for (int u = 0; u < 8; u++) {
+            const char* row = reinterpret_cast<const char*>(vram.dendrite_weights + (size_t)(s + 8 + u) * N);
+            wn[u] = *reinterpret_cast<const int16_t*>(row + woff);
+          }
+        } else {
+#pragma unroll
+          for (int u = 0; u < 8; u++) { tn[u] = 0; wn[u] = 0; }
+        }
+        if (FUSED && pend) {
+          // Pending GSOP of the previous tick first, so that its snapshot
+          // values and this tick's are never live together.
+          uint4 q[8];
+#pragma unroll
+          for (int u = 0; u < 8; u++) q[u] = *reinterpret_cast<const uint4*>(pbase + ao[u]);
+#pragma unroll
+          for (int u = 0; u < 8; u++) {
+            const int16_t nw = gsop_weight(w[u], min_dist16(q[u], t[u] >> 24), g.len, g.raw_pot, g.final_dep, g.bm,
+                                           p.inertia_curve);
+            if ((uint32_t)u < nv) {
+              w[u] = nw;
+              char* row = reinterpret_cast<char*>(vram.dendrite_weights + (size_t)(s + u) * N);
+              *reinterpret_cast<int16_t*>(row + woff) = nw;
+            }
+          }
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        if (!refr) {
+          uint4 c[8];
+#pragma unroll
+          for (int u = 0; u < 8; u++) c[u] = *reinterpret_cast<const uint4*>(cbase + ao[u]);
+#pragma unroll
+          for (int u = 0; u < 8; u++) {
+            // The inert entry never hits, so slots past the end need no mask.
+            const bool hit = min_dist16(c[u], t[u] >> 24) < prop;
+            n_hits += hit;
+            i_in = wadd(i_in, hit ? (int32_t)w[u] : 0);
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++) { t[u] = tn[u]; w[u] = wn[u]; }
+      }
+      const uint32_t c_i = filled < kMaxDendrites ? filled + 1 : kMaxDendrites;
+      if (!refr) {
+        n_scan = filled;
+        n_tread = c_i;
+      }
+      if (FUSED && pend) {
+        n_gslots = filled;
+        n_gtgt = c_i;
+      }
+    }
+
+    if (refr) {
       vram.timers[i] = (uint8_t)(timer - 1);
       vram.soma_flags[i] = flags;
     } else {
       n_int = 1;
-      const AxcVariantParameters& p = lut[(flags >> 4) & 0x0F];
-      const uint32_t prop = p.signal_propagation_length;
-      int32_t i_in = 0;
-      const uint32_t* __restrict__ tcol = vram.dendrite_targets + i;
-      const int16_t* __restrict__ wcol = vram.dendrite_weights + i;
-      uint32_t s = 0;
-      bool alive = true;
-      // Four slots per step: the four target loads and the four head gathers
-      // are independent, which keeps several requests in flight per lane.
-      for (; s < kMaxDendrites && alive; s += 4) {
-        uint32_t t[4];
-        int16_t w[4];
-#pragma unroll
-        for (int u = 0; u < 4; u++) t[u] = tcol[(size_t)(s + u) * N];
-#pragma unroll
-        for (int u = 0; u < 4; u++) w[u] = wcol[(size_t)(s + u) * N];
-#pragma unroll
-        for (int u = 0; u < 4; u++) {
-          alive = alive && (t[u] != 0);
-          if (alive) {
-            Heads8 h = target_heads(vram.axon_heads, A, t[u]);
-            bool hit = any_hit(h, t[u] >> 24, prop);
-            n_hits += hit;
-            i_in = wadd(i_in, hit ? (int32_t)w[u] : 0);
-            n_scan++;
-          }
-        }
-      }
-      n_tread = n_scan < kMaxDendrites ? n_scan + 1 : kMaxDendrites;
-
       int32_t thr = vram.threshold_offset[i];
       int32_t decayed = wsub(thr, (int32_t)p.homeostasis_decay);
       thr = decayed & ~(decayed >> 31);
@@ -261,20 +419,15 @@ __global__ __launch_bounds__(256) void neuron_tick_kernel(ShardVramPtrs vram, ui
     wave_cnt[0][wv] = (uint32_t)__popcll(bs);
     wave_cnt[1][wv] = (uint32_t)__popcll(bo);
   }
-  // Block partial sums of the counters.
-  uint32_t r0 = n_int, r1 = n_scan, r2 = n_tread, r3 = n_hits;
+  uint32_t r[6] = {n_int, n_scan, n_tread, n_hits, n_gslots, n_gtgt};
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    r0 += __shfl_xor(r0, off, 64);
-    r1 += __shfl_xor(r1, off, 64);
-    r2 += __shfl_xor(r2, off, 64);
-    r3 += __shfl_xor(r3, off, 64);
+  for (int k = 0; k < 6; k++) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) r[k] += __shfl_xor(r[k], off, 64);
   }
   if (lane == 0) {
-    red[wv][0] = r0;
-    red[wv][1] = r1;
-    red[wv][2] = r2;
-    red[wv][3] = r3;
+#pragma unroll
+    for (int k = 0; k < 6; k++) red[wv][k] = r[k];
   }
   __syncthreads();
   if (tid == 0) {
@@ -287,6 +440,10 @@ __global__ __launch_bounds__(256) void neuron_tick_kernel(ShardVramPtrs vram, ui
     part[kStScanned] += red[0][1] + red[1][1] + red[2][1] + red[3][1];
     part[kStTargets] += red[0][2] + red[1][2] + red[2][2] + red[3][2];
     part[kStHits] += red[0][3] + red[1][3] + red[2][3] + red[3][3];
+    part[kStGsopSlots] += red[0][4] + red[1][4] + red[2][4] + red[3][4];
+    part[kStGsopTargets] += red[0][5] + red[1][5] + red[2][5] + red[3][5];
+    part[kStSpikes] += cs;
+    if (blockIdx.x == 0) part[kStTicks] += 1;
   }
   __syncthreads();
   uint32_t pre_s = blk_base[0], pre_o = blk_base[1];
@@ -299,8 +456,10 @@ __global__ __launch_bounds__(256) void neuron_tick_kernel(ShardVramPtrs vram, ui
 }
 
 // Applies the counted pushes of this tick's spikes (physics.cu:253-262 with
-// the snapshot rule). One lane per listed axon; clears the count.
-__global__ __launch_bounds__(256) void push_apply_kernel(AxcBurstHeads8* __restrict__ heads, ShardScratch sc) {
+// the snapshot rule). One lane per listed axon; clears the count and refreshes
+// the axon's compact snapshot.
+__global__ __launch_bounds__(256) void push_apply_kernel(AxcBurstHeads8* __restrict__ heads, ShardScratch sc,
+                                                         uint4* __restrict__ comp) {
   const uint32_t n = sc.counters[kCntOwners];
   for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
     uint32_t a = sc.own_list[j];
@@ -309,18 +468,20 @@ __global__ __launch_bounds__(256) void push_apply_kernel(AxcBurstHeads8* __restr
     Heads8 h = load_heads(heads, a);
     push_heads(h, c > 8 ? 8 : c);
     store_heads(heads, a, h);
+    if (comp) comp[a] = compact_heads(h);
   }
 }
 
 // ---------------------------------------------------------------------------
-// K6 (physics.cu:281-373): one wave per spiking soma, lane = dendrite slot.
+// K6 (physics.cu:281-373) over the spiker list: one wave per spiking soma,
+// lane = dendrite slot, heads from the post-push snapshot.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void gsop_kernel(ShardVramPtrs vram, uint32_t N, uint32_t A,
                                                    const AxcVariantParameters* __restrict__ lut_g,
                                                    const TickParams* __restrict__ prm, int32_t dop_arg,
-                                                   ShardScratch sc) {
+                                                   ShardScratch sc, const uint4* __restrict__ comp) {
   __shared__ AxcVariantParameters lut[16];
-  __shared__ uint64_t red[4][3];
+  __shared__ uint32_t red[4][2];
   const uint32_t tid = threadIdx.x;
   if (tid < 64) reinterpret_cast<uint4*>(lut)[tid] = reinterpret_cast<const uint4*>(lut_g)[tid];
   __syncthreads();
@@ -329,20 +490,13 @@ __global__ __launch_bounds__(256) void gsop_kernel(ShardVramPtrs vram, uint32_t 
   const uint32_t n_spk = sc.counters[kCntSpikes];
   const int32_t dop = dopamine_of(prm, dop_arg);
   const uint32_t waves = gridDim.x * 4;
-  uint32_t cnt_spk = 0, cnt_slots = 0, cnt_tgt = 0;
+  uint32_t cnt_slots = 0, cnt_tgt = 0;
 
   for (uint32_t k = blockIdx.x * 4 + wv; k < n_spk; k += waves) {
     const uint32_t i = sc.spk_list[k];
     const uint8_t flags = vram.soma_flags[i];
-    const uint8_t bc = (flags >> 1) & 0x07;
-    const int32_t bm = bc > 0 ? bc : 1;
     const AxcVariantParameters& p = lut[(flags >> 4) & 0x0F];
-    const uint32_t len = p.signal_propagation_length;
-    const int32_t pot_mod = wmul(dop, (int32_t)p.d1_affinity) >> 7;
-    const int32_t dep_mod = wmul(dop, (int32_t)p.d2_affinity) >> 7;
-    const int32_t raw_pot = wadd((int32_t)p.gsop_potentiation, pot_mod);
-    const int32_t raw_dep = wsub((int32_t)p.gsop_depression, dep_mod);
-    const int32_t final_dep = raw_dep & ~(raw_dep >> 31);
+    const GsopTerms g = gsop_terms(flags, p, dop);
     uint32_t filled = 0;
     bool done = false;
     for (uint32_t base = 0; base < kMaxDendrites && !done; base += 64) {
@@ -352,54 +506,35 @@ __global__ __launch_bounds__(256) void gsop_kernel(ShardVramPtrs vram, uint32_t 
       const uint64_t empty = __ballot(t == 0);
       const uint32_t first_empty = empty ? (uint32_t)__ffsll((long long)empty) - 1u : 64u;
       if (lane < first_empty) {
-        Heads8 h = target_heads(vram.axon_heads, A, t);
-        const uint32_t seg = t >> 24;
-        uint32_t md = dist_or_max(h.lo.x, seg, len);
-        md = min(md, dist_or_max(h.lo.y, seg, len));
-        md = min(md, dist_or_max(h.lo.z, seg, len));
-        md = min(md, dist_or_max(h.lo.w, seg, len));
-        md = min(md, dist_or_max(h.hi.x, seg, len));
-        md = min(md, dist_or_max(h.hi.y, seg, len));
-        md = min(md, dist_or_max(h.hi.z, seg, len));
-        md = min(md, dist_or_max(h.hi.w, seg, len));
-        const bool act = md != 0xFFFFFFFFu;
+        const uint4 c = comp[snap_index(t, A)];
         const int16_t w = vram.dendrite_weights[col];
-        const int32_t sign = (w >= 0) ? 1 : -1;
-        int32_t abs_w = (int32_t)w;
-        abs_w = abs_w < 0 ? -abs_w : abs_w;
-        uint32_t rank = (uint32_t)abs_w >> 11;
-        rank = rank > 15 ? 15 : rank;
-        const int32_t inertia = p.inertia_curve[rank];
-        const int32_t dpot = wmul(wmul(raw_pot, inertia), bm) >> 7;
-        const int32_t ddep = wmul(wmul(final_dep, inertia), bm) >> 7;
-        const uint32_t cs = act ? (md >> 4) : 0u;
-        int32_t delta = act ? (dpot >> cs) : wsub(0, ddep);
-        delta = wmul(delta, 128) >> (7 + cs);
-        int32_t na = wadd(abs_w, delta);
-        na = na & ~(na >> 31);
-        na = na > 32767 ? 32767 : na;
-        vram.dendrite_weights[col] = (int16_t)wmul(na, sign);
+        vram.dendrite_weights[col] =
+            gsop_weight(w, min_dist16(c, t >> 24), g.len, g.raw_pot, g.final_dep, g.bm, p.inertia_curve);
       }
       filled += first_empty;
       done = first_empty < 64;
     }
-    cnt_spk += 1;
     cnt_slots += filled;
     cnt_tgt += filled < kMaxDendrites ? filled + 1 : kMaxDendrites;
   }
   if (lane == 0) {
-    red[wv][0] = cnt_spk;
-    red[wv][1] = cnt_slots;
-    red[wv][2] = cnt_tgt;
+    red[wv][0] = cnt_slots;
+    red[wv][1] = cnt_tgt;
   }
   __syncthreads();
   if (tid == 0) {
     uint64_t* part = sc.stats_part2 + (size_t)blockIdx.x * kStatsPerBlock;
-    part[0] += red[0][0] + red[1][0] + red[2][0] + red[3][0];
-    part[1] += red[0][1] + red[1][1] + red[2][1] + red[3][1];
-    part[2] += red[0][2] + red[1][2] + red[2][2] + red[3][2];
-    if (blockIdx.x == 0) part[3] += 1;  // ticks
+    part[kStGsopSlots] += red[0][0] + red[1][0] + red[2][0] + red[3][0];
+    part[kStGsopTargets] += red[0][1] + red[1][1] + red[2][1] + red[3][1];
   }
+}
+
+// Fills a compact snapshot (all entries plus the inert entry A) from the heads.
+__global__ __launch_bounds__(256) void compact_all_kernel(const AxcBurstHeads8* __restrict__ heads, uint32_t A,
+                                                          uint4* __restrict__ comp) {
+  uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a < A) comp[a] = compact_heads(load_heads(heads, a));
+  else if (a == A) comp[a] = make_uint4(0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu);
 }
 
 // K7 (physics.cu:378-395); ids outside [0, N) read 0.

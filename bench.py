@@ -83,6 +83,8 @@ def main() -> int:
     ap.add_argument("--cpu-max-ticks", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--timing-ticks", type=int, default=10, help="per-tick launches timed with HIP events")
+    ap.add_argument("--copy-calibration-mb", type=int, default=0,
+                    help="also run one device-to-device copy of this size (reference for HBM byte counters)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -164,10 +166,19 @@ def main() -> int:
     eng.synchronize()
     log(f"warmup done at tick {tick}")
 
+    hip = C.CDLL("libamdhip64.so")
+    if args.copy_calibration_mb:
+        nb = args.copy_calibration_mb << 20
+        src, dst = DeviceBuffer(nb), DeviceBuffer(nb)
+        hip.hipMemcpy(C.c_void_p(dst.ptr), C.c_void_p(src.ptr), C.c_size_t(nb), 3)  # hipMemcpyDeviceToDevice
+        hip.hipDeviceSynchronize()
+        src.free()
+        dst.free()
+        log(f"copy calibration: {args.copy_calibration_mb} MiB device-to-device")
+
     eng.stats_reset()
     start = C.c_void_p()
     stop = C.c_void_p()
-    hip = C.CDLL("libamdhip64.so")
     hip.hipEventCreate(C.byref(start))
     hip.hipEventCreate(C.byref(stop))
     eng.synchronize()

@@ -156,3 +156,29 @@ def test_counters_match_oracle(gpu):
         total = st if total is None else {kk: total[kk] + st[kk] for kk in st}
     got = p.eng.stats()
     assert got == total, (got, total)
+
+
+def test_c1_graph_batches_with_night(gpu):
+    """C1 through the batch entry point (HIP graph, GSOP of tick t fused into
+    tick t+1): 50-tick batches to tick 1,000, Night every 250 ticks, nonzero
+    dopamine, outputs mapped."""
+    p = Pair(B.generate(B.spec_c1()), n_out=64)
+    while p.tick < 1000:
+        out_ref, _ = p.step(50, use_graph=True, dopamine=120)
+        np.testing.assert_array_equal(p.outputs(50), out_ref, err_msg=f"graph outputs ending {p.tick}")
+        p.check(f"graph batch ending tick {p.tick}")
+        if p.tick % 250 == 0:
+            p.night(15)
+            p.check(f"night at tick {p.tick}")
+
+
+def test_graph_and_per_tick_paths_agree_on_counters(gpu):
+    """Work counters of the fused batch path equal the oracle's."""
+    p = Pair(B.generate(B.spec_small(3000, 64, 200, seed=12)))
+    p.eng.stats_reset()
+    total = None
+    for k in range(3):
+        _, st = p.step(17, use_graph=True)
+        total = st if total is None else {kk: total[kk] + st[kk] for kk in st}
+    assert p.eng.stats() == total
+    p.check("graph counters")
