@@ -146,6 +146,7 @@ class GenSpec(C.Structure):
         ("slots_filled", C.c_uint32), ("axon_len_min", C.c_uint32), ("axon_len_max", C.c_uint32),
         ("w_min", C.c_int32), ("w_max", C.c_int32), ("v_seg", C.c_uint32), ("n_types", C.c_uint32),
         ("type_ppm_cdf", C.c_uint32 * 16), ("mean_dist_milli", C.c_uint32), ("master_seed", C.c_uint64),
+        ("brick", C.c_uint32),
     ]
 
 

@@ -8,7 +8,8 @@ variant tables and the named workloads:
   256 inputs at p=0.05 per tick.
 * ``c2`` — 1.5M neurons of 4 types (excitatory/inhibitory x fast/slow leak),
   100 slots each (150M synapses), 128x128x64 grid, axons 1-255 segments,
-  |w| in [64,512], 5% of neurons as inputs (75,000) at p=0.02 per tick.
+  |w| in [64,512], 5% of neurons as inputs (75,000) at p=0.02 per tick;
+  somas numbered in 8x8x8-voxel bricks.
 
 Parameter values follow the reference template (_template/blueprints.toml:13-37)
 and the blueprint defaults (genesis-core/src/config/blueprints.rs:131-152).
@@ -76,13 +77,20 @@ def lut_c1() -> np.ndarray:
 
 
 def lut_c2() -> np.ndarray:
-    # Fast leak: template. Slow leak: half the leak with the adaptive mode on
-    # (leak grows with the homeostatic offset up to 3000).
+    """Four types: excitatory/inhibitory x fast/slow leak. Fast leak: the
+    template leak. Slow leak: half the leak with the adaptive mode on (the leak
+    grows with the homeostatic offset up to 3000). All four use the blueprint
+    default propagation length of 10 segments (blueprints.rs:131) and a
+    128-tick heartbeat; with these values the 1.5M-neuron brain fires at about
+    1% per tick for its first ~1,500 ticks (the template's 20 segments drive it
+    to the refractory-limited ~6%)."""
+    common = dict(signal_propagation_length=10, spontaneous_firing_period_ticks=128)
+    slow = dict(leak_rate=750, adaptive_mode=1, adaptive_leak_gain=64, adaptive_leak_max=3000)
     return lut_from([
-        template_variant(False),
-        template_variant(False, leak_rate=750, adaptive_mode=1, adaptive_leak_gain=64, adaptive_leak_max=3000),
-        template_variant(True),
-        template_variant(True, leak_rate=750, adaptive_mode=1, adaptive_leak_gain=64, adaptive_leak_max=3000),
+        template_variant(False, **common),
+        template_variant(False, **common, **slow),
+        template_variant(True, **common),
+        template_variant(True, **common, **slow),
     ])
 
 
@@ -100,6 +108,7 @@ class BrainSpec:
     seed: int = DEFAULT_MASTER_SEED
     input_ppm: int = 50_000
     lut: np.ndarray | None = None
+    brick: int = 0  # 0: voxels numbered x-fastest; B: BxBxB bricks (a spatial sort of the somas)
 
     def to_c(self) -> GenSpec:
         g = GenSpec()
@@ -117,6 +126,7 @@ class BrainSpec:
             g.type_ppm_cdf[i] = acc
         g.mean_dist_milli = int(round(self.mean_dist * 1000))
         g.master_seed = self.seed & 0xFFFFFFFFFFFFFFFF
+        g.brick = self.brick
         return g
 
 
@@ -124,9 +134,11 @@ def spec_c1(seed: int = DEFAULT_MASTER_SEED) -> BrainSpec:
     return BrainSpec(4096, 256, (16, 16, 16), 64, (1, 32), seed=seed, input_ppm=50_000, lut=lut_c1())
 
 
-def spec_c2(seed: int = DEFAULT_MASTER_SEED) -> BrainSpec:
+def spec_c2(seed: int = DEFAULT_MASTER_SEED, brick: int = 8) -> BrainSpec:
+    """Somas numbered brick by brick (8x8x8 voxels) unless brick=0."""
     return BrainSpec(1_500_000, 75_000, (128, 128, 64), 100, (1, 255),
-                     type_ppm=[400_000, 400_000, 100_000, 100_000], seed=seed, input_ppm=20_000, lut=lut_c2())
+                     type_ppm=[400_000, 400_000, 100_000, 100_000], seed=seed, input_ppm=20_000, lut=lut_c2(),
+                     brick=brick)
 
 
 def spec_small(n: int, slots: int, n_inputs: int, seed: int, axon_len=(1, 32), input_ppm=50_000,

@@ -9,7 +9,8 @@
 //
 // Topology model (a generator of our own; the reference baker's cone-traced
 // growth is not reproduced): neurons fill a gx*gy*gz voxel grid in index
-// order, neuron i sits in voxel floor(i*V/n). Axon i belongs to neuron i
+// order, neuron i sits in the voxel of rank floor(i*V/n), voxels ranked
+// x-fastest or brick by brick (GenSpec::brick). Axon i belongs to neuron i
 // (soma_to_axon[i] = i); input axons sit at [padded_n, padded_n+n_in)
 // (genesis-compute/src/memory.rs:128-130 with zero ghosts) spread evenly over
 // the grid. Each filled dendrite slot picks a presynaptic axon whose voxel is
@@ -80,6 +81,8 @@ struct GenSpec {
   uint32_t type_ppm_cdf[16];  // cumulative ppm, last = 1'000'000
   uint32_t mean_dist_milli;  // mean Chebyshev distance of targets, in 1/1000 voxel
   uint64_t master_seed;
+  uint32_t brick;            // 0: voxels numbered x-fastest; B > 0: BxBxB bricks, x-fastest
+                             // inside a brick and over bricks (dims multiples of B)
 };
 
 // padded_n and total_axons of a spec (memory.rs:32 align_to_warp, boot.rs:65-71).
@@ -90,6 +93,35 @@ void axc_gen_sizes(const GenSpec* spec, uint32_t* padded_n, uint32_t* total_axon
   *padded_n = pn;
   *total_axons = a;
 }
+
+// Voxel numbering: rank <-> (x, y, z). With bricks, the voxels of one brick
+// get consecutive ranks, so consecutive somas are spatial neighbours in all
+// three axes.
+struct VoxelOrder {
+  uint32_t gx, gy, gz, b;
+  void unrank(uint64_t r, uint32_t* x, uint32_t* y, uint32_t* z) const {
+    if (b == 0) {
+      *x = (uint32_t)(r % gx);
+      *y = (uint32_t)((r / gx) % gy);
+      *z = (uint32_t)(r / ((uint64_t)gx * gy));
+      return;
+    }
+    const uint64_t b3 = (uint64_t)b * b * b;
+    const uint64_t br = r / b3, off = r % b3;
+    const uint32_t nbx = gx / b, nby = gy / b;
+    const uint32_t bx = (uint32_t)(br % nbx), by = (uint32_t)((br / nbx) % nby), bz = (uint32_t)(br / ((uint64_t)nbx * nby));
+    *x = bx * b + (uint32_t)(off % b);
+    *y = by * b + (uint32_t)((off / b) % b);
+    *z = bz * b + (uint32_t)(off / ((uint64_t)b * b));
+  }
+  uint64_t rank(uint32_t x, uint32_t y, uint32_t z) const {
+    if (b == 0) return (uint64_t)x + (uint64_t)gx * ((uint64_t)y + (uint64_t)gy * (uint64_t)z);
+    const uint32_t nbx = gx / b, nby = gy / b;
+    const uint64_t br = (uint64_t)(x / b) + (uint64_t)nbx * ((uint64_t)(y / b) + (uint64_t)nby * (uint64_t)(z / b));
+    const uint64_t off = (uint64_t)(x % b) + (uint64_t)b * ((uint64_t)(y % b) + (uint64_t)b * (uint64_t)(z % b));
+    return br * b * b * b + off;
+  }
+};
 
 // Fills `state` (910*padded_n bytes), `heads` (8*total_axons u32) and
 // optionally `axon_len` (total_axons bytes, 0 = no axon) and `soma_pos`
@@ -103,6 +135,8 @@ int axc_gen_brain(const GenSpec* spec, const GenVariant* lut, uint8_t* state, ui
       spec->w_max > 32767 || spec->w_min > spec->w_max || spec->n_types == 0 || spec->n_types > 16 ||
       spec->gx == 0 || spec->gy == 0 || spec->gz == 0)
     return -1;
+  if (spec->brick && (spec->gx % spec->brick || spec->gy % spec->brick || spec->gz % spec->brick)) return -1;
+  const VoxelOrder vo{spec->gx, spec->gy, spec->gz, spec->brick};
   uint32_t pn, A;
   axc_gen_sizes(spec, &pn, &A);
   const uint32_t n = spec->n_neurons, nin = spec->n_inputs;
@@ -162,7 +196,8 @@ int axc_gen_brain(const GenSpec* spec, const GenVariant* lut, uint8_t* state, ui
     timers[i] = 0;
     s2a[i] = i;
     uint64_t vox = (uint64_t)i * V / n;
-    uint32_t x = (uint32_t)(vox % gx), y = (uint32_t)((vox / gx) % gy), z = (uint32_t)(vox / ((uint64_t)gx * gy));
+    uint32_t x, y, z;
+    vo.unrank(vox, &x, &y, &z);
     if (soma_pos) soma_pos[i] = (x & 0x3FF) | ((y & 0x3FF) << 10) | ((z & 0xFF) << 20) | ((uint32_t)(type[i] & 0xF) << 28);
 
     uint64_t ds = entity_seed(ms ^ kTagDend, i);
@@ -182,7 +217,7 @@ int axc_gen_brain(const GenSpec* spec, const GenVariant* lut, uint8_t* state, ui
       tx = tx < 0 ? 0 : (tx >= gx ? gx - 1 : tx);
       ty = ty < 0 ? 0 : (ty >= gy ? gy - 1 : ty);
       tz = tz < 0 ? 0 : (tz >= gz ? gz - 1 : tz);
-      uint64_t tv = (uint64_t)tx + (uint64_t)gx * ((uint64_t)ty + (uint64_t)gy * (uint64_t)tz);
+      uint64_t tv = vo.rank((uint32_t)tx, (uint32_t)ty, (uint32_t)tz);
       uint64_t h6 = entity_seed(ds, s * 8u + 6u);
       // Source population: inputs with probability nin/(n+nin).
       uint32_t axon;

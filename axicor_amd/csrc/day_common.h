@@ -35,6 +35,23 @@ constexpr uint32_t kStTicks = 7;
 // the last axon) is all 0xFFFF and stands for an axon with no live head.
 constexpr uint32_t kInertHead = 0xFFFFu;
 
+// Dendrite scan shape of neuron_tick (slots per step, next-step prefetch).
+#ifndef AXC_SCAN_UNROLL
+#define AXC_SCAN_UNROLL 8
+#endif
+#ifndef AXC_SCAN_PIPELINE
+#define AXC_SCAN_PIPELINE 0
+#endif
+#ifndef AXC_NEURON_ATTR
+#define AXC_NEURON_ATTR
+#endif
+constexpr int kScanUnroll = AXC_SCAN_UNROLL;
+constexpr bool kScanPipeline = AXC_SCAN_PIPELINE != 0;
+static_assert(AXC_MAX_DENDRITES % AXC_SCAN_UNROLL == 0, "the scan steps must tile the 128 slots");
+
+// XCDs of an MI355X; workgroups are dealt to them round-robin.
+constexpr uint32_t kNumXcd = 8;
+
 // Workgroups of the list-driven kernels (gsop, push_apply): fixed so that a
 // HIP graph of a batch does not depend on the number of spikes.
 constexpr uint32_t kListBlocks = 2048;

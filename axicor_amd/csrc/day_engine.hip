@@ -41,6 +41,8 @@ namespace {
 constexpr uint32_t kThreads = 256;
 
 inline uint32_t blocks_for(uint64_t n, uint32_t t = kThreads) { return (uint32_t)((n + t - 1) / t); }
+// Soma blocks of neuron_tick, padded to a multiple of the XCD count.
+inline uint32_t neuron_blocks(uint32_t N) { return (blocks_for(N) + kNumXcd - 1) / kNumXcd * kNumXcd; }
 
 // ---- per-device variant table (the reference's __constant__ VARIANT_LUT[16],
 // physics.cu:82; process-global per device like constant memory) -----------
@@ -149,7 +151,7 @@ void free_scratch(Shard* s) {
 
 hipError_t alloc_scratch(Shard* s) {
   const size_t N = s->N ? s->N : 1, A = s->A ? s->A : 1;
-  s->stats_blocks = blocks_for(N);
+  s->stats_blocks = neuron_blocks((uint32_t)N);
   hipError_t e;
   if ((e = hipMalloc((void**)&s->sc.spk_list, N * 4)) != hipSuccess) return e;
   if ((e = hipMalloc((void**)&s->sc.own_list, A * 4)) != hipSuccess) return e;
@@ -241,10 +243,10 @@ hipError_t enqueue_tick(const ShardVramPtrs& v, Shard* s, uint32_t v_seg, uint32
   if (N > 0) {
     TimedScope ts(kKNeuron, st, timed);
     if (fuse_prev)
-      neuron_tick_kernel<true><<<blocks_for(N), kThreads, 0, st>>>(v, N, A, lut, prm, tick, s->sc, cur, prev,
+      neuron_tick_kernel<true><<<neuron_blocks(N), kThreads, 0, st>>>(v, N, A, lut, prm, tick, s->sc, cur, prev,
                                                                    dopamine);
     else
-      neuron_tick_kernel<false><<<blocks_for(N), kThreads, 0, st>>>(v, N, A, lut, prm, tick, s->sc, cur, prev,
+      neuron_tick_kernel<false><<<neuron_blocks(N), kThreads, 0, st>>>(v, N, A, lut, prm, tick, s->sc, cur, prev,
                                                                     dopamine);
   }
   {
@@ -621,7 +623,7 @@ void launch_update_neurons(AxcLegacyVramState v, const void* constants_ptr, uint
   if (!s || !lut) return;
   // Needs the snapshot of the current heads: rebuilt from them first.
   compact_all_kernel<<<blocks_for(v.total_axons + 1), kThreads>>>(p.axon_heads, v.total_axons, s->sc.comp[0]);
-  neuron_tick_kernel<false><<<blocks_for(v.padded_n), kThreads>>>(p, v.padded_n, v.total_axons, lut, nullptr,
+  neuron_tick_kernel<false><<<neuron_blocks(v.padded_n), kThreads>>>(p, v.padded_n, v.total_axons, lut, nullptr,
                                                                   current_tick, s->sc, s->sc.comp[0], s->sc.comp[1], 0);
   push_apply_kernel<<<kListBlocks / 4, kThreads>>>(p.axon_heads, s->sc, s->sc.comp[0]);
 }

@@ -93,19 +93,16 @@ __device__ __forceinline__ uint4 compact_heads(const Heads8& h) {
 // Smallest (c_k - seg) mod 2^16 over the eight compact heads: < prop means
 // "some head in the active tail" (physics.cu:198-201), and for GSOP it is the
 // freshest distance min_dist of physics.cu:311-318 whenever it is < len.
+// Packed 16-bit subtract and min: nine VALU instructions for eight heads.
+typedef unsigned short axc_us2 __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ uint32_t min_dist16(const uint4& c, uint32_t seg) {
-  const uint32_t s2 = seg | (seg << 16);
-  // Per-halfword subtraction without borrow across halves.
-  auto sub2 = [&](uint32_t x) -> uint32_t {
-    uint32_t lo = ((x & 0xFFFFu) - seg) & 0xFFFFu;
-    uint32_t hi = ((x >> 16) - seg) & 0xFFFFu;
-    return lo | (hi << 16);
-  };
-  (void)s2;
-  uint32_t d0 = sub2(c.x), d1 = sub2(c.y), d2 = sub2(c.z), d3 = sub2(c.w);
-  uint32_t m0 = min(d0 & 0xFFFFu, d0 >> 16), m1 = min(d1 & 0xFFFFu, d1 >> 16);
-  uint32_t m2 = min(d2 & 0xFFFFu, d2 >> 16), m3 = min(d3 & 0xFFFFu, d3 >> 16);
-  return min(min(m0, m1), min(m2, m3));
+  const axc_us2 s = {(unsigned short)seg, (unsigned short)seg};
+  const axc_us2 d0 = __builtin_bit_cast(axc_us2, c.x) - s;
+  const axc_us2 d1 = __builtin_bit_cast(axc_us2, c.y) - s;
+  const axc_us2 d2 = __builtin_bit_cast(axc_us2, c.z) - s;
+  const axc_us2 d3 = __builtin_bit_cast(axc_us2, c.w) - s;
+  const axc_us2 m = __builtin_elementwise_min(__builtin_elementwise_min(d0, d1), __builtin_elementwise_min(d2, d3));
+  return min((uint32_t)m.x, (uint32_t)m.y);
 }
 
 // Axon index of a packed target for the snapshot: out-of-range indices (and
@@ -228,7 +225,7 @@ __global__ __launch_bounds__(256) void spike_count_kernel(const uint32_t* __rest
 // can observe the state between ticks; the batch ends with a plain gsop launch.
 // ---------------------------------------------------------------------------
 template <bool FUSED>
-__global__ __launch_bounds__(256) void neuron_tick_kernel(ShardVramPtrs vram, uint32_t N, uint32_t A,
+__global__ __launch_bounds__(256) AXC_NEURON_ATTR void neuron_tick_kernel(ShardVramPtrs vram, uint32_t N, uint32_t A,
                                                           const AxcVariantParameters* __restrict__ lut_g,
                                                           const TickParams* __restrict__ prm, uint32_t tick_arg,
                                                           ShardScratch sc, const uint4* __restrict__ cur,
@@ -242,7 +239,14 @@ __global__ __launch_bounds__(256) void neuron_tick_kernel(ShardVramPtrs vram, ui
   if (tid < 64) reinterpret_cast<uint4*>(lut)[tid] = reinterpret_cast<const uint4*>(lut_g)[tid];
   __syncthreads();
 
-  const uint32_t i = blockIdx.x * blockDim.x + tid;
+  // XCD-aware order: workgroups are dealt round-robin to the 8 XCDs, so
+  // workgroup b runs on XCD b % 8. Each XCD gets one contiguous eighth of the
+  // somas and walks it in order, which keeps the snapshot entries its somas
+  // gather (axons of nearby somas) in that XCD's own L2. The grid is padded
+  // to a multiple of 8; workgroups past the last soma block have no somas.
+  const uint32_t per_xcd = gridDim.x / kNumXcd;
+  const uint32_t lblock = (blockIdx.x % kNumXcd) * per_xcd + blockIdx.x / kNumXcd;
+  const uint32_t i = lblock * blockDim.x + tid;
   const uint32_t tick = tick_of(prm, tick_arg);
   bool fin = false;
   bool push_own = false;
@@ -270,56 +274,54 @@ __global__ __launch_bounds__(256) void neuron_tick_kernel(ShardVramPtrs vram, ui
       const char* __restrict__ pbase = reinterpret_cast<const char*>(prev);
       uint32_t filled = 0;
       bool alive = true;
-      // Software pipeline: the targets and weights of the next eight slots
-      // are requested before this group's snapshot gathers, so each lane has
-      // two groups of loads in flight.
-      uint32_t t[8];
-      int16_t w[8];
+      // kScanUnroll slots per step. With kScanPipeline the targets and
+      // weights of the next step are requested before this step's snapshot
+      // gathers, so each lane has two steps of loads in flight.
+      constexpr int U = kScanUnroll;
+      uint32_t t[U];
+      int16_t w[U];
 #pragma unroll
-      for (int u = 0; u < 8; u++) {
+      for (int u = 0; u < U; u++) {
         const char* row = reinterpret_cast<const char*>(vram.dendrite_targets + (size_t)u * N);
         t[u] = __builtin_nontemporal_load(reinterpret_cast<const uint32_t*>(row + toff));
       }
 #pragma unroll
-      for (int u = 0; u < 8; u++) {
+      for (int u = 0; u < U; u++) {
         const char* row = reinterpret_cast<const char*>(vram.dendrite_weights + (size_t)u * N);
         w[u] = *reinterpret_cast<const int16_t*>(row + woff);
       }
-      for (uint32_t s = 0; s < kMaxDendrites && alive; s += 8) {
-        uint32_t nv = 8;
+      for (uint32_t s = 0; s < kMaxDendrites && alive; s += U) {
+        uint32_t nv = U;
 #pragma unroll
-        for (int u = 7; u >= 0; u--) nv = (t[u] == 0) ? (uint32_t)u : nv;
-        alive = nv == 8;
+        for (int u = U - 1; u >= 0; u--) nv = (t[u] == 0) ? (uint32_t)u : nv;
+        alive = nv == U;
         filled += nv;
-        uint32_t ao[8];  // byte offset of the slot's snapshot entry; slots past the first empty one read the inert entry
+        uint32_t ao[U];  // byte offset of the slot's snapshot entry; slots past the first empty one read the inert entry
 #pragma unroll
-        for (int u = 0; u < 8; u++) ao[u] = (((uint32_t)u < nv) ? snap_index(t[u], A) : A) << 4;
-        uint32_t tn[8];
-        int16_t wn[8];
-        const bool more = alive && (s + 8 < kMaxDendrites);
-        if (more) {
+        for (int u = 0; u < U; u++) ao[u] = (((uint32_t)u < nv) ? snap_index(t[u], A) : A) << 4;
+        uint32_t tn[U];
+        int16_t wn[U];
+        const bool more = alive && (s + U < kMaxDendrites);
+        if (kScanPipeline && more) {
 #pragma unroll
-          for (int u = 0; u < 8; u++) {
-            const char* row = reinterpret_cast<const char*>(vram.dendrite_targets + (size_t)(s + 8 + u) * N);
+          for (int u = 0; u < U; u++) {
+            const char* row = reinterpret_cast<const char*>(vram.dendrite_targets + (size_t)(s + U + u) * N);
             tn[u] = __builtin_nontemporal_load(reinterpret_cast<const uint32_t*>(row + toff));
           }
 #pragma unroll
-          for (int u = 0; u < 8; u++) {
-            const char* row = reinterpret_cast<const char*>(vram.dendrite_weights + (size_t)(s + 8 + u) * N);
+          for (int u = 0; u < U; u++) {
+            const char* row = reinterpret_cast<const char*>(vram.dendrite_weights + (size_t)(s + U + u) * N);
             wn[u] = *reinterpret_cast<const int16_t*>(row + woff);
           }
-        } else {
-#pragma unroll
-          for (int u = 0; u < 8; u++) { tn[u] = 0; wn[u] = 0; }
         }
         if (FUSED && pend) {
           // Pending GSOP of the previous tick first, so that its snapshot
           // values and this tick's are never live together.
-          uint4 q[8];
+          uint4 q[U];
 #pragma unroll
-          for (int u = 0; u < 8; u++) q[u] = *reinterpret_cast<const uint4*>(pbase + ao[u]);
+          for (int u = 0; u < U; u++) q[u] = *reinterpret_cast<const uint4*>(pbase + ao[u]);
 #pragma unroll
-          for (int u = 0; u < 8; u++) {
+          for (int u = 0; u < U; u++) {
             const int16_t nw = gsop_weight(w[u], min_dist16(q[u], t[u] >> 24), g.len, g.raw_pot, g.final_dep, g.bm,
                                            p.inertia_curve);
             if ((uint32_t)u < nv) {
@@ -331,19 +333,32 @@ __global__ __launch_bounds__(256) void neuron_tick_kernel(ShardVramPtrs vram, ui
         }
         __builtin_amdgcn_sched_barrier(0);
         if (!refr) {
-          uint4 c[8];
+          uint4 c[U];
 #pragma unroll
-          for (int u = 0; u < 8; u++) c[u] = *reinterpret_cast<const uint4*>(cbase + ao[u]);
+          for (int u = 0; u < U; u++) c[u] = *reinterpret_cast<const uint4*>(cbase + ao[u]);
 #pragma unroll
-          for (int u = 0; u < 8; u++) {
+          for (int u = 0; u < U; u++) {
             // The inert entry never hits, so slots past the end need no mask.
             const bool hit = min_dist16(c[u], t[u] >> 24) < prop;
             n_hits += hit;
             i_in = wadd(i_in, hit ? (int32_t)w[u] : 0);
           }
         }
+        if (kScanPipeline) {
 #pragma unroll
-        for (int u = 0; u < 8; u++) { t[u] = tn[u]; w[u] = wn[u]; }
+          for (int u = 0; u < U; u++) { t[u] = tn[u]; w[u] = wn[u]; }
+        } else if (alive && s + U < kMaxDendrites) {
+#pragma unroll
+          for (int u = 0; u < U; u++) {
+            const char* row = reinterpret_cast<const char*>(vram.dendrite_targets + (size_t)(s + U + u) * N);
+            t[u] = __builtin_nontemporal_load(reinterpret_cast<const uint32_t*>(row + toff));
+          }
+#pragma unroll
+          for (int u = 0; u < U; u++) {
+            const char* row = reinterpret_cast<const char*>(vram.dendrite_weights + (size_t)(s + U + u) * N);
+            w[u] = *reinterpret_cast<const int16_t*>(row + woff);
+          }
+        }
       }
       const uint32_t c_i = filled < kMaxDendrites ? filled + 1 : kMaxDendrites;
       if (!refr) {

@@ -78,7 +78,8 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--ticks", type=int, default=50, help="ticks per step (one BSP batch)")
-    ap.add_argument("--workload", default="c2", choices=["c2", "c1"])
+    ap.add_argument("--workload", default="c2", choices=["c2", "c1", "c2-linear"],
+                    help="c2-linear: the c2 brain with somas numbered x-fastest instead of in bricks")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--cpu-max-ticks", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
@@ -99,7 +100,7 @@ def main() -> int:
     lib = day_lib()
     lib.gpu_set_device(0)
 
-    spec = B.spec_c2() if args.workload == "c2" else B.spec_c1()
+    spec = {"c2": B.spec_c2, "c1": B.spec_c1, "c2-linear": lambda: B.spec_c2(brick=0)}[args.workload]()
     t0 = time.perf_counter()
     brain = B.generate(spec)
     N, A = brain.padded_n, brain.total_axons
@@ -255,7 +256,8 @@ def main() -> int:
         "config": {
             "workload": (f"{args.workload}: {spec.n_neurons} GLIF neurons x {spec.slots_filled} slots, "
                          f"{spec.n_inputs} inputs at {spec.input_ppm / 1e6:g}/tick, axons "
-                         f"{spec.axon_len[0]}-{spec.axon_len[1]} segments, GSOP on"),
+                         f"{spec.axon_len[0]}-{spec.axon_len[1]} segments, GSOP on, "
+                         f"soma numbering {'bricks of %d^3 voxels' % spec.brick if spec.brick else 'x-fastest'}"),
             "padded_n": N, "total_axons": A, "ticks_per_step": T, "parallelism": "single GPU",
         },
         "roofline": {
