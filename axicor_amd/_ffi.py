@@ -56,6 +56,18 @@ class AxcShardStats(C.Structure):
         "ticks", "integrated", "slots_scanned", "targets_read", "hits", "spikes", "gsop_slots", "gsop_targets")]
 
 
+class AxcBrainDesc(C.Structure):
+    """One member of a batch (include/axicor_day.h)."""
+    _fields_ = [("vram", ShardVramPtrs), ("padded_n", C.c_uint32), ("total_axons", C.c_uint32),
+                ("v_seg", C.c_uint32), ("virtual_offset", C.c_uint32), ("num_virtual_axons", C.c_uint32),
+                ("lut", C.c_void_p), ("d_input_bitmask", C.c_void_p), ("input_words", C.c_uint32),
+                ("input_ticks", C.c_uint32), ("dopamine", C.c_int16)]
+
+
+class AxcBatchTuning(C.Structure):
+    _fields_ = [("axon_blocks", C.c_uint32), ("soma_blocks", C.c_uint32), ("list_blocks", C.c_uint32)]
+
+
 _P = C.c_void_p
 _U32 = C.c_uint32
 _I32 = C.c_int32
@@ -109,6 +121,13 @@ SIGNATURES = {
     "axc_host_sort_and_prune": (None, [_P, _P, _P, _P, _U32, _I16]),
     "axc_kernel_timing_enable": (None, [C.c_int]),
     "axc_kernel_timing_read": (_I32, [_P, _P, C.c_int]),
+    "axc_batch_create": (_I32, [C.POINTER(AxcBrainDesc), _U32, C.POINTER(_P)]),
+    "axc_batch_step": (_I32, [_P, _U32, _U32, _P]),
+    "axc_batch_autotune": (_I32, [_P, _U32, _P, C.POINTER(AxcBatchTuning)]),
+    "axc_batch_set_tuning": (_I32, [_P, C.POINTER(AxcBatchTuning)]),
+    "axc_batch_get_tuning": (_I32, [_P, C.POINTER(AxcBatchTuning)]),
+    "axc_batch_stats": (_I32, [_P, _U32, C.POINTER(AxcShardStats), _P]),
+    "axc_batch_destroy": (None, [_P]),
     "axc_version": (C.c_char_p, []),
 }
 

@@ -217,3 +217,55 @@ def gen_inputs(seed: int, n_inputs: int, ppm: int, tick0: int, ticks: int) -> np
     if n_inputs > 0 and ticks > 0:
         gen_lib().axc_gen_inputs(seed & 0xFFFFFFFFFFFFFFFF, n_inputs, ppm, tick0, ticks, ptr(out))
     return out[:, :words] if words else out[:, :0]
+
+
+# ---------------------------------------------------------------------------
+# Per-brain parameter vectors (batched brains, parameter search)
+# ---------------------------------------------------------------------------
+# The twelve GLIF/GSOP parameters of one brain and their ranges. The first
+# six are the reference's search space (examples/bayesian_search.py:65-77:
+# leak 100-3000, homeostasis penalty 500-10000, decay 1-100, D1/D2 affinity
+# 64-255, prune threshold 2-30) plus its dopamine (-255..255); the rest are
+# the template value +-50% (_template/blueprints.toml:13-37), propagation
+# around the blueprint default of 10 segments.
+PARAM_NAMES = ["leak_rate", "homeostasis_penalty", "homeostasis_decay", "d1_affinity", "d2_affinity",
+               "prune_threshold", "dopamine", "threshold", "refractory_period", "gsop_potentiation",
+               "gsop_depression", "signal_propagation_length"]
+PARAM_RANGES = np.array([[100, 3000], [500, 10000], [1, 100], [64, 255], [64, 255], [2, 30], [-255, 255],
+                         [15000, 45000], [8, 22], [75, 225], [30, 90], [5, 15]], dtype=np.int64)
+
+
+def random_params(rng: np.random.Generator, n: int = 1) -> np.ndarray:
+    """Integer parameter vectors, uniform within PARAM_RANGES (inclusive)."""
+    lo, hi = PARAM_RANGES[:, 0], PARAM_RANGES[:, 1]
+    return rng.integers(lo, hi + 1, size=(n, len(PARAM_NAMES)))
+
+
+def lut_from_params(p) -> np.ndarray:
+    """Two-type table (0 excitatory, 1 inhibitory) from one parameter vector;
+    the inhibitory type gets 1.5x leak and 0.8x penalty, as the reference
+    search sets them (examples/bayesian_search.py:86-87)."""
+    p = [int(x) for x in p]
+    leak, pen, dec, d1, d2, _prune, _dop, thr, refr, pot, dep, prop = p
+    common = dict(homeostasis_decay=dec, d1_affinity=d1, d2_affinity=d2, threshold=thr, refractory_period=refr,
+                  gsop_potentiation=pot, gsop_depression=dep, signal_propagation_length=prop,
+                  spontaneous_firing_period_ticks=128)
+    return lut_from([template_variant(False, leak_rate=leak, homeostasis_penalty=pen, **common),
+                     template_variant(True, leak_rate=int(leak * 1.5), homeostasis_penalty=int(pen * 0.8), **common)])
+
+
+def spec_batch_member(rng: np.random.Generator, n_min: int = 8_000, n_max: int = 250_000,
+                      slot_choices=(32, 64, 128), params=None, seed: int | None = None) -> tuple[BrainSpec, np.ndarray]:
+    """One brain of the batched workload: neuron count log-uniform in
+    [n_min, n_max], slots from slot_choices, 5% inputs at p ~ U[0.01, 0.05],
+    its own parameter vector. Returns (spec, params)."""
+    n = int(round(np.exp(rng.uniform(np.log(n_min), np.log(n_max)))))
+    slots = int(rng.choice(slot_choices))
+    if params is None:
+        params = random_params(rng)[0]
+    side = max(8, int(round((n / 1.4) ** (1 / 3) / 8)) * 8)
+    gz = max(8, (n + side * side - 1) // (side * side) // 8 * 8)
+    p_in = int(rng.integers(10_000, 50_001))
+    spec = BrainSpec(n, max(1, n // 20), (side, side, gz), slots, (1, 255), seed=int(seed if seed is not None else rng.integers(1, 2**62)),
+                     input_ppm=p_in, lut=lut_from_params(params), brick=8)
+    return spec, params

@@ -1,0 +1,143 @@
+// batch_engine.hip — many independent brains through one set of work-list
+// kernels per tick (no counterpart in the reference, whose host runs one
+// ShardEngine per shard, genesis-compute/src/compute/shard.rs:18-125).
+//
+// Each brain keeps the reference layout (its own cu_allocate_shard arrays)
+// and its own 16-entry parameter table. Per tick the batch launches three
+// fixed-grid kernels for all brains together — axon phase, soma phase (with
+// the previous tick's GSOP fused in), push phase — that walk flat work lists
+// of (brain, 256-item chunk) entries; a batch of ticks is captured in one HIP
+// graph and ends with a GSOP launch. Each brain runs exactly the device code
+// of the single-shard path (axon_step, neuron_block, push_apply_one,
+// gsop_soma), so every brain is bit-identical to running it alone.
+//
+// Included by day_engine.hip (one translation unit).
+
+namespace axc {
+
+// Device view of one brain.
+struct BrainDev {
+  ShardVramPtrs vram;
+  uint32_t N, A, v_seg, virt_off, n_virt, words, input_ticks;
+  int32_t dop;
+  const AxcVariantParameters* lut;  // device, 16 entries
+  const uint32_t* bitmask;          // device [input_ticks][words] or null
+  ShardScratch sc;
+  uint64_t* stats;                  // [kStatsPerBlock] accumulated with atomics
+  uint64_t* type_spikes;            // [16] spikes per variant (flags >> 4)
+};
+
+// Work item: brain index and first element (axon or soma) of a 256 chunk.
+struct WorkItem {
+  uint32_t brain;
+  uint32_t start;
+};
+
+struct BatchTick {
+  uint32_t tick_base;
+};
+
+// Axon phase over all brains (K2 + K4 with the counters reset, see
+// axon_tick_kernel). Persistent: block g takes items g, g + G, ...
+__global__ __launch_bounds__(256) void batch_axon_kernel(const BrainDev* __restrict__ brains,
+                                                         const WorkItem* __restrict__ items, uint32_t n_items,
+                                                         const BatchTick* __restrict__ bt, uint32_t t_off,
+                                                         uint32_t snap) {
+  const uint32_t tick = bt->tick_base + t_off;
+  for (uint32_t it = blockIdx.x; it < n_items; it += gridDim.x) {
+    const WorkItem w = items[it];
+    const BrainDev& b = brains[w.brain];
+    const uint32_t a = w.start + threadIdx.x;
+    if (w.start == 0 && threadIdx.x < kNumCounters) b.sc.counters[threadIdx.x] = 0;
+    if (a >= b.A) continue;
+    const uint32_t* bm = b.bitmask ? b.bitmask + (size_t)(tick % b.input_ticks) * b.words : nullptr;
+    axon_step(b.vram.axon_heads, a, b.v_seg, bm, b.virt_off, bm ? b.n_virt : 0u, b.sc.push_cnt, false,
+              b.sc.comp[snap & 1]);
+  }
+}
+
+// Soma phase over all brains. Block g on XCD g % 8 walks a contiguous eighth
+// of the item list (the XCD-aware order of neuron_tick_kernel).
+template <bool FUSED>
+__global__ __launch_bounds__(256) AXC_NEURON_ATTR void batch_neuron_kernel(const BrainDev* __restrict__ brains,
+                                                                        const WorkItem* __restrict__ items,
+                                                                        uint32_t n_items,
+                                                                        const BatchTick* __restrict__ bt,
+                                                                        uint32_t t_off, uint32_t snap) {
+  __shared__ AxcVariantParameters lut[16];
+  __shared__ uint64_t cnt[kStatsPerBlock];
+  const uint32_t tick = bt->tick_base + t_off;
+  const uint32_t G = gridDim.x;
+  const uint32_t per_xcd_blocks = G / kNumXcd;
+  const uint32_t xcd = blockIdx.x % kNumXcd, k = blockIdx.x / kNumXcd;
+  const uint32_t span = (n_items + kNumXcd - 1) / kNumXcd;
+  const uint32_t lo = xcd * span, hi = min(n_items, lo + span);
+  for (uint32_t it = lo + k; it < hi; it += per_xcd_blocks) {
+    const WorkItem w = items[it];
+    const BrainDev& b = brains[w.brain];
+    __syncthreads();  // the previous item's LDS table and counters are done with
+    if (threadIdx.x < 64) reinterpret_cast<uint4*>(lut)[threadIdx.x] = reinterpret_cast<const uint4*>(b.lut)[threadIdx.x];
+    if (threadIdx.x < kStatsPerBlock) cnt[threadIdx.x] = 0;
+    __syncthreads();
+    neuron_block<FUSED>(b.vram, b.N, b.A, lut, tick, b.dop, b.sc, b.sc.comp[snap & 1], b.sc.comp[(snap & 1) ^ 1],
+                        w.start, cnt, w.start == 0);
+    __syncthreads();
+    if (threadIdx.x < kStatsPerBlock && cnt[threadIdx.x]) atomicAdd((unsigned long long*)&b.stats[threadIdx.x],
+                                                                   (unsigned long long)cnt[threadIdx.x]);
+    // Spikes per variant type, for per-population firing rates.
+    const uint32_t i = w.start + threadIdx.x;
+    if (i < b.N) {
+      const uint8_t f = b.vram.soma_flags[i];
+      if (f & 1) atomicAdd((unsigned long long*)&b.type_spikes[f >> 4], 1ull);
+    }
+  }
+}
+
+// Push phase: every thread of the grid strides over each brain's owner list.
+__global__ __launch_bounds__(256) void batch_push_kernel(const BrainDev* __restrict__ brains, uint32_t n_brains,
+                                                         uint32_t snap) {
+  const uint32_t gt = blockIdx.x * blockDim.x + threadIdx.x, gs = gridDim.x * blockDim.x;
+  for (uint32_t bi = 0; bi < n_brains; bi++) {
+    const BrainDev& b = brains[bi];
+    const uint32_t n = b.sc.counters[kCntOwners];
+    for (uint32_t j = gt; j < n; j += gs) push_apply_one(b.vram.axon_heads, b.sc, b.sc.comp[snap & 1], j);
+  }
+}
+
+// GSOP of the last tick of a batch for every brain: every wave of the grid
+// strides over each brain's spike list.
+__global__ __launch_bounds__(256) void batch_gsop_kernel(const BrainDev* __restrict__ brains, uint32_t n_brains,
+                                                         uint32_t snap) {
+  __shared__ AxcVariantParameters lut[16];
+  __shared__ uint32_t red[4][2];
+  const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
+  const uint32_t gw = blockIdx.x * 4 + wv, W = gridDim.x * 4;
+  for (uint32_t bi = 0; bi < n_brains; bi++) {
+    const BrainDev& b = brains[bi];
+    __syncthreads();
+    if (threadIdx.x < 64) reinterpret_cast<uint4*>(lut)[threadIdx.x] = reinterpret_cast<const uint4*>(b.lut)[threadIdx.x];
+    __syncthreads();
+    const uint32_t n = b.sc.counters[kCntSpikes];
+    uint32_t cs = 0, ct = 0;
+    for (uint32_t k = gw; k < n; k += W) {
+      const uint32_t filled = gsop_soma(b.vram, b.N, b.A, lut, b.dop, b.sc.spk_list[k], b.sc.comp[snap & 1], lane);
+      cs += filled;
+      ct += filled < kMaxDendrites ? filled + 1 : kMaxDendrites;
+    }
+    if (lane == 0) {
+      red[wv][0] = cs;
+      red[wv][1] = ct;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const uint64_t s = red[0][0] + red[1][0] + red[2][0] + red[3][0];
+      const uint64_t t = red[0][1] + red[1][1] + red[2][1] + red[3][1];
+      if (s) atomicAdd((unsigned long long*)&b.stats[kStGsopSlots], (unsigned long long)s);
+      if (t) atomicAdd((unsigned long long*)&b.stats[kStGsopTargets], (unsigned long long)t);
+    }
+  }
+}
+
+__global__ void batch_set_tick_kernel(BatchTick* __restrict__ bt, uint32_t tick_base) { bt->tick_base = tick_base; }
+
+}  // namespace axc

@@ -19,6 +19,7 @@
 #include <algorithm>
 #include "day_common.h"
 #include "day_kernels.hip"
+#include "batch_engine.hip"
 
 using namespace axc;
 
@@ -856,5 +857,293 @@ int32_t axc_kernel_timing_read(double* ms_out, uint64_t* launches_out, int reset
 }
 
 const char* axc_version(void) { return "axicor_amd day-phase engine 0.1 (gfx950)"; }
+
+}  // extern "C"
+
+// ===========================================================================
+// Batched brains (no counterpart in the reference)
+// ===========================================================================
+namespace {
+
+struct Batch {
+  int device = 0;
+  std::vector<AxcBrainDesc> desc;
+  std::vector<BrainDev> host_dev;
+  std::vector<Shard*> shards;          // per-brain workspaces (not in the shard registry)
+  BrainDev* d_brains = nullptr;
+  AxcVariantParameters* d_luts = nullptr;
+  uint64_t* d_stats = nullptr;         // [n][kStatsPerBlock]
+  uint64_t* d_type_spikes = nullptr;   // [n][16]
+  WorkItem* d_axon_items = nullptr;
+  WorkItem* d_soma_items = nullptr;
+  uint32_t n_axon_items = 0, n_soma_items = 0;
+  BatchTick* d_tick = nullptr;
+  AxcBatchTuning tune{0, 0, 0};
+  hipGraphExec_t gexec = nullptr;
+  hipGraph_t graph = nullptr;
+  uint32_t graph_T = 0;
+  AxcBatchTuning graph_tune{0, 0, 0};
+};
+
+hipError_t batch_capture(Batch* b, uint32_t T) {
+  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->gexec = nullptr;
+  b->graph = nullptr;
+  hipStream_t cap;
+  hipError_t e = hipStreamCreateWithFlags(&cap, hipStreamNonBlocking);
+  if (e != hipSuccess) return e;
+  const uint32_t n = (uint32_t)b->desc.size();
+  e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
+  if (e == hipSuccess) {
+    for (uint32_t t = 0; t < T; t++) {
+      batch_axon_kernel<<<b->tune.axon_blocks, kThreads, 0, cap>>>(b->d_brains, b->d_axon_items, b->n_axon_items,
+                                                                   b->d_tick, t, t);
+      if (t == 0)
+        batch_neuron_kernel<false><<<b->tune.soma_blocks, kThreads, 0, cap>>>(b->d_brains, b->d_soma_items,
+                                                                              b->n_soma_items, b->d_tick, t, t);
+      else
+        batch_neuron_kernel<true><<<b->tune.soma_blocks, kThreads, 0, cap>>>(b->d_brains, b->d_soma_items,
+                                                                             b->n_soma_items, b->d_tick, t, t);
+      batch_push_kernel<<<b->tune.list_blocks, kThreads, 0, cap>>>(b->d_brains, n, t);
+    }
+    batch_gsop_kernel<<<b->tune.list_blocks, kThreads, 0, cap>>>(b->d_brains, n, T - 1);
+    e = hipGetLastError();
+    hipGraph_t g = nullptr;
+    hipError_t e2 = hipStreamEndCapture(cap, &g);
+    if (e == hipSuccess) e = e2;
+    if (e == hipSuccess) {
+      e = hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0);
+      b->graph = g;
+    } else if (g) {
+      (void)hipGraphDestroy(g);
+    }
+  }
+  (void)hipStreamDestroy(cap);
+  if (e == hipSuccess) {
+    b->graph_T = T;
+    b->graph_tune = b->tune;
+  }
+  return e;
+}
+
+hipError_t batch_run(Batch* b, uint32_t tick_base, uint32_t T, hipStream_t st) {
+  if (!(b->gexec && b->graph_T == T && b->graph_tune.axon_blocks == b->tune.axon_blocks &&
+        b->graph_tune.soma_blocks == b->tune.soma_blocks && b->graph_tune.list_blocks == b->tune.list_blocks)) {
+    hipError_t e = batch_capture(b, T);
+    if (e != hipSuccess) return e;
+  }
+  batch_set_tick_kernel<<<1, 1, 0, st>>>(b->d_tick, tick_base);
+  return hipGraphLaunch(b->gexec, st);
+}
+
+void batch_free(Batch* b) {
+  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  for (Shard* s : b->shards) {
+    free_scratch(s);
+    delete s;
+  }
+  (void)hipFree(b->d_brains);
+  (void)hipFree(b->d_luts);
+  (void)hipFree(b->d_stats);
+  (void)hipFree(b->d_type_spikes);
+  (void)hipFree(b->d_axon_items);
+  (void)hipFree(b->d_soma_items);
+  (void)hipFree(b->d_tick);
+  delete b;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t axc_batch_create(const AxcBrainDesc* brains, uint32_t n, void** out) {
+  if (!brains || n == 0 || !out) return (int32_t)hipErrorInvalidValue;
+  Batch* b = new Batch();
+  (void)hipGetDevice(&b->device);
+  b->desc.assign(brains, brains + n);
+  hipError_t e = hipSuccess;
+  std::vector<WorkItem> ax, so;
+  for (uint32_t k = 0; k < n && e == hipSuccess; k++) {
+    const AxcBrainDesc& d = brains[k];
+    if (d.num_virtual_axons && (uint64_t)d.virtual_offset + d.num_virtual_axons > d.total_axons) {
+      e = hipErrorInvalidValue;
+      break;
+    }
+    if (d.d_input_bitmask && d.input_ticks == 0) {
+      e = hipErrorInvalidValue;
+      break;
+    }
+    Shard* s = new Shard();
+    s->N = d.padded_n;
+    s->A = d.total_axons;
+    s->device = b->device;
+    b->shards.push_back(s);
+    if ((e = alloc_scratch(s)) != hipSuccess) break;
+    for (uint32_t a = 0; a < d.total_axons; a += kThreads) ax.push_back(WorkItem{k, a});
+    for (uint32_t i = 0; i < d.padded_n; i += kThreads) so.push_back(WorkItem{k, i});
+  }
+  if (e == hipSuccess) e = hipMalloc((void**)&b->d_luts, (size_t)n * 16 * sizeof(AxcVariantParameters));
+  if (e == hipSuccess) e = hipMalloc((void**)&b->d_stats, (size_t)n * kStatsPerBlock * 8);
+  if (e == hipSuccess) e = hipMalloc((void**)&b->d_type_spikes, (size_t)n * 16 * 8);
+  if (e == hipSuccess) e = hipMemset(b->d_stats, 0, (size_t)n * kStatsPerBlock * 8);
+  if (e == hipSuccess) e = hipMemset(b->d_type_spikes, 0, (size_t)n * 16 * 8);
+  if (e == hipSuccess) e = hipMalloc((void**)&b->d_tick, sizeof(BatchTick));
+  if (e == hipSuccess) {
+    b->n_axon_items = (uint32_t)ax.size();
+    b->n_soma_items = (uint32_t)so.size();
+    e = hipMalloc((void**)&b->d_axon_items, ax.size() * sizeof(WorkItem));
+    if (e == hipSuccess) e = hipMalloc((void**)&b->d_soma_items, so.size() * sizeof(WorkItem));
+    if (e == hipSuccess) e = hipMemcpy(b->d_axon_items, ax.data(), ax.size() * sizeof(WorkItem), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(b->d_soma_items, so.data(), so.size() * sizeof(WorkItem), hipMemcpyHostToDevice);
+  }
+  if (e == hipSuccess) {
+    b->host_dev.resize(n);
+    for (uint32_t k = 0; k < n && e == hipSuccess; k++) {
+      const AxcBrainDesc& d = brains[k];
+      e = hipMemcpy(b->d_luts + (size_t)k * 16, d.lut, 16 * sizeof(AxcVariantParameters), hipMemcpyHostToDevice);
+      BrainDev& h = b->host_dev[k];
+      h.vram = d.vram;
+      h.N = d.padded_n;
+      h.A = d.total_axons;
+      h.v_seg = d.v_seg;
+      h.virt_off = d.virtual_offset;
+      h.n_virt = d.num_virtual_axons;
+      h.words = d.input_words;
+      h.input_ticks = d.input_ticks ? d.input_ticks : 1;
+      h.dop = d.dopamine;
+      h.lut = b->d_luts + (size_t)k * 16;
+      h.bitmask = d.d_input_bitmask;
+      h.sc = b->shards[k]->sc;
+      h.stats = b->d_stats + (size_t)k * kStatsPerBlock;
+      h.type_spikes = b->d_type_spikes + (size_t)k * 16;
+    }
+  }
+  if (e == hipSuccess) e = hipMalloc((void**)&b->d_brains, n * sizeof(BrainDev));
+  if (e == hipSuccess) e = hipMemcpy(b->d_brains, b->host_dev.data(), n * sizeof(BrainDev), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    batch_free(b);
+    return (int32_t)e;
+  }
+  int cus = 256;
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device);
+  b->tune.axon_blocks = (uint32_t)cus * 8;
+  b->tune.soma_blocks = (uint32_t)cus * 4;
+  b->tune.list_blocks = (uint32_t)cus * 2;
+  *out = b;
+  return 0;
+}
+
+int32_t axc_batch_step(void* h, uint32_t tick_base, uint32_t ticks, void* stream) {
+  if (!h || ticks == 0) return (int32_t)hipErrorInvalidValue;
+  return (int32_t)batch_run((Batch*)h, tick_base, ticks, (hipStream_t)stream);
+}
+
+int32_t axc_batch_set_tuning(void* h, const AxcBatchTuning* t) {
+  if (!h || !t || t->axon_blocks == 0 || t->list_blocks == 0 || t->soma_blocks == 0 ||
+      t->soma_blocks % kNumXcd != 0)
+    return (int32_t)hipErrorInvalidValue;
+  ((Batch*)h)->tune = *t;
+  return 0;
+}
+
+int32_t axc_batch_get_tuning(void* h, AxcBatchTuning* t) {
+  if (!h || !t) return (int32_t)hipErrorInvalidValue;
+  *t = ((Batch*)h)->tune;
+  return 0;
+}
+
+int32_t axc_batch_autotune(void* h, uint32_t ticks, void* stream, AxcBatchTuning* best_out) {
+  if (!h || ticks == 0) return (int32_t)hipErrorInvalidValue;
+  Batch* b = (Batch*)h;
+  hipStream_t st = (hipStream_t)stream;
+  const uint32_t n = (uint32_t)b->desc.size();
+  // Snapshot every brain (state, heads) so that tuning leaves no trace.
+  std::vector<void*> bak(2 * n, nullptr);
+  hipError_t e = hipSuccess;
+  for (uint32_t k = 0; k < n && e == hipSuccess; k++) {
+    const AxcBrainDesc& d = b->desc[k];
+    const size_t sb = (size_t)d.padded_n * AXC_STATE_BYTES_PER_NEURON, hb = (size_t)d.total_axons * 32;
+    e = hipMalloc(&bak[2 * k], sb);
+    if (e == hipSuccess) e = hipMalloc(&bak[2 * k + 1], hb);
+    if (e == hipSuccess) e = hipMemcpyAsync(bak[2 * k], d.vram.soma_voltage, sb, hipMemcpyDeviceToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(bak[2 * k + 1], d.vram.axon_heads, hb, hipMemcpyDeviceToDevice, st);
+  }
+  int cus = 256;
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device);
+  const AxcBatchTuning base = b->tune;
+  AxcBatchTuning best = base;
+  float best_ms = 1e30f;
+  hipEvent_t e0, e1;
+  (void)hipEventCreate(&e0);
+  (void)hipEventCreate(&e1);
+  const uint32_t soma_mult[] = {2, 4, 8, 16};
+  const uint32_t axon_mult[] = {4, 16};
+  for (uint32_t sm : soma_mult) {
+    for (uint32_t am : axon_mult) {
+      if (e != hipSuccess) break;
+      b->tune.soma_blocks = (uint32_t)cus * sm;
+      b->tune.axon_blocks = (uint32_t)cus * am;
+      b->tune.list_blocks = base.list_blocks;
+      e = batch_run(b, 0, ticks, st);  // capture + warm
+      if (e == hipSuccess) (void)hipEventRecord(e0, st);
+      if (e == hipSuccess) e = batch_run(b, ticks, ticks, st);
+      if (e == hipSuccess) (void)hipEventRecord(e1, st);
+      if (e == hipSuccess) e = hipEventSynchronize(e1);
+      float ms = 0;
+      if (e == hipSuccess) (void)hipEventElapsedTime(&ms, e0, e1);
+      if (e == hipSuccess && ms < best_ms) {
+        best_ms = ms;
+        best = b->tune;
+      }
+      // Restore before the next candidate.
+      for (uint32_t k = 0; k < n && e == hipSuccess; k++) {
+        const AxcBrainDesc& d = b->desc[k];
+        const size_t sb = (size_t)d.padded_n * AXC_STATE_BYTES_PER_NEURON, hb = (size_t)d.total_axons * 32;
+        e = hipMemcpyAsync(d.vram.soma_voltage, bak[2 * k], sb, hipMemcpyDeviceToDevice, st);
+        if (e == hipSuccess) e = hipMemcpyAsync(d.vram.axon_heads, bak[2 * k + 1], hb, hipMemcpyDeviceToDevice, st);
+      }
+    }
+  }
+  // The snapshot buffers follow the restored state; counters restart at zero.
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  for (void* p : bak) (void)hipFree(p);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (e == hipSuccess) {
+    e = hipMemset(b->d_stats, 0, (size_t)n * kStatsPerBlock * 8);
+    if (e == hipSuccess) e = hipMemset(b->d_type_spikes, 0, (size_t)n * 16 * 8);
+  }
+  b->tune = best;
+  if (best_out) *best_out = best;
+  return (int32_t)e;
+}
+
+int32_t axc_batch_stats(void* h, uint32_t brain, AxcShardStats* out, uint64_t* type_spikes16) {
+  if (!h || !out) return (int32_t)hipErrorInvalidValue;
+  Batch* b = (Batch*)h;
+  if (brain >= b->desc.size()) return (int32_t)hipErrorInvalidValue;
+  uint64_t v[kStatsPerBlock];
+  hipError_t e = hipMemcpy(v, b->d_stats + (size_t)brain * kStatsPerBlock, sizeof(v), hipMemcpyDeviceToHost);
+  if (e == hipSuccess && type_spikes16)
+    e = hipMemcpy(type_spikes16, b->d_type_spikes + (size_t)brain * 16, 16 * 8, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return (int32_t)e;
+  out->integrated = v[kStIntegrated];
+  out->slots_scanned = v[kStScanned];
+  out->targets_read = v[kStTargets];
+  out->hits = v[kStHits];
+  out->spikes = v[kStSpikes];
+  out->gsop_slots = v[kStGsopSlots];
+  out->gsop_targets = v[kStGsopTargets];
+  out->ticks = v[kStTicks];
+  return 0;
+}
+
+void axc_batch_destroy(void* h) {
+  if (!h) return;
+  (void)hipDeviceSynchronize();
+  batch_free((Batch*)h);
+}
 
 }  // extern "C"

@@ -159,20 +159,9 @@ __device__ __forceinline__ GsopTerms gsop_terms(uint8_t flags, const AxcVariantP
 // k pushes an axon receives this tick are applied at once, before propagation.
 // Also writes the compact snapshot of the propagated heads.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void axon_tick_kernel(AxcBurstHeads8* __restrict__ heads, uint32_t A,
-                                                        uint32_t v_seg, const uint32_t* __restrict__ bitmask_arg,
-                                                        uint32_t virt_off, uint32_t n_virt,
-                                                        uint32_t* __restrict__ push_cnt, int use_cnt,
-                                                        uint32_t* __restrict__ counters,
-                                                        const TickParams* __restrict__ prm, uint32_t t_off,
-                                                        uint32_t words, uint4* __restrict__ comp) {
-  uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
-  const uint32_t* __restrict__ bitmask =
-      prm ? (prm->bitmask ? prm->bitmask + (size_t)t_off * words : nullptr) : bitmask_arg;
-  // The list counters of the previous tick were consumed by its gsop and
-  // push_apply launches; this tick's neuron_tick fills them again.
-  if (a < kNumCounters) counters[a] = 0;
-  if (a >= A) return;
+__device__ __forceinline__ void axon_step(AxcBurstHeads8* __restrict__ heads, uint32_t a, uint32_t v_seg,
+                                          const uint32_t* __restrict__ bitmask, uint32_t virt_off, uint32_t n_virt,
+                                          uint32_t* __restrict__ push_cnt, bool use_cnt, uint4* __restrict__ comp) {
   Heads8 h = load_heads(heads, a);
   uint32_t k = 0;
   if (bitmask) {
@@ -197,6 +186,23 @@ __global__ __launch_bounds__(256) void axon_tick_kernel(AxcBurstHeads8* __restri
   h.hi.w += (h.hi.w != kSentinel) ? v_seg : 0u;
   store_heads(heads, a, h);
   if (comp) comp[a] = compact_heads(h);
+}
+
+__global__ __launch_bounds__(256) void axon_tick_kernel(AxcBurstHeads8* __restrict__ heads, uint32_t A,
+                                                        uint32_t v_seg, const uint32_t* __restrict__ bitmask_arg,
+                                                        uint32_t virt_off, uint32_t n_virt,
+                                                        uint32_t* __restrict__ push_cnt, int use_cnt,
+                                                        uint32_t* __restrict__ counters,
+                                                        const TickParams* __restrict__ prm, uint32_t t_off,
+                                                        uint32_t words, uint4* __restrict__ comp) {
+  uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t* __restrict__ bitmask =
+      prm ? (prm->bitmask ? prm->bitmask + (size_t)t_off * words : nullptr) : bitmask_arg;
+  // The list counters of the previous tick were consumed by its gsop and
+  // push_apply launches; this tick's neuron_tick fills them again.
+  if (a < kNumCounters) counters[a] = 0;
+  if (a >= A) return;
+  axon_step(heads, a, v_seg, bitmask, virt_off, n_virt, push_cnt, use_cnt != 0, comp);
 }
 
 // K3 (physics.cu:114-132): one count per occurrence of an id < A.
@@ -224,30 +230,21 @@ __global__ __launch_bounds__(256) void spike_count_kernel(const uint32_t* __rest
 // launch followed by the next tick's K5. Used inside a batch, where no caller
 // can observe the state between ticks; the batch ends with a plain gsop launch.
 // ---------------------------------------------------------------------------
+// One workgroup's 256 somas [i0, i0+256) of one shard: the per-soma tick and
+// the workgroup-level compaction and counters. `lut` is the shard's table,
+// already in LDS. `part` (may be null) receives the workgroup's counters.
+// Called by the single-shard kernel and by the batched work-list kernel.
 template <bool FUSED>
-__global__ __launch_bounds__(256) AXC_NEURON_ATTR void neuron_tick_kernel(ShardVramPtrs vram, uint32_t N, uint32_t A,
-                                                          const AxcVariantParameters* __restrict__ lut_g,
-                                                          const TickParams* __restrict__ prm, uint32_t tick_arg,
-                                                          ShardScratch sc, const uint4* __restrict__ cur,
-                                                          const uint4* __restrict__ prev, int32_t dop_arg) {
-  __shared__ AxcVariantParameters lut[16];
+__device__ __forceinline__ void neuron_block(const ShardVramPtrs& vram, uint32_t N, uint32_t A,
+                                             const AxcVariantParameters* lut, uint32_t tick, int32_t dop,
+                                             const ShardScratch& sc, const uint4* __restrict__ cur,
+                                             const uint4* __restrict__ prev, uint32_t i0, uint64_t* part,
+                                             bool count_tick) {
   __shared__ uint32_t wave_cnt[2][4];
   __shared__ uint32_t blk_base[2];
   __shared__ uint32_t red[4][8];
-
   const uint32_t tid = threadIdx.x;
-  if (tid < 64) reinterpret_cast<uint4*>(lut)[tid] = reinterpret_cast<const uint4*>(lut_g)[tid];
-  __syncthreads();
-
-  // XCD-aware order: workgroups are dealt round-robin to the 8 XCDs, so
-  // workgroup b runs on XCD b % 8. Each XCD gets one contiguous eighth of the
-  // somas and walks it in order, which keeps the snapshot entries its somas
-  // gather (axons of nearby somas) in that XCD's own L2. The grid is padded
-  // to a multiple of 8; workgroups past the last soma block have no somas.
-  const uint32_t per_xcd = gridDim.x / kNumXcd;
-  const uint32_t lblock = (blockIdx.x % kNumXcd) * per_xcd + blockIdx.x / kNumXcd;
-  const uint32_t i = lblock * blockDim.x + tid;
-  const uint32_t tick = tick_of(prm, tick_arg);
+  const uint32_t i = i0 + tid;
   bool fin = false;
   bool push_own = false;
   uint32_t my_axon = 0;
@@ -266,7 +263,7 @@ __global__ __launch_bounds__(256) AXC_NEURON_ATTR void neuron_tick_kernel(ShardV
 
     if (!refr || pend) {
       GsopTerms g;
-      if (FUSED && pend) g = gsop_terms(flags_in, p, dopamine_of(prm, dop_arg));
+      if (FUSED && pend) g = gsop_terms(flags_in, p, dop);
       // Byte offsets below are 32-bit lane offsets from wave-uniform bases,
       // so loads use a scalar base register plus one VGPR each.
       const uint32_t toff = i * 4u, woff = i * 2u;
@@ -450,15 +447,16 @@ __global__ __launch_bounds__(256) AXC_NEURON_ATTR void neuron_tick_kernel(ShardV
     uint32_t co = wave_cnt[1][0] + wave_cnt[1][1] + wave_cnt[1][2] + wave_cnt[1][3];
     blk_base[0] = cs ? atomicAdd(&sc.counters[kCntSpikes], cs) : 0u;
     blk_base[1] = co ? atomicAdd(&sc.counters[kCntOwners], co) : 0u;
-    uint64_t* part = sc.stats_part + (size_t)blockIdx.x * kStatsPerBlock;
-    part[kStIntegrated] += red[0][0] + red[1][0] + red[2][0] + red[3][0];
-    part[kStScanned] += red[0][1] + red[1][1] + red[2][1] + red[3][1];
-    part[kStTargets] += red[0][2] + red[1][2] + red[2][2] + red[3][2];
-    part[kStHits] += red[0][3] + red[1][3] + red[2][3] + red[3][3];
-    part[kStGsopSlots] += red[0][4] + red[1][4] + red[2][4] + red[3][4];
-    part[kStGsopTargets] += red[0][5] + red[1][5] + red[2][5] + red[3][5];
-    part[kStSpikes] += cs;
-    if (blockIdx.x == 0) part[kStTicks] += 1;
+    if (part) {
+      part[kStIntegrated] += red[0][0] + red[1][0] + red[2][0] + red[3][0];
+      part[kStScanned] += red[0][1] + red[1][1] + red[2][1] + red[3][1];
+      part[kStTargets] += red[0][2] + red[1][2] + red[2][2] + red[3][2];
+      part[kStHits] += red[0][3] + red[1][3] + red[2][3] + red[3][3];
+      part[kStGsopSlots] += red[0][4] + red[1][4] + red[2][4] + red[3][4];
+      part[kStGsopTargets] += red[0][5] + red[1][5] + red[2][5] + red[3][5];
+      part[kStSpikes] += cs;
+      if (count_tick) part[kStTicks] += 1;
+    }
   }
   __syncthreads();
   uint32_t pre_s = blk_base[0], pre_o = blk_base[1];
@@ -470,27 +468,79 @@ __global__ __launch_bounds__(256) AXC_NEURON_ATTR void neuron_tick_kernel(ShardV
   if (own_first) sc.own_list[pre_o + (uint32_t)__popcll(bo & below)] = my_axon;
 }
 
+template <bool FUSED>
+__global__ __launch_bounds__(256) AXC_NEURON_ATTR void neuron_tick_kernel(ShardVramPtrs vram, uint32_t N, uint32_t A,
+                                                          const AxcVariantParameters* __restrict__ lut_g,
+                                                          const TickParams* __restrict__ prm, uint32_t tick_arg,
+                                                          ShardScratch sc, const uint4* __restrict__ cur,
+                                                          const uint4* __restrict__ prev, int32_t dop_arg) {
+  __shared__ AxcVariantParameters lut[16];
+  const uint32_t tid = threadIdx.x;
+  if (tid < 64) reinterpret_cast<uint4*>(lut)[tid] = reinterpret_cast<const uint4*>(lut_g)[tid];
+  __syncthreads();
+  // XCD-aware order: workgroups are dealt round-robin to the 8 XCDs, so
+  // workgroup b runs on XCD b % 8. Each XCD gets one contiguous eighth of the
+  // somas and walks it in order, which keeps the snapshot entries its somas
+  // gather (axons of nearby somas) in that XCD's own L2. The grid is padded
+  // to a multiple of 8; workgroups past the last soma block have no somas.
+  const uint32_t per_xcd = gridDim.x / kNumXcd;
+  const uint32_t lblock = (blockIdx.x % kNumXcd) * per_xcd + blockIdx.x / kNumXcd;
+  neuron_block<FUSED>(vram, N, A, lut, tick_of(prm, tick_arg), dopamine_of(prm, dop_arg), sc, cur, prev,
+                      lblock * blockDim.x, sc.stats_part + (size_t)blockIdx.x * kStatsPerBlock, blockIdx.x == 0);
+}
+
 // Applies the counted pushes of this tick's spikes (physics.cu:253-262 with
 // the snapshot rule). One lane per listed axon; clears the count and refreshes
 // the axon's compact snapshot.
+__device__ __forceinline__ void push_apply_one(AxcBurstHeads8* __restrict__ heads, const ShardScratch& sc,
+                                               uint4* __restrict__ comp, uint32_t j) {
+  uint32_t a = sc.own_list[j];
+  uint32_t c = sc.push_cnt[a];
+  sc.push_cnt[a] = 0;
+  Heads8 h = load_heads(heads, a);
+  push_heads(h, c > 8 ? 8 : c);
+  store_heads(heads, a, h);
+  if (comp) comp[a] = compact_heads(h);
+}
+
 __global__ __launch_bounds__(256) void push_apply_kernel(AxcBurstHeads8* __restrict__ heads, ShardScratch sc,
                                                          uint4* __restrict__ comp) {
   const uint32_t n = sc.counters[kCntOwners];
-  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
-    uint32_t a = sc.own_list[j];
-    uint32_t c = sc.push_cnt[a];
-    sc.push_cnt[a] = 0;
-    Heads8 h = load_heads(heads, a);
-    push_heads(h, c > 8 ? 8 : c);
-    store_heads(heads, a, h);
-    if (comp) comp[a] = compact_heads(h);
-  }
+  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x)
+    push_apply_one(heads, sc, comp, j);
 }
 
 // ---------------------------------------------------------------------------
 // K6 (physics.cu:281-373) over the spiker list: one wave per spiking soma,
 // lane = dendrite slot, heads from the post-push snapshot.
 // ---------------------------------------------------------------------------
+// GSOP of one spiking soma i by one wave; returns its filled-slot count.
+__device__ __forceinline__ uint32_t gsop_soma(const ShardVramPtrs& vram, uint32_t N, uint32_t A,
+                                              const AxcVariantParameters* lut, int32_t dop, uint32_t i,
+                                              const uint4* __restrict__ comp, uint32_t lane) {
+  const uint8_t flags = vram.soma_flags[i];
+  const AxcVariantParameters& p = lut[(flags >> 4) & 0x0F];
+  const GsopTerms g = gsop_terms(flags, p, dop);
+  uint32_t filled = 0;
+  bool done = false;
+  for (uint32_t base = 0; base < kMaxDendrites && !done; base += 64) {
+    const uint32_t s = base + lane;
+    const size_t col = (size_t)s * N + i;
+    const uint32_t t = vram.dendrite_targets[col];
+    const uint64_t empty = __ballot(t == 0);
+    const uint32_t first_empty = empty ? (uint32_t)__ffsll((long long)empty) - 1u : 64u;
+    if (lane < first_empty) {
+      const uint4 c = comp[snap_index(t, A)];
+      const int16_t w = vram.dendrite_weights[col];
+      vram.dendrite_weights[col] =
+          gsop_weight(w, min_dist16(c, t >> 24), g.len, g.raw_pot, g.final_dep, g.bm, p.inertia_curve);
+    }
+    filled += first_empty;
+    done = first_empty < 64;
+  }
+  return filled;
+}
+
 __global__ __launch_bounds__(256) void gsop_kernel(ShardVramPtrs vram, uint32_t N, uint32_t A,
                                                    const AxcVariantParameters* __restrict__ lut_g,
                                                    const TickParams* __restrict__ prm, int32_t dop_arg,
@@ -508,27 +558,7 @@ __global__ __launch_bounds__(256) void gsop_kernel(ShardVramPtrs vram, uint32_t 
   uint32_t cnt_slots = 0, cnt_tgt = 0;
 
   for (uint32_t k = blockIdx.x * 4 + wv; k < n_spk; k += waves) {
-    const uint32_t i = sc.spk_list[k];
-    const uint8_t flags = vram.soma_flags[i];
-    const AxcVariantParameters& p = lut[(flags >> 4) & 0x0F];
-    const GsopTerms g = gsop_terms(flags, p, dop);
-    uint32_t filled = 0;
-    bool done = false;
-    for (uint32_t base = 0; base < kMaxDendrites && !done; base += 64) {
-      const uint32_t s = base + lane;
-      const size_t col = (size_t)s * N + i;
-      const uint32_t t = vram.dendrite_targets[col];
-      const uint64_t empty = __ballot(t == 0);
-      const uint32_t first_empty = empty ? (uint32_t)__ffsll((long long)empty) - 1u : 64u;
-      if (lane < first_empty) {
-        const uint4 c = comp[snap_index(t, A)];
-        const int16_t w = vram.dendrite_weights[col];
-        vram.dendrite_weights[col] =
-            gsop_weight(w, min_dist16(c, t >> 24), g.len, g.raw_pot, g.final_dep, g.bm, p.inertia_curve);
-      }
-      filled += first_empty;
-      done = first_empty < 64;
-    }
+    const uint32_t filled = gsop_soma(vram, N, A, lut, dop, sc.spk_list[k], comp, lane);
     cnt_slots += filled;
     cnt_tgt += filled < kMaxDendrites ? filled + 1 : kMaxDendrites;
   }

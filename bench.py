@@ -72,6 +72,85 @@ def cpu_baseline(brain, inputs_host: np.ndarray, seconds: float, max_ticks: int)
     return state, heads, ticks, time.perf_counter() - t0, tot
 
 
+def bench_batched(n_brains: int, ticks: int, input_ticks: int, batch_ticks: int, seed: int) -> dict:
+    """Batched brains (128 brains of 8k-250k neurons, slots in {32,64,128},
+    their own parameter vectors and input rates) stepped together, against the
+    same brains stepped one after another with the plain engine. Every brain's
+    final state is compared bit for bit between the two runs."""
+    from axicor_amd import brain as B
+    from axicor_amd._ffi import day_lib
+    from axicor_amd.batch import BatchEngine
+    from axicor_amd.engine import DeviceBuffer, ShardEngine, VramState, upload_constant_memory
+    lib = day_lib()
+    rng = np.random.default_rng(seed)
+    t0 = time.perf_counter()
+    members = []
+    for k in range(n_brains):
+        spec, params = B.spec_batch_member(rng)
+        b = B.generate(spec)
+        members.append((b, int(params[6]), b.inputs(0, input_ticks)))
+    neurons = sum(b.spec.n_neurons for b, _, _ in members)
+    synapses = sum(b.spec.n_neurons * b.spec.slots_filled for b, _, _ in members)
+    log(f"batched: {n_brains} brains, {neurons} neurons, {synapses} synapses, generated in "
+        f"{time.perf_counter() - t0:.1f}s")
+    mems = []
+    for b, dop, inp in members:
+        v = VramState(b.padded_n, b.total_axons)
+        v.upload_state(b.state)
+        v.upload_axon_heads(b.heads)
+        mems.append({"vram": v, "lut": b.lut, "d_inputs": DeviceBuffer.from_array(inp), "input_words": b.input_words,
+                     "input_ticks": input_ticks, "v_seg": 1, "virtual_offset": b.virtual_offset,
+                     "n_inputs": b.n_inputs, "dopamine": dop})
+    stream = C.c_void_p()
+    lib.gpu_stream_create(C.byref(stream))
+    st = stream.value
+    be = BatchEngine(mems)
+    tune = be.autotune(8, st)
+    log(f"batched: autotuned launch shape {tune}")
+    lib.gpu_stream_synchronize(st)
+    w0 = time.perf_counter()
+    for t in range(0, ticks, batch_ticks):
+        be.step(t, min(batch_ticks, ticks - t), st)
+    lib.gpu_stream_synchronize(st)
+    t_batched = time.perf_counter() - w0
+    stats = [be.stats(k) for k in range(n_brains)]
+    be.destroy()
+    res_b = [(m["vram"].download_state(), m["vram"].download_axons()) for m in mems]
+    log(f"batched: {ticks} ticks in {t_batched:.2f}s")
+    # Serial: the same brains from their initial state, one after another.
+    for (b, _, _), m in zip(members, mems):
+        m["vram"].upload_state(b.state)
+        m["vram"].upload_axon_heads(b.heads)
+    w0 = time.perf_counter()
+    for (b, dop, _), m in zip(members, mems):
+        upload_constant_memory(b.lut)
+        v = m["vram"]
+        for t in range(0, ticks, batch_ticks):
+            T = min(batch_ticks, ticks - t)
+            row = t % input_ticks
+            err = lib.axc_step_day_phase_batch(C.byref(v.ptrs), v.padded_n, v.total_axons, 1, t, T,
+                                               m["d_inputs"].ptr + row * b.input_words * 4, b.input_words,
+                                               b.virtual_offset, b.n_inputs, None, 0, None, None, None, 0, dop, st)
+            if err:
+                raise RuntimeError(f"serial step failed {err}")
+        lib.gpu_stream_synchronize(st)
+    t_serial = time.perf_counter() - w0
+    log(f"serial: {ticks} ticks x {n_brains} brains in {t_serial:.2f}s")
+    identical = all(np.array_equal(rb[0], m["vram"].download_state()) and
+                    np.array_equal(rb[1], m["vram"].download_axons()) for rb, m in zip(res_b, mems))
+    log(f"batched vs serial: {'bit-identical' if identical else 'MISMATCH'} for all {n_brains} brains")
+    lib.gpu_stream_destroy(st)
+    rates = [s[0]["spikes"] / max(1, s[0]["ticks"] * b.padded_n) for s, (b, _, _) in zip(stats, members)]
+    return {
+        "brains": n_brains, "ticks_per_brain": ticks, "neurons": neurons, "synapses": synapses,
+        "batched_brains_per_s": n_brains / t_batched, "serial_brains_per_s": n_brains / t_serial,
+        "speedup": t_serial / t_batched, "bit_identical_all_brains": identical, "tuning": tune,
+        "batched_seconds": t_batched, "serial_seconds": t_serial,
+        "serial": "each brain alone through axc_step_day_phase_batch (HIP-graph batches), one after another",
+        "firing_rate_median": float(np.median(rates)),
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -84,6 +163,8 @@ def main() -> int:
     ap.add_argument("--cpu-max-ticks", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--timing-ticks", type=int, default=10, help="per-tick launches timed with HIP events")
+    ap.add_argument("--batched-brains", type=int, default=128, help="0 skips the batched-brains leg")
+    ap.add_argument("--batched-ticks", type=int, default=2000)
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="also run one device-to-device copy of this size (reference for HBM byte counters)")
     args = ap.parse_args()
@@ -275,6 +356,9 @@ def main() -> int:
         "per_kernel": per_kernel,
         "parity": parity,
     }
+    if args.batched_brains:
+        del vram, eng, d_in
+        out["batched"] = bench_batched(args.batched_brains, args.batched_ticks, 1000, 50, 20261203)
     if cpu is not None:
         out["cpu_baseline"] = {
             "value": cpu["stats"]["slots_scanned"] / cpu["elapsed"], "unit": "synaptic events/s",

@@ -304,6 +304,43 @@ void axc_host_sort_and_prune(uint8_t *flags, uint32_t *targets, int16_t *weights
 void axc_kernel_timing_enable(int enable);
 int32_t axc_kernel_timing_read(double *ms_out, uint64_t *launches_out, int reset);
 
+/* ---- Batched brains -------------------------------------------------- */
+/* Many independent brains, each in its own cu_allocate_shard arrays with its
+ * own 16-entry parameter table, stepped together: per tick three work-list
+ * kernels cover every brain, a batch of ticks is one captured HIP graph.
+ * Every brain's result is bit-identical to stepping it alone with
+ * axc_step_day_phase_batch. For callers that run many brains at once
+ * (parameter searches, ensembles). */
+typedef struct AxcBrainDesc {
+    ShardVramPtrs vram;                 /* from cu_allocate_shard */
+    uint32_t padded_n, total_axons, v_seg;
+    uint32_t virtual_offset, num_virtual_axons;
+    const AxcVariantParameters *lut;    /* host pointer, 16 entries, copied at create */
+    const uint32_t *d_input_bitmask;    /* device [input_ticks][input_words], or NULL */
+    uint32_t input_words;               /* u32 words per tick */
+    uint32_t input_ticks;               /* tick t reads row t % input_ticks */
+    int16_t dopamine;
+} AxcBrainDesc;
+
+/* Launch shape: workgroups of the axon, soma and list kernels (the soma count
+ * must be a multiple of 8, one share per XCD). */
+typedef struct AxcBatchTuning {
+    uint32_t axon_blocks, soma_blocks, list_blocks;
+} AxcBatchTuning;
+
+int32_t axc_batch_create(const AxcBrainDesc *brains, uint32_t n_brains, void **out_batch);
+/* Ticks tick_base .. tick_base+ticks-1 for every brain, asynchronously. */
+int32_t axc_batch_step(void *batch, uint32_t tick_base, uint32_t ticks, void *stream);
+/* Times a few launch shapes on `ticks` ticks from a snapshot of every brain,
+ * restores the snapshot, keeps the fastest shape (also returned). */
+int32_t axc_batch_autotune(void *batch, uint32_t ticks, void *stream, AxcBatchTuning *best);
+int32_t axc_batch_set_tuning(void *batch, const AxcBatchTuning *tuning);
+int32_t axc_batch_get_tuning(void *batch, AxcBatchTuning *tuning);
+/* Counters of one brain since creation (or the last autotune) and its spikes
+ * per variant type (16 entries, may be NULL). */
+int32_t axc_batch_stats(void *batch, uint32_t brain, AxcShardStats *out, uint64_t *type_spikes16);
+void axc_batch_destroy(void *batch);
+
 /* Library/version string. */
 const char *axc_version(void);
 

@@ -1,0 +1,94 @@
+"""Batched brains: every brain of a batch is bit-identical to the same brain
+stepped alone (axc_step_day_phase_batch), and to the CPU oracle."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from axicor_amd import brain as B
+from axicor_amd._ffi import day_lib
+from axicor_amd.batch import BatchEngine
+from axicor_amd.engine import DeviceBuffer, ShardEngine, VramState, upload_constant_memory
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def make_members(seed: int, n_brains: int, ticks: int):
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n_brains):
+        spec, params = B.spec_batch_member(rng, n_min=1500, n_max=12000)
+        b = B.generate(spec)
+        out.append((b, int(params[6]), b.inputs(0, ticks)))
+    return out
+
+
+def load(b, dop, inputs):
+    v = VramState(b.padded_n, b.total_axons)
+    v.upload_state(b.state)
+    v.upload_axon_heads(b.heads)
+    d_in = DeviceBuffer.from_array(inputs)
+    return {"vram": v, "lut": b.lut, "d_inputs": d_in, "input_words": b.input_words, "input_ticks": inputs.shape[0],
+            "v_seg": 1, "virtual_offset": b.virtual_offset, "n_inputs": b.n_inputs, "dopamine": dop}
+
+
+def test_batch_bit_identical_to_solo_and_oracle(gpu):
+    T, nb = 30, 2
+    members = make_members(5, 7, T * nb)
+    lib = day_lib()
+    # Batched
+    mem_b = [load(b, dop, inp) for b, dop, inp in members]
+    be = BatchEngine(mem_b)
+    tune = be.autotune(4)
+    assert tune["soma_blocks"] % 8 == 0
+    for k in range(nb):
+        be.step(k * T, T)
+    lib.gpu_device_synchronize()
+    # Solo: the plain engine, one brain at a time (each with its own table)
+    for k, (b, dop, inp) in enumerate(members):
+        m = load(b, dop, inp)
+        upload_constant_memory(b.lut)
+        eng = ShardEngine(m["vram"])
+        for j in range(nb):
+            err = lib.axc_step_day_phase_batch(C.byref(m["vram"].ptrs), b.padded_n, b.total_axons, 1, j * T, T,
+                                               m["d_inputs"].ptr + j * T * b.input_words * 4, b.input_words,
+                                               b.virtual_offset, b.n_inputs, None, 0, None, None, None, 0, dop,
+                                               eng.stream)
+            assert err == 0
+        eng.synchronize()
+        sb, hb = mem_b[k]["vram"].download_state(), mem_b[k]["vram"].download_axons()
+        ss, hs = m["vram"].download_state(), m["vram"].download_axons()
+        assert np.array_equal(sb, ss), f"brain {k}: batched state != solo state"
+        assert np.array_equal(hb, hs), f"brain {k}: batched heads != solo heads"
+        if k < 2:
+            st, hd = b.state.copy(), b.heads.copy()
+            O.step_batch(st, b.padded_n, hd, b.total_axons, b.lut, 1, 0, T * nb, inp, b.virtual_offset, b.n_inputs,
+                         dopamine=dop)
+            assert np.array_equal(sb, st) and np.array_equal(hb, hd), f"brain {k}: batched != oracle"
+        s, ts = be.stats(k)
+        assert s["ticks"] == T * nb and ts.sum() == s["spikes"]
+    be.destroy()
+
+
+def test_batch_graph_replays_and_autotune_restore(gpu):
+    """Autotuning leaves the state as it was; repeated graph replays keep
+    matching the oracle."""
+    members = make_members(9, 3, 40)
+    mem = [load(b, dop, inp) for b, dop, inp in members]
+    be = BatchEngine(mem)
+    before = [m["vram"].download_state() for m in mem]
+    be.autotune(3)
+    after = [m["vram"].download_state() for m in mem]
+    for x, y in zip(before, after):
+        assert np.array_equal(x, y)
+    for k in range(4):
+        be.step(k * 10, 10)
+    day_lib().gpu_device_synchronize()
+    for (b, dop, inp), m in zip(members, mem):
+        st, hd = b.state.copy(), b.heads.copy()
+        O.step_batch(st, b.padded_n, hd, b.total_axons, b.lut, 1, 0, 40, inp, b.virtual_offset, b.n_inputs,
+                     dopamine=dop)
+        assert np.array_equal(m["vram"].download_state(), st)
+        assert np.array_equal(m["vram"].download_axons(), hd)
+    be.destroy()
