@@ -53,7 +53,8 @@ class ShardVramPtrs(C.Structure):
 
 class AxcShardStats(C.Structure):
     _fields_ = [(n, C.c_uint64) for n in (
-        "ticks", "integrated", "slots_scanned", "targets_read", "hits", "spikes", "gsop_slots", "gsop_targets")]
+        "ticks", "integrated", "slots_scanned", "targets_read", "hits", "spikes", "gsop_slots", "gsop_targets",
+        "gsop_slots_fused", "gsop_targets_fused")]
 
 
 class AxcBrainDesc(C.Structure):
@@ -172,7 +173,7 @@ class GenSpec(C.Structure):
 def gen_lib() -> C.CDLL:
     global _gen
     if _gen is None:
-        lib = _load(_build.GEN_LIB, "libaxicor_gen.so")
+        lib = _load(_build.GEN_LIB, "libaxicor_host.so")
         lib.axc_gen_sizes.argtypes = [C.POINTER(GenSpec), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
         lib.axc_gen_sizes.restype = None
         lib.axc_gen_brain.argtypes = [C.POINTER(GenSpec), _P, _P, _P, _P, _P]
@@ -181,6 +182,12 @@ def gen_lib() -> C.CDLL:
         lib.axc_gen_inputs.restype = None
         lib.axc_gen_entity_seed.argtypes = [C.c_uint64, C.c_uint32]
         lib.axc_gen_entity_seed.restype = C.c_uint64
+        lib.axc_gen_paths.argtypes = [C.POINTER(GenSpec), _P, _P, _P, _P, _P]
+        lib.axc_gen_paths.restype = C.c_int
+        lib.axc_sprout_pass.argtypes = [_P, _P, _P, _P, _P, _U32, _U32, _P, _P, _P, _U32, _U32, _U32, _U32, _U32,
+                                        _P, _U32, C.c_uint64, _P, _P, _P, C.c_uint64, _U32, C.c_uint16, _P, _P,
+                                        _U32, _P]
+        lib.axc_sprout_pass.restype = C.c_int
         _gen = lib
     return _gen
 

@@ -162,6 +162,9 @@ class Brain:
     lut: np.ndarray       # VARIANT_DTYPE[16]
     axon_len: np.ndarray  # uint8[total_axons]
     soma_pos: np.ndarray  # uint32[padded_n]
+    paths: np.ndarray | None = None  # uint32[total_axons, 256] (Night-Phase geometry)
+    tips: np.ndarray | None = None   # uint32[total_axons]
+    dirs: np.ndarray | None = None   # uint32[total_axons]
 
     @property
     def virtual_offset(self) -> int:
@@ -180,8 +183,10 @@ class Brain:
         return StateView(self.state, self.padded_n)
 
     def copy(self) -> "Brain":
+        cp = lambda a: None if a is None else a.copy()  # noqa: E731
         return Brain(self.spec, self.padded_n, self.total_axons, self.state.copy(), self.heads.copy(),
-                     self.lut.copy(), self.axon_len.copy(), self.soma_pos.copy())
+                     self.lut.copy(), self.axon_len.copy(), self.soma_pos.copy(), cp(self.paths), cp(self.tips),
+                     cp(self.dirs))
 
     def inputs(self, tick0: int, ticks: int) -> np.ndarray:
         """uint32[ticks, words] Bernoulli input bitmasks for ticks tick0.."""
@@ -195,7 +200,7 @@ def sizes(spec: BrainSpec) -> tuple[int, int]:
     return pn.value, a.value
 
 
-def generate(spec: BrainSpec) -> Brain:
+def generate(spec: BrainSpec, with_paths: bool = False) -> Brain:
     g = spec.to_c()
     pn, a = sizes(spec)
     _, nbytes = calculate_state_blob_size(pn)
@@ -208,7 +213,13 @@ def generate(spec: BrainSpec) -> Brain:
     rc = gen_lib().axc_gen_brain(C.byref(g), ptr(lut), ptr(state), ptr(heads), ptr(alen), ptr(pos))
     if rc != 0:
         raise ValueError(f"bad brain spec: {spec}")
-    return Brain(spec, pn, a, state, heads, lut, alen, pos)
+    b = Brain(spec, pn, a, state, heads, lut, alen, pos)
+    if with_paths:
+        b.paths = np.empty((a, 256), dtype=np.uint32)
+        b.tips = np.empty(a, dtype=np.uint32)
+        b.dirs = np.empty(a, dtype=np.uint32)
+        gen_lib().axc_gen_paths(C.byref(g), ptr(pos), ptr(alen), ptr(b.paths), ptr(b.tips), ptr(b.dirs))
+    return b
 
 
 def gen_inputs(seed: int, n_inputs: int, ppm: int, tick0: int, ticks: int) -> np.ndarray:

@@ -2,8 +2,9 @@
 
 * ``axicor_amd/lib/libaxicor_day.so`` — the Day-Phase engine and its C-ABI
   (``include/axicor_day.h``), HIP for gfx950 only.
-* ``axicor_amd/lib/libaxicor_gen.so`` — host C++: the seeded synthetic brain
-  generator (``.state``/``.axons`` blobs, input bitmasks).
+* ``axicor_amd/lib/libaxicor_host.so`` — host C++: the seeded synthetic brain
+  generator (``.state``/``.axons`` blobs, axon paths, input bitmasks) and the
+  Night-Phase sprouting pass.
 
 Run ``python -m axicor_amd.build`` (or ``__graft_entry__.build()``). Every
 call compiles; the loaders in :mod:`axicor_amd._ffi` only load what is there.
@@ -21,7 +22,7 @@ CSRC = os.path.join(ROOT, "csrc")
 LIBDIR = os.path.join(ROOT, "lib")
 
 DAY_LIB = os.path.join(LIBDIR, "libaxicor_day.so")
-GEN_LIB = os.path.join(LIBDIR, "libaxicor_gen.so")
+GEN_LIB = os.path.join(LIBDIR, "libaxicor_host.so")
 
 # gfx950 only. Packed-FP32 instructions are switched off for every HIP file.
 HIPCC_FLAGS = [
@@ -56,8 +57,10 @@ def build_day_lib() -> str:
 def build_gen_lib() -> str:
     os.makedirs(LIBDIR, exist_ok=True)
     tmp = GEN_LIB + ".tmp"
-    _run(["g++", "-O3", "-std=c++17", "-fPIC", "-shared", "-fopenmp", "-Wall",
-          "-o", tmp, os.path.join(CSRC, "brain_gen.cpp")])
+    # -ffp-contract=off: the sprouting scores are f32 sums evaluated as the
+    # reference writes them, without fused multiply-adds.
+    _run(["g++", "-O3", "-std=c++17", "-fPIC", "-shared", "-fopenmp", "-ffp-contract=off", "-Wall",
+          "-o", tmp, os.path.join(CSRC, "brain_gen.cpp"), os.path.join(CSRC, "night_host.cpp")])
     os.replace(tmp, GEN_LIB)
     return GEN_LIB
 

@@ -25,6 +25,7 @@
 #include <cstdlib>
 #include <vector>
 #include <cmath>
+#include <algorithm>
 
 namespace {
 
@@ -251,6 +252,70 @@ int axc_gen_brain(const GenSpec* spec, const GenVariant* lut, uint8_t* state, ui
     if (len[a]) h[0] = 0x80000000u - (uint32_t)len[a] * spec->v_seg;
   }
   if (axon_len) std::memcpy(axon_len, len.data(), A);
+  return 0;
+}
+
+// Axon geometry for the Night Phase: paths [A][256] (PackedPosition of each
+// segment, types.rs:16-46), tips [A] (last segment) and dirs [A] (packed i8
+// growth direction, sprouting.rs:72-75). Axon a < n starts at its soma's
+// voxel, input axon k at the voxel of rank floor(k*V/n_in); each segment moves
+// one voxel along every axis whose direction component exceeds 64 in
+// magnitude, plus a one-voxel seeded jitter on one axis, clamped to the grid.
+// Requires the soma_pos and axon_len of axc_gen_brain.
+int axc_gen_paths(const GenSpec* spec, const uint32_t* soma_pos, const uint8_t* axon_len, uint32_t* paths,
+                  uint32_t* tips, uint32_t* dirs) {
+  uint32_t pn, A;
+  axc_gen_sizes(spec, &pn, &A);
+  const uint32_t n = spec->n_neurons, nin = spec->n_inputs;
+  const uint64_t V = (uint64_t)spec->gx * spec->gy * spec->gz;
+  const VoxelOrder vo{spec->gx, spec->gy, spec->gz, spec->brick};
+  const uint64_t ms = spec->master_seed ^ 0x7061746800000006ull;
+#pragma omp parallel for schedule(static)
+  for (int64_t ai = 0; ai < (int64_t)A; ai++) {
+    const uint32_t a = (uint32_t)ai;
+    uint32_t* row = paths + (size_t)a * 256;
+    std::memset(row, 0, 256 * sizeof(uint32_t));
+    tips[a] = 0;
+    dirs[a] = 0;
+    const uint32_t L = axon_len[a];
+    if (L == 0) continue;
+    int32_t x, y, z;
+    uint32_t type;
+    if (a < n) {
+      const uint32_t p = soma_pos[a];
+      x = (int32_t)(p & 0x3FF);
+      y = (int32_t)((p >> 10) & 0x3FF);
+      z = (int32_t)((p >> 20) & 0xFF);
+      type = p >> 28;
+    } else {
+      const uint32_t k = a - pn;
+      uint32_t ux, uy, uz;
+      vo.unrank((uint64_t)k * V / (nin ? nin : 1), &ux, &uy, &uz);
+      x = (int32_t)ux;
+      y = (int32_t)uy;
+      z = (int32_t)uz;
+      type = 0;
+    }
+    const uint64_t h = entity_seed(ms, a);
+    const int8_t d[3] = {(int8_t)(uniform(h, 255) - 127), (int8_t)(uniform(entity_seed(h, 1), 255) - 127),
+                         (int8_t)(uniform(entity_seed(h, 2), 255) - 127)};
+    int32_t c[3] = {x, y, z};
+    const int32_t lim[3] = {(int32_t)spec->gx - 1, (int32_t)spec->gy - 1, std::min<int32_t>((int32_t)spec->gz - 1, 255)};
+    for (uint32_t s = 0; s < L; s++) {
+      row[s] = (uint32_t)(c[0] & 0x3FF) | ((uint32_t)(c[1] & 0x3FF) << 10) | ((uint32_t)(c[2] & 0xFF) << 20) |
+               ((type & 0xF) << 28);
+      const uint64_t j = entity_seed(h, 16 + s);
+      const uint32_t axis = uniform(j, 3);
+      const int32_t jit = (int32_t)uniform(entity_seed(j, 1), 3) - 1;
+      for (int k = 0; k < 3; k++) {
+        int32_t step = d[k] > 64 ? 1 : (d[k] < -64 ? -1 : 0);
+        if ((uint32_t)k == axis) step += jit;
+        c[k] = std::min(std::max(c[k] + step, 0), lim[k]);
+      }
+    }
+    tips[a] = row[L - 1];
+    dirs[a] = ((uint32_t)(uint8_t)d[2] << 16) | ((uint32_t)(uint8_t)d[1] << 8) | (uint32_t)(uint8_t)d[0];
+  }
   return 0;
 }
 

@@ -692,6 +692,10 @@ int32_t axc_shard_stats_read(const ShardVramPtrs* vram, AxcShardStats* out) {
     }
   };
   add(a, s->stats_blocks);
+  for (size_t k = 0; k < s->stats_blocks; k++) {
+    r.gsop_slots_fused += a[k * kStatsPerBlock + kStGsopSlots];
+    r.gsop_targets_fused += a[k * kStatsPerBlock + kStGsopTargets];
+  }
   add(b, kListBlocks);
   *out = r;
   return 0;
@@ -731,6 +735,22 @@ int32_t axc_step_day_phase_batch(const ShardVramPtrs* vram, uint32_t padded_n, u
 
   // The per-batch burst reset of shard.rs:71 (idempotent with the per-tick one).
   reset_burst_kernel<<<blocks_for(padded_n ? padded_n : 1), kThreads, 0, st>>>(vram->soma_flags, padded_n);
+  bool timed;
+  {
+    std::lock_guard<std::mutex> lk(g_tmu);
+    timed = g_timing.enabled;
+  }
+  if (timed) {
+    // Kernel by kernel, so that each launch can be timed: the same kernels
+    // and tick structure as the graph below, arguments passed directly.
+    for (uint32_t t = 0; t < T && e == hipSuccess; t++) {
+      e = enqueue_tick(*vram, s, v_seg, tick_base + t, d_bitmask ? d_bitmask + (size_t)t * words : nullptr, words,
+                       virt_off, n_virt, d_spikes ? d_spikes + (size_t)t * max_spk : nullptr, max_spk,
+                       key.spike_counts[t], d_mapped, d_out ? d_out + (size_t)t * n_out : nullptr, n_out,
+                       (int32_t)dopamine, nullptr, st, lut, t, t > 0, t + 1 == T);
+    }
+    return (int32_t)e;
+  }
   TickParams pv;
   pv.tick_base = tick_base;
   pv.dopamine = (int32_t)dopamine;
@@ -1137,6 +1157,8 @@ int32_t axc_batch_stats(void* h, uint32_t brain, AxcShardStats* out, uint64_t* t
   out->gsop_slots = v[kStGsopSlots];
   out->gsop_targets = v[kStGsopTargets];
   out->ticks = v[kStTicks];
+  out->gsop_slots_fused = 0;
+  out->gsop_targets_fused = 0;
   return 0;
 }
 

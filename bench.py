@@ -162,9 +162,10 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--cpu-max-ticks", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--timing-ticks", type=int, default=10, help="per-tick launches timed with HIP events")
+    ap.add_argument("--timing-ticks", type=int, default=10,
+                    help="ticks per batch of the two batches whose kernels are timed one by one with HIP events")
     ap.add_argument("--batched-brains", type=int, default=128, help="0 skips the batched-brains leg")
-    ap.add_argument("--batched-ticks", type=int, default=2000)
+    ap.add_argument("--batched-ticks", type=int, default=5000)
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="also run one device-to-device copy of this size (reference for HBM byte counters)")
     args = ap.parse_args()
@@ -283,15 +284,15 @@ def main() -> int:
     tick_frac = ab["total"] / elapsed / (HBM_PEAK_GBS * 1e9)
 
     # ---- per-kernel durations (HIP events on the engine's stream) ----------
+    # The same batches, launched kernel by kernel so each launch is timed.
     lib.axc_kernel_timing_read(None, None, 1)
     lib.axc_kernel_timing_enable(1)
     eng.stats_reset()
-    for k in range(args.timing_ticks):
-        err = lib.cu_step_day_phase(ptrs, N, A, brain.spec.v_seg, tick, d_in.ptr + (tick % n_in_ticks) * words * 4,
-                                    brain.virtual_offset, brain.n_inputs, None, 0, None, None, 0, 0, eng.stream)
-        if err != 0:
-            raise RuntimeError(f"cu_step_day_phase failed: {err}")
-        tick += 1
+    tb = max(2, args.timing_ticks)
+    for k in range(2):
+        if tick + tb > n_in_ticks:
+            tick = 0  # reuse resident inputs from the start; the state just keeps running
+        batch(tb)
     eng.synchronize()
     lib.axc_kernel_timing_enable(0)
     kms = (C.c_double * 6)()
@@ -299,18 +300,21 @@ def main() -> int:
     lib.axc_kernel_timing_read(kms, kn, 1)
     st_t = eng.stats()
     ab_t = algorithmic_bytes(st_t, N, A, words)
+    fused = 4 * st_t["gsop_targets_fused"] + 4 * st_t["gsop_slots_fused"]
     per_kernel = {TIMED_KERNELS[k]: {"avg_us": kms[k] * 1e3 / kn[k], "launches": int(kn[k])}
                   for k in range(6) if kn[k]}
     nt_us = per_kernel["neuron_tick"]["avg_us"]
-    nt_bytes = ab_t["neuron_tick"] / st_t["ticks"]
+    nt_bytes = (ab_t["neuron_tick"] + fused) / per_kernel["neuron_tick"]["launches"]
     achieved = nt_bytes / (nt_us * 1e-6) / 1e9
-    for k in ("axon_tick", "gsop"):
-        if k in per_kernel:
-            b = ab_t[k] / st_t["ticks"]
-            per_kernel[k]["algorithmic_bytes"] = b
-            per_kernel[k]["achieved_GBs"] = b / (per_kernel[k]["avg_us"] * 1e-6) / 1e9
     per_kernel["neuron_tick"]["algorithmic_bytes"] = nt_bytes
     per_kernel["neuron_tick"]["achieved_GBs"] = achieved
+    per_kernel["axon_tick"]["algorithmic_bytes"] = ab_t["axon_tick"] / per_kernel["axon_tick"]["launches"]
+    per_kernel["axon_tick"]["achieved_GBs"] = (per_kernel["axon_tick"]["algorithmic_bytes"] /
+                                               (per_kernel["axon_tick"]["avg_us"] * 1e-6) / 1e9)
+    if "gsop" in per_kernel:
+        g = (ab_t["gsop"] - fused) / per_kernel["gsop"]["launches"]
+        per_kernel["gsop"]["algorithmic_bytes"] = g
+        per_kernel["gsop"]["achieved_GBs"] = g / (per_kernel["gsop"]["avg_us"] * 1e-6) / 1e9
     log("per-kernel: " + json.dumps(per_kernel))
 
     traffic = None
@@ -342,8 +346,8 @@ def main() -> int:
             "padded_n": N, "total_axons": A, "ticks_per_step": T, "parallelism": "single GPU",
         },
         "roofline": {
-            "bound": "hbm", "kernel": "neuron_tick", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+            "bound": "hbm", "kernel": "neuron_tick (previous tick's GSOP fused)", "achieved": achieved,
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
         },
         "ticks_per_s": ticks_timed / elapsed,
         "neuron_ticks_per_s": N * ticks_timed / elapsed,

@@ -267,6 +267,8 @@ typedef struct AxcShardStats {
     uint64_t spikes;
     uint64_t gsop_slots;
     uint64_t gsop_targets;
+    uint64_t gsop_slots_fused;    /* the part of gsop_slots applied inside neuron_tick */
+    uint64_t gsop_targets_fused;  /* the part of gsop_targets applied inside neuron_tick */
 } AxcShardStats;
 
 int32_t axc_shard_stats_reset(const ShardVramPtrs *vram, void *stream);
@@ -295,12 +297,13 @@ int32_t axc_step_day_phase_batch(const ShardVramPtrs *vram, uint32_t padded_n, u
 void axc_host_sort_and_prune(uint8_t *flags, uint32_t *targets, int16_t *weights, uint8_t *timers,
                              uint32_t padded_n, int16_t prune_threshold);
 
-/* Per-kernel timing of the per-tick path (cu_step_day_phase), for
- * benchmarks: when enabled, HIP events are recorded on the caller's stream
- * around each launch. axc_kernel_timing_read waits for them and returns the
- * summed milliseconds and launch counts of 6 kernels in this order:
- * axon_tick, neuron_tick, push_apply, gsop, readout, spike_count.
- * Graph-replayed batches are never timed. */
+/* Per-kernel timing, for benchmarks: when enabled, HIP events are recorded
+ * on the caller's stream around each launch of cu_step_day_phase, and
+ * axc_step_day_phase_batch launches its kernels one by one instead of
+ * replaying its graph (same kernels, same results) so that they can be timed.
+ * axc_kernel_timing_read waits for the events and returns the summed
+ * milliseconds and launch counts of 6 kernels in this order:
+ * axon_tick, neuron_tick, push_apply, gsop, readout, spike_count. */
 void axc_kernel_timing_enable(int enable);
 int32_t axc_kernel_timing_read(double *ms_out, uint64_t *launches_out, int reset);
 
