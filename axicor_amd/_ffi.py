@@ -134,8 +134,23 @@ SIGNATURES = {
 
 TIMED_KERNELS = ["axon_tick", "neuron_tick", "push_apply", "gsop", "readout", "spike_count"]
 
+SURROGATE_SIGNATURES = {
+    "axc_mlp_create": (_I32, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint64, C.POINTER(C.c_void_p)]),
+    "axc_mlp_forward": (_I32, [_P, _P, C.c_int64, _P, _P]),
+    "axc_mlp_train_step": (_I32, [_P, _P, _P, C.c_int, C.c_float, C.POINTER(C.c_float), _P]),
+    "axc_mlp_gradients": (_I32, [_P, _P, _P, C.c_int, _P, C.POINTER(C.c_float), _P]),
+    "axc_mlp_layout": (_I32, [_P, C.POINTER(C.c_int32), _P, _P, C.POINTER(C.c_int64)]),
+    "axc_mlp_get_params": (_I32, [_P, _P]),
+    "axc_mlp_set_params": (_I32, [_P, _P]),
+    "axc_mlp_reset_optimizer": (_I32, [_P]),
+    "axc_mlp_gemm": (_I32, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _P, C.c_int, _P, C.c_int, _P, C.c_int,
+                            _P]),
+    "axc_mlp_destroy": (None, [_P]),
+}
+
 _day = None
 _gen = None
+_sur = None
 
 
 def _load(path: str, what: str) -> C.CDLL:
@@ -156,6 +171,19 @@ def day_lib() -> C.CDLL:
                 fn.argtypes = args
         _day = lib
     return _day
+
+
+def surrogate_lib() -> C.CDLL:
+    """The surrogate library (include/axicor_surrogate.h), argtypes set."""
+    global _sur
+    if _sur is None:
+        lib = _load(_build.SUR_LIB, "libaxicor_surrogate.so")
+        for name, (res, args) in SURROGATE_SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _sur = lib
+    return _sur
 
 
 class GenSpec(C.Structure):

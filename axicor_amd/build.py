@@ -5,6 +5,8 @@
 * ``axicor_amd/lib/libaxicor_host.so`` — host C++: the seeded synthetic brain
   generator (``.state``/``.axons`` blobs, axon paths, input bitmasks) and the
   Night-Phase sprouting pass.
+* ``axicor_amd/lib/libaxicor_surrogate.so`` — the parameter-search surrogate
+  (``include/axicor_surrogate.h``): MFMA BF16 GEMMs, GELU, Adam; HIP for gfx950.
 
 Run ``python -m axicor_amd.build`` (or ``__graft_entry__.build()``). Every
 call compiles; the loaders in :mod:`axicor_amd._ffi` only load what is there.
@@ -23,6 +25,7 @@ LIBDIR = os.path.join(ROOT, "lib")
 
 DAY_LIB = os.path.join(LIBDIR, "libaxicor_day.so")
 GEN_LIB = os.path.join(LIBDIR, "libaxicor_host.so")
+SUR_LIB = os.path.join(LIBDIR, "libaxicor_surrogate.so")
 
 # gfx950 only. Packed-FP32 instructions are switched off for every HIP file.
 HIPCC_FLAGS = [
@@ -65,8 +68,16 @@ def build_gen_lib() -> str:
     return GEN_LIB
 
 
+def build_surrogate_lib() -> str:
+    os.makedirs(LIBDIR, exist_ok=True)
+    tmp = SUR_LIB + ".tmp"
+    _run([_hipcc(), *HIPCC_FLAGS, "-o", tmp, os.path.join(CSRC, "surrogate.hip")])
+    os.replace(tmp, SUR_LIB)
+    return SUR_LIB
+
+
 def build_all() -> list[str]:
-    return [build_day_lib(), build_gen_lib()]
+    return [build_day_lib(), build_gen_lib(), build_surrogate_lib()]
 
 
 if __name__ == "__main__":

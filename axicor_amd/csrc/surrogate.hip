@@ -1,0 +1,560 @@
+// surrogate.hip — the parameter-search surrogate: an MLP
+// 16 -> 512 -> 512 -> 512 -> 8 with GELU, BF16 weights and activations, FP32
+// accumulation, trained with Adam, all on hand-written gfx950 kernels. It has
+// no counterpart in the reference, whose search is Optuna TPE driving a live
+// brain (examples/bayesian_search.py:63-187); the C-ABI is include/axicor_surrogate.h.
+//
+// GEMMs run on v_mfma_f32_16x16x32_bf16. A workgroup of 4 waves computes a
+// 64x64 tile of C, each wave a 32x32 quarter as 2x2 MFMA tiles; K advances in
+// steps of 32 staged through LDS with 16-byte global loads. One kernel serves
+// the forward product (X W), the input gradient (dZ W^T) and the weight
+// gradient (X^T dZ) through the transpose flags, with the elementwise step of
+// each (bias, GELU, GELU') fused into its epilogue.
+//
+// MFMA 16x16x32 operand layout, lane l of 64: the A fragment is 8 bf16 of row
+// l%16, columns 8*(l/16)..+7; the B fragment is 8 bf16 of column l%16, rows
+// 8*(l/16)..+7; accumulator r of lane l is row 4*(l/16)+r, column l%16.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <math.h>
+#include <cmath>
+#include <algorithm>
+#include <vector>
+
+namespace axcmlp {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int BM = 64, BN = 64, BK = 32;
+constexpr int LDSK = BK + 8;      // row pitch of the LDS tiles in bf16 (80 B): spreads rows over banks
+constexpr int kColsumChunks = 32; // row chunks of the two-pass column sum
+
+enum Epi { EPI_BIAS_GELU = 0, EPI_BIAS = 1, EPI_GELU_GRAD = 2, EPI_STORE = 3, EPI_BIAS_GELU_NOZ = 4 };
+
+__device__ __forceinline__ float gelu_f(float z) {
+  // GELU, tanh form: 0.5 z (1 + tanh(sqrt(2/pi) (z + 0.044715 z^3)))
+  const float c = 0.7978845608028654f;
+  return 0.5f * z * (1.0f + tanhf(c * (z + 0.044715f * z * z * z)));
+}
+__device__ __forceinline__ float gelu_grad_f(float z) {
+  const float c = 0.7978845608028654f;
+  const float t = tanhf(c * (z + 0.044715f * z * z * z));
+  const float du = c * (1.0f + 3.0f * 0.044715f * z * z);
+  return 0.5f * (1.0f + t) + 0.5f * z * (1.0f - t * t) * du;
+}
+
+// Stages a 64 (rows r of the tile) x 32 (k) block into T[r][k] from a matrix
+// whose element (r, k) lives at P[r*ld + k] (row-major in k: one 16-byte load
+// per thread) or, when KMAJOR, at P[k*ld + r] (one 16-byte load of 8 rows per
+// thread, scattered into LDS). Out-of-range elements read as zero.
+template <bool KMAJOR>
+__device__ __forceinline__ void stage_tile(__bf16 (*T)[LDSK], const __bf16* __restrict__ P, int ld, int r0, int R,
+                                           int k0, int K, int tid) {
+  const __bf16 zero = (__bf16)0.0f;
+  if (!KMAJOR) {
+    const int r = tid >> 2, c = (tid & 3) * 8;
+    const int gr = r0 + r, gk = k0 + c;
+    if (gr < R && gk + 8 <= K) {
+      *reinterpret_cast<uint4*>(&T[r][c]) = *reinterpret_cast<const uint4*>(P + (size_t)gr * ld + gk);
+    } else {
+      for (int j = 0; j < 8; j++) T[r][c + j] = (gr < R && gk + j < K) ? P[(size_t)gr * ld + gk + j] : zero;
+    }
+  } else {
+    const int kk = tid >> 3, rr = (tid & 7) * 8;
+    const int gk = k0 + kk, gr = r0 + rr;
+    if (gk < K && gr + 8 <= R) {
+      const uint4 v = *reinterpret_cast<const uint4*>(P + (size_t)gk * ld + gr);
+      const bf16x8 e = __builtin_bit_cast(bf16x8, v);
+      for (int j = 0; j < 8; j++) T[rr + j][kk] = e[j];
+    } else {
+      for (int j = 0; j < 8; j++) T[rr + j][kk] = (gk < K && gr + j < R) ? P[(size_t)gk * ld + gr + j] : zero;
+    }
+  }
+}
+
+// C[M,N] = op(A)[M,K] op(B)[K,N], then the epilogue. op(A)(m,k) is A[m*lda+k],
+// or A[k*lda+m] when TA; op(B)(k,n) is B[k*ldb+n], or B[n*ldb+k] when TB.
+// Leading dimensions are multiples of 8 elements; M, N, K are arbitrary.
+template <bool TA, bool TB, int EPI>
+__global__ __launch_bounds__(256) void gemm_kernel(int M, int N, int K, const __bf16* __restrict__ A, int lda,
+                                                   const __bf16* __restrict__ B, int ldb,
+                                                   const float* __restrict__ bias, const float* __restrict__ aux,
+                                                   int ldaux, float* __restrict__ C32, __bf16* __restrict__ C16,
+                                                   int ldc) {
+  __shared__ __attribute__((aligned(16))) __bf16 As[BM][LDSK];
+  __shared__ __attribute__((aligned(16))) __bf16 Bs[BN][LDSK];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int wm = w >> 1, wn = w & 1;
+  const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+  f32x4 acc[2][2];
+  for (int i = 0; i < 2; i++)
+    for (int j = 0; j < 2; j++) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+  for (int k0 = 0; k0 < K; k0 += BK) {
+    stage_tile<TA>(As, A, lda, m0, M, k0, K, tid);
+    stage_tile<!TB>(Bs, B, ldb, n0, N, k0, K, tid);
+    __syncthreads();
+    bf16x8 a[2], b[2];
+    for (int i = 0; i < 2; i++)
+      a[i] = *reinterpret_cast<const bf16x8*>(&As[wm * 32 + i * 16 + (lane & 15)][8 * (lane >> 4)]);
+    for (int j = 0; j < 2; j++)
+      b[j] = *reinterpret_cast<const bf16x8*>(&Bs[wn * 32 + j * 16 + (lane & 15)][8 * (lane >> 4)]);
+    for (int i = 0; i < 2; i++)
+      for (int j = 0; j < 2; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+    __syncthreads();
+  }
+
+  for (int i = 0; i < 2; i++) {
+    for (int j = 0; j < 2; j++) {
+      const int col = n0 + wn * 32 + j * 16 + (lane & 15);
+      if (col >= N) continue;
+      for (int r = 0; r < 4; r++) {
+        const int row = m0 + wm * 32 + i * 16 + 4 * (lane >> 4) + r;
+        if (row >= M) continue;
+        float v = acc[i][j][r];
+        const size_t o = (size_t)row * ldc + col;
+        if (EPI == EPI_BIAS_GELU || EPI == EPI_BIAS_GELU_NOZ) {
+          v += bias[col];
+          if (EPI == EPI_BIAS_GELU) C32[o] = v;  // pre-activation, kept for the backward pass
+          C16[o] = (__bf16)gelu_f(v);
+        } else if (EPI == EPI_BIAS) {
+          C32[o] = v + bias[col];
+        } else if (EPI == EPI_GELU_GRAD) {
+          C16[o] = (__bf16)(v * gelu_grad_f(aux[(size_t)row * ldaux + col]));
+        } else {
+          C32[o] = v;
+        }
+      }
+    }
+  }
+}
+
+// Gradient of the mean squared error over the first n_out columns:
+// dY = 2 (Y - T) / (Bn n_out) as bf16, zero in the padding columns; one
+// partial sum of squared errors per workgroup.
+__global__ __launch_bounds__(256) void mse_grad_kernel(const float* __restrict__ Y, int ldy, const float* __restrict__ T,
+                                                       int n_out, int Bn, __bf16* __restrict__ dY, int ldd,
+                                                       float* __restrict__ part) {
+  __shared__ float red[256];
+  const int idx = blockIdx.x * 256 + threadIdx.x;
+  float se = 0.f;
+  if (idx < Bn * ldd) {
+    const int r = idx / ldd, c = idx % ldd;
+    float g = 0.f;
+    if (c < n_out) {
+      const float d = Y[(size_t)r * ldy + c] - T[(size_t)r * n_out + c];
+      se = d * d;
+      g = 2.0f * d / (float)(Bn * n_out);
+    }
+    dY[idx] = (__bf16)g;
+  }
+  red[threadIdx.x] = se;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// Column sums of a bf16 [rows][ld] matrix in two deterministic passes:
+// partial[chunk][c] over row chunks, then db[c] = sum over chunks.
+// Block: 64 columns x 4 row lanes; grid (ceil(cols/64), kColsumChunks).
+__global__ __launch_bounds__(256) void colsum_part_kernel(const __bf16* __restrict__ dZ, int rows, int cols, int ld,
+                                                          float* __restrict__ partial) {
+  __shared__ float red[4][64];
+  const int cl = threadIdx.x & 63, rl = threadIdx.x >> 6;
+  const int c = blockIdx.x * 64 + cl;
+  const int per = (rows + kColsumChunks - 1) / kColsumChunks;
+  const int r0 = blockIdx.y * per, r1 = min(rows, r0 + per);
+  float s = 0.f;
+  if (c < cols)
+    for (int r = r0 + rl; r < r1; r += 4) s += (float)dZ[(size_t)r * ld + c];
+  red[rl][cl] = s;
+  __syncthreads();
+  if (rl == 0 && c < cols) partial[(size_t)blockIdx.y * cols + c] = red[0][cl] + red[1][cl] + red[2][cl] + red[3][cl];
+}
+
+__global__ __launch_bounds__(256) void colsum_final_kernel(const float* __restrict__ partial, int cols,
+                                                           float* __restrict__ db) {
+  const int c = blockIdx.x * 256 + threadIdx.x;
+  if (c >= cols) return;
+  float s = 0.f;
+  for (int k = 0; k < kColsumChunks; k++) s += partial[(size_t)k * cols + c];
+  db[c] = s;
+}
+
+// Rows of an fp32 [rows][ldx] matrix to bf16 [rows][ldy], zero past `cols`.
+__global__ __launch_bounds__(256) void cast_rows_kernel(const float* __restrict__ X, int rows, int cols, int ldx,
+                                                        __bf16* __restrict__ Y, int ldy) {
+  const int idx = blockIdx.x * 256 + threadIdx.x;
+  if (idx >= rows * ldy) return;
+  const int r = idx / ldy, c = idx % ldy;
+  Y[idx] = (__bf16)(c < cols ? X[(size_t)r * ldx + c] : 0.0f);
+}
+
+// Adam on the fp32 master parameters (bias-corrected, c1 = 1 - b1^t,
+// c2 = 1 - b2^t); refreshes the bf16 copy the GEMMs read. Entries with
+// mask 0 (padding) are left untouched.
+__global__ __launch_bounds__(256) void adam_kernel(float* __restrict__ p, const float* __restrict__ g,
+                                                   float* __restrict__ m, float* __restrict__ v, __bf16* __restrict__ p16,
+                                                   int n, float lr, float b1, float b2, float eps, float c1, float c2,
+                                                   const uint8_t* __restrict__ mask) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n || !mask[i]) return;
+  const float gi = g[i];
+  const float mi = b1 * m[i] + (1.f - b1) * gi;
+  const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  const float pi = p[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+  p[i] = pi;
+  if (p16) p16[i] = (__bf16)pi;
+}
+
+__global__ __launch_bounds__(256) void to_bf16_kernel(const float* __restrict__ x, __bf16* __restrict__ y, int n) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) y[i] = (__bf16)x[i];
+}
+
+}  // namespace axcmlp
+
+using namespace axcmlp;
+
+namespace {
+
+struct Layer {
+  int in, out;             // padded sizes (in: multiple of 32, out: multiple of 16)
+  float *W, *b;            // fp32 master parameters, W[in][out] row-major
+  __bf16* W16;             // bf16 copy read by the GEMMs
+  float *gW, *gb, *mW, *vW, *mb, *vb;
+  uint8_t *maskW, *maskb;  // 1 on real entries, 0 on padding
+};
+
+struct Mlp {
+  int d_in, d_out, hidden, depth;  // real sizes
+  int max_batch;
+  std::vector<Layer> L;
+  std::vector<float*> Z;      // [max_batch][out_l] fp32: pre-activations (hidden), outputs (last)
+  std::vector<__bf16*> H;     // [max_batch][in_l] bf16: H[0] = input, H[l+1] = GELU(Z[l])
+  std::vector<__bf16*> dZ;    // [max_batch][out_l] bf16: gradient wrt Z[l]
+  float* part = nullptr;      // loss partial sums
+  float* cpart = nullptr;     // column-sum partials [kColsumChunks][max out]
+  int step = 0;
+  int max_out = 0;
+};
+
+inline int pad(int x, int m) { return (x + m - 1) / m * m; }
+inline unsigned grid1(size_t n) { return (unsigned)((n + 255) / 256); }
+
+template <bool TA, bool TB, int EPI>
+void gemm(hipStream_t st, int M, int N, int K, const __bf16* A, int lda, const __bf16* B, int ldb, const float* bias,
+          const float* aux, int ldaux, float* C32, __bf16* C16, int ldc) {
+  dim3 grid((N + BN - 1) / BN, (M + BM - 1) / BM);
+  gemm_kernel<TA, TB, EPI><<<grid, 256, 0, st>>>(M, N, K, A, lda, B, ldb, bias, aux, ldaux, C32, C16, ldc);
+}
+
+void forward(Mlp* m, int Bn, hipStream_t st, bool keep_z) {
+  const int nl = (int)m->L.size();
+  for (int l = 0; l < nl; l++) {
+    Layer& y = m->L[l];
+    if (l + 1 < nl) {
+      if (keep_z)
+        gemm<false, false, EPI_BIAS_GELU>(st, Bn, y.out, y.in, m->H[l], y.in, y.W16, y.out, y.b, nullptr, 0, m->Z[l],
+                                          m->H[l + 1], y.out);
+      else
+        gemm<false, false, EPI_BIAS_GELU_NOZ>(st, Bn, y.out, y.in, m->H[l], y.in, y.W16, y.out, y.b, nullptr, 0,
+                                              nullptr, m->H[l + 1], y.out);
+    } else {
+      gemm<false, false, EPI_BIAS>(st, Bn, y.out, y.in, m->H[l], y.in, y.W16, y.out, y.b, nullptr, 0, m->Z[l],
+                                   nullptr, y.out);
+    }
+  }
+}
+
+// Gradients of the batch's mean squared error wrt every parameter into gW/gb.
+void backward(Mlp* m, const float* d_t, int Bn, hipStream_t st) {
+  const int nl = (int)m->L.size();
+  Layer& last = m->L[nl - 1];
+  mse_grad_kernel<<<grid1((size_t)Bn * last.out), 256, 0, st>>>(m->Z[nl - 1], last.out, d_t, m->d_out, Bn,
+                                                                  m->dZ[nl - 1], last.out, m->part);
+  for (int l = nl - 1; l >= 0; l--) {
+    Layer& y = m->L[l];
+    // dW_l = H_l^T dZ_l (in x out); db_l = column sums of dZ_l.
+    gemm<true, false, EPI_STORE>(st, y.in, y.out, Bn, m->H[l], y.in, m->dZ[l], y.out, nullptr, nullptr, 0, y.gW,
+                                 nullptr, y.out);
+    colsum_part_kernel<<<dim3((y.out + 63) / 64, kColsumChunks), 256, 0, st>>>(m->dZ[l], Bn, y.out, y.out, m->cpart);
+    colsum_final_kernel<<<grid1(y.out), 256, 0, st>>>(m->cpart, y.out, y.gb);
+    if (l > 0) {
+      // dZ_{l-1} = (dZ_l W_l^T) * GELU'(Z_{l-1})
+      Layer& p = m->L[l - 1];
+      gemm<false, true, EPI_GELU_GRAD>(st, Bn, y.in, y.out, m->dZ[l], y.out, y.W16, y.out, nullptr, m->Z[l - 1],
+                                       p.out, nullptr, m->dZ[l - 1], p.out);
+    }
+  }
+}
+
+void load_input(Mlp* m, const float* d_x, int b, hipStream_t st) {
+  cast_rows_kernel<<<grid1((size_t)b * m->L[0].in), 256, 0, st>>>(d_x, b, m->d_in, m->d_in, m->H[0], m->L[0].in);
+}
+
+uint64_t splitmix(uint64_t& s) {
+  uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+void release(Mlp* m) {
+  for (Layer& y : m->L)
+    for (void* p : {(void*)y.W, (void*)y.b, (void*)y.W16, (void*)y.gW, (void*)y.gb, (void*)y.mW, (void*)y.vW,
+                    (void*)y.mb, (void*)y.vb, (void*)y.maskW, (void*)y.maskb})
+      if (p) (void)hipFree(p);
+  for (auto p : m->H) if (p) (void)hipFree(p);
+  for (auto p : m->Z) if (p) (void)hipFree(p);
+  for (auto p : m->dZ) if (p) (void)hipFree(p);
+  if (m->part) (void)hipFree(m->part);
+  if (m->cpart) (void)hipFree(m->cpart);
+  delete m;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t axc_mlp_create(int d_in, int hidden, int depth, int d_out, int max_batch, uint64_t seed, void** out) {
+  if (d_in <= 0 || hidden <= 0 || depth < 1 || d_out <= 0 || max_batch <= 0 || !out) return (int32_t)hipErrorInvalidValue;
+  *out = nullptr;
+  Mlp* m = new Mlp();
+  m->d_in = d_in;
+  m->d_out = d_out;
+  m->hidden = hidden;
+  m->depth = depth;
+  m->max_batch = max_batch;
+  std::vector<int> dims{pad(d_in, 32)}, real{d_in};
+  for (int k = 0; k < depth; k++) {
+    dims.push_back(pad(hidden, 32));
+    real.push_back(hidden);
+  }
+  dims.push_back(pad(d_out, 16));
+  real.push_back(d_out);
+  hipError_t e = hipSuccess;
+  uint64_t rs = seed;
+  for (size_t l = 0; l + 1 < dims.size() && e == hipSuccess; l++) {
+    m->L.push_back(Layer{});
+    Layer& y = m->L.back();
+    y.in = dims[l];
+    y.out = dims[l + 1];
+    m->max_out = std::max(m->max_out, y.out);
+    const size_t nw = (size_t)y.in * y.out;
+    std::vector<float> hw(nw, 0.f), hb(y.out, 0.f);
+    std::vector<uint8_t> mw(nw, 0), mb(y.out, 0);
+    // He-uniform: U(-sqrt(6/fan_in), sqrt(6/fan_in)) on the real entries.
+    const double lim = std::sqrt(6.0 / (double)real[l]);
+    for (int i = 0; i < real[l]; i++)
+      for (int j = 0; j < real[l + 1]; j++) {
+        const double u = (double)(splitmix(rs) >> 11) * (1.0 / 9007199254740992.0);
+        hw[(size_t)i * y.out + j] = (float)((2.0 * u - 1.0) * lim);
+        mw[(size_t)i * y.out + j] = 1;
+      }
+    for (int j = 0; j < real[l + 1]; j++) mb[j] = 1;
+    for (float** p : {&y.W, &y.gW, &y.mW, &y.vW})
+      if (e == hipSuccess) e = hipMalloc((void**)p, nw * 4);
+    for (float** p : {&y.b, &y.gb, &y.mb, &y.vb})
+      if (e == hipSuccess) e = hipMalloc((void**)p, (size_t)y.out * 4);
+    if (e == hipSuccess) e = hipMalloc((void**)&y.W16, nw * 2);
+    if (e == hipSuccess) e = hipMalloc((void**)&y.maskW, nw);
+    if (e == hipSuccess) e = hipMalloc((void**)&y.maskb, y.out);
+    if (e == hipSuccess) e = hipMemcpy(y.W, hw.data(), nw * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(y.b, hb.data(), (size_t)y.out * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(y.maskW, mw.data(), nw, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(y.maskb, mb.data(), y.out, hipMemcpyHostToDevice);
+    for (float* p : {y.gW, y.mW, y.vW})
+      if (e == hipSuccess) e = hipMemset(p, 0, nw * 4);
+    for (float* p : {y.gb, y.mb, y.vb})
+      if (e == hipSuccess) e = hipMemset(p, 0, (size_t)y.out * 4);
+    if (e == hipSuccess) {
+      to_bf16_kernel<<<grid1(nw), 256>>>(y.W, y.W16, (int)nw);
+      e = hipGetLastError();
+    }
+  }
+  const size_t B = (size_t)max_batch;
+  for (size_t l = 0; l < dims.size() && e == hipSuccess; l++) {
+    m->H.push_back(nullptr);
+    e = hipMalloc((void**)&m->H.back(), B * dims[l] * 2);
+  }
+  for (size_t l = 0; l + 1 < dims.size() && e == hipSuccess; l++) {
+    m->Z.push_back(nullptr);
+    m->dZ.push_back(nullptr);
+    e = hipMalloc((void**)&m->Z.back(), B * dims[l + 1] * 4);
+    if (e == hipSuccess) e = hipMalloc((void**)&m->dZ.back(), B * dims[l + 1] * 2);
+  }
+  if (e == hipSuccess) e = hipMalloc((void**)&m->part, (size_t)grid1(B * dims.back()) * 4);
+  if (e == hipSuccess) e = hipMalloc((void**)&m->cpart, (size_t)kColsumChunks * m->max_out * 4);
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e != hipSuccess) {
+    release(m);
+    return (int32_t)e;
+  }
+  *out = m;
+  return 0;
+}
+
+int32_t axc_mlp_forward(void* h, const float* d_x, int64_t n, float* d_y, void* stream) {
+  Mlp* m = (Mlp*)h;
+  if (!m || n < 0 || (n > 0 && (!d_x || !d_y))) return (int32_t)hipErrorInvalidValue;
+  hipStream_t st = (hipStream_t)stream;
+  const int nl = (int)m->L.size();
+  for (int64_t r0 = 0; r0 < n; r0 += m->max_batch) {
+    const int b = (int)std::min<int64_t>(m->max_batch, n - r0);
+    load_input(m, d_x + r0 * m->d_in, b, st);
+    forward(m, b, st, false);
+    hipError_t e = hipMemcpy2DAsync(d_y + r0 * m->d_out, (size_t)m->d_out * 4, m->Z[nl - 1],
+                                    (size_t)m->L[nl - 1].out * 4, (size_t)m->d_out * 4, b, hipMemcpyDeviceToDevice, st);
+    if (e != hipSuccess) return (int32_t)e;
+  }
+  return (int32_t)hipGetLastError();
+}
+
+int32_t axc_mlp_train_step(void* h, const float* d_x, const float* d_t, int n, float lr, float* loss, void* stream) {
+  Mlp* m = (Mlp*)h;
+  if (!m || n <= 0 || n > m->max_batch || !d_x || !d_t || !(lr >= 0.f)) return (int32_t)hipErrorInvalidValue;
+  hipStream_t st = (hipStream_t)stream;
+  load_input(m, d_x, n, st);
+  forward(m, n, st, true);
+  backward(m, d_t, n, st);
+  m->step += 1;
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  const float c1 = 1.0f - powf(b1, (float)m->step), c2 = 1.0f - powf(b2, (float)m->step);
+  for (Layer& y : m->L) {
+    const int nw = y.in * y.out;
+    adam_kernel<<<grid1(nw), 256, 0, st>>>(y.W, y.gW, y.mW, y.vW, y.W16, nw, lr, b1, b2, eps, c1, c2, y.maskW);
+    adam_kernel<<<grid1(y.out), 256, 0, st>>>(y.b, y.gb, y.mb, y.vb, nullptr, y.out, lr, b1, b2, eps, c1, c2, y.maskb);
+  }
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess && loss) {
+    const int nb = (int)grid1((size_t)n * m->L.back().out);
+    std::vector<float> p(nb);
+    e = hipMemcpyAsync(p.data(), m->part, (size_t)nb * 4, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    double s = 0;
+    for (float x : p) s += x;
+    *loss = (float)(s / ((double)n * m->d_out));
+  }
+  return (int32_t)e;
+}
+
+int32_t axc_mlp_gradients(void* h, const float* d_x, const float* d_t, int n, float* host_grads, float* loss,
+                          void* stream) {
+  Mlp* m = (Mlp*)h;
+  if (!m || n <= 0 || n > m->max_batch || !d_x || !d_t || !host_grads) return (int32_t)hipErrorInvalidValue;
+  hipStream_t st = (hipStream_t)stream;
+  load_input(m, d_x, n, st);
+  forward(m, n, st, true);
+  backward(m, d_t, n, st);
+  hipError_t e = hipStreamSynchronize(st);
+  size_t off = 0;
+  for (Layer& y : m->L) {
+    if (e == hipSuccess) e = hipMemcpy(host_grads + off, y.gW, (size_t)y.in * y.out * 4, hipMemcpyDeviceToHost);
+    off += (size_t)y.in * y.out;
+    if (e == hipSuccess) e = hipMemcpy(host_grads + off, y.gb, (size_t)y.out * 4, hipMemcpyDeviceToHost);
+    off += y.out;
+  }
+  if (e == hipSuccess && loss) {
+    const int nb = (int)grid1((size_t)n * m->L.back().out);
+    std::vector<float> p(nb);
+    e = hipMemcpy(p.data(), m->part, (size_t)nb * 4, hipMemcpyDeviceToHost);
+    double s = 0;
+    for (float x : p) s += x;
+    *loss = (float)(s / ((double)n * m->d_out));
+  }
+  return (int32_t)e;
+}
+
+int32_t axc_mlp_layout(void* h, int32_t* n_layers, int32_t* ins, int32_t* outs, int64_t* n_params) {
+  Mlp* m = (Mlp*)h;
+  if (!m) return (int32_t)hipErrorInvalidValue;
+  int64_t n = 0;
+  int k = 0;
+  for (Layer& y : m->L) {
+    if (ins) ins[k] = y.in;
+    if (outs) outs[k] = y.out;
+    n += (int64_t)y.in * y.out + y.out;
+    k++;
+  }
+  if (n_layers) *n_layers = k;
+  if (n_params) *n_params = n;
+  return 0;
+}
+
+int32_t axc_mlp_get_params(void* h, float* host) {
+  Mlp* m = (Mlp*)h;
+  if (!m || !host) return (int32_t)hipErrorInvalidValue;
+  size_t off = 0;
+  hipError_t e = hipDeviceSynchronize();
+  for (Layer& y : m->L) {
+    if (e == hipSuccess) e = hipMemcpy(host + off, y.W, (size_t)y.in * y.out * 4, hipMemcpyDeviceToHost);
+    off += (size_t)y.in * y.out;
+    if (e == hipSuccess) e = hipMemcpy(host + off, y.b, (size_t)y.out * 4, hipMemcpyDeviceToHost);
+    off += y.out;
+  }
+  return (int32_t)e;
+}
+
+int32_t axc_mlp_set_params(void* h, const float* host) {
+  Mlp* m = (Mlp*)h;
+  if (!m || !host) return (int32_t)hipErrorInvalidValue;
+  size_t off = 0;
+  hipError_t e = hipDeviceSynchronize();
+  for (Layer& y : m->L) {
+    const size_t nw = (size_t)y.in * y.out;
+    if (e == hipSuccess) e = hipMemcpy(y.W, host + off, nw * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+      to_bf16_kernel<<<grid1(nw), 256>>>(y.W, y.W16, (int)nw);
+      e = hipGetLastError();
+    }
+    off += nw;
+    if (e == hipSuccess) e = hipMemcpy(y.b, host + off, (size_t)y.out * 4, hipMemcpyHostToDevice);
+    off += y.out;
+  }
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  return (int32_t)e;
+}
+
+int32_t axc_mlp_reset_optimizer(void* h) {
+  Mlp* m = (Mlp*)h;
+  if (!m) return (int32_t)hipErrorInvalidValue;
+  hipError_t e = hipDeviceSynchronize();
+  for (Layer& y : m->L) {
+    for (float* p : {y.mW, y.vW})
+      if (e == hipSuccess) e = hipMemset(p, 0, (size_t)y.in * y.out * 4);
+    for (float* p : {y.mb, y.vb})
+      if (e == hipSuccess) e = hipMemset(p, 0, (size_t)y.out * 4);
+  }
+  m->step = 0;
+  return (int32_t)e;
+}
+
+int32_t axc_mlp_gemm(int M, int N, int K, int ta, int tb, const void* d_a, int lda, const void* d_b, int ldb,
+                     float* d_c, int ldc, void* stream) {
+  if (M <= 0 || N <= 0 || K <= 0 || !d_a || !d_b || !d_c || lda % 8 || ldb % 8 || ldc < N) return (int32_t)hipErrorInvalidValue;
+  if (lda < (ta ? M : K) || ldb < (tb ? K : N)) return (int32_t)hipErrorInvalidValue;
+  hipStream_t st = (hipStream_t)stream;
+  const __bf16* A = (const __bf16*)d_a;
+  const __bf16* B = (const __bf16*)d_b;
+  if (ta && tb) gemm<true, true, EPI_STORE>(st, M, N, K, A, lda, B, ldb, nullptr, nullptr, 0, d_c, nullptr, ldc);
+  else if (ta) gemm<true, false, EPI_STORE>(st, M, N, K, A, lda, B, ldb, nullptr, nullptr, 0, d_c, nullptr, ldc);
+  else if (tb) gemm<false, true, EPI_STORE>(st, M, N, K, A, lda, B, ldb, nullptr, nullptr, 0, d_c, nullptr, ldc);
+  else gemm<false, false, EPI_STORE>(st, M, N, K, A, lda, B, ldb, nullptr, nullptr, 0, d_c, nullptr, ldc);
+  return (int32_t)hipGetLastError();
+}
+
+void axc_mlp_destroy(void* h) {
+  Mlp* m = (Mlp*)h;
+  if (!m) return;
+  (void)hipDeviceSynchronize();
+  release(m);
+}
+
+}  // extern "C"

@@ -85,3 +85,16 @@ def test_host_sort_and_prune_matches_oracle():
     day_lib().axc_host_sort_and_prune(v.flags.ctypes.data, v.targets.ctypes.data, v.weights.ctypes.data,
                                       v.dendrite_timers.ctypes.data, b.padded_n, 20)
     np.testing.assert_array_equal(b.state, ref)
+
+
+def test_surrogate_library_exports_every_declared_symbol():
+    from axicor_amd._ffi import SURROGATE_SIGNATURES, surrogate_lib
+    src = open(os.path.join(build.REPO, "include", "axicor_surrogate.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    names = set(re.findall(r"\b(axc_mlp_[a-z_]+)\s*\(", src))
+    assert names == set(SURROGATE_SIGNATURES), names ^ set(SURROGATE_SIGNATURES)
+    lib = surrogate_lib()
+    out = subprocess.run(["nm", "-D", "--defined-only", build.SUR_LIB], capture_output=True, text=True).stdout
+    exported = set(line.split()[-1] for line in out.splitlines() if line.strip())
+    for name in names:
+        assert hasattr(lib, name) and name in exported, name
