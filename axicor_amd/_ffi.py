@@ -69,6 +69,11 @@ class AxcBatchTuning(C.Structure):
     _fields_ = [("axon_blocks", C.c_uint32), ("soma_blocks", C.c_uint32), ("list_blocks", C.c_uint32)]
 
 
+class AxcSynapseStats(C.Structure):
+    _fields_ = [("filled", C.c_uint64), ("sum_abs_weight", C.c_uint64), ("sum_sq_weight", C.c_uint64),
+                ("below_prune", C.c_uint64), ("sum_threshold_offset", C.c_int64)]
+
+
 _P = C.c_void_p
 _U32 = C.c_uint32
 _I32 = C.c_int32
@@ -128,6 +133,7 @@ SIGNATURES = {
     "axc_batch_set_tuning": (_I32, [_P, C.POINTER(AxcBatchTuning)]),
     "axc_batch_get_tuning": (_I32, [_P, C.POINTER(AxcBatchTuning)]),
     "axc_batch_stats": (_I32, [_P, _U32, C.POINTER(AxcShardStats), _P]),
+    "axc_batch_synapse_stats": (_I32, [_P, _P, _P, _P]),
     "axc_batch_destroy": (None, [_P]),
     "axc_version": (C.c_char_p, []),
 }

@@ -7,7 +7,7 @@ import ctypes as C
 
 import numpy as np
 
-from ._ffi import AxcBatchTuning, AxcBrainDesc, AxcShardStats, VARIANT_DTYPE, day_lib, ptr
+from ._ffi import AxcBatchTuning, AxcBrainDesc, AxcShardStats, AxcSynapseStats, VARIANT_DTYPE, day_lib, ptr
 from .engine import DeviceBuffer, GpuError, VramState
 
 
@@ -71,6 +71,17 @@ class BatchEngine:
         if err:
             raise GpuError(f"axc_batch_stats failed ({err})")
         return {n: getattr(s, n) for n, _ in AxcShardStats._fields_}, ts
+
+    def synapse_stats(self, prune_thresholds=None, stream=None) -> np.ndarray:
+        """[n][5] int64: filled slots, sum |w|, sum w^2, slots with |w| below
+        the brain's prune threshold, sum of threshold offsets."""
+        arr = (AxcSynapseStats * self.n)()
+        pr = None if prune_thresholds is None else np.ascontiguousarray(prune_thresholds, np.int16)
+        err = day_lib().axc_batch_synapse_stats(self.h, ptr(pr), arr, stream)
+        if err:
+            raise GpuError(f"axc_batch_synapse_stats failed ({err})")
+        return np.array([[a.filled, a.sum_abs_weight, a.sum_sq_weight, a.below_prune, a.sum_threshold_offset]
+                         for a in arr], dtype=np.int64)
 
     def destroy(self) -> None:
         if self.h:

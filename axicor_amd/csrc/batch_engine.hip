@@ -140,4 +140,44 @@ __global__ __launch_bounds__(256) void batch_gsop_kernel(const BrainDev* __restr
 
 __global__ void batch_set_tick_kernel(BatchTick* __restrict__ bt, uint32_t tick_base) { bt->tick_base = tick_base; }
 
+// Synapse statistics per brain, one workgroup per soma work item: over the
+// slots with a target, the count, sum |w|, sum w^2 and the count with
+// |w| < prune[brain]; plus the sum of the threshold offsets. Integer sums, so
+// the result does not depend on the order of the atomics.
+constexpr int kSynStats = 5;
+__global__ __launch_bounds__(256) void batch_synapse_stats_kernel(const BrainDev* __restrict__ brains,
+                                                                  const WorkItem* __restrict__ items,
+                                                                  const int16_t* __restrict__ prune,
+                                                                  unsigned long long* __restrict__ out) {
+  __shared__ unsigned long long red[kSynStats][256];
+  const WorkItem it = items[blockIdx.x];
+  const BrainDev& b = brains[it.brain];
+  const uint32_t i = it.start + threadIdx.x;
+  unsigned long long v[kSynStats] = {0, 0, 0, 0, 0};
+  if (i < b.N) {
+    const uint32_t* __restrict__ tg = b.vram.dendrite_targets;
+    const int16_t* __restrict__ wt = b.vram.dendrite_weights;
+    const int32_t pr = prune[it.brain];
+    for (uint32_t s = 0; s < 128; s++) {
+      const size_t o = (size_t)s * b.N + i;
+      if (tg[o] == 0) continue;
+      const int32_t w = wt[o];
+      const uint32_t aw = (uint32_t)(w < 0 ? -w : w);
+      v[0] += 1;
+      v[1] += aw;
+      v[2] += (unsigned long long)aw * aw;
+      v[3] += (int32_t)aw < pr ? 1 : 0;
+    }
+    v[4] = (unsigned long long)(long long)b.vram.threshold_offset[i];
+  }
+  for (int k = 0; k < kSynStats; k++) red[k][threadIdx.x] = v[k];
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st)
+      for (int k = 0; k < kSynStats; k++) red[k][threadIdx.x] += red[k][threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x < kSynStats) atomicAdd(&out[(size_t)it.brain * kSynStats + threadIdx.x], red[threadIdx.x][0]);
+}
+
 }  // namespace axc

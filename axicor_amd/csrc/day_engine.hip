@@ -894,6 +894,8 @@ struct Batch {
   AxcVariantParameters* d_luts = nullptr;
   uint64_t* d_stats = nullptr;         // [n][kStatsPerBlock]
   uint64_t* d_type_spikes = nullptr;   // [n][16]
+  unsigned long long* d_syn = nullptr; // [n][kSynStats] synapse statistics
+  int16_t* d_prune = nullptr;          // [n] prune thresholds of the last axc_batch_synapse_stats
   WorkItem* d_axon_items = nullptr;
   WorkItem* d_soma_items = nullptr;
   uint32_t n_axon_items = 0, n_soma_items = 0;
@@ -968,6 +970,8 @@ void batch_free(Batch* b) {
   (void)hipFree(b->d_luts);
   (void)hipFree(b->d_stats);
   (void)hipFree(b->d_type_spikes);
+  (void)hipFree(b->d_syn);
+  (void)hipFree(b->d_prune);
   (void)hipFree(b->d_axon_items);
   (void)hipFree(b->d_soma_items);
   (void)hipFree(b->d_tick);
@@ -1010,6 +1014,8 @@ int32_t axc_batch_create(const AxcBrainDesc* brains, uint32_t n, void** out) {
   if (e == hipSuccess) e = hipMemset(b->d_stats, 0, (size_t)n * kStatsPerBlock * 8);
   if (e == hipSuccess) e = hipMemset(b->d_type_spikes, 0, (size_t)n * 16 * 8);
   if (e == hipSuccess) e = hipMalloc((void**)&b->d_tick, sizeof(BatchTick));
+  if (e == hipSuccess) e = hipMalloc((void**)&b->d_syn, (size_t)n * kSynStats * 8);
+  if (e == hipSuccess) e = hipMalloc((void**)&b->d_prune, (size_t)n * sizeof(int16_t));
   if (e == hipSuccess) {
     b->n_axon_items = (uint32_t)ax.size();
     b->n_soma_items = (uint32_t)so.size();
@@ -1138,6 +1144,34 @@ int32_t axc_batch_autotune(void* h, uint32_t ticks, void* stream, AxcBatchTuning
   b->tune = best;
   if (best_out) *best_out = best;
   return (int32_t)e;
+}
+
+int32_t axc_batch_synapse_stats(void* h, const int16_t* prune_thresholds, AxcSynapseStats* out, void* stream) {
+  if (!h || !out) return (int32_t)hipErrorInvalidValue;
+  Batch* b = (Batch*)h;
+  const size_t n = b->desc.size();
+  hipStream_t st = (hipStream_t)stream;
+  std::vector<int16_t> pr(n, 0);
+  if (prune_thresholds) pr.assign(prune_thresholds, prune_thresholds + n);
+  std::vector<unsigned long long> v(n * kSynStats);
+  hipError_t e = hipMemcpyAsync(b->d_prune, pr.data(), n * sizeof(int16_t), hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipMemsetAsync(b->d_syn, 0, n * kSynStats * 8, st);
+  if (e == hipSuccess) {
+    batch_synapse_stats_kernel<<<b->n_soma_items, kThreads, 0, st>>>(b->d_brains, b->d_soma_items, b->d_prune,
+                                                                     b->d_syn);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(v.data(), b->d_syn, n * kSynStats * 8, hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e != hipSuccess) return (int32_t)e;
+  for (size_t k = 0; k < n; k++) {
+    out[k].filled = v[k * kSynStats + 0];
+    out[k].sum_abs_weight = v[k * kSynStats + 1];
+    out[k].sum_sq_weight = v[k * kSynStats + 2];
+    out[k].below_prune = v[k * kSynStats + 3];
+    out[k].sum_threshold_offset = (int64_t)v[k * kSynStats + 4];
+  }
+  return 0;
 }
 
 int32_t axc_batch_stats(void* h, uint32_t brain, AxcShardStats* out, uint64_t* type_spikes16) {

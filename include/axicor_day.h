@@ -342,6 +342,18 @@ int32_t axc_batch_get_tuning(void *batch, AxcBatchTuning *tuning);
 /* Counters of one brain since creation (or the last autotune) and its spikes
  * per variant type (16 entries, may be NULL). */
 int32_t axc_batch_stats(void *batch, uint32_t brain, AxcShardStats *out, uint64_t *type_spikes16);
+
+/* Synapse statistics of every brain in its current state: over the dendrite
+ * slots with a target (target != 0), their number, sum |w|, sum w^2 and the
+ * number with |w| below the brain's prune threshold (what the next Night's
+ * sort&prune, bindings.cu:497-505, would remove), plus the sum of the soma
+ * threshold offsets. prune_thresholds: host array of n_brains entries, or
+ * NULL for 0. out: host array of n_brains entries. Synchronises `stream`. */
+typedef struct AxcSynapseStats {
+    uint64_t filled, sum_abs_weight, sum_sq_weight, below_prune;
+    int64_t sum_threshold_offset;
+} AxcSynapseStats;
+int32_t axc_batch_synapse_stats(void *batch, const int16_t *prune_thresholds, AxcSynapseStats *out, void *stream);
 void axc_batch_destroy(void *batch);
 
 /* Library/version string. */

@@ -92,3 +92,27 @@ def test_batch_graph_replays_and_autotune_restore(gpu):
         assert np.array_equal(m["vram"].download_state(), st)
         assert np.array_equal(m["vram"].download_axons(), hd)
     be.destroy()
+
+
+def test_batch_synapse_stats_match_the_downloaded_state(gpu):
+    """axc_batch_synapse_stats against the same sums over the state blob."""
+    from axicor_amd.layout import compute_state_offsets
+    members = make_members(9, 5, 40)
+    mems = [load(b, dop, inp) for b, dop, inp in members]
+    be = BatchEngine(mems)
+    be.step(0, 40)
+    day_lib().gpu_device_synchronize()
+    prune = np.array([2, 15, 30, 100, 0], np.int16)
+    got = be.synapse_stats(prune)
+    be.destroy()
+    for k, m in enumerate(mems):
+        v = m["vram"]
+        st = v.download_state()
+        off = compute_state_offsets(v.padded_n)
+        N = v.padded_n
+        t = st[off.dendrite_targets:off.dendrite_targets + 4 * 128 * N].view(np.uint32)
+        w = st[off.dendrite_weights:off.dendrite_weights + 2 * 128 * N].view(np.int16).astype(np.int64)
+        thr = st[off.threshold_offset:off.threshold_offset + 4 * N].view(np.int32).astype(np.int64)
+        aw = np.abs(w[t != 0])
+        want = [aw.size, aw.sum(), (aw * aw).sum(), np.count_nonzero(aw < prune[k]), thr.sum()]
+        assert got[k].tolist() == want, (k, got[k].tolist(), want)
