@@ -136,6 +136,8 @@ SIGNATURES = {
     "axc_batch_synapse_stats": (_I32, [_P, _P, _P, _P]),
     "axc_batch_destroy": (None, [_P]),
     "axc_version": (C.c_char_p, []),
+    "axc_staging_enable": (None, [C.c_int]),
+    "axc_shard_targets_changed": (None, [C.POINTER(ShardVramPtrs)]),
 }
 
 TIMED_KERNELS = ["axon_tick", "neuron_tick", "push_apply", "gsop", "readout", "spike_count"]

@@ -19,7 +19,9 @@
 #include <algorithm>
 #include "day_common.h"
 #include "day_kernels.hip"
+#include "day_staged.hip"
 #include "batch_engine.hip"
+#include <atomic>
 
 using namespace axc;
 
@@ -72,9 +74,10 @@ struct GraphKey {
   bool has_bitmask, has_spikes, has_out;
   const void* mapped;
   uint32_t v_seg;
+  bool staged;
   std::vector<uint32_t> spike_counts;
   bool operator==(const GraphKey& o) const {
-    return T == o.T && words == o.words && virt_off == o.virt_off && n_virt == o.n_virt &&
+    return T == o.T && words == o.words && virt_off == o.virt_off && n_virt == o.n_virt && staged == o.staged &&
            max_spk == o.max_spk && n_out == o.n_out && has_bitmask == o.has_bitmask &&
            has_spikes == o.has_spikes && has_out == o.has_out && mapped == o.mapped && v_seg == o.v_seg &&
            spike_counts == o.spike_counts;
@@ -96,7 +99,14 @@ struct Shard {
   GraphKey gkey{};
   hipGraphExec_t gexec = nullptr;
   hipGraph_t graph = nullptr;
+  // Staged dendrite scan (day_staged.hip): the plan, allocated at the first
+  // step, and whether the targets changed since it was built.
+  StagePlan sp{};
+  std::atomic<bool> stage_dirty{true};
 };
+
+// Staged scan on/off for the whole process (on by default).
+std::atomic<bool> g_staging{true};
 
 std::mutex g_mu;
 std::unordered_map<const void*, Shard*> g_shards;
@@ -148,6 +158,81 @@ void free_scratch(Shard* s) {
   (void)hipFree(s->prm);
   s->sc = ShardScratch{};
   s->prm = nullptr;
+  for (void* q : {(void*)s->sp.code, (void*)s->sp.seg, (void*)s->sp.key, (void*)s->sp.fill, (void*)s->sp.set,
+                  (void*)s->sp.cnt, (void*)s->sp.comp8})
+    if (q) (void)hipFree(q);
+  s->sp = StagePlan{};
+  s->stage_dirty = true;
+}
+
+// Allocates the staging plan of a shard (first step) and rebuilds it on
+// `st` when the targets changed. Returns whether the staged scan is used.
+bool ensure_stage(Shard* s, const ShardVramPtrs& v, hipStream_t st, hipError_t* err) {
+  *err = hipSuccess;
+  if (!g_staging.load() || s->N == 0) return false;
+  const size_t N = s->N, A = s->A;
+  const size_t blocks = neuron_blocks((uint32_t)N);
+  if (!s->sp.comp8) {
+    static std::once_flag attr_once;
+    std::call_once(attr_once, [] {
+      const int bytes = (int)((kStageCap + 1) * sizeof(uint2));
+      (void)hipFuncSetAttribute((const void*)neuron_staged_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                bytes);
+    });
+    StagePlan p{};
+    hipError_t e = hipSuccess;
+    if (kStageKey32) {
+      e = hipMalloc((void**)&p.key, kMaxDendrites * N * sizeof(uint32_t));
+    } else {
+      e = hipMalloc((void**)&p.code, kMaxDendrites * N * sizeof(uint16_t));
+      if (e == hipSuccess) e = hipMalloc((void**)&p.seg, kMaxDendrites * N);
+    }
+    if (e == hipSuccess) e = hipMalloc((void**)&p.fill, N);
+    if (e == hipSuccess) e = hipMalloc((void**)&p.set, blocks * kStageCap * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc((void**)&p.cnt, blocks * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc((void**)&p.comp8, (A + 1) * sizeof(uint2));
+    // Padding blocks keep an empty set; entry A is the inert axon.
+    if (e == hipSuccess) e = hipMemset(p.cnt, 0, blocks * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMemset(p.comp8, 0xFF, (A + 1) * sizeof(uint2));
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e != hipSuccess) {
+      for (void* q : {(void*)p.code, (void*)p.seg, (void*)p.key, (void*)p.fill, (void*)p.set, (void*)p.cnt,
+                      (void*)p.comp8})
+        if (q) (void)hipFree(q);
+      *err = e;
+      return false;
+    }
+    s->sp = p;
+    s->stage_dirty = true;
+  }
+  if (s->stage_dirty.exchange(false)) {
+    stage_build_kernel<<<blocks_for(N), kThreads, 0, st>>>(v.dendrite_targets, (uint32_t)N, (uint32_t)A, s->sp);
+    *err = hipGetLastError();
+    if (*err != hipSuccess) {
+      s->stage_dirty = true;
+      return false;
+    }
+  }
+  return true;
+}
+
+// A host write of [dst, dst+size) may have changed some shard's targets.
+void note_device_write(const void* dst, size_t size) {
+  if (!dst || size == 0) return;
+  const char* a = (const char*)dst;
+  std::lock_guard<std::mutex> lk(g_mu);
+  for (auto& kv : g_shards) {
+    Shard* s = kv.second;
+    const char* t0 = (const char*)s->state_base + (size_t)s->N * 14;
+    const char* t1 = t0 + (size_t)s->N * 4 * kMaxDendrites;
+    if (a < t1 && a + size > t0) s->stage_dirty = true;
+  }
+}
+
+void mark_stage_dirty(const void* state_base) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  auto it = g_shards.find(state_base);
+  if (it != g_shards.end()) it->second->stage_dirty = true;
 }
 
 hipError_t alloc_scratch(Shard* s) {
@@ -219,7 +304,8 @@ hipError_t enqueue_tick(const ShardVramPtrs& v, Shard* s, uint32_t v_seg, uint32
                         const uint32_t* bitmask, uint32_t words, uint32_t virt_off, uint32_t n_virt,
                         const uint32_t* spikes, uint32_t max_spk, uint32_t n_spk, const uint32_t* mapped,
                         uint8_t* out, uint32_t n_out, int32_t dopamine, const TickParams* prm, hipStream_t st,
-                        const AxcVariantParameters* lut, uint32_t snap, bool fuse_prev, bool gsop_now) {
+                        const AxcVariantParameters* lut, uint32_t snap, bool fuse_prev, bool gsop_now,
+                        bool staged) {
   const uint32_t N = s->N, A = s->A;
   uint4* cur = s->sc.comp[snap & 1];
   uint4* prev = s->sc.comp[(snap & 1) ^ 1];
@@ -239,9 +325,15 @@ hipError_t enqueue_tick(const ShardVramPtrs& v, Shard* s, uint32_t v_seg, uint32
     TimedScope ts(kKAxon, st, timed);
     axon_tick_kernel<<<blocks_for(A > kNumCounters ? A : kNumCounters), kThreads, 0, st>>>(
         v.axon_heads, A, v_seg, k2 ? bitmask : nullptr, virt_off, k2 ? n_virt : 0u, s->sc.push_cnt, k3 ? 1 : 0,
-        s->sc.counters, prm, tick, words, cur);
+        s->sc.counters, prm, tick, words, cur, staged ? s->sp.comp8 : nullptr);
   }
-  if (N > 0) {
+  if (N > 0 && staged) {
+    // The staged pass never fuses the previous tick's GSOP (gsop_now is set
+    // for every tick of a staged batch).
+    TimedScope ts(kKNeuron, st, timed);
+    const size_t lds = (kStageCap + 1) * sizeof(uint2);
+    neuron_staged_kernel<<<neuron_blocks(N), kStageThreads, lds, st>>>(v, N, A, lut, prm, tick, s->sc, cur, s->sp);
+  } else if (N > 0) {
     TimedScope ts(kKNeuron, st, timed);
     if (fuse_prev)
       neuron_tick_kernel<true><<<neuron_blocks(N), kThreads, 0, st>>>(v, N, A, lut, prm, tick, s->sc, cur, prev,
@@ -256,7 +348,8 @@ hipError_t enqueue_tick(const ShardVramPtrs& v, Shard* s, uint32_t v_seg, uint32
   }
   if (gsop_now) {
     TimedScope ts(kKGsop, st, timed);
-    gsop_kernel<<<kListBlocks, kThreads, 0, st>>>(v, N, A, lut, prm, dopamine, s->sc, cur);
+    gsop_kernel<<<kListBlocks, kThreads, 0, st>>>(v, N, A, lut, prm, dopamine, s->sc, cur,
+                                                  staged ? s->sp.fill : nullptr);
   }
   if (n_out > 0 && mapped != nullptr && out != nullptr) {
     TimedScope ts(kKReadout, st, timed);
@@ -320,6 +413,7 @@ int32_t cu_upload_state_blob(const ShardVramPtrs* vram, const void* blob, size_t
       return (int32_t)hipErrorInvalidValue;
   }
   hipError_t e = hipMemcpy(vram->soma_voltage, blob, size, hipMemcpyHostToDevice);
+  mark_stage_dirty(vram->soma_voltage);
   if (e != hipSuccess) {
     fprintf(stderr, "[cu_upload_state_blob] hipMemcpy failed: %s\n", hipGetErrorString(e));
     return (int32_t)e;
@@ -394,9 +488,11 @@ int32_t cu_step_day_phase(const ShardVramPtrs* vram, uint32_t padded_n, uint32_t
   if (!s) return (int32_t)e;
   const AxcVariantParameters* lut = device_lut();
   if (!lut) return (int32_t)hipErrorOutOfMemory;
+  const bool staged = ensure_stage(s, *vram, (hipStream_t)stream, &e);
+  if (e != hipSuccess) return (int32_t)e;
   e = enqueue_tick(*vram, s, v_seg, current_tick, input_bitmask, 0, virtual_offset, num_virtual_axons,
                    incoming_spikes, 0, num_incoming_spikes, mapped_soma_ids, output_history, num_outputs,
-                   (int32_t)dopamine, nullptr, (hipStream_t)stream, lut, s->flip, false, true);
+                   (int32_t)dopamine, nullptr, (hipStream_t)stream, lut, s->flip, false, true, staged);
   s->flip ^= 1u;
   return (int32_t)e;
 }
@@ -433,12 +529,14 @@ void gpu_host_free(void* p) {
   if (p) (void)hipHostFree(p);
 }
 void gpu_memcpy_host_to_device_async(void* dst, const void* src, size_t size, void* stream) {
+  note_device_write(dst, size);
   (void)hipMemcpyAsync(dst, src, size, hipMemcpyHostToDevice, (hipStream_t)stream);
 }
 void gpu_memcpy_device_to_host_async(void* dst, const void* src, size_t size, void* stream) {
   (void)hipMemcpyAsync(dst, src, size, hipMemcpyDeviceToHost, (hipStream_t)stream);
 }
 bool gpu_memcpy_host_to_device(void* dst, const void* src, size_t size) {
+  note_device_write(dst, size);
   hipError_t e = hipMemcpy(dst, src, size, hipMemcpyHostToDevice);
   if (e != hipSuccess) fprintf(stderr, "[gpu_memcpy_host_to_device] %s (size %zu)\n", hipGetErrorString(e), size);
   return e == hipSuccess;
@@ -474,6 +572,7 @@ void update_constant_memory_hot_reload(const AxcVariantParameters* nv, void* str
 // ===========================================================================
 void launch_sort_and_prune(const ShardVramPtrs* ptrs, uint32_t padded_n, int16_t prune_threshold) {
   if (!ptrs || padded_n == 0) return;
+  mark_stage_dirty(ptrs->soma_voltage);
   sort_prune_kernel<<<blocks_for(padded_n, 64), kThreads>>>(ptrs->soma_flags, ptrs->dendrite_targets,
                                                              ptrs->dendrite_weights, ptrs->dendrite_timers,
                                                              padded_n, (int32_t)prune_threshold);
@@ -592,7 +691,7 @@ void launch_inject_inputs(AxcLegacyVramState v, const uint32_t* bitmask, uint32_
   if (!s) return;
   uint32_t n = total_virtual_axons < v.total_axons ? total_virtual_axons : v.total_axons;
   axon_tick_kernel<<<blocks_for(v.total_axons), kThreads>>>(v.axon_heads, n, 0u, bitmask, 0u, n, s->sc.push_cnt, 0,
-                                                            s->sc.counters, nullptr, 0u, 0u, nullptr);
+                                                            s->sc.counters, nullptr, 0u, 0u, nullptr, nullptr);
 }
 void launch_apply_spike_batch(AxcLegacyVramState v, const void* schedule, uint32_t count) {
   if (!schedule || count == 0 || v.total_axons == 0) return;
@@ -603,7 +702,7 @@ void launch_apply_spike_batch(AxcLegacyVramState v, const void* schedule, uint32
   spike_count_kernel<<<blocks_for(count), kThreads>>>((const uint32_t*)schedule, count, v.total_axons, s->sc.push_cnt,
                                                       nullptr, 0u, 0u);
   axon_tick_kernel<<<blocks_for(v.total_axons), kThreads>>>(v.axon_heads, v.total_axons, 0u, nullptr, 0u, 0u,
-                                                            s->sc.push_cnt, 1, s->sc.counters, nullptr, 0u, 0u, nullptr);
+                                                            s->sc.push_cnt, 1, s->sc.counters, nullptr, 0u, 0u, nullptr, nullptr);
 }
 void launch_propagate_axons(AxcLegacyVramState v, uint32_t v_seg) {
   if (v.total_axons == 0) return;
@@ -612,7 +711,7 @@ void launch_propagate_axons(AxcLegacyVramState v, uint32_t v_seg) {
   Shard* s = shard_for(&p, v.padded_n, v.total_axons, &e);
   if (!s) return;
   axon_tick_kernel<<<blocks_for(v.total_axons), kThreads>>>(v.axon_heads, v.total_axons, v_seg, nullptr, 0u, 0u,
-                                                            s->sc.push_cnt, 0, s->sc.counters, nullptr, 0u, 0u, nullptr);
+                                                            s->sc.push_cnt, 0, s->sc.counters, nullptr, 0u, 0u, nullptr, nullptr);
 }
 void launch_update_neurons(AxcLegacyVramState v, const void* constants_ptr, uint32_t current_tick) {
   (void)constants_ptr;
@@ -729,6 +828,11 @@ int32_t axc_step_day_phase_batch(const ShardVramPtrs* vram, uint32_t padded_n, u
   key.has_out = d_out != nullptr && d_mapped != nullptr && n_out > 0;
   key.mapped = d_mapped;
   key.v_seg = v_seg;
+  key.staged = ensure_stage(s, *vram, st, &e);
+  if (e != hipSuccess) return (int32_t)e;
+  // The direct pass applies tick t's GSOP inside tick t+1's soma pass; with
+  // the staged pass every tick ends with the GSOP kernel.
+  const bool fuse = !key.staged;
   key.spike_counts.assign(T, 0);
   if (d_spikes && h_counts)
     for (uint32_t t = 0; t < T; t++) key.spike_counts[t] = h_counts[t];
@@ -747,7 +851,7 @@ int32_t axc_step_day_phase_batch(const ShardVramPtrs* vram, uint32_t padded_n, u
       e = enqueue_tick(*vram, s, v_seg, tick_base + t, d_bitmask ? d_bitmask + (size_t)t * words : nullptr, words,
                        virt_off, n_virt, d_spikes ? d_spikes + (size_t)t * max_spk : nullptr, max_spk,
                        key.spike_counts[t], d_mapped, d_out ? d_out + (size_t)t * n_out : nullptr, n_out,
-                       (int32_t)dopamine, nullptr, st, lut, t, t > 0, t + 1 == T);
+                       (int32_t)dopamine, nullptr, st, lut, t, t > 0 && fuse, t + 1 == T || !fuse, key.staged);
     }
     return (int32_t)e;
   }
@@ -773,7 +877,8 @@ int32_t axc_step_day_phase_batch(const ShardVramPtrs* vram, uint32_t padded_n, u
         // GSOP of tick t runs inside tick t+1's neuron pass; the last tick's
         // GSOP is launched on its own so the batch leaves a complete state.
         e = enqueue_tick(*vram, s, v_seg, t, d_bitmask, words, virt_off, n_virt, d_spikes, max_spk,
-                         key.spike_counts[t], d_mapped, d_out, n_out, 0, s->prm, cap, lut, t, t > 0, t + 1 == T);
+                         key.spike_counts[t], d_mapped, d_out, n_out, 0, s->prm, cap, lut, t, t > 0 && fuse,
+                         t + 1 == T || !fuse, key.staged);
       }
       hipGraph_t g = nullptr;
       hipError_t e2 = hipStreamEndCapture(cap, &g);
@@ -839,6 +944,12 @@ void axc_host_sort_and_prune(uint8_t* flags, uint32_t* targets, int16_t* weights
     th.emplace_back(work, lo, hi);
   }
   for (auto& x : th) x.join();
+}
+
+void axc_staging_enable(int enable) { g_staging = enable != 0; }
+
+void axc_shard_targets_changed(const ShardVramPtrs* vram) {
+  if (vram) mark_stage_dirty(vram->soma_voltage);
 }
 
 void axc_kernel_timing_enable(int enable) {

@@ -90,6 +90,16 @@ __device__ __forceinline__ uint4 compact_heads(const Heads8& h) {
                     c16(h.hi.x) | (c16(h.hi.y) << 16), c16(h.hi.z) | (c16(h.hi.w) << 16));
 }
 
+// Byte form of eight heads, min(h, 255) each, for the staged scan: for a
+// segment s and length L with s + L <= 255 (the u8 ranges of both make every
+// other case rare), "(h - s) < L" and the freshest distance below L come out
+// the same from min(h, 255) as from h, since any h >= 255 fails both tests.
+__device__ __forceinline__ uint32_t b8(uint32_t h) { return h < 0xFFu ? h : 0xFFu; }
+__device__ __forceinline__ uint2 byte_heads(const Heads8& h) {
+  return make_uint2(b8(h.lo.x) | (b8(h.lo.y) << 8) | (b8(h.lo.z) << 16) | (b8(h.lo.w) << 24),
+                    b8(h.hi.x) | (b8(h.hi.y) << 8) | (b8(h.hi.z) << 16) | (b8(h.hi.w) << 24));
+}
+
 // Smallest (c_k - seg) mod 2^16 over the eight compact heads: < prop means
 // "some head in the active tail" (physics.cu:198-201), and for GSOP it is the
 // freshest distance min_dist of physics.cu:311-318 whenever it is < len.
@@ -101,6 +111,21 @@ __device__ __forceinline__ uint32_t min_dist16(const uint4& c, uint32_t seg) {
   const axc_us2 d1 = __builtin_bit_cast(axc_us2, c.y) - s;
   const axc_us2 d2 = __builtin_bit_cast(axc_us2, c.z) - s;
   const axc_us2 d3 = __builtin_bit_cast(axc_us2, c.w) - s;
+  const axc_us2 m = __builtin_elementwise_min(__builtin_elementwise_min(d0, d1), __builtin_elementwise_min(d2, d3));
+  return min((uint32_t)m.x, (uint32_t)m.y);
+}
+
+// min_dist16 over the byte form: the eight bytes are spread to u16 lanes with
+// v_perm_b32, then the same packed subtract and min. Equal to min_dist16 on
+// the compact form whenever seg + L <= 255 and the result is compared with L.
+__device__ __forceinline__ uint32_t min_dist8(const uint2& c, uint32_t seg) {
+  const uint32_t p0 = __builtin_amdgcn_perm(0u, c.x, 0x0c010c00u), p1 = __builtin_amdgcn_perm(0u, c.x, 0x0c030c02u);
+  const uint32_t p2 = __builtin_amdgcn_perm(0u, c.y, 0x0c010c00u), p3 = __builtin_amdgcn_perm(0u, c.y, 0x0c030c02u);
+  const axc_us2 s = {(unsigned short)seg, (unsigned short)seg};
+  const axc_us2 d0 = __builtin_bit_cast(axc_us2, p0) - s;
+  const axc_us2 d1 = __builtin_bit_cast(axc_us2, p1) - s;
+  const axc_us2 d2 = __builtin_bit_cast(axc_us2, p2) - s;
+  const axc_us2 d3 = __builtin_bit_cast(axc_us2, p3) - s;
   const axc_us2 m = __builtin_elementwise_min(__builtin_elementwise_min(d0, d1), __builtin_elementwise_min(d2, d3));
   return min((uint32_t)m.x, (uint32_t)m.y);
 }
@@ -161,7 +186,8 @@ __device__ __forceinline__ GsopTerms gsop_terms(uint8_t flags, const AxcVariantP
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void axon_step(AxcBurstHeads8* __restrict__ heads, uint32_t a, uint32_t v_seg,
                                           const uint32_t* __restrict__ bitmask, uint32_t virt_off, uint32_t n_virt,
-                                          uint32_t* __restrict__ push_cnt, bool use_cnt, uint4* __restrict__ comp) {
+                                          uint32_t* __restrict__ push_cnt, bool use_cnt, uint4* __restrict__ comp,
+                                          uint2* __restrict__ comp8 = nullptr) {
   Heads8 h = load_heads(heads, a);
   uint32_t k = 0;
   if (bitmask) {
@@ -186,6 +212,7 @@ __device__ __forceinline__ void axon_step(AxcBurstHeads8* __restrict__ heads, ui
   h.hi.w += (h.hi.w != kSentinel) ? v_seg : 0u;
   store_heads(heads, a, h);
   if (comp) comp[a] = compact_heads(h);
+  if (comp8) comp8[a] = byte_heads(h);
 }
 
 __global__ __launch_bounds__(256) void axon_tick_kernel(AxcBurstHeads8* __restrict__ heads, uint32_t A,
@@ -194,7 +221,8 @@ __global__ __launch_bounds__(256) void axon_tick_kernel(AxcBurstHeads8* __restri
                                                         uint32_t* __restrict__ push_cnt, int use_cnt,
                                                         uint32_t* __restrict__ counters,
                                                         const TickParams* __restrict__ prm, uint32_t t_off,
-                                                        uint32_t words, uint4* __restrict__ comp) {
+                                                        uint32_t words, uint4* __restrict__ comp,
+                                                        uint2* __restrict__ comp8) {
   uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t* __restrict__ bitmask =
       prm ? (prm->bitmask ? prm->bitmask + (size_t)t_off * words : nullptr) : bitmask_arg;
@@ -202,7 +230,7 @@ __global__ __launch_bounds__(256) void axon_tick_kernel(AxcBurstHeads8* __restri
   // push_apply launches; this tick's neuron_tick fills them again.
   if (a < kNumCounters) counters[a] = 0;
   if (a >= A) return;
-  axon_step(heads, a, v_seg, bitmask, virt_off, n_virt, push_cnt, use_cnt != 0, comp);
+  axon_step(heads, a, v_seg, bitmask, virt_off, n_virt, push_cnt, use_cnt != 0, comp, comp8);
 }
 
 // K3 (physics.cu:114-132): one count per occurrence of an id < A.
@@ -215,6 +243,62 @@ __global__ __launch_bounds__(256) void spike_count_kernel(const uint32_t* __rest
   const uint32_t* __restrict__ ids = prm ? prm->spikes + (size_t)t_off * stride : ids_arg;
   uint32_t a = ids[j];
   if (a < A) atomicAdd(&push_cnt[a], 1u);
+}
+
+// The soma part of K5 (physics.cu:165-174, 209-275) for soma i after its
+// dendritic input i_in: refractory countdown, or homeostasis decay, adaptive
+// leak, leak toward rest, threshold test, heartbeat, resets, burst count and
+// flags. `flags` already has K1 and the spike bit cleared. Sets fin (spike)
+// and, for a spike, the soma's own axon to push.
+__device__ __forceinline__ void soma_tick(const ShardVramPtrs& vram, uint32_t i, uint32_t A,
+                                          const AxcVariantParameters& p, uint8_t flags, uint8_t timer, bool refr,
+                                          uint32_t tick, int32_t i_in, bool& fin, bool& push_own,
+                                          uint32_t& my_axon) {
+  if (refr) {
+    vram.timers[i] = (uint8_t)(timer - 1);
+    vram.soma_flags[i] = flags;
+    return;
+  }
+  int32_t thr = vram.threshold_offset[i];
+  int32_t decayed = wsub(thr, (int32_t)p.homeostasis_decay);
+  thr = decayed & ~(decayed >> 31);
+  int32_t leak = p.leak_rate;
+  if (p.adaptive_mode == 1) {
+    int32_t add = wmul(thr, (int32_t)p.adaptive_leak_gain) >> 8;
+    leak = wadd(leak, add);
+    int32_t over = wsub(leak, p.adaptive_leak_max);
+    leak = wsub(leak, over & ~(over >> 31));
+  }
+  int32_t v = wadd(vram.soma_voltage[i], i_in);
+  int32_t diff = wsub(v, p.rest_potential);
+  int32_t sign = (diff > 0) - (diff < 0);
+  int32_t abs_diff = wmul(diff, sign);
+  int32_t leaked = wsub(abs_diff, leak);
+  leaked = leaked & ~(leaked >> 31);
+  v = wadd(p.rest_potential, wmul(sign, leaked));
+  int32_t eff = wadd(p.threshold, thr);
+  bool glif = v >= eff;
+  uint32_t period = p.spontaneous_firing_period_ticks;
+  bool hb = period > 0 && ((tick + i * 104729u) % period) == 0;
+  fin = glif || hb;
+  if (glif) {
+    v = p.rest_potential;
+    thr = wadd(thr, p.homeostasis_penalty);
+  }
+  uint8_t new_timer = glif ? p.refractory_period : (uint8_t)0;
+  if (fin) {
+    flags = (uint8_t)((flags & 0xFC) | 0x03);
+    my_axon = vram.soma_to_axon[i];
+    push_own = (my_axon != kNoAxon) && (my_axon < A);
+  } else {
+    flags = (uint8_t)(flags & ~0x01);
+  }
+  uint8_t bc = (flags >> 1) & 0x07;
+  bc = (uint8_t)(bc + ((fin && bc < 7) ? 1 : 0));
+  vram.soma_voltage[i] = v;
+  vram.soma_flags[i] = (uint8_t)((flags & 0xF0) | (bc << 1) | (fin ? 1 : 0));
+  vram.threshold_offset[i] = thr;
+  vram.timers[i] = new_timer;
 }
 
 // ---------------------------------------------------------------------------
@@ -368,52 +452,8 @@ __device__ __forceinline__ void neuron_block(const ShardVramPtrs& vram, uint32_t
       }
     }
 
-    if (refr) {
-      vram.timers[i] = (uint8_t)(timer - 1);
-      vram.soma_flags[i] = flags;
-    } else {
-      n_int = 1;
-      int32_t thr = vram.threshold_offset[i];
-      int32_t decayed = wsub(thr, (int32_t)p.homeostasis_decay);
-      thr = decayed & ~(decayed >> 31);
-      int32_t leak = p.leak_rate;
-      if (p.adaptive_mode == 1) {
-        int32_t add = wmul(thr, (int32_t)p.adaptive_leak_gain) >> 8;
-        leak = wadd(leak, add);
-        int32_t over = wsub(leak, p.adaptive_leak_max);
-        leak = wsub(leak, over & ~(over >> 31));
-      }
-      int32_t v = wadd(vram.soma_voltage[i], i_in);
-      int32_t diff = wsub(v, p.rest_potential);
-      int32_t sign = (diff > 0) - (diff < 0);
-      int32_t abs_diff = wmul(diff, sign);
-      int32_t leaked = wsub(abs_diff, leak);
-      leaked = leaked & ~(leaked >> 31);
-      v = wadd(p.rest_potential, wmul(sign, leaked));
-      int32_t eff = wadd(p.threshold, thr);
-      bool glif = v >= eff;
-      uint32_t period = p.spontaneous_firing_period_ticks;
-      bool hb = period > 0 && ((tick + i * 104729u) % period) == 0;
-      fin = glif || hb;
-      if (glif) {
-        v = p.rest_potential;
-        thr = wadd(thr, p.homeostasis_penalty);
-      }
-      uint8_t new_timer = glif ? p.refractory_period : (uint8_t)0;
-      if (fin) {
-        flags = (uint8_t)((flags & 0xFC) | 0x03);
-        my_axon = vram.soma_to_axon[i];
-        push_own = (my_axon != kNoAxon) && (my_axon < A);
-      } else {
-        flags = (uint8_t)(flags & ~0x01);
-      }
-      uint8_t bc = (flags >> 1) & 0x07;
-      bc = (uint8_t)(bc + ((fin && bc < 7) ? 1 : 0));
-      vram.soma_voltage[i] = v;
-      vram.soma_flags[i] = (uint8_t)((flags & 0xF0) | (bc << 1) | (fin ? 1 : 0));
-      vram.threshold_offset[i] = thr;
-      vram.timers[i] = new_timer;
-    }
+    soma_tick(vram, i, A, p, flags, timer, refr, tick, i_in, fin, push_own, my_axon);
+    n_int = refr ? 0u : 1u;
   }
 
   // Push counts: one atomic per spiking soma with an axon, on distinct
@@ -541,10 +581,32 @@ __device__ __forceinline__ uint32_t gsop_soma(const ShardVramPtrs& vram, uint32_
   return filled;
 }
 
+// GSOP of spiking soma i with its filled prefix F known (the staged plan's
+// fill): lane l takes slots l and l + 64 at once.
+__device__ __forceinline__ void gsop_soma_filled(const ShardVramPtrs& vram, uint32_t N, uint32_t A,
+                                                 const AxcVariantParameters* lut, int32_t dop, uint32_t i, uint32_t F,
+                                                 const uint4* __restrict__ comp, uint32_t lane) {
+  const uint8_t flags = vram.soma_flags[i];
+  const AxcVariantParameters& p = lut[(flags >> 4) & 0x0F];
+  const GsopTerms g = gsop_terms(flags, p, dop);
+  const bool l0 = lane < F, l1 = lane + 64 < F;
+  const size_t c0 = (size_t)lane * N + i, c1 = (size_t)(lane + 64) * N + i;
+  const uint32_t t0 = l0 ? vram.dendrite_targets[c0] : 0u, t1 = l1 ? vram.dendrite_targets[c1] : 0u;
+  const int16_t w0 = l0 ? vram.dendrite_weights[c0] : (int16_t)0, w1 = l1 ? vram.dendrite_weights[c1] : (int16_t)0;
+  const uint4 q0 = comp[l0 ? snap_index(t0, A) : A], q1 = comp[l1 ? snap_index(t1, A) : A];
+  if (l0) vram.dendrite_weights[c0] = gsop_weight(w0, min_dist16(q0, t0 >> 24), g.len, g.raw_pot, g.final_dep, g.bm,
+                                                 p.inertia_curve);
+  if (l1) vram.dendrite_weights[c1] = gsop_weight(w1, min_dist16(q1, t1 >> 24), g.len, g.raw_pot, g.final_dep, g.bm,
+                                                 p.inertia_curve);
+}
+
+// fill: the filled prefix of every soma (the staged plan's), or null to find
+// the first empty target of each spiker by ballot.
 __global__ __launch_bounds__(256) void gsop_kernel(ShardVramPtrs vram, uint32_t N, uint32_t A,
                                                    const AxcVariantParameters* __restrict__ lut_g,
                                                    const TickParams* __restrict__ prm, int32_t dop_arg,
-                                                   ShardScratch sc, const uint4* __restrict__ comp) {
+                                                   ShardScratch sc, const uint4* __restrict__ comp,
+                                                   const uint8_t* __restrict__ fill = nullptr) {
   __shared__ AxcVariantParameters lut[16];
   __shared__ uint32_t red[4][2];
   const uint32_t tid = threadIdx.x;
@@ -558,7 +620,14 @@ __global__ __launch_bounds__(256) void gsop_kernel(ShardVramPtrs vram, uint32_t 
   uint32_t cnt_slots = 0, cnt_tgt = 0;
 
   for (uint32_t k = blockIdx.x * 4 + wv; k < n_spk; k += waves) {
-    const uint32_t filled = gsop_soma(vram, N, A, lut, dop, sc.spk_list[k], comp, lane);
+    const uint32_t i = sc.spk_list[k];
+    uint32_t filled;
+    if (fill) {
+      filled = fill[i];
+      gsop_soma_filled(vram, N, A, lut, dop, i, filled, comp, lane);
+    } else {
+      filled = gsop_soma(vram, N, A, lut, dop, i, comp, lane);
+    }
     cnt_slots += filled;
     cnt_tgt += filled < kMaxDendrites ? filled + 1 : kMaxDendrites;
   }

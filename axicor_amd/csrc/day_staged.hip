@@ -1,0 +1,354 @@
+// day_staged.hip — the dendrite scan of K5 (physics.cu:182-206) with each
+// soma block's presynaptic snapshot entries staged in LDS.
+//
+// A block of 256 consecutive somas reads its dendrite columns once per tick
+// and gathers one snapshot entry per filled slot. Its distinct presynaptic
+// axons are few (about 6,500 for the 25,600 slots of a 256-soma block of the
+// 1.5M-neuron brain with somas numbered by 8x8x8 bricks). So the gathers that
+// limit the direct scan become ~6,500 global loads per block plus LDS reads:
+//   stage_build_kernel  (whenever the targets change) per block: the distinct
+//                       axons of the filled slots (LDS hash set), each slot's
+//                       block-local entry (u16) and segment (u8), and each
+//                       soma's filled prefix;
+//   neuron_staged_kernel (every tick) loads the block's byte snapshot entries
+//                       into LDS, then four groups of 256 lanes scan slot
+//                       chunks 8g..8g+7, 8(g+4).. of the block's somas (the
+//                       slots before the first empty target, as the reference
+//                       loop's `break`), and group 0 finishes each soma with
+//                       the GLIF update and the spike compaction;
+// Each tick's GSOP (K6) runs after the pushes in gsop_kernel, one wave per
+// spiking soma: applying it inside the next tick's staged pass (as the
+// direct pass does) makes the few spiking lanes of a wave wait on a chain of
+// dependent gathers (entry -> axon -> previous snapshot) that holds up their
+// whole block, and measured slower.
+// The snapshot entries are the byte form of the heads (min(h, 255) x 8, see
+// byte_heads); a slot whose segment + propagation length exceeds 255 reads the
+// 16-byte compact form from global memory instead, so every hit is exact.
+// Blocks whose distinct axons exceed kStageCap read the reference targets and
+// gather the byte snapshot from global memory.
+#include "day_common.h"
+
+namespace axc {
+
+constexpr uint32_t kStageGroups = 4;
+constexpr uint32_t kStageThreads = 256 * kStageGroups;
+constexpr uint32_t kHashSize = 16384;     // LDS hash set of the build kernel
+constexpr uint32_t kHashEmpty = 0xFFFFFFFFu;
+constexpr uint32_t kHashMaxProbe = 256;
+
+__device__ __forceinline__ uint32_t hash_slot(uint32_t key) { return (key * 2654435761u) >> (32 - 14); }
+
+// One workgroup of 256 lanes per soma block (grid = ceil(N/256)).
+__global__ __launch_bounds__(256) void stage_build_kernel(const uint32_t* __restrict__ targets, uint32_t N,
+                                                          uint32_t A, StagePlan sp) {
+  __shared__ uint32_t keys[kHashSize];
+  __shared__ uint16_t local[kHashSize];
+  __shared__ uint32_t scan[256];
+  __shared__ uint32_t overflow;
+  const uint32_t tid = threadIdx.x, b = blockIdx.x;
+  const uint32_t i = b * 256 + tid;
+  for (uint32_t j = tid; j < kHashSize; j += 256) keys[j] = kHashEmpty;
+  if (tid == 0) overflow = 0;
+  __syncthreads();
+  uint32_t F = 0;
+  if (i < N) {
+    for (; F < kMaxDendrites; F++) {
+      const uint32_t t = targets[(size_t)F * N + i];
+      if (t == 0) break;
+      const uint32_t key = snap_index(t, A);
+      uint32_t h = hash_slot(key), probe = 0;
+      for (; probe < kHashMaxProbe; probe++) {
+        const uint32_t prev = atomicCAS(&keys[h], kHashEmpty, key);
+        if (prev == kHashEmpty || prev == key) break;
+        h = (h + 1) & (kHashSize - 1);
+      }
+      if (probe == kHashMaxProbe) overflow = 1;
+    }
+    sp.fill[i] = (uint8_t)F;
+  }
+  __syncthreads();
+  // Block-local entries in table order: lane tid owns table positions
+  // [64 tid, 64 tid + 64).
+  constexpr uint32_t kPer = kHashSize / 256;
+  uint32_t mine = 0;
+  for (uint32_t j = 0; j < kPer; j++) mine += keys[tid * kPer + j] != kHashEmpty;
+  scan[tid] = mine;
+  __syncthreads();
+  for (uint32_t off = 1; off < 256; off <<= 1) {
+    const uint32_t v = tid >= off ? scan[tid - off] : 0u;
+    __syncthreads();
+    scan[tid] += v;
+    __syncthreads();
+  }
+  const uint32_t total = scan[255];
+  const bool fits = overflow == 0 && total <= kStageCap;
+  uint32_t next = scan[tid] - mine;
+  for (uint32_t j = 0; j < kPer; j++) {
+    const uint32_t pos = tid * kPer + j;
+    const uint32_t key = keys[pos];
+    if (key != kHashEmpty) {
+      local[pos] = (uint16_t)(next < 0xFFFFu ? next : 0xFFFFu);
+      if (fits) sp.set[(size_t)b * kStageCap + next] = key;
+      next++;
+    }
+  }
+  if (tid == 0) sp.cnt[b] = fits ? total : kStageOverflow;
+  __syncthreads();
+  if (!fits || i >= N) return;
+  for (uint32_t s = 0; s < F; s++) {
+    const size_t o = (size_t)s * N + i;
+    const uint32_t t = targets[o];
+    const uint32_t key = snap_index(t, A);
+    uint32_t h = hash_slot(key);
+    while (keys[h] != key) h = (h + 1) & (kHashSize - 1);  // present: inserted above
+    if (kStageKey32) {
+      sp.key[o] = (uint32_t)local[h] | ((t >> 24) << 16);
+    } else {
+      sp.code[o] = local[h];
+      sp.seg[o] = (uint8_t)(t >> 24);
+    }
+  }
+}
+
+// Slot keys. Staged: block-local entry | segment << 16 (from the plan);
+// otherwise the reference target (segment << 24 | axon + 1).
+template <bool STAGED>
+__device__ __forceinline__ uint32_t slot_key(const ShardVramPtrs& vram, const StagePlan& sp, uint32_t N, uint32_t s,
+                                             uint32_t i) {
+  if (STAGED && kStageKey32) {
+    const char* krow = reinterpret_cast<const char*>(sp.key + (size_t)s * N);
+    return __builtin_nontemporal_load(reinterpret_cast<const uint32_t*>(krow + i * 4u));
+  }
+  if (STAGED) {
+    const char* crow = reinterpret_cast<const char*>(sp.code + (size_t)s * N);
+    const char* srow = reinterpret_cast<const char*>(sp.seg + (size_t)s * N);
+    return (uint32_t)*reinterpret_cast<const uint16_t*>(crow + i * 2u) |
+           ((uint32_t)*reinterpret_cast<const uint8_t*>(srow + i) << 16);
+  }
+  const char* trow = reinterpret_cast<const char*>(vram.dendrite_targets + (size_t)s * N);
+  return __builtin_nontemporal_load(reinterpret_cast<const uint32_t*>(trow + i * 4u));
+}
+template <bool STAGED>
+__device__ __forceinline__ uint32_t key_seg(uint32_t kv) { return STAGED ? (kv >> 16) : (kv >> 24); }
+// Staged: the block-local entry; otherwise the axon's snapshot index.
+template <bool STAGED>
+__device__ __forceinline__ uint32_t key_entry(uint32_t kv, uint32_t A) {
+  return STAGED ? (kv & 0xFFFFu) : snap_index(kv, A);
+}
+template <bool STAGED>
+__device__ __forceinline__ uint32_t key_axon(uint32_t kv, const uint32_t* __restrict__ bset, uint32_t A) {
+  return STAGED ? *reinterpret_cast<const uint32_t*>(reinterpret_cast<const char*>(bset) + (kv & 0xFFFFu) * 4u)
+                : snap_index(kv, A);
+}
+// Snapshot entry of axon a (< 2^28): a scalar base plus a 32-bit lane offset.
+__device__ __forceinline__ uint4 snap_at(const uint4* __restrict__ snap, uint32_t a) {
+  return *reinterpret_cast<const uint4*>(reinterpret_cast<const char*>(snap) + (a << 4));
+}
+__device__ __forceinline__ char* weight_row(const ShardVramPtrs& vram, uint32_t N, uint32_t s) {
+  return reinterpret_cast<char*>(vram.dendrite_weights + (size_t)s * N);
+}
+
+// The hit sum of soma i over the slots of group gu: chunks of 8 slots
+// starting at 8 ((gu + 3) % 4 + 4k), below the soma's filled prefix F.
+template <bool STAGED>
+__device__ __forceinline__ void scan_slots(const ShardVramPtrs& vram, uint32_t N, uint32_t A, const StagePlan& sp,
+                                           const uint2* ent, const uint32_t* __restrict__ bset,
+                                           const uint4* __restrict__ cur, uint32_t gu, uint32_t i, uint32_t F,
+                                           uint32_t prop, int32_t& i_in, uint32_t& hits) {
+  constexpr int U = 8;
+  // Group 0, which finishes the somas afterwards, takes chunks 3, 7, 11, ...
+  // so that a partly filled last chunk lands on another group.
+  const uint32_t c0 = (gu + kStageGroups - 1) % kStageGroups;
+#pragma unroll 1
+  for (uint32_t k = 0; k < kMaxDendrites / (U * kStageGroups); k++) {
+    const uint32_t s0 = (c0 + kStageGroups * k) * U;
+    if (s0 >= F) break;
+    const uint32_t nv = F - s0 < (uint32_t)U ? F - s0 : (uint32_t)U;
+    uint32_t key[U];
+    uint32_t w2[U / 2];  // weights of slots 2j (low half) and 2j+1 (high half)
+#pragma unroll
+    for (int u = 0; u < U; u++) key[u] = slot_key<STAGED>(vram, sp, N, s0 + u, i);
+#pragma unroll
+    for (int j = 0; j < U / 2; j++) {
+      const uint32_t lo = *reinterpret_cast<const uint16_t*>(weight_row(vram, N, s0 + 2 * j) + i * 2u);
+      const uint32_t hi = *reinterpret_cast<const uint16_t*>(weight_row(vram, N, s0 + 2 * j + 1) + i * 2u);
+      w2[j] = lo | (hi << 16);
+    }
+    uint32_t hit_mask = 0, wide = 0;
+    constexpr int H = 4;  // entries read in two halves of four
+#pragma unroll
+    for (int h0 = 0; h0 < U; h0 += H) {
+      uint2 c[H];
+#pragma unroll
+      for (int u = 0; u < H; u++) {
+        const bool live = (uint32_t)(h0 + u) < nv;
+        if (STAGED) c[u] = ent[live ? key_entry<true>(key[h0 + u], A) : kStageCap];
+        else c[u] = sp.comp8[live ? key_entry<false>(key[h0 + u], A) : A];
+      }
+#pragma unroll
+      for (int u = 0; u < H; u++) {
+        const bool live = (uint32_t)(h0 + u) < nv;
+        const uint32_t sgv = key_seg<STAGED>(key[h0 + u]);
+        hit_mask |= (live && min_dist8(c[u], sgv) < prop) ? (1u << (h0 + u)) : 0u;
+        wide |= (live && sgv + prop > 255u) ? (1u << (h0 + u)) : 0u;
+      }
+    }
+    if (wide) {
+      // Segment + length beyond the byte form: the 16-byte compact entry.
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        if (wide & (1u << u)) {
+          const bool h = min_dist16(snap_at(cur, key_axon<STAGED>(key[u], bset, A)), key_seg<STAGED>(key[u])) < prop;
+          hit_mask = h ? (hit_mask | (1u << u)) : (hit_mask & ~(1u << u));
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int32_t wu = (int32_t)(int16_t)(uint16_t)(w2[u >> 1] >> (16 * (u & 1)));
+      i_in = wadd(i_in, (hit_mask >> u) & 1u ? wu : 0);
+    }
+    hits += __builtin_popcount(hit_mask);
+  }
+}
+
+// Loads the block's byte snapshot entries into ent[0, cnt); ent[kStageCap]
+// is the inert entry. Eight independent loads in flight per lane.
+__device__ __forceinline__ void stage_entries(uint2* ent, const uint32_t* __restrict__ bset, uint32_t cnt,
+                                              const uint2* __restrict__ comp8, uint32_t A, uint32_t tid) {
+  constexpr uint32_t SU = 8;
+  for (uint32_t j0 = 0; j0 < cnt; j0 += kStageThreads * SU) {
+    uint32_t ix[SU];
+    uint2 v[SU];
+#pragma unroll
+    for (uint32_t k = 0; k < SU; k++) {
+      const uint32_t j = j0 + k * kStageThreads + tid;
+      ix[k] = j < cnt ? bset[j] : A;
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < SU; k++) v[k] = comp8[ix[k]];
+#pragma unroll
+    for (uint32_t k = 0; k < SU; k++) {
+      const uint32_t j = j0 + k * kStageThreads + tid;
+      if (j < cnt) ent[j] = v[k];
+    }
+  }
+  if (tid == 0) ent[kStageCap] = make_uint2(0xFFFFFFFFu, 0xFFFFFFFFu);
+}
+
+// K1 + K5 of one tick for a block of 256 somas, 1024 lanes (see the file
+// comment). The tick's GSOP (K6) is left to gsop_kernel, after the pushes.
+__global__ __launch_bounds__(kStageThreads) __attribute__((amdgpu_waves_per_eu(8, 8))) void neuron_staged_kernel(
+    ShardVramPtrs vram, uint32_t N, uint32_t A, const AxcVariantParameters* __restrict__ lut_g,
+    const TickParams* __restrict__ prm, uint32_t tick_arg, ShardScratch sc, const uint4* __restrict__ cur,
+    StagePlan sp) {
+  extern __shared__ uint2 ent[];  // kStageCap + 1 entries
+  __shared__ AxcVariantParameters lut[16];
+  __shared__ int32_t red_in[kStageGroups][256];
+  __shared__ uint16_t red_hits[kStageGroups][256];
+  __shared__ uint32_t wave_cnt[2][4];
+  __shared__ uint32_t blk_base[2];
+  __shared__ uint32_t red[4][8];
+  const uint32_t tid = threadIdx.x;
+  const uint32_t per_xcd = gridDim.x / kNumXcd;
+  const uint32_t lb = (blockIdx.x % kNumXcd) * per_xcd + blockIdx.x / kNumXcd;
+  const uint32_t tick = tick_of(prm, tick_arg);
+  if (tid < 64) reinterpret_cast<uint4*>(lut)[tid] = reinterpret_cast<const uint4*>(lut_g)[tid];
+  const uint32_t cnt = sp.cnt[lb];
+  const bool staged = cnt != kStageOverflow;
+  const uint32_t* __restrict__ bset = sp.set + (size_t)lb * kStageCap;
+  if (staged) stage_entries(ent, bset, cnt, sp.comp8, A, tid);
+  __syncthreads();
+
+  const uint32_t g = tid >> 8, li = tid & 255u;
+  const uint32_t i = lb * 256 + li;
+  int32_t i_in = 0;
+  uint32_t hits = 0, F = 0;
+  uint8_t flags_in = 0, timer = 0;
+  bool refr = true;
+  if (i < N) {
+    flags_in = vram.soma_flags[i];
+    timer = vram.timers[i];
+    refr = timer > 0;
+    F = sp.fill[i];
+  }
+  const AxcVariantParameters& p = lut[(flags_in >> 4) & 0x0F];
+  // The group index is the same for the 64 lanes of a wave: scalar.
+  const uint32_t gu = __builtin_amdgcn_readfirstlane(g);
+  if (i < N && !refr) {
+    if (staged) scan_slots<true>(vram, N, A, sp, ent, bset, cur, gu, i, F, p.signal_propagation_length, i_in, hits);
+    else scan_slots<false>(vram, N, A, sp, ent, bset, cur, gu, i, F, p.signal_propagation_length, i_in, hits);
+  }
+  red_in[g][li] = i_in;
+  red_hits[g][li] = (uint16_t)hits;
+  __syncthreads();
+
+  // Group 0 finishes its 256 somas; the other groups only join the barriers.
+  bool fin = false, push_own = false;
+  uint32_t my_axon = 0;
+  uint32_t n_int = 0, n_scan = 0, n_tread = 0, n_hits = 0;
+  if (g == 0 && i < N) {
+    int32_t sum = 0;
+    uint32_t hsum = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < kStageGroups; k++) {
+      sum = wadd(sum, red_in[k][li]);
+      hsum += red_hits[k][li];
+    }
+    if (!refr) {
+      n_scan = F;
+      n_tread = F < kMaxDendrites ? F + 1 : kMaxDendrites;
+      n_hits = hsum;
+    }
+    uint8_t flags = flags_in & 0xF1;  // K1
+    flags &= (uint8_t)~0x01;
+    soma_tick(vram, i, A, p, flags, timer, refr, tick, sum, fin, push_own, my_axon);
+    n_int = refr ? 0u : 1u;
+  }
+  bool own_first = false;
+  if (push_own) own_first = atomicAdd(&sc.push_cnt[my_axon], 1u) == 0u;
+
+  // Compaction of group 0's spikers and first owners, as in neuron_block.
+  const uint32_t lane = tid & 63u, wv = tid >> 6;
+  const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  const uint64_t bs = __ballot(fin);
+  const uint64_t bo = __ballot(own_first);
+  uint32_t r[4] = {n_int, n_scan, n_tread, n_hits};
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) r[k] += __shfl_xor(r[k], off, 64);
+  }
+  if (g == 0 && lane == 0) {
+    wave_cnt[0][wv] = (uint32_t)__popcll(bs);
+    wave_cnt[1][wv] = (uint32_t)__popcll(bo);
+#pragma unroll
+    for (int k = 0; k < 4; k++) red[wv][k] = r[k];
+  }
+  __syncthreads();
+  if (tid == 0) {
+    const uint32_t cs = wave_cnt[0][0] + wave_cnt[0][1] + wave_cnt[0][2] + wave_cnt[0][3];
+    const uint32_t co = wave_cnt[1][0] + wave_cnt[1][1] + wave_cnt[1][2] + wave_cnt[1][3];
+    blk_base[0] = cs ? atomicAdd(&sc.counters[kCntSpikes], cs) : 0u;
+    blk_base[1] = co ? atomicAdd(&sc.counters[kCntOwners], co) : 0u;
+    uint64_t* part = sc.stats_part + (size_t)blockIdx.x * kStatsPerBlock;
+    part[kStIntegrated] += red[0][0] + red[1][0] + red[2][0] + red[3][0];
+    part[kStScanned] += red[0][1] + red[1][1] + red[2][1] + red[3][1];
+    part[kStTargets] += red[0][2] + red[1][2] + red[2][2] + red[3][2];
+    part[kStHits] += red[0][3] + red[1][3] + red[2][3] + red[3][3];
+    part[kStSpikes] += cs;
+    if (blockIdx.x == 0) part[kStTicks] += 1;
+  }
+  __syncthreads();
+  if (g == 0) {
+    uint32_t pre_s = blk_base[0], pre_o = blk_base[1];
+    for (uint32_t w = 0; w < wv; w++) {
+      pre_s += wave_cnt[0][w];
+      pre_o += wave_cnt[1][w];
+    }
+    if (fin) sc.spk_list[pre_s + (uint32_t)__popcll(bs & below)] = i;
+    if (own_first) sc.own_list[pre_o + (uint32_t)__popcll(bo & below)] = my_axon;
+  }
+}
+
+}  // namespace axc

@@ -297,6 +297,18 @@ int32_t axc_step_day_phase_batch(const ShardVramPtrs *vram, uint32_t padded_n, u
 void axc_host_sort_and_prune(uint8_t *flags, uint32_t *targets, int16_t *weights, uint8_t *timers,
                              uint32_t padded_n, int16_t prune_threshold);
 
+/* Staged dendrite scan. The engine keeps a private plan derived from each
+ * shard's dendrite_targets: per block of 256 somas, the distinct presynaptic
+ * axons (whose head snapshots the soma pass stages in LDS) and each slot's
+ * block-local index and segment. The plan is rebuilt at the next step after
+ * the targets change through cu_upload_state_blob, launch_sort_and_prune or
+ * gpu_memcpy_host_to_device(_async) into the targets array. A caller that
+ * writes dendrite_targets any other way (its own copies or kernels) calls
+ * axc_shard_targets_changed before the next step. axc_staging_enable(0)
+ * switches the whole process to the direct scan (same results). */
+void axc_staging_enable(int enable);
+void axc_shard_targets_changed(const ShardVramPtrs *vram);
+
 /* Per-kernel timing, for benchmarks: when enabled, HIP events are recorded
  * on the caller's stream around each launch of cu_step_day_phase, and
  * axc_step_day_phase_batch launches its kernels one by one instead of
