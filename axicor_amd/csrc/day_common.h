@@ -85,18 +85,12 @@ struct ShardScratch {
 // (somas [256b, 256b+256)) reads the byte snapshot entries of the distinct
 // presynaptic axons of its filled slots from LDS: set[b*kStageCap + j] is the
 // axon of block-local entry j (A for targets outside [0, A)).
-#ifndef AXC_STAGE_KEY32
-#define AXC_STAGE_KEY32 0
-#endif
-// Slot keys of the staged plan as one u32 per slot (entry | segment << 16)
-// instead of a u16 entry array and a u8 segment array.
-constexpr bool kStageKey32 = AXC_STAGE_KEY32 != 0;
 constexpr uint32_t kStageCap = 8704;             // LDS entries per block (8 B each)
 constexpr uint32_t kStageOverflow = 0xFFFFFFFFu; // cnt of a block whose set does not fit
 struct StagePlan {
-  uint16_t* code;  // [128][N] block-local entry of each filled slot's axon
-  uint8_t* seg;    // [128][N] segment of each filled slot (target >> 24)
-  uint32_t* key;   // [128][N] entry | segment << 16 (instead of code and seg)
+  uint16_t* code;  // [16][N][8] block-local entry of each filled slot's axon: slot s of soma i at
+                   // ((s / 8) * N + i) * 8 + s % 8, so one soma's chunk of 8 slots is 16 contiguous bytes
+  uint8_t* seg;    // [16][N][8] segment of each filled slot (target >> 24), same order
   uint8_t* fill;   // [N] filled prefix: slots before the first empty target
   uint32_t* set;   // [blocks][kStageCap]
   uint32_t* cnt;   // [blocks] entries, or kStageOverflow

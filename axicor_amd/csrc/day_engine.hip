@@ -158,8 +158,8 @@ void free_scratch(Shard* s) {
   (void)hipFree(s->prm);
   s->sc = ShardScratch{};
   s->prm = nullptr;
-  for (void* q : {(void*)s->sp.code, (void*)s->sp.seg, (void*)s->sp.key, (void*)s->sp.fill, (void*)s->sp.set,
-                  (void*)s->sp.cnt, (void*)s->sp.comp8})
+  for (void* q : {(void*)s->sp.code, (void*)s->sp.seg, (void*)s->sp.fill, (void*)s->sp.set, (void*)s->sp.cnt,
+                  (void*)s->sp.comp8})
     if (q) (void)hipFree(q);
   s->sp = StagePlan{};
   s->stage_dirty = true;
@@ -180,13 +180,8 @@ bool ensure_stage(Shard* s, const ShardVramPtrs& v, hipStream_t st, hipError_t* 
                                 bytes);
     });
     StagePlan p{};
-    hipError_t e = hipSuccess;
-    if (kStageKey32) {
-      e = hipMalloc((void**)&p.key, kMaxDendrites * N * sizeof(uint32_t));
-    } else {
-      e = hipMalloc((void**)&p.code, kMaxDendrites * N * sizeof(uint16_t));
-      if (e == hipSuccess) e = hipMalloc((void**)&p.seg, kMaxDendrites * N);
-    }
+    hipError_t e = hipMalloc((void**)&p.code, kMaxDendrites * N * sizeof(uint16_t));
+    if (e == hipSuccess) e = hipMalloc((void**)&p.seg, kMaxDendrites * N);
     if (e == hipSuccess) e = hipMalloc((void**)&p.fill, N);
     if (e == hipSuccess) e = hipMalloc((void**)&p.set, blocks * kStageCap * sizeof(uint32_t));
     if (e == hipSuccess) e = hipMalloc((void**)&p.cnt, blocks * sizeof(uint32_t));
@@ -196,8 +191,7 @@ bool ensure_stage(Shard* s, const ShardVramPtrs& v, hipStream_t st, hipError_t* 
     if (e == hipSuccess) e = hipMemset(p.comp8, 0xFF, (A + 1) * sizeof(uint2));
     if (e == hipSuccess) e = hipDeviceSynchronize();
     if (e != hipSuccess) {
-      for (void* q : {(void*)p.code, (void*)p.seg, (void*)p.key, (void*)p.fill, (void*)p.set, (void*)p.cnt,
-                      (void*)p.comp8})
+      for (void* q : {(void*)p.code, (void*)p.seg, (void*)p.fill, (void*)p.set, (void*)p.cnt, (void*)p.comp8})
         if (q) (void)hipFree(q);
       *err = e;
       return false;

@@ -30,7 +30,17 @@
 
 namespace axc {
 
-constexpr uint32_t kStageGroups = 4;
+#ifndef AXC_STAGE_GROUPS
+#define AXC_STAGE_GROUPS 4
+#endif
+#ifndef AXC_STAGE_PREFETCH
+#define AXC_STAGE_PREFETCH 0
+#endif
+// Lane groups per 256-soma block (each scans every kStageGroups-th chunk of
+// 8 slots) and whether a group loads its next chunk before testing the
+// current one (which needs more registers than 8 waves per SIMD allow).
+constexpr uint32_t kStageGroups = AXC_STAGE_GROUPS;
+constexpr bool kStagePrefetch = AXC_STAGE_PREFETCH != 0;
 constexpr uint32_t kStageThreads = 256 * kStageGroups;
 constexpr uint32_t kHashSize = 16384;     // LDS hash set of the build kernel
 constexpr uint32_t kHashEmpty = 0xFFFFFFFFu;
@@ -38,11 +48,15 @@ constexpr uint32_t kHashMaxProbe = 256;
 
 __device__ __forceinline__ uint32_t hash_slot(uint32_t key) { return (key * 2654435761u) >> (32 - 14); }
 
-// One workgroup of 256 lanes per soma block (grid = ceil(N/256)).
+// One workgroup of 256 lanes per soma block (grid = ceil(N/256)). The
+// block's distinct axons are gathered in an LDS hash set, then sorted, so
+// that the set (and the LDS entries) are in axon order: consecutive lanes of
+// the per-tick staging loop then gather neighbouring snapshot entries, which
+// share cache lines. A slot's block-local entry is the rank of its axon.
 __global__ __launch_bounds__(256) void stage_build_kernel(const uint32_t* __restrict__ targets, uint32_t N,
                                                           uint32_t A, StagePlan sp) {
   __shared__ uint32_t keys[kHashSize];
-  __shared__ uint16_t local[kHashSize];
+  __shared__ uint32_t sorted[kHashSize];
   __shared__ uint32_t scan[256];
   __shared__ uint32_t overflow;
   const uint32_t tid = threadIdx.x, b = blockIdx.x;
@@ -67,8 +81,7 @@ __global__ __launch_bounds__(256) void stage_build_kernel(const uint32_t* __rest
     sp.fill[i] = (uint8_t)F;
   }
   __syncthreads();
-  // Block-local entries in table order: lane tid owns table positions
-  // [64 tid, 64 tid + 64).
+  // Compact the table (lane tid owns positions [64 tid, 64 tid + 64)).
   constexpr uint32_t kPer = kHashSize / 256;
   uint32_t mine = 0;
   for (uint32_t j = 0; j < kPer; j++) mine += keys[tid * kPer + j] != kHashEmpty;
@@ -82,74 +95,113 @@ __global__ __launch_bounds__(256) void stage_build_kernel(const uint32_t* __rest
   }
   const uint32_t total = scan[255];
   const bool fits = overflow == 0 && total <= kStageCap;
+  if (tid == 0) sp.cnt[b] = fits ? total : kStageOverflow;
+  if (!fits) return;  // block-uniform
   uint32_t next = scan[tid] - mine;
   for (uint32_t j = 0; j < kPer; j++) {
-    const uint32_t pos = tid * kPer + j;
-    const uint32_t key = keys[pos];
-    if (key != kHashEmpty) {
-      local[pos] = (uint16_t)(next < 0xFFFFu ? next : 0xFFFFu);
-      if (fits) sp.set[(size_t)b * kStageCap + next] = key;
-      next++;
+    const uint32_t key = keys[tid * kPer + j];
+    if (key != kHashEmpty) sorted[next++] = key;
+  }
+  // Bitonic sort of sorted[0, S), S = the next power of two >= total,
+  // padded with kHashEmpty (which sorts last).
+  uint32_t S = 1;
+  while (S < total) S <<= 1;
+  for (uint32_t j = total + tid; j < S; j += 256) sorted[j] = kHashEmpty;
+  __syncthreads();
+  for (uint32_t k = 2; k <= S; k <<= 1) {
+    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+      for (uint32_t x = tid; x < S; x += 256) {
+        const uint32_t y = x ^ j;
+        if (y > x) {
+          const uint32_t a = sorted[x], c = sorted[y];
+          const bool up = (x & k) == 0;
+          if (up ? a > c : a < c) {
+            sorted[x] = c;
+            sorted[y] = a;
+          }
+        }
+      }
+      __syncthreads();
     }
   }
-  if (tid == 0) sp.cnt[b] = fits ? total : kStageOverflow;
-  __syncthreads();
-  if (!fits || i >= N) return;
+  __syncthreads();  // also when S == 1 and the loops above did not run
+  for (uint32_t j = tid; j < total; j += 256) sp.set[(size_t)b * kStageCap + j] = sorted[j];
+  if (i >= N) return;
   for (uint32_t s = 0; s < F; s++) {
     const size_t o = (size_t)s * N + i;
     const uint32_t t = targets[o];
     const uint32_t key = snap_index(t, A);
-    uint32_t h = hash_slot(key);
-    while (keys[h] != key) h = (h + 1) & (kHashSize - 1);  // present: inserted above
-    if (kStageKey32) {
-      sp.key[o] = (uint32_t)local[h] | ((t >> 24) << 16);
-    } else {
-      sp.code[o] = local[h];
-      sp.seg[o] = (uint8_t)(t >> 24);
+    uint32_t lo = 0, hi = total;  // the key is present: sorted[lo] == key at the end
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (sorted[mid] <= key) lo = mid;
+      else hi = mid;
     }
+    const size_t c = ((size_t)(s >> 3) * N + i) * 8 + (s & 7);  // chunk-interleaved
+    sp.code[c] = (uint16_t)lo;
+    sp.seg[c] = (uint8_t)(t >> 24);
   }
 }
 
-// Slot keys. Staged: block-local entry | segment << 16 (from the plan);
-// otherwise the reference target (segment << 24 | axon + 1).
-template <bool STAGED>
-__device__ __forceinline__ uint32_t slot_key(const ShardVramPtrs& vram, const StagePlan& sp, uint32_t N, uint32_t s,
-                                             uint32_t i) {
-  if (STAGED && kStageKey32) {
-    const char* krow = reinterpret_cast<const char*>(sp.key + (size_t)s * N);
-    return __builtin_nontemporal_load(reinterpret_cast<const uint32_t*>(krow + i * 4u));
-  }
-  if (STAGED) {
-    const char* crow = reinterpret_cast<const char*>(sp.code + (size_t)s * N);
-    const char* srow = reinterpret_cast<const char*>(sp.seg + (size_t)s * N);
-    return (uint32_t)*reinterpret_cast<const uint16_t*>(crow + i * 2u) |
-           ((uint32_t)*reinterpret_cast<const uint8_t*>(srow + i) << 16);
-  }
-  const char* trow = reinterpret_cast<const char*>(vram.dendrite_targets + (size_t)s * N);
-  return __builtin_nontemporal_load(reinterpret_cast<const uint32_t*>(trow + i * 4u));
-}
-template <bool STAGED>
-__device__ __forceinline__ uint32_t key_seg(uint32_t kv) { return STAGED ? (kv >> 16) : (kv >> 24); }
-// Staged: the block-local entry; otherwise the axon's snapshot index.
-template <bool STAGED>
-__device__ __forceinline__ uint32_t key_entry(uint32_t kv, uint32_t A) {
-  return STAGED ? (kv & 0xFFFFu) : snap_index(kv, A);
-}
-template <bool STAGED>
-__device__ __forceinline__ uint32_t key_axon(uint32_t kv, const uint32_t* __restrict__ bset, uint32_t A) {
-  return STAGED ? *reinterpret_cast<const uint32_t*>(reinterpret_cast<const char*>(bset) + (kv & 0xFFFFu) * 4u)
-                : snap_index(kv, A);
+__device__ __forceinline__ char* weight_row(const ShardVramPtrs& vram, uint32_t N, uint32_t s) {
+  return reinterpret_cast<char*>(vram.dendrite_weights + (size_t)s * N);
 }
 // Snapshot entry of axon a (< 2^28): a scalar base plus a 32-bit lane offset.
 __device__ __forceinline__ uint4 snap_at(const uint4* __restrict__ snap, uint32_t a) {
   return *reinterpret_cast<const uint4*>(reinterpret_cast<const char*>(snap) + (a << 4));
 }
-__device__ __forceinline__ char* weight_row(const ShardVramPtrs& vram, uint32_t N, uint32_t s) {
-  return reinterpret_cast<char*>(vram.dendrite_weights + (size_t)s * N);
-}
+
+// One chunk of 8 slots of soma i as loaded: staged, the plan's 8 entries
+// (u16, one 16-byte load) and 8 segments (u8, one 8-byte load); otherwise
+// the 8 reference targets (segment << 24 | axon + 1). Plus the 8 weights,
+// two per register.
+template <bool STAGED>
+struct Chunk {
+  uint32_t k[STAGED ? 6 : 8];
+  uint32_t w2[4];
+  __device__ __forceinline__ void load(const ShardVramPtrs& vram, const StagePlan& sp, uint32_t N, uint32_t c,
+                                       uint32_t i) {
+    if (STAGED) {
+      const uint4 e = *reinterpret_cast<const uint4*>(reinterpret_cast<const char*>(sp.code + (size_t)c * N * 8) +
+                                                      i * 16u);
+      const uint2 g = *reinterpret_cast<const uint2*>(reinterpret_cast<const char*>(sp.seg + (size_t)c * N * 8) +
+                                                      i * 8u);
+      k[0] = e.x; k[1] = e.y; k[2] = e.z; k[3] = e.w; k[4] = g.x; k[5] = g.y;
+    } else {
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        const char* row = reinterpret_cast<const char*>(vram.dendrite_targets + (size_t)(c * 8 + u) * N);
+        k[u] = __builtin_nontemporal_load(reinterpret_cast<const uint32_t*>(row + i * 4u));
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const char* r0 = reinterpret_cast<const char*>(vram.dendrite_weights + (size_t)(c * 8 + 2 * j) * N);
+      const char* r1 = reinterpret_cast<const char*>(vram.dendrite_weights + (size_t)(c * 8 + 2 * j + 1) * N);
+      w2[j] = (uint32_t)*reinterpret_cast<const uint16_t*>(r0 + i * 2u) |
+              ((uint32_t)*reinterpret_cast<const uint16_t*>(r1 + i * 2u) << 16);
+    }
+  }
+  __device__ __forceinline__ uint32_t seg(int u) const {
+    return STAGED ? ((k[4 + (u >> 2)] >> (8 * (u & 3))) & 0xFFu) : (k[u] >> 24);
+  }
+  // Staged: the block-local entry; otherwise the axon's snapshot index.
+  __device__ __forceinline__ uint32_t entry(int u, uint32_t A) const {
+    return STAGED ? ((k[u >> 1] >> (16 * (u & 1))) & 0xFFFFu) : snap_index(k[u], A);
+  }
+  __device__ __forceinline__ uint32_t axon(int u, const uint32_t* __restrict__ bset, uint32_t A) const {
+    return STAGED ? *reinterpret_cast<const uint32_t*>(reinterpret_cast<const char*>(bset) + entry(u, A) * 4u)
+                  : snap_index(k[u], A);
+  }
+  __device__ __forceinline__ int32_t weight(int u) const {
+    return (int32_t)(int16_t)(uint16_t)(w2[u >> 1] >> (16 * (u & 1)));
+  }
+};
 
 // The hit sum of soma i over the slots of group gu: chunks of 8 slots
-// starting at 8 ((gu + 3) % 4 + 4k), below the soma's filled prefix F.
+// c = (gu + G - 1) % G + G k below the soma's filled prefix F (G groups);
+// with kStagePrefetch the next chunk's loads are issued before the current
+// chunk is tested.
 template <bool STAGED>
 __device__ __forceinline__ void scan_slots(const ShardVramPtrs& vram, uint32_t N, uint32_t A, const StagePlan& sp,
                                            const uint2* ent, const uint32_t* __restrict__ bset,
@@ -158,57 +210,54 @@ __device__ __forceinline__ void scan_slots(const ShardVramPtrs& vram, uint32_t N
   constexpr int U = 8;
   // Group 0, which finishes the somas afterwards, takes chunks 3, 7, 11, ...
   // so that a partly filled last chunk lands on another group.
-  const uint32_t c0 = (gu + kStageGroups - 1) % kStageGroups;
+  const uint32_t cg = (gu + kStageGroups - 1) % kStageGroups;
+  constexpr bool kPipe = STAGED && kStagePrefetch;
+  Chunk<STAGED> now;
+  if (kPipe && cg * U < F) now.load(vram, sp, N, cg, i);
 #pragma unroll 1
-  for (uint32_t k = 0; k < kMaxDendrites / (U * kStageGroups); k++) {
-    const uint32_t s0 = (c0 + kStageGroups * k) * U;
-    if (s0 >= F) break;
+  for (uint32_t c = cg; c * U < F; c += kStageGroups) {
+    const uint32_t s0 = c * U;
     const uint32_t nv = F - s0 < (uint32_t)U ? F - s0 : (uint32_t)U;
-    uint32_t key[U];
-    uint32_t w2[U / 2];  // weights of slots 2j (low half) and 2j+1 (high half)
-#pragma unroll
-    for (int u = 0; u < U; u++) key[u] = slot_key<STAGED>(vram, sp, N, s0 + u, i);
-#pragma unroll
-    for (int j = 0; j < U / 2; j++) {
-      const uint32_t lo = *reinterpret_cast<const uint16_t*>(weight_row(vram, N, s0 + 2 * j) + i * 2u);
-      const uint32_t hi = *reinterpret_cast<const uint16_t*>(weight_row(vram, N, s0 + 2 * j + 1) + i * 2u);
-      w2[j] = lo | (hi << 16);
-    }
+    Chunk<STAGED> nxt;
+    const bool more = kPipe && s0 + kStageGroups * U < F;
+    if (!kPipe) now.load(vram, sp, N, c, i);
+    else if (more) nxt.load(vram, sp, N, c + kStageGroups, i);
     uint32_t hit_mask = 0, wide = 0;
     constexpr int H = 4;  // entries read in two halves of four
 #pragma unroll
     for (int h0 = 0; h0 < U; h0 += H) {
-      uint2 c[H];
+      uint2 e[H];
 #pragma unroll
       for (int u = 0; u < H; u++) {
         const bool live = (uint32_t)(h0 + u) < nv;
-        if (STAGED) c[u] = ent[live ? key_entry<true>(key[h0 + u], A) : kStageCap];
-        else c[u] = sp.comp8[live ? key_entry<false>(key[h0 + u], A) : A];
+        if (STAGED) e[u] = ent[live ? now.entry(h0 + u, A) : kStageCap];
+        else e[u] = sp.comp8[live ? now.entry(h0 + u, A) : A];
       }
 #pragma unroll
       for (int u = 0; u < H; u++) {
         const bool live = (uint32_t)(h0 + u) < nv;
-        const uint32_t sgv = key_seg<STAGED>(key[h0 + u]);
-        hit_mask |= (live && min_dist8(c[u], sgv) < prop) ? (1u << (h0 + u)) : 0u;
+        const uint32_t sgv = now.seg(h0 + u);
+        hit_mask |= (live && min_dist8(e[u], sgv) < prop) ? (1u << (h0 + u)) : 0u;
         wide |= (live && sgv + prop > 255u) ? (1u << (h0 + u)) : 0u;
       }
     }
     if (wide) {
-      // Segment + length beyond the byte form: the 16-byte compact entry.
+      // Segment + length beyond the byte form (rare): the slot's reference
+      // target and the 16-byte compact entry (scalar bases, lane offsets).
 #pragma unroll
       for (int u = 0; u < U; u++) {
         if (wide & (1u << u)) {
-          const bool h = min_dist16(snap_at(cur, key_axon<STAGED>(key[u], bset, A)), key_seg<STAGED>(key[u])) < prop;
+          const char* trow = reinterpret_cast<const char*>(vram.dendrite_targets + (size_t)(s0 + u) * N);
+          const uint32_t t = *reinterpret_cast<const uint32_t*>(trow + i * 4u);
+          const bool h = min_dist16(snap_at(cur, snap_index(t, A)), t >> 24) < prop;
           hit_mask = h ? (hit_mask | (1u << u)) : (hit_mask & ~(1u << u));
         }
       }
     }
 #pragma unroll
-    for (int u = 0; u < U; u++) {
-      const int32_t wu = (int32_t)(int16_t)(uint16_t)(w2[u >> 1] >> (16 * (u & 1)));
-      i_in = wadd(i_in, (hit_mask >> u) & 1u ? wu : 0);
-    }
+    for (int u = 0; u < U; u++) i_in = wadd(i_in, (hit_mask >> u) & 1u ? now.weight(u) : 0);
     hits += __builtin_popcount(hit_mask);
+    if (more) now = nxt;
   }
 }
 
@@ -238,7 +287,9 @@ __device__ __forceinline__ void stage_entries(uint2* ent, const uint32_t* __rest
 
 // K1 + K5 of one tick for a block of 256 somas, 1024 lanes (see the file
 // comment). The tick's GSOP (K6) is left to gsop_kernel, after the pushes.
-__global__ __launch_bounds__(kStageThreads) __attribute__((amdgpu_waves_per_eu(8, 8))) void neuron_staged_kernel(
+// Two blocks per CU: 2 * kStageThreads / 64 waves over 4 SIMDs.
+__global__ __launch_bounds__(kStageThreads) __attribute__((amdgpu_waves_per_eu(2 * kStageGroups, 2 * kStageGroups)))
+void neuron_staged_kernel(
     ShardVramPtrs vram, uint32_t N, uint32_t A, const AxcVariantParameters* __restrict__ lut_g,
     const TickParams* __restrict__ prm, uint32_t tick_arg, ShardScratch sc, const uint4* __restrict__ cur,
     StagePlan sp) {
