@@ -338,12 +338,12 @@ hipError_t enqueue_tick(const ShardVramPtrs& v, Shard* s, uint32_t v_seg, uint32
   }
   {
     TimedScope ts(kKPush, st, timed);
-    push_apply_kernel<<<kListBlocks / 4, kThreads, 0, st>>>(v.axon_heads, s->sc, cur);
+    push_apply_kernel<<<kListBlocks / 4, kThreads, 0, st>>>(v.axon_heads, s->sc, cur, staged ? s->sp.comp8 : nullptr);
   }
   if (gsop_now) {
     TimedScope ts(kKGsop, st, timed);
     gsop_kernel<<<kListBlocks, kThreads, 0, st>>>(v, N, A, lut, prm, dopamine, s->sc, cur,
-                                                  staged ? s->sp.fill : nullptr);
+                                                  staged ? s->sp.fill : nullptr, staged ? s->sp.comp8 : nullptr);
   }
   if (n_out > 0 && mapped != nullptr && out != nullptr) {
     TimedScope ts(kKReadout, st, timed);

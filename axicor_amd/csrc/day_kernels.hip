@@ -532,8 +532,10 @@ __global__ __launch_bounds__(256) AXC_NEURON_ATTR void neuron_tick_kernel(ShardV
 // Applies the counted pushes of this tick's spikes (physics.cu:253-262 with
 // the snapshot rule). One lane per listed axon; clears the count and refreshes
 // the axon's compact snapshot.
+// comp8 (may be null): the byte snapshot, refreshed too once the soma pass
+// that read this tick's byte entries is done (for gsop_kernel).
 __device__ __forceinline__ void push_apply_one(AxcBurstHeads8* __restrict__ heads, const ShardScratch& sc,
-                                               uint4* __restrict__ comp, uint32_t j) {
+                                               uint4* __restrict__ comp, uint32_t j, uint2* __restrict__ comp8 = nullptr) {
   uint32_t a = sc.own_list[j];
   uint32_t c = sc.push_cnt[a];
   sc.push_cnt[a] = 0;
@@ -541,13 +543,14 @@ __device__ __forceinline__ void push_apply_one(AxcBurstHeads8* __restrict__ head
   push_heads(h, c > 8 ? 8 : c);
   store_heads(heads, a, h);
   if (comp) comp[a] = compact_heads(h);
+  if (comp8) comp8[a] = byte_heads(h);
 }
 
 __global__ __launch_bounds__(256) void push_apply_kernel(AxcBurstHeads8* __restrict__ heads, ShardScratch sc,
-                                                         uint4* __restrict__ comp) {
+                                                         uint4* __restrict__ comp, uint2* __restrict__ comp8 = nullptr) {
   const uint32_t n = sc.counters[kCntOwners];
   for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x)
-    push_apply_one(heads, sc, comp, j);
+    push_apply_one(heads, sc, comp, j, comp8);
 }
 
 // ---------------------------------------------------------------------------
@@ -582,10 +585,17 @@ __device__ __forceinline__ uint32_t gsop_soma(const ShardVramPtrs& vram, uint32_
 }
 
 // GSOP of spiking soma i with its filled prefix F known (the staged plan's
-// fill): lane l takes slots l and l + 64 at once.
+// fill): lane l takes slots l and l + 64 at once. comp8: the byte snapshot
+// after the pushes (push_apply refreshed it), exact for segment + length
+// <= 255; other slots read the compact form.
+__device__ __forceinline__ uint32_t gsop_dist(const uint4* __restrict__ comp, const uint2* __restrict__ comp8,
+                                              uint32_t a, uint32_t seg, uint32_t len) {
+  return seg + len > 255u ? min_dist16(comp[a], seg) : min_dist8(comp8[a], seg);
+}
 __device__ __forceinline__ void gsop_soma_filled(const ShardVramPtrs& vram, uint32_t N, uint32_t A,
                                                  const AxcVariantParameters* lut, int32_t dop, uint32_t i, uint32_t F,
-                                                 const uint4* __restrict__ comp, uint32_t lane) {
+                                                 const uint4* __restrict__ comp, const uint2* __restrict__ comp8,
+                                                 uint32_t lane) {
   const uint8_t flags = vram.soma_flags[i];
   const AxcVariantParameters& p = lut[(flags >> 4) & 0x0F];
   const GsopTerms g = gsop_terms(flags, p, dop);
@@ -593,20 +603,21 @@ __device__ __forceinline__ void gsop_soma_filled(const ShardVramPtrs& vram, uint
   const size_t c0 = (size_t)lane * N + i, c1 = (size_t)(lane + 64) * N + i;
   const uint32_t t0 = l0 ? vram.dendrite_targets[c0] : 0u, t1 = l1 ? vram.dendrite_targets[c1] : 0u;
   const int16_t w0 = l0 ? vram.dendrite_weights[c0] : (int16_t)0, w1 = l1 ? vram.dendrite_weights[c1] : (int16_t)0;
-  const uint4 q0 = comp[l0 ? snap_index(t0, A) : A], q1 = comp[l1 ? snap_index(t1, A) : A];
-  if (l0) vram.dendrite_weights[c0] = gsop_weight(w0, min_dist16(q0, t0 >> 24), g.len, g.raw_pot, g.final_dep, g.bm,
-                                                 p.inertia_curve);
-  if (l1) vram.dendrite_weights[c1] = gsop_weight(w1, min_dist16(q1, t1 >> 24), g.len, g.raw_pot, g.final_dep, g.bm,
-                                                 p.inertia_curve);
+  const uint32_t d0 = gsop_dist(comp, comp8, l0 ? snap_index(t0, A) : A, t0 >> 24, g.len);
+  const uint32_t d1 = gsop_dist(comp, comp8, l1 ? snap_index(t1, A) : A, t1 >> 24, g.len);
+  if (l0) vram.dendrite_weights[c0] = gsop_weight(w0, d0, g.len, g.raw_pot, g.final_dep, g.bm, p.inertia_curve);
+  if (l1) vram.dendrite_weights[c1] = gsop_weight(w1, d1, g.len, g.raw_pot, g.final_dep, g.bm, p.inertia_curve);
 }
 
 // fill: the filled prefix of every soma (the staged plan's), or null to find
-// the first empty target of each spiker by ballot.
+// the first empty target of each spiker by ballot. With fill, comp8 is the
+// post-push byte snapshot.
 __global__ __launch_bounds__(256) void gsop_kernel(ShardVramPtrs vram, uint32_t N, uint32_t A,
                                                    const AxcVariantParameters* __restrict__ lut_g,
                                                    const TickParams* __restrict__ prm, int32_t dop_arg,
                                                    ShardScratch sc, const uint4* __restrict__ comp,
-                                                   const uint8_t* __restrict__ fill = nullptr) {
+                                                   const uint8_t* __restrict__ fill = nullptr,
+                                                   const uint2* __restrict__ comp8 = nullptr) {
   __shared__ AxcVariantParameters lut[16];
   __shared__ uint32_t red[4][2];
   const uint32_t tid = threadIdx.x;
@@ -624,7 +635,7 @@ __global__ __launch_bounds__(256) void gsop_kernel(ShardVramPtrs vram, uint32_t 
     uint32_t filled;
     if (fill) {
       filled = fill[i];
-      gsop_soma_filled(vram, N, A, lut, dop, i, filled, comp, lane);
+      gsop_soma_filled(vram, N, A, lut, dop, i, filled, comp, comp8, lane);
     } else {
       filled = gsop_soma(vram, N, A, lut, dop, i, comp, lane);
     }

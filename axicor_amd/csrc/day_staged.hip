@@ -30,17 +30,10 @@
 
 namespace axc {
 
-#ifndef AXC_STAGE_GROUPS
-#define AXC_STAGE_GROUPS 4
-#endif
-#ifndef AXC_STAGE_PREFETCH
-#define AXC_STAGE_PREFETCH 0
-#endif
-// Lane groups per 256-soma block (each scans every kStageGroups-th chunk of
-// 8 slots) and whether a group loads its next chunk before testing the
-// current one (which needs more registers than 8 waves per SIMD allow).
-constexpr uint32_t kStageGroups = AXC_STAGE_GROUPS;
-constexpr bool kStagePrefetch = AXC_STAGE_PREFETCH != 0;
+// Lane groups per 256-soma block; each scans every kStageGroups-th chunk of
+// 8 slots. (Two groups at half the occupancy, with the next chunk's loads
+// issued early, measured slower.)
+constexpr uint32_t kStageGroups = 4;
 constexpr uint32_t kStageThreads = 256 * kStageGroups;
 constexpr uint32_t kHashSize = 16384;     // LDS hash set of the build kernel
 constexpr uint32_t kHashEmpty = 0xFFFFFFFFu;
@@ -199,9 +192,7 @@ struct Chunk {
 };
 
 // The hit sum of soma i over the slots of group gu: chunks of 8 slots
-// c = (gu + G - 1) % G + G k below the soma's filled prefix F (G groups);
-// with kStagePrefetch the next chunk's loads are issued before the current
-// chunk is tested.
+// c = (gu + 3) % 4 + 4k below the soma's filled prefix F.
 template <bool STAGED>
 __device__ __forceinline__ void scan_slots(const ShardVramPtrs& vram, uint32_t N, uint32_t A, const StagePlan& sp,
                                            const uint2* ent, const uint32_t* __restrict__ bset,
@@ -211,17 +202,12 @@ __device__ __forceinline__ void scan_slots(const ShardVramPtrs& vram, uint32_t N
   // Group 0, which finishes the somas afterwards, takes chunks 3, 7, 11, ...
   // so that a partly filled last chunk lands on another group.
   const uint32_t cg = (gu + kStageGroups - 1) % kStageGroups;
-  constexpr bool kPipe = STAGED && kStagePrefetch;
-  Chunk<STAGED> now;
-  if (kPipe && cg * U < F) now.load(vram, sp, N, cg, i);
 #pragma unroll 1
   for (uint32_t c = cg; c * U < F; c += kStageGroups) {
     const uint32_t s0 = c * U;
     const uint32_t nv = F - s0 < (uint32_t)U ? F - s0 : (uint32_t)U;
-    Chunk<STAGED> nxt;
-    const bool more = kPipe && s0 + kStageGroups * U < F;
-    if (!kPipe) now.load(vram, sp, N, c, i);
-    else if (more) nxt.load(vram, sp, N, c + kStageGroups, i);
+    Chunk<STAGED> now;
+    now.load(vram, sp, N, c, i);
     uint32_t hit_mask = 0, wide = 0;
     constexpr int H = 4;  // entries read in two halves of four
 #pragma unroll
@@ -257,7 +243,6 @@ __device__ __forceinline__ void scan_slots(const ShardVramPtrs& vram, uint32_t N
 #pragma unroll
     for (int u = 0; u < U; u++) i_in = wadd(i_in, (hit_mask >> u) & 1u ? now.weight(u) : 0);
     hits += __builtin_popcount(hit_mask);
-    if (more) now = nxt;
   }
 }
 
