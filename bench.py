@@ -346,7 +346,8 @@ def main() -> int:
             "padded_n": N, "total_axons": A, "ticks_per_step": T, "parallelism": "single GPU",
         },
         "roofline": {
-            "bound": "hbm", "kernel": "neuron_tick (previous tick's GSOP fused)", "achieved": achieved,
+            "bound": "hbm", "kernel": "neuron_tick (K1 + K5: staged dendrite scan, GLIF soma update)",
+            "achieved": achieved,
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
         },
         "ticks_per_s": ticks_timed / elapsed,
