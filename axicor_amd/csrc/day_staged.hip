@@ -148,6 +148,9 @@ __device__ __forceinline__ uint4 snap_at(const uint4* __restrict__ snap, uint32_
 // (u16, one 16-byte load) and 8 segments (u8, one 8-byte load); otherwise
 // the 8 reference targets (segment << 24 | axon + 1). Plus the 8 weights,
 // two per register.
+typedef uint32_t v4u __attribute__((ext_vector_type(4)));
+typedef uint32_t v2u __attribute__((ext_vector_type(2)));
+
 template <bool STAGED>
 struct Chunk {
   uint32_t k[STAGED ? 6 : 8];
@@ -155,10 +158,12 @@ struct Chunk {
   __device__ __forceinline__ void load(const ShardVramPtrs& vram, const StagePlan& sp, uint32_t N, uint32_t c,
                                        uint32_t i) {
     if (STAGED) {
-      const uint4 e = *reinterpret_cast<const uint4*>(reinterpret_cast<const char*>(sp.code + (size_t)c * N * 8) +
-                                                      i * 16u);
-      const uint2 g = *reinterpret_cast<const uint2*>(reinterpret_cast<const char*>(sp.seg + (size_t)c * N * 8) +
-                                                      i * 8u);
+      // Streamed once per tick: nontemporal, so that the byte snapshot and
+      // the block sets the staging gathers read keep their places in L2.
+      const v4u e = __builtin_nontemporal_load(
+          reinterpret_cast<const v4u*>(reinterpret_cast<const char*>(sp.code + (size_t)c * N * 8) + i * 16u));
+      const v2u g = __builtin_nontemporal_load(
+          reinterpret_cast<const v2u*>(reinterpret_cast<const char*>(sp.seg + (size_t)c * N * 8) + i * 8u));
       k[0] = e.x; k[1] = e.y; k[2] = e.z; k[3] = e.w; k[4] = g.x; k[5] = g.y;
     } else {
 #pragma unroll
@@ -171,8 +176,8 @@ struct Chunk {
     for (int j = 0; j < 4; j++) {
       const char* r0 = reinterpret_cast<const char*>(vram.dendrite_weights + (size_t)(c * 8 + 2 * j) * N);
       const char* r1 = reinterpret_cast<const char*>(vram.dendrite_weights + (size_t)(c * 8 + 2 * j + 1) * N);
-      w2[j] = (uint32_t)*reinterpret_cast<const uint16_t*>(r0 + i * 2u) |
-              ((uint32_t)*reinterpret_cast<const uint16_t*>(r1 + i * 2u) << 16);
+      w2[j] = (uint32_t)__builtin_nontemporal_load(reinterpret_cast<const uint16_t*>(r0 + i * 2u)) |
+              ((uint32_t)__builtin_nontemporal_load(reinterpret_cast<const uint16_t*>(r1 + i * 2u)) << 16);
     }
   }
   __device__ __forceinline__ uint32_t seg(int u) const {
