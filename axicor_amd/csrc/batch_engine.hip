@@ -25,6 +25,7 @@ struct BrainDev {
   ShardScratch sc;
   uint64_t* stats;                  // [kStatsPerBlock] accumulated with atomics
   uint64_t* type_spikes;            // [16] spikes per variant (flags >> 4)
+  StagePlan sp;                     // staged soma pass (comp8 null when the batch does not stage)
 };
 
 // Work item: brain index and first element (axon or soma) of a 256 chunk.
@@ -52,7 +53,7 @@ __global__ __launch_bounds__(256) void batch_axon_kernel(const BrainDev* __restr
     if (a >= b.A) continue;
     const uint32_t* bm = b.bitmask ? b.bitmask + (size_t)(tick % b.input_ticks) * b.words : nullptr;
     axon_step(b.vram.axon_heads, a, b.v_seg, bm, b.virt_off, bm ? b.n_virt : 0u, b.sc.push_cnt, false,
-              b.sc.comp[snap & 1]);
+              b.sc.comp[snap & 1], b.sp.comp8);
   }
 }
 
@@ -100,7 +101,7 @@ __global__ __launch_bounds__(256) void batch_push_kernel(const BrainDev* __restr
   for (uint32_t bi = 0; bi < n_brains; bi++) {
     const BrainDev& b = brains[bi];
     const uint32_t n = b.sc.counters[kCntOwners];
-    for (uint32_t j = gt; j < n; j += gs) push_apply_one(b.vram.axon_heads, b.sc, b.sc.comp[snap & 1], j);
+    for (uint32_t j = gt; j < n; j += gs) push_apply_one(b.vram.axon_heads, b.sc, b.sc.comp[snap & 1], j, b.sp.comp8);
   }
 }
 
@@ -139,6 +140,93 @@ __global__ __launch_bounds__(256) void batch_gsop_kernel(const BrainDev* __restr
 }
 
 __global__ void batch_set_tick_kernel(BatchTick* __restrict__ bt, uint32_t tick_base) { bt->tick_base = tick_base; }
+
+// Staged soma phase over all brains (staged_block per work item of 256
+// somas; the items of one XCD are a contiguous eighth of the list). GSOP
+// counts are taken here, for batch_gsop_flat_kernel.
+__global__ __launch_bounds__(kStageThreads) __attribute__((amdgpu_waves_per_eu(2 * kStageGroups, 2 * kStageGroups)))
+void batch_staged_kernel(const BrainDev* __restrict__ brains, const WorkItem* __restrict__ items, uint32_t n_items,
+                         const BatchTick* __restrict__ bt, uint32_t t_off, uint32_t snap) {
+  __shared__ AxcVariantParameters lut[16];
+  const uint32_t tick = bt->tick_base + t_off;
+  const uint32_t per_xcd_blocks = gridDim.x / kNumXcd;
+  const uint32_t xcd = blockIdx.x % kNumXcd, k = blockIdx.x / kNumXcd;
+  const uint32_t span = (n_items + kNumXcd - 1) / kNumXcd;
+  const uint32_t lo = xcd * span, hi = min(n_items, lo + span);
+  for (uint32_t it = lo + k; it < hi; it += per_xcd_blocks) {
+    const WorkItem w = items[it];
+    const BrainDev& b = brains[w.brain];
+    __syncthreads();  // the previous item is done with the LDS table and entries
+    if (threadIdx.x < 64)
+      reinterpret_cast<uint4*>(lut)[threadIdx.x] = reinterpret_cast<const uint4*>(b.lut)[threadIdx.x];
+    staged_block(b.vram, b.N, b.A, lut, tick, b.sc, b.sc.comp[snap & 1], b.sp, w.start >> 8, b.stats, true,
+                 w.start == 0, b.type_spikes, true);
+  }
+}
+
+// Exclusive prefix of the brains' list counts of this tick (one workgroup):
+// counter kCntSpikes or kCntOwners of each brain's workspace.
+__global__ __launch_bounds__(1024) void batch_spike_prefix_kernel(const BrainDev* __restrict__ brains,
+                                                                  uint32_t n_brains, uint32_t* __restrict__ prefix,
+                                                                  uint32_t counter = kCntSpikes) {
+  __shared__ uint32_t part[1024];
+  uint32_t carry = 0;
+  for (uint32_t b0 = 0; b0 < n_brains; b0 += 1024) {
+    const uint32_t bi = b0 + threadIdx.x;
+    const uint32_t v = bi < n_brains ? brains[bi].sc.counters[counter] : 0u;
+    part[threadIdx.x] = v;
+    __syncthreads();
+    for (uint32_t off = 1; off < 1024; off <<= 1) {
+      const uint32_t x = threadIdx.x >= off ? part[threadIdx.x - off] : 0u;
+      __syncthreads();
+      part[threadIdx.x] += x;
+      __syncthreads();
+    }
+    if (bi < n_brains) prefix[bi] = carry + part[threadIdx.x] - v;
+    carry += part[1023];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) prefix[n_brains] = carry;
+}
+
+// The brain whose range [prefix[b], prefix[b+1]) holds k (k < prefix[n]).
+__device__ __forceinline__ uint32_t brain_of(const uint32_t* __restrict__ prefix, uint32_t n_brains, uint32_t k) {
+  uint32_t lo = 0, hi = n_brains;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (prefix[mid] <= k) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// Push phase over the flattened owner lists of all brains (prefix of the
+// owner counts from batch_spike_prefix_kernel).
+__global__ __launch_bounds__(256) void batch_push_flat_kernel(const BrainDev* __restrict__ brains, uint32_t n_brains,
+                                                              const uint32_t* __restrict__ prefix, uint32_t snap) {
+  const uint32_t total = prefix[n_brains];
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < total; k += gridDim.x * blockDim.x) {
+    const uint32_t bi = brain_of(prefix, n_brains, k);
+    const BrainDev& b = brains[bi];
+    push_apply_one(b.vram.axon_heads, b.sc, b.sc.comp[snap & 1], k - prefix[bi], b.sp.comp8);
+  }
+}
+
+// GSOP of this tick for every brain's spikers, one wave per spiker over the
+// flattened spike lists (prefix from batch_spike_prefix_kernel), with each
+// brain's filled prefixes and post-push byte snapshot.
+__global__ __launch_bounds__(256) void batch_gsop_flat_kernel(const BrainDev* __restrict__ brains, uint32_t n_brains,
+                                                              const uint32_t* __restrict__ prefix, uint32_t snap) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t gw = blockIdx.x * 4 + (threadIdx.x >> 6), W = gridDim.x * 4;
+  const uint32_t total = prefix[n_brains];
+  for (uint32_t k = gw; k < total; k += W) {
+    const uint32_t bi = brain_of(prefix, n_brains, k);
+    const BrainDev& b = brains[bi];
+    const uint32_t i = b.sc.spk_list[k - prefix[bi]];
+    gsop_soma_filled(b.vram, b.N, b.A, b.lut, b.dop, i, b.sp.fill[i], b.sc.comp[snap & 1], b.sp.comp8, lane);
+  }
+}
 
 // Synapse statistics per brain, one workgroup per soma work item: over the
 // slots with a target, the count, sum |w|, sum w^2 and the count with

@@ -103,6 +103,9 @@ struct Shard {
   // step, and whether the targets changed since it was built.
   StagePlan sp{};
   std::atomic<bool> stage_dirty{true};
+  // Bumped whenever the targets may have changed (for plans kept elsewhere:
+  // the batches that include this shard's arrays).
+  std::atomic<uint64_t> targets_version{1};
 };
 
 // Staged scan on/off for the whole process (on by default).
@@ -178,6 +181,7 @@ bool ensure_stage(Shard* s, const ShardVramPtrs& v, hipStream_t st, hipError_t* 
       const int bytes = (int)((kStageCap + 1) * sizeof(uint2));
       (void)hipFuncSetAttribute((const void*)neuron_staged_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 bytes);
+      (void)hipFuncSetAttribute((const void*)batch_staged_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
     });
     StagePlan p{};
     hipError_t e = hipMalloc((void**)&p.code, kMaxDendrites * N * sizeof(uint16_t));
@@ -219,14 +223,27 @@ void note_device_write(const void* dst, size_t size) {
     Shard* s = kv.second;
     const char* t0 = (const char*)s->state_base + (size_t)s->N * 14;
     const char* t1 = t0 + (size_t)s->N * 4 * kMaxDendrites;
-    if (a < t1 && a + size > t0) s->stage_dirty = true;
+    if (a < t1 && a + size > t0) {
+      s->stage_dirty = true;
+      s->targets_version++;
+    }
   }
 }
 
 void mark_stage_dirty(const void* state_base) {
   std::lock_guard<std::mutex> lk(g_mu);
   auto it = g_shards.find(state_base);
-  if (it != g_shards.end()) it->second->stage_dirty = true;
+  if (it != g_shards.end()) {
+    it->second->stage_dirty = true;
+    it->second->targets_version++;
+  }
+}
+
+// Targets version of the registered shard at state_base (0: not registered).
+uint64_t targets_version_of(const void* state_base) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  auto it = g_shards.find(state_base);
+  return it != g_shards.end() ? it->second->targets_version.load() : 0;
 }
 
 hipError_t alloc_scratch(Shard* s) {
@@ -1010,6 +1027,11 @@ struct Batch {
   hipGraph_t graph = nullptr;
   uint32_t graph_T = 0;
   AxcBatchTuning graph_tune{0, 0, 0};
+  // Staged soma pass: each brain's plan lives in its workspace (shards[k]);
+  // built_version[k] is the targets version it was built from.
+  bool staged = false;
+  uint32_t* d_prefix = nullptr;  // [2][n+1] spike- and owner-count prefixes of the current tick
+  std::vector<uint64_t> built_version;
 };
 
 hipError_t batch_capture(Batch* b, uint32_t T) {
@@ -1023,6 +1045,20 @@ hipError_t batch_capture(Batch* b, uint32_t T) {
   const uint32_t n = (uint32_t)b->desc.size();
   e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
   if (e == hipSuccess) {
+    if (b->staged) {
+      // Per tick: axons, staged somas, pushes, GSOP after the pushes.
+      const size_t lds = (kStageCap + 1) * sizeof(uint2);
+      for (uint32_t t = 0; t < T; t++) {
+        batch_axon_kernel<<<b->tune.axon_blocks, kThreads, 0, cap>>>(b->d_brains, b->d_axon_items, b->n_axon_items,
+                                                                     b->d_tick, t, t);
+        batch_staged_kernel<<<b->tune.soma_blocks, kStageThreads, lds, cap>>>(b->d_brains, b->d_soma_items,
+                                                                             b->n_soma_items, b->d_tick, t, t);
+        batch_spike_prefix_kernel<<<1, 1024, 0, cap>>>(b->d_brains, n, b->d_prefix + n + 1, kCntOwners);
+        batch_push_flat_kernel<<<b->tune.list_blocks, kThreads, 0, cap>>>(b->d_brains, n, b->d_prefix + n + 1, t);
+        batch_spike_prefix_kernel<<<1, 1024, 0, cap>>>(b->d_brains, n, b->d_prefix, kCntSpikes);
+        batch_gsop_flat_kernel<<<b->tune.list_blocks * 8, kThreads, 0, cap>>>(b->d_brains, n, b->d_prefix, t);
+      }
+    } else {
     for (uint32_t t = 0; t < T; t++) {
       batch_axon_kernel<<<b->tune.axon_blocks, kThreads, 0, cap>>>(b->d_brains, b->d_axon_items, b->n_axon_items,
                                                                    b->d_tick, t, t);
@@ -1035,6 +1071,7 @@ hipError_t batch_capture(Batch* b, uint32_t T) {
       batch_push_kernel<<<b->tune.list_blocks, kThreads, 0, cap>>>(b->d_brains, n, t);
     }
     batch_gsop_kernel<<<b->tune.list_blocks, kThreads, 0, cap>>>(b->d_brains, n, T - 1);
+    }
     e = hipGetLastError();
     hipGraph_t g = nullptr;
     hipError_t e2 = hipStreamEndCapture(cap, &g);
@@ -1079,6 +1116,7 @@ void batch_free(Batch* b) {
   (void)hipFree(b->d_prune);
   (void)hipFree(b->d_axon_items);
   (void)hipFree(b->d_soma_items);
+  (void)hipFree(b->d_prefix);
   (void)hipFree(b->d_tick);
   delete b;
 }
@@ -1110,6 +1148,13 @@ int32_t axc_batch_create(const AxcBrainDesc* brains, uint32_t n, void** out) {
     s->device = b->device;
     b->shards.push_back(s);
     if ((e = alloc_scratch(s)) != hipSuccess) break;
+    // Staging plan of this brain, built now from its current targets.
+    if (g_staging.load() && d.padded_n > 0) {
+      ensure_stage(s, d.vram, nullptr, &e);
+      if (e != hipSuccess) break;
+      b->staged = true;
+    }
+    b->built_version.push_back(targets_version_of(d.vram.soma_voltage));
     for (uint32_t a = 0; a < d.total_axons; a += kThreads) ax.push_back(WorkItem{k, a});
     for (uint32_t i = 0; i < d.padded_n; i += kThreads) so.push_back(WorkItem{k, i});
   }
@@ -1149,8 +1194,10 @@ int32_t axc_batch_create(const AxcBrainDesc* brains, uint32_t n, void** out) {
       h.sc = b->shards[k]->sc;
       h.stats = b->d_stats + (size_t)k * kStatsPerBlock;
       h.type_spikes = b->d_type_spikes + (size_t)k * 16;
+      h.sp = b->shards[k]->sp;
     }
   }
+  if (e == hipSuccess) e = hipMalloc((void**)&b->d_prefix, (size_t)2 * (n + 1) * sizeof(uint32_t));
   if (e == hipSuccess) e = hipMalloc((void**)&b->d_brains, n * sizeof(BrainDev));
   if (e == hipSuccess) e = hipMemcpy(b->d_brains, b->host_dev.data(), n * sizeof(BrainDev), hipMemcpyHostToDevice);
   if (e != hipSuccess) {
@@ -1168,7 +1215,23 @@ int32_t axc_batch_create(const AxcBrainDesc* brains, uint32_t n, void** out) {
 
 int32_t axc_batch_step(void* h, uint32_t tick_base, uint32_t ticks, void* stream) {
   if (!h || ticks == 0) return (int32_t)hipErrorInvalidValue;
-  return (int32_t)batch_run((Batch*)h, tick_base, ticks, (hipStream_t)stream);
+  Batch* b = (Batch*)h;
+  hipStream_t st = (hipStream_t)stream;
+  if (b->staged) {
+    // Rebuild the plans of brains whose targets changed since (Night uploads,
+    // sort&prune, state uploads).
+    for (size_t k = 0; k < b->desc.size(); k++) {
+      const AxcBrainDesc& d = b->desc[k];
+      const uint64_t v = targets_version_of(d.vram.soma_voltage);
+      if (v == b->built_version[k] || d.padded_n == 0) continue;
+      stage_build_kernel<<<blocks_for(d.padded_n), kThreads, 0, st>>>(d.vram.dendrite_targets, d.padded_n,
+                                                                      d.total_axons, b->shards[k]->sp);
+      hipError_t e = hipGetLastError();
+      if (e != hipSuccess) return (int32_t)e;
+      b->built_version[k] = v;
+    }
+  }
+  return (int32_t)batch_run(b, tick_base, ticks, st);
 }
 
 int32_t axc_batch_set_tuning(void* h, const AxcBatchTuning* t) {

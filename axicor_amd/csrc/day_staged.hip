@@ -275,26 +275,26 @@ __device__ __forceinline__ void stage_entries(uint2* ent, const uint32_t* __rest
   if (tid == 0) ent[kStageCap] = make_uint2(0xFFFFFFFFu, 0xFFFFFFFFu);
 }
 
-// K1 + K5 of one tick for a block of 256 somas, 1024 lanes (see the file
-// comment). The tick's GSOP (K6) is left to gsop_kernel, after the pushes.
-// Two blocks per CU: 2 * kStageThreads / 64 waves over 4 SIMDs.
-__global__ __launch_bounds__(kStageThreads) __attribute__((amdgpu_waves_per_eu(2 * kStageGroups, 2 * kStageGroups)))
-void neuron_staged_kernel(
-    ShardVramPtrs vram, uint32_t N, uint32_t A, const AxcVariantParameters* __restrict__ lut_g,
-    const TickParams* __restrict__ prm, uint32_t tick_arg, ShardScratch sc, const uint4* __restrict__ cur,
-    StagePlan sp) {
+// K1 + K5 of one tick for soma block lb (256 somas) of one shard, by a
+// workgroup of kStageThreads lanes (see the file comment). `lut` is the
+// shard's table, already in LDS (visible after the first barrier here).
+// Counters go to part[] (this workgroup's own partials) or, with
+// atomic_stats, are added atomically to a shared per-shard array;
+// type_spikes (may be null) counts spikes per variant type. The tick's GSOP
+// (K6) is left to gsop_kernel, after the pushes; count_gsop adds its slot
+// and target counts here (for callers whose GSOP kernel does not count).
+__device__ __forceinline__ void staged_block(const ShardVramPtrs& vram, uint32_t N, uint32_t A,
+                                             const AxcVariantParameters* lut, uint32_t tick, const ShardScratch& sc,
+                                             const uint4* __restrict__ cur, const StagePlan& sp, uint32_t lb,
+                                             uint64_t* part, bool atomic_stats, bool count_tick,
+                                             uint64_t* type_spikes, bool count_gsop = false) {
   extern __shared__ uint2 ent[];  // kStageCap + 1 entries
-  __shared__ AxcVariantParameters lut[16];
   __shared__ int32_t red_in[kStageGroups][256];
   __shared__ uint16_t red_hits[kStageGroups][256];
   __shared__ uint32_t wave_cnt[2][4];
   __shared__ uint32_t blk_base[2];
   __shared__ uint32_t red[4][8];
   const uint32_t tid = threadIdx.x;
-  const uint32_t per_xcd = gridDim.x / kNumXcd;
-  const uint32_t lb = (blockIdx.x % kNumXcd) * per_xcd + blockIdx.x / kNumXcd;
-  const uint32_t tick = tick_of(prm, tick_arg);
-  if (tid < 64) reinterpret_cast<uint4*>(lut)[tid] = reinterpret_cast<const uint4*>(lut_g)[tid];
   const uint32_t cnt = sp.cnt[lb];
   const bool staged = cnt != kStageOverflow;
   const uint32_t* __restrict__ bset = sp.set + (size_t)lb * kStageCap;
@@ -327,7 +327,7 @@ void neuron_staged_kernel(
   // Group 0 finishes its 256 somas; the other groups only join the barriers.
   bool fin = false, push_own = false;
   uint32_t my_axon = 0;
-  uint32_t n_int = 0, n_scan = 0, n_tread = 0, n_hits = 0;
+  uint32_t n_int = 0, n_scan = 0, n_tread = 0, n_hits = 0, n_gslots = 0, n_gtgt = 0;
   if (g == 0 && i < N) {
     int32_t sum = 0;
     uint32_t hsum = 0;
@@ -345,6 +345,11 @@ void neuron_staged_kernel(
     flags &= (uint8_t)~0x01;
     soma_tick(vram, i, A, p, flags, timer, refr, tick, sum, fin, push_own, my_axon);
     n_int = refr ? 0u : 1u;
+    if (fin && type_spikes) atomicAdd((unsigned long long*)&type_spikes[flags_in >> 4], 1ull);
+    if (fin && count_gsop) {
+      n_gslots = F;
+      n_gtgt = F < kMaxDendrites ? F + 1 : kMaxDendrites;
+    }
   }
   bool own_first = false;
   if (push_own) own_first = atomicAdd(&sc.push_cnt[my_axon], 1u) == 0u;
@@ -354,9 +359,9 @@ void neuron_staged_kernel(
   const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   const uint64_t bs = __ballot(fin);
   const uint64_t bo = __ballot(own_first);
-  uint32_t r[4] = {n_int, n_scan, n_tread, n_hits};
+  uint32_t r[6] = {n_int, n_scan, n_tread, n_hits, n_gslots, n_gtgt};
 #pragma unroll
-  for (int k = 0; k < 4; k++) {
+  for (int k = 0; k < 6; k++) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) r[k] += __shfl_xor(r[k], off, 64);
   }
@@ -364,7 +369,7 @@ void neuron_staged_kernel(
     wave_cnt[0][wv] = (uint32_t)__popcll(bs);
     wave_cnt[1][wv] = (uint32_t)__popcll(bo);
 #pragma unroll
-    for (int k = 0; k < 4; k++) red[wv][k] = r[k];
+    for (int k = 0; k < 6; k++) red[wv][k] = r[k];
   }
   __syncthreads();
   if (tid == 0) {
@@ -372,13 +377,24 @@ void neuron_staged_kernel(
     const uint32_t co = wave_cnt[1][0] + wave_cnt[1][1] + wave_cnt[1][2] + wave_cnt[1][3];
     blk_base[0] = cs ? atomicAdd(&sc.counters[kCntSpikes], cs) : 0u;
     blk_base[1] = co ? atomicAdd(&sc.counters[kCntOwners], co) : 0u;
-    uint64_t* part = sc.stats_part + (size_t)blockIdx.x * kStatsPerBlock;
-    part[kStIntegrated] += red[0][0] + red[1][0] + red[2][0] + red[3][0];
-    part[kStScanned] += red[0][1] + red[1][1] + red[2][1] + red[3][1];
-    part[kStTargets] += red[0][2] + red[1][2] + red[2][2] + red[3][2];
-    part[kStHits] += red[0][3] + red[1][3] + red[2][3] + red[3][3];
-    part[kStSpikes] += cs;
-    if (blockIdx.x == 0) part[kStTicks] += 1;
+    const uint64_t v[kStatsPerBlock] = {};
+    uint64_t add[kStatsPerBlock];
+#pragma unroll
+    for (uint32_t k = 0; k < kStatsPerBlock; k++) add[k] = v[k];
+    add[kStIntegrated] = red[0][0] + red[1][0] + red[2][0] + red[3][0];
+    add[kStScanned] = red[0][1] + red[1][1] + red[2][1] + red[3][1];
+    add[kStTargets] = red[0][2] + red[1][2] + red[2][2] + red[3][2];
+    add[kStHits] = red[0][3] + red[1][3] + red[2][3] + red[3][3];
+    add[kStGsopSlots] = red[0][4] + red[1][4] + red[2][4] + red[3][4];
+    add[kStGsopTargets] = red[0][5] + red[1][5] + red[2][5] + red[3][5];
+    add[kStSpikes] = cs;
+    add[kStTicks] = count_tick ? 1 : 0;
+#pragma unroll
+    for (uint32_t k = 0; k < kStatsPerBlock; k++) {
+      if (!add[k]) continue;
+      if (atomic_stats) atomicAdd((unsigned long long*)&part[k], (unsigned long long)add[k]);
+      else part[k] += add[k];
+    }
   }
   __syncthreads();
   if (g == 0) {
@@ -390,6 +406,21 @@ void neuron_staged_kernel(
     if (fin) sc.spk_list[pre_s + (uint32_t)__popcll(bs & below)] = i;
     if (own_first) sc.own_list[pre_o + (uint32_t)__popcll(bo & below)] = my_axon;
   }
+}
+
+// The single-shard soma pass: one workgroup per soma block, blocks dealt to
+// XCDs in contiguous eighths (see neuron_tick_kernel). Two workgroups per CU:
+// 2 * kStageThreads / 64 waves over 4 SIMDs.
+__global__ __launch_bounds__(kStageThreads) __attribute__((amdgpu_waves_per_eu(2 * kStageGroups, 2 * kStageGroups)))
+void neuron_staged_kernel(ShardVramPtrs vram, uint32_t N, uint32_t A, const AxcVariantParameters* __restrict__ lut_g,
+                          const TickParams* __restrict__ prm, uint32_t tick_arg, ShardScratch sc,
+                          const uint4* __restrict__ cur, StagePlan sp) {
+  __shared__ AxcVariantParameters lut[16];
+  const uint32_t per_xcd = gridDim.x / kNumXcd;
+  const uint32_t lb = (blockIdx.x % kNumXcd) * per_xcd + blockIdx.x / kNumXcd;
+  if (threadIdx.x < 64) reinterpret_cast<uint4*>(lut)[threadIdx.x] = reinterpret_cast<const uint4*>(lut_g)[threadIdx.x];
+  staged_block(vram, N, A, lut, tick_of(prm, tick_arg), sc, cur, sp, lb,
+               sc.stats_part + (size_t)blockIdx.x * kStatsPerBlock, false, blockIdx.x == 0, nullptr);
 }
 
 }  // namespace axc
