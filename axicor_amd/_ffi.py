@@ -140,7 +140,7 @@ SIGNATURES = {
     "axc_shard_targets_changed": (None, [C.POINTER(ShardVramPtrs)]),
 }
 
-TIMED_KERNELS = ["axon_tick", "neuron_tick", "push_apply", "gsop", "readout", "spike_count"]
+TIMED_KERNELS = ["axon_tick", "neuron_tick", "push_apply", "gsop", "readout", "spike_count", "weights_copy"]
 
 SURROGATE_SIGNATURES = {
     "axc_mlp_create": (_I32, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint64, C.POINTER(C.c_void_p)]),

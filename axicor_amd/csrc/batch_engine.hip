@@ -142,8 +142,9 @@ __global__ __launch_bounds__(256) void batch_gsop_kernel(const BrainDev* __restr
 __global__ void batch_set_tick_kernel(BatchTick* __restrict__ bt, uint32_t tick_base) { bt->tick_base = tick_base; }
 
 // Staged soma phase over all brains (staged_block per work item of 256
-// somas; the items of one XCD are a contiguous eighth of the list). GSOP
-// counts are taken here, for batch_gsop_flat_kernel.
+// somas, weights from each brain's working copy; the items of one XCD are a
+// contiguous eighth of the list). GSOP counts are taken here, for
+// batch_gsop_chunks_kernel.
 __global__ __launch_bounds__(kStageThreads) __attribute__((amdgpu_waves_per_eu(2 * kStageGroups, 2 * kStageGroups)))
 void batch_staged_kernel(const BrainDev* __restrict__ brains, const WorkItem* __restrict__ items, uint32_t n_items,
                          const BatchTick* __restrict__ bt, uint32_t t_off, uint32_t snap) {
@@ -159,8 +160,8 @@ void batch_staged_kernel(const BrainDev* __restrict__ brains, const WorkItem* __
     __syncthreads();  // the previous item is done with the LDS table and entries
     if (threadIdx.x < 64)
       reinterpret_cast<uint4*>(lut)[threadIdx.x] = reinterpret_cast<const uint4*>(b.lut)[threadIdx.x];
-    staged_block(b.vram, b.N, b.A, lut, tick, b.sc, b.sc.comp[snap & 1], b.sp, w.start >> 8, b.stats, true,
-                 w.start == 0, b.type_spikes, true);
+    staged_block<true>(b.vram, b.N, b.A, lut, tick, b.sc, b.sc.comp[snap & 1], b.sp, w.start >> 8, b.stats, true,
+                       w.start == 0, b.type_spikes, true);
   }
 }
 
@@ -212,19 +213,33 @@ __global__ __launch_bounds__(256) void batch_push_flat_kernel(const BrainDev* __
   }
 }
 
-// GSOP of this tick for every brain's spikers, one wave per spiker over the
-// flattened spike lists (prefix from batch_spike_prefix_kernel), with each
-// brain's filled prefixes and post-push byte snapshot.
-__global__ __launch_bounds__(256) void batch_gsop_flat_kernel(const BrainDev* __restrict__ brains, uint32_t n_brains,
-                                                              const uint32_t* __restrict__ prefix, uint32_t snap) {
-  const uint32_t lane = threadIdx.x & 63u;
-  const uint32_t gw = blockIdx.x * 4 + (threadIdx.x >> 6), W = gridDim.x * 4;
-  const uint32_t total = prefix[n_brains];
-  for (uint32_t k = gw; k < total; k += W) {
+// GSOP of this tick for every brain's spikers over the flattened spike lists
+// (prefix from batch_spike_prefix_kernel), kGsopLanes lanes per spiker, one
+// chunk of 8 slots each, on the brains' working copies of the weights.
+__global__ __launch_bounds__(256) void batch_gsop_chunks_kernel(const BrainDev* __restrict__ brains,
+                                                                uint32_t n_brains, const uint32_t* __restrict__ prefix,
+                                                                uint32_t snap) {
+  const uint32_t total = prefix[n_brains] * kGsopLanes;
+  for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < total; q += gridDim.x * blockDim.x) {
+    const uint32_t k = q / kGsopLanes, c = q % kGsopLanes;
     const uint32_t bi = brain_of(prefix, n_brains, k);
     const BrainDev& b = brains[bi];
     const uint32_t i = b.sc.spk_list[k - prefix[bi]];
-    gsop_soma_filled(b.vram, b.N, b.A, b.lut, b.dop, i, b.sp.fill[i], b.sc.comp[snap & 1], b.sp.comp8, lane);
+    const uint32_t F = b.sp.fill[i];
+    if (c * 8 < F) gsop_chunk(b.N, b.A, b.lut, b.dop, b.vram.soma_flags, b.sp, b.sc.comp[snap & 1], i, c, F);
+  }
+}
+
+// Every brain's working copy of the weights, loaded from (IN) or stored to
+// its reference rows: one workgroup per soma work item.
+template <bool IN>
+__global__ __launch_bounds__(256) void batch_copy_weights_kernel(const BrainDev* __restrict__ brains,
+                                                                 const WorkItem* __restrict__ items, uint32_t n_items) {
+  for (uint32_t it = blockIdx.x; it < n_items; it += gridDim.x) {
+    const WorkItem w = items[it];
+    const BrainDev& b = brains[w.brain];
+    const uint32_t i = w.start + threadIdx.x;
+    if (i < b.N) copy_weights_soma<IN>(b.vram.dendrite_weights, b.N, b.sp, i);
   }
 }
 

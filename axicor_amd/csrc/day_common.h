@@ -95,6 +95,10 @@ struct StagePlan {
   uint32_t* set;   // [blocks][kStageCap]
   uint32_t* cnt;   // [blocks] entries, or kStageOverflow
   uint2* comp8;    // [A+1] byte snapshot: per axon 8 x min(h, 255); entry A inert
+  uint32_t* tgt;   // [16][N][8] targets of the filled slots (reference format), same order
+  int16_t* w;      // [16][N][8] weights of the filled slots' chunks, same order: the working copy of
+                   // the batch graphs, loaded from the reference rows at a batch's start and stored
+                   // back at its end
 };
 
 }  // namespace axc

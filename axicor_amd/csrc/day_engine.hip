@@ -116,7 +116,7 @@ std::unordered_map<const void*, Shard*> g_shards;
 
 // Optional per-kernel timing of the per-tick path: HIP events recorded on the
 // caller's stream around each launch, summed by axc_kernel_timing_read.
-enum TimedKernel { kKAxon = 0, kKNeuron, kKPush, kKGsop, kKReadout, kKSpikes, kNumTimed };
+enum TimedKernel { kKAxon = 0, kKNeuron, kKPush, kKGsop, kKReadout, kKSpikes, kKWeights, kNumTimed };
 struct TimingState {
   bool enabled = false;
   std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending;
@@ -162,7 +162,7 @@ void free_scratch(Shard* s) {
   s->sc = ShardScratch{};
   s->prm = nullptr;
   for (void* q : {(void*)s->sp.code, (void*)s->sp.seg, (void*)s->sp.fill, (void*)s->sp.set, (void*)s->sp.cnt,
-                  (void*)s->sp.comp8})
+                  (void*)s->sp.comp8, (void*)s->sp.tgt, (void*)s->sp.w})
     if (q) (void)hipFree(q);
   s->sp = StagePlan{};
   s->stage_dirty = true;
@@ -179,9 +179,9 @@ bool ensure_stage(Shard* s, const ShardVramPtrs& v, hipStream_t st, hipError_t* 
     static std::once_flag attr_once;
     std::call_once(attr_once, [] {
       const int bytes = (int)((kStageCap + 1) * sizeof(uint2));
-      (void)hipFuncSetAttribute((const void*)neuron_staged_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                bytes);
-      (void)hipFuncSetAttribute((const void*)batch_staged_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+      for (const void* f : {(const void*)neuron_staged_kernel<false>, (const void*)neuron_staged_kernel<true>,
+                            (const void*)batch_staged_kernel})
+        (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
     });
     StagePlan p{};
     hipError_t e = hipMalloc((void**)&p.code, kMaxDendrites * N * sizeof(uint16_t));
@@ -190,12 +190,15 @@ bool ensure_stage(Shard* s, const ShardVramPtrs& v, hipStream_t st, hipError_t* 
     if (e == hipSuccess) e = hipMalloc((void**)&p.set, blocks * kStageCap * sizeof(uint32_t));
     if (e == hipSuccess) e = hipMalloc((void**)&p.cnt, blocks * sizeof(uint32_t));
     if (e == hipSuccess) e = hipMalloc((void**)&p.comp8, (A + 1) * sizeof(uint2));
+    if (e == hipSuccess) e = hipMalloc((void**)&p.tgt, kMaxDendrites * N * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc((void**)&p.w, kMaxDendrites * N * sizeof(int16_t));
     // Padding blocks keep an empty set; entry A is the inert axon.
     if (e == hipSuccess) e = hipMemset(p.cnt, 0, blocks * sizeof(uint32_t));
     if (e == hipSuccess) e = hipMemset(p.comp8, 0xFF, (A + 1) * sizeof(uint2));
     if (e == hipSuccess) e = hipDeviceSynchronize();
     if (e != hipSuccess) {
-      for (void* q : {(void*)p.code, (void*)p.seg, (void*)p.fill, (void*)p.set, (void*)p.cnt, (void*)p.comp8})
+      for (void* q : {(void*)p.code, (void*)p.seg, (void*)p.fill, (void*)p.set, (void*)p.cnt, (void*)p.comp8,
+                      (void*)p.tgt, (void*)p.w})
         if (q) (void)hipFree(q);
       *err = e;
       return false;
@@ -310,13 +313,14 @@ Shard* shard_for(const ShardVramPtrs* v, uint32_t N, uint32_t A, hipError_t* err
 // snap: compact snapshot buffer of this tick (the other one holds the
 // previous tick's). fuse_prev: apply the previous tick's GSOP inside this
 // tick's neuron pass. gsop_now: launch this tick's GSOP (otherwise the next
-// tick of the batch applies it).
+// tick of the batch applies it). staged: the staged soma pass; pw: with the
+// plan's working copy of the weights (loaded by the caller).
 hipError_t enqueue_tick(const ShardVramPtrs& v, Shard* s, uint32_t v_seg, uint32_t tick,
                         const uint32_t* bitmask, uint32_t words, uint32_t virt_off, uint32_t n_virt,
                         const uint32_t* spikes, uint32_t max_spk, uint32_t n_spk, const uint32_t* mapped,
                         uint8_t* out, uint32_t n_out, int32_t dopamine, const TickParams* prm, hipStream_t st,
                         const AxcVariantParameters* lut, uint32_t snap, bool fuse_prev, bool gsop_now,
-                        bool staged) {
+                        bool staged, bool pw = false) {
   const uint32_t N = s->N, A = s->A;
   uint4* cur = s->sc.comp[snap & 1];
   uint4* prev = s->sc.comp[(snap & 1) ^ 1];
@@ -343,7 +347,12 @@ hipError_t enqueue_tick(const ShardVramPtrs& v, Shard* s, uint32_t v_seg, uint32
     // for every tick of a staged batch).
     TimedScope ts(kKNeuron, st, timed);
     const size_t lds = (kStageCap + 1) * sizeof(uint2);
-    neuron_staged_kernel<<<neuron_blocks(N), kStageThreads, lds, st>>>(v, N, A, lut, prm, tick, s->sc, cur, s->sp);
+    if (pw)
+      neuron_staged_kernel<true><<<neuron_blocks(N), kStageThreads, lds, st>>>(v, N, A, lut, prm, tick, s->sc, cur,
+                                                                               s->sp);
+    else
+      neuron_staged_kernel<false><<<neuron_blocks(N), kStageThreads, lds, st>>>(v, N, A, lut, prm, tick, s->sc, cur,
+                                                                                s->sp);
   } else if (N > 0) {
     TimedScope ts(kKNeuron, st, timed);
     if (fuse_prev)
@@ -359,8 +368,11 @@ hipError_t enqueue_tick(const ShardVramPtrs& v, Shard* s, uint32_t v_seg, uint32
   }
   if (gsop_now) {
     TimedScope ts(kKGsop, st, timed);
-    gsop_kernel<<<kListBlocks, kThreads, 0, st>>>(v, N, A, lut, prm, dopamine, s->sc, cur,
-                                                  staged ? s->sp.fill : nullptr, staged ? s->sp.comp8 : nullptr);
+    if (pw)
+      gsop_chunks_kernel<<<kListBlocks, kThreads, 0, st>>>(v, N, A, lut, prm, dopamine, s->sc, cur, s->sp);
+    else
+      gsop_kernel<<<kListBlocks, kThreads, 0, st>>>(v, N, A, lut, prm, dopamine, s->sc, cur,
+                                                    staged ? s->sp.fill : nullptr, staged ? s->sp.comp8 : nullptr);
   }
   if (n_out > 0 && mapped != nullptr && out != nullptr) {
     TimedScope ts(kKReadout, st, timed);
@@ -842,8 +854,11 @@ int32_t axc_step_day_phase_batch(const ShardVramPtrs* vram, uint32_t padded_n, u
   key.staged = ensure_stage(s, *vram, st, &e);
   if (e != hipSuccess) return (int32_t)e;
   // The direct pass applies tick t's GSOP inside tick t+1's soma pass; with
-  // the staged pass every tick ends with the GSOP kernel.
+  // the staged pass every tick ends with the GSOP kernel, on the plan's
+  // working copy of the weights, loaded before the first tick and stored
+  // back after the last.
   const bool fuse = !key.staged;
+  const bool pw = key.staged;
   key.spike_counts.assign(T, 0);
   if (d_spikes && h_counts)
     for (uint32_t t = 0; t < T; t++) key.spike_counts[t] = h_counts[t];
@@ -858,11 +873,20 @@ int32_t axc_step_day_phase_batch(const ShardVramPtrs* vram, uint32_t padded_n, u
   if (timed) {
     // Kernel by kernel, so that each launch can be timed: the same kernels
     // and tick structure as the graph below, arguments passed directly.
+    if (pw) {
+      TimedScope ts(kKWeights, st, true);
+      copy_weights_kernel<true><<<blocks_for(padded_n), kThreads, 0, st>>>(vram->dendrite_weights, padded_n, s->sp);
+    }
     for (uint32_t t = 0; t < T && e == hipSuccess; t++) {
       e = enqueue_tick(*vram, s, v_seg, tick_base + t, d_bitmask ? d_bitmask + (size_t)t * words : nullptr, words,
                        virt_off, n_virt, d_spikes ? d_spikes + (size_t)t * max_spk : nullptr, max_spk,
                        key.spike_counts[t], d_mapped, d_out ? d_out + (size_t)t * n_out : nullptr, n_out,
-                       (int32_t)dopamine, nullptr, st, lut, t, t > 0 && fuse, t + 1 == T || !fuse, key.staged);
+                       (int32_t)dopamine, nullptr, st, lut, t, t > 0 && fuse, t + 1 == T || !fuse, key.staged, pw);
+    }
+    if (pw && e == hipSuccess) {
+      TimedScope ts(kKWeights, st, true);
+      copy_weights_kernel<false><<<blocks_for(padded_n), kThreads, 0, st>>>(vram->dendrite_weights, padded_n, s->sp);
+      e = hipGetLastError();
     }
     return (int32_t)e;
   }
@@ -884,13 +908,19 @@ int32_t axc_step_day_phase_batch(const ShardVramPtrs* vram, uint32_t padded_n, u
     if ((e = hipStreamCreateWithFlags(&cap, hipStreamNonBlocking)) != hipSuccess) return (int32_t)e;
     e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
     if (e == hipSuccess) {
+      if (pw)
+        copy_weights_kernel<true><<<blocks_for(padded_n), kThreads, 0, cap>>>(vram->dendrite_weights, padded_n,
+                                                                               s->sp);
       for (uint32_t t = 0; t < T && e == hipSuccess; t++) {
         // GSOP of tick t runs inside tick t+1's neuron pass; the last tick's
         // GSOP is launched on its own so the batch leaves a complete state.
         e = enqueue_tick(*vram, s, v_seg, t, d_bitmask, words, virt_off, n_virt, d_spikes, max_spk,
                          key.spike_counts[t], d_mapped, d_out, n_out, 0, s->prm, cap, lut, t, t > 0 && fuse,
-                         t + 1 == T || !fuse, key.staged);
+                         t + 1 == T || !fuse, key.staged, pw);
       }
+      if (pw && e == hipSuccess)
+        copy_weights_kernel<false><<<blocks_for(padded_n), kThreads, 0, cap>>>(vram->dendrite_weights, padded_n,
+                                                                                s->sp);
       hipGraph_t g = nullptr;
       hipError_t e2 = hipStreamEndCapture(cap, &g);
       if (e == hipSuccess) e = e2;
@@ -1046,8 +1076,11 @@ hipError_t batch_capture(Batch* b, uint32_t T) {
   e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
   if (e == hipSuccess) {
     if (b->staged) {
-      // Per tick: axons, staged somas, pushes, GSOP after the pushes.
+      // Per tick: axons, staged somas, pushes, GSOP after the pushes, on the
+      // brains' working copies of the weights (loaded first, stored last).
       const size_t lds = (kStageCap + 1) * sizeof(uint2);
+      batch_copy_weights_kernel<true><<<b->tune.soma_blocks, kThreads, 0, cap>>>(b->d_brains, b->d_soma_items,
+                                                                                 b->n_soma_items);
       for (uint32_t t = 0; t < T; t++) {
         batch_axon_kernel<<<b->tune.axon_blocks, kThreads, 0, cap>>>(b->d_brains, b->d_axon_items, b->n_axon_items,
                                                                      b->d_tick, t, t);
@@ -1056,8 +1089,10 @@ hipError_t batch_capture(Batch* b, uint32_t T) {
         batch_spike_prefix_kernel<<<1, 1024, 0, cap>>>(b->d_brains, n, b->d_prefix + n + 1, kCntOwners);
         batch_push_flat_kernel<<<b->tune.list_blocks, kThreads, 0, cap>>>(b->d_brains, n, b->d_prefix + n + 1, t);
         batch_spike_prefix_kernel<<<1, 1024, 0, cap>>>(b->d_brains, n, b->d_prefix, kCntSpikes);
-        batch_gsop_flat_kernel<<<b->tune.list_blocks * 8, kThreads, 0, cap>>>(b->d_brains, n, b->d_prefix, t);
+        batch_gsop_chunks_kernel<<<b->tune.list_blocks * 8, kThreads, 0, cap>>>(b->d_brains, n, b->d_prefix, t);
       }
+      batch_copy_weights_kernel<false><<<b->tune.soma_blocks, kThreads, 0, cap>>>(b->d_brains, b->d_soma_items,
+                                                                                  b->n_soma_items);
     } else {
     for (uint32_t t = 0; t < T; t++) {
       batch_axon_kernel<<<b->tune.axon_blocks, kThreads, 0, cap>>>(b->d_brains, b->d_axon_items, b->n_axon_items,

@@ -17,10 +17,12 @@
 //                       loop's `break`), and group 0 finishes each soma with
 //                       the GLIF update and the spike compaction;
 // Each tick's GSOP (K6) runs after the pushes in gsop_kernel, one wave per
-// spiking soma: applying it inside the next tick's staged pass (as the
-// direct pass does) makes the few spiking lanes of a wave wait on a chain of
-// dependent gathers (entry -> axon -> previous snapshot) that holds up their
-// whole block, and measured slower.
+// spiking soma. Applying it inside the next tick's staged pass (as the
+// direct pass does) measured slower even from the staged byte entries (this
+// tick's heads at segment + v_seg, no dependent gathers): at 1% firing most
+// waves hold one or two somas that spiked, whose slot pass runs with the
+// other lanes idle (a pass before the scan: 575 us against 306 + 136 us for
+// scan + GSOP at 1.5M neurons; inside the scan loop the registers spill).
 // The snapshot entries are the byte form of the heads (min(h, 255) x 8, see
 // byte_heads); a slot whose segment + propagation length exceeds 255 reads the
 // 16-byte compact form from global memory instead, so every hit is exact.
@@ -62,6 +64,7 @@ __global__ __launch_bounds__(256) void stage_build_kernel(const uint32_t* __rest
     for (; F < kMaxDendrites; F++) {
       const uint32_t t = targets[(size_t)F * N + i];
       if (t == 0) break;
+      sp.tgt[((size_t)(F >> 3) * N + i) * 8 + (F & 7)] = t;
       const uint32_t key = snap_index(t, A);
       uint32_t h = hash_slot(key), probe = 0;
       for (; probe < kHashMaxProbe; probe++) {
@@ -146,38 +149,46 @@ __device__ __forceinline__ uint4 snap_at(const uint4* __restrict__ snap, uint32_
 
 // One chunk of 8 slots of soma i as loaded: staged, the plan's 8 entries
 // (u16, one 16-byte load) and 8 segments (u8, one 8-byte load); otherwise
-// the 8 reference targets (segment << 24 | axon + 1). Plus the 8 weights,
-// two per register.
+// the plan's 8 targets (segment << 24 | axon + 1, two 16-byte loads). Plus
+// the 8 weights, two per register: PW, from the plan's working copy (one
+// 16-byte load); otherwise from the 8 reference rows.
 typedef uint32_t v4u __attribute__((ext_vector_type(4)));
 typedef uint32_t v2u __attribute__((ext_vector_type(2)));
 
-template <bool STAGED>
+__device__ __forceinline__ size_t chunk_at(uint32_t N, uint32_t c, uint32_t i) { return ((size_t)c * N + i) * 8; }
+
+template <bool STAGED, bool PW>
 struct Chunk {
   uint32_t k[STAGED ? 6 : 8];
   uint32_t w2[4];
   __device__ __forceinline__ void load(const ShardVramPtrs& vram, const StagePlan& sp, uint32_t N, uint32_t c,
                                        uint32_t i) {
+    // Streamed once per tick: nontemporal, so that the byte snapshot and
+    // the block sets the staging gathers read keep their places in L2.
     if (STAGED) {
-      // Streamed once per tick: nontemporal, so that the byte snapshot and
-      // the block sets the staging gathers read keep their places in L2.
       const v4u e = __builtin_nontemporal_load(
           reinterpret_cast<const v4u*>(reinterpret_cast<const char*>(sp.code + (size_t)c * N * 8) + i * 16u));
       const v2u g = __builtin_nontemporal_load(
           reinterpret_cast<const v2u*>(reinterpret_cast<const char*>(sp.seg + (size_t)c * N * 8) + i * 8u));
       k[0] = e.x; k[1] = e.y; k[2] = e.z; k[3] = e.w; k[4] = g.x; k[5] = g.y;
     } else {
-#pragma unroll
-      for (int u = 0; u < 8; u++) {
-        const char* row = reinterpret_cast<const char*>(vram.dendrite_targets + (size_t)(c * 8 + u) * N);
-        k[u] = __builtin_nontemporal_load(reinterpret_cast<const uint32_t*>(row + i * 4u));
-      }
+      const v4u* q =
+          reinterpret_cast<const v4u*>(reinterpret_cast<const char*>(sp.tgt + (size_t)c * N * 8) + i * 32u);
+      const v4u t0 = __builtin_nontemporal_load(q), t1 = __builtin_nontemporal_load(q + 1);
+      k[0] = t0.x; k[1] = t0.y; k[2] = t0.z; k[3] = t0.w; k[4] = t1.x; k[5] = t1.y; k[6] = t1.z; k[7] = t1.w;
     }
+    if (PW) {
+      const v4u wv = __builtin_nontemporal_load(
+          reinterpret_cast<const v4u*>(reinterpret_cast<const char*>(sp.w + (size_t)c * N * 8) + i * 16u));
+      w2[0] = wv.x; w2[1] = wv.y; w2[2] = wv.z; w2[3] = wv.w;
+    } else {
 #pragma unroll
-    for (int j = 0; j < 4; j++) {
-      const char* r0 = reinterpret_cast<const char*>(vram.dendrite_weights + (size_t)(c * 8 + 2 * j) * N);
-      const char* r1 = reinterpret_cast<const char*>(vram.dendrite_weights + (size_t)(c * 8 + 2 * j + 1) * N);
-      w2[j] = (uint32_t)__builtin_nontemporal_load(reinterpret_cast<const uint16_t*>(r0 + i * 2u)) |
-              ((uint32_t)__builtin_nontemporal_load(reinterpret_cast<const uint16_t*>(r1 + i * 2u)) << 16);
+      for (int j = 0; j < 4; j++) {
+        const char* r0 = reinterpret_cast<const char*>(vram.dendrite_weights + (size_t)(c * 8 + 2 * j) * N);
+        const char* r1 = reinterpret_cast<const char*>(vram.dendrite_weights + (size_t)(c * 8 + 2 * j + 1) * N);
+        w2[j] = (uint32_t)__builtin_nontemporal_load(reinterpret_cast<const uint16_t*>(r0 + i * 2u)) |
+                ((uint32_t)__builtin_nontemporal_load(reinterpret_cast<const uint16_t*>(r1 + i * 2u)) << 16);
+      }
     }
   }
   __device__ __forceinline__ uint32_t seg(int u) const {
@@ -187,10 +198,6 @@ struct Chunk {
   __device__ __forceinline__ uint32_t entry(int u, uint32_t A) const {
     return STAGED ? ((k[u >> 1] >> (16 * (u & 1))) & 0xFFFFu) : snap_index(k[u], A);
   }
-  __device__ __forceinline__ uint32_t axon(int u, const uint32_t* __restrict__ bset, uint32_t A) const {
-    return STAGED ? *reinterpret_cast<const uint32_t*>(reinterpret_cast<const char*>(bset) + entry(u, A) * 4u)
-                  : snap_index(k[u], A);
-  }
   __device__ __forceinline__ int32_t weight(int u) const {
     return (int32_t)(int16_t)(uint16_t)(w2[u >> 1] >> (16 * (u & 1)));
   }
@@ -198,11 +205,10 @@ struct Chunk {
 
 // The hit sum of soma i over the slots of group gu: chunks of 8 slots
 // c = (gu + 3) % 4 + 4k below the soma's filled prefix F.
-template <bool STAGED>
+template <bool STAGED, bool PW>
 __device__ __forceinline__ void scan_slots(const ShardVramPtrs& vram, uint32_t N, uint32_t A, const StagePlan& sp,
-                                           const uint2* ent, const uint32_t* __restrict__ bset,
-                                           const uint4* __restrict__ cur, uint32_t gu, uint32_t i, uint32_t F,
-                                           uint32_t prop, int32_t& i_in, uint32_t& hits) {
+                                           const uint2* ent, const uint4* __restrict__ cur, uint32_t gu, uint32_t i,
+                                           uint32_t F, uint32_t prop, int32_t& i_in, uint32_t& hits) {
   constexpr int U = 8;
   // Group 0, which finishes the somas afterwards, takes chunks 3, 7, 11, ...
   // so that a partly filled last chunk lands on another group.
@@ -211,7 +217,7 @@ __device__ __forceinline__ void scan_slots(const ShardVramPtrs& vram, uint32_t N
   for (uint32_t c = cg; c * U < F; c += kStageGroups) {
     const uint32_t s0 = c * U;
     const uint32_t nv = F - s0 < (uint32_t)U ? F - s0 : (uint32_t)U;
-    Chunk<STAGED> now;
+    Chunk<STAGED, PW> now;
     now.load(vram, sp, N, c, i);
     uint32_t hit_mask = 0, wide = 0;
     constexpr int H = 4;  // entries read in two halves of four
@@ -233,13 +239,17 @@ __device__ __forceinline__ void scan_slots(const ShardVramPtrs& vram, uint32_t N
       }
     }
     if (wide) {
-      // Segment + length beyond the byte form (rare): the slot's reference
-      // target and the 16-byte compact entry (scalar bases, lane offsets).
+      // Segment + length beyond the byte form (rare): the slot's target and
+      // the 16-byte compact entry (scalar bases, lane offsets).
 #pragma unroll
       for (int u = 0; u < U; u++) {
         if (wide & (1u << u)) {
-          const char* trow = reinterpret_cast<const char*>(vram.dendrite_targets + (size_t)(s0 + u) * N);
-          const uint32_t t = *reinterpret_cast<const uint32_t*>(trow + i * 4u);
+          uint32_t t;
+          if constexpr (STAGED)
+            t = *reinterpret_cast<const uint32_t*>(reinterpret_cast<const char*>(sp.tgt + (size_t)c * N * 8) +
+                                                   i * 32u + u * 4u);
+          else
+            t = now.k[u];
           const bool h = min_dist16(snap_at(cur, snap_index(t, A)), t >> 24) < prop;
           hit_mask = h ? (hit_mask | (1u << u)) : (hit_mask & ~(1u << u));
         }
@@ -283,6 +293,8 @@ __device__ __forceinline__ void stage_entries(uint2* ent, const uint32_t* __rest
 // type_spikes (may be null) counts spikes per variant type. The tick's GSOP
 // (K6) is left to gsop_kernel, after the pushes; count_gsop adds its slot
 // and target counts here (for callers whose GSOP kernel does not count).
+// PW: the weights are read from the plan's working copy.
+template <bool PW>
 __device__ __forceinline__ void staged_block(const ShardVramPtrs& vram, uint32_t N, uint32_t A,
                                              const AxcVariantParameters* lut, uint32_t tick, const ShardScratch& sc,
                                              const uint4* __restrict__ cur, const StagePlan& sp, uint32_t lb,
@@ -317,8 +329,8 @@ __device__ __forceinline__ void staged_block(const ShardVramPtrs& vram, uint32_t
   // The group index is the same for the 64 lanes of a wave: scalar.
   const uint32_t gu = __builtin_amdgcn_readfirstlane(g);
   if (i < N && !refr) {
-    if (staged) scan_slots<true>(vram, N, A, sp, ent, bset, cur, gu, i, F, p.signal_propagation_length, i_in, hits);
-    else scan_slots<false>(vram, N, A, sp, ent, bset, cur, gu, i, F, p.signal_propagation_length, i_in, hits);
+    if (staged) scan_slots<true, PW>(vram, N, A, sp, ent, cur, gu, i, F, p.signal_propagation_length, i_in, hits);
+    else scan_slots<false, PW>(vram, N, A, sp, ent, cur, gu, i, F, p.signal_propagation_length, i_in, hits);
   }
   red_in[g][li] = i_in;
   red_hits[g][li] = (uint16_t)hits;
@@ -411,6 +423,7 @@ __device__ __forceinline__ void staged_block(const ShardVramPtrs& vram, uint32_t
 // The single-shard soma pass: one workgroup per soma block, blocks dealt to
 // XCDs in contiguous eighths (see neuron_tick_kernel). Two workgroups per CU:
 // 2 * kStageThreads / 64 waves over 4 SIMDs.
+template <bool PW>
 __global__ __launch_bounds__(kStageThreads) __attribute__((amdgpu_waves_per_eu(2 * kStageGroups, 2 * kStageGroups)))
 void neuron_staged_kernel(ShardVramPtrs vram, uint32_t N, uint32_t A, const AxcVariantParameters* __restrict__ lut_g,
                           const TickParams* __restrict__ prm, uint32_t tick_arg, ShardScratch sc,
@@ -419,8 +432,116 @@ void neuron_staged_kernel(ShardVramPtrs vram, uint32_t N, uint32_t A, const AxcV
   const uint32_t per_xcd = gridDim.x / kNumXcd;
   const uint32_t lb = (blockIdx.x % kNumXcd) * per_xcd + blockIdx.x / kNumXcd;
   if (threadIdx.x < 64) reinterpret_cast<uint4*>(lut)[threadIdx.x] = reinterpret_cast<const uint4*>(lut_g)[threadIdx.x];
-  staged_block(vram, N, A, lut, tick_of(prm, tick_arg), sc, cur, sp, lb,
-               sc.stats_part + (size_t)blockIdx.x * kStatsPerBlock, false, blockIdx.x == 0, nullptr);
+  staged_block<PW>(vram, N, A, lut, tick_of(prm, tick_arg), sc, cur, sp, lb,
+                   sc.stats_part + (size_t)blockIdx.x * kStatsPerBlock, false, blockIdx.x == 0, nullptr);
+}
+
+// The working copy of the weights: soma i's chunks c with 8c < fill[i], to
+// (IN) or from the reference rows. Slots of such a chunk at or past the fill
+// are copied as they are (no pass writes them). One lane per soma, chunks in
+// turn: each reference row is read or written by consecutive lanes.
+template <bool IN>
+__device__ __forceinline__ void copy_weights_soma(int16_t* __restrict__ rows, uint32_t N, const StagePlan& sp,
+                                                  uint32_t i) {
+  const uint32_t F = sp.fill[i];
+  for (uint32_t c = 0; c * 8 < F; c++) {
+    v4u* q = reinterpret_cast<v4u*>(sp.w + chunk_at(N, c, i));
+    if (IN) {
+      uint32_t w2[4];
+#pragma unroll
+      for (int j = 0; j < 4; j++)
+        w2[j] = (uint32_t)(uint16_t)rows[(size_t)(c * 8 + 2 * j) * N + i] |
+                ((uint32_t)(uint16_t)rows[(size_t)(c * 8 + 2 * j + 1) * N + i] << 16);
+      *q = v4u{w2[0], w2[1], w2[2], w2[3]};
+    } else {
+      const v4u v = *q;
+      const uint32_t w2[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        rows[(size_t)(c * 8 + 2 * j) * N + i] = (int16_t)(uint16_t)(w2[j] & 0xFFFFu);
+        rows[(size_t)(c * 8 + 2 * j + 1) * N + i] = (int16_t)(uint16_t)(w2[j] >> 16);
+      }
+    }
+  }
+}
+
+template <bool IN>
+__global__ __launch_bounds__(256) void copy_weights_kernel(int16_t* __restrict__ rows, uint32_t N, StagePlan sp) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i < N) copy_weights_soma<IN>(rows, N, sp, i);
+}
+
+// GSOP (K6, physics.cu:281-373) of one chunk c of spiking soma i on the
+// working copy: 8 slots from one 16-byte weight load and two 16-byte target
+// loads; the heads from the post-push byte snapshot (comp8), or the compact
+// one (comp) where segment + length > 255. F: the soma's fill (> 8c).
+__device__ __forceinline__ void gsop_chunk(uint32_t N, uint32_t A, const AxcVariantParameters* lut, int32_t dop,
+                                           const uint8_t* __restrict__ flags_all, const StagePlan& sp,
+                                           const uint4* __restrict__ comp, uint32_t i, uint32_t c, uint32_t F) {
+  const uint8_t flags = flags_all[i];
+  const AxcVariantParameters& p = lut[(flags >> 4) & 0x0F];
+  const GsopTerms g = gsop_terms(flags, p, dop);
+  const uint32_t nv = F - c * 8 < 8u ? F - c * 8 : 8u;
+  const v4u* tq = reinterpret_cast<const v4u*>(sp.tgt + chunk_at(N, c, i));
+  const v4u t0 = tq[0], t1 = tq[1];
+  const uint32_t t[8] = {t0.x, t0.y, t0.z, t0.w, t1.x, t1.y, t1.z, t1.w};
+  v4u* wq = reinterpret_cast<v4u*>(sp.w + chunk_at(N, c, i));
+  const v4u wv = *wq;
+  uint32_t w2[4] = {wv.x, wv.y, wv.z, wv.w};
+  uint32_t d[8];
+#pragma unroll
+  for (int u = 0; u < 8; u++) d[u] = gsop_dist(comp, sp.comp8, (uint32_t)u < nv ? snap_index(t[u], A) : A, t[u] >> 24, g.len);
+#pragma unroll
+  for (int u = 0; u < 8; u++) {
+    if ((uint32_t)u >= nv) continue;
+    const uint32_t sh = 16 * (u & 1);
+    const int16_t w = (int16_t)(uint16_t)(w2[u >> 1] >> sh);
+    const int16_t nw = gsop_weight(w, d[u], g.len, g.raw_pot, g.final_dep, g.bm, p.inertia_curve);
+    w2[u >> 1] = (w2[u >> 1] & ~(0xFFFFu << sh)) | ((uint32_t)(uint16_t)nw << sh);
+  }
+  *wq = v4u{w2[0], w2[1], w2[2], w2[3]};
+}
+
+// GSOP of a tick on the working copy: lane group of kGsopLanes per spiking
+// soma, lane j of a group takes chunk j (the chunks below the soma's fill).
+constexpr uint32_t kGsopLanes = kMaxDendrites / 8;
+__global__ __launch_bounds__(256) void gsop_chunks_kernel(ShardVramPtrs vram, uint32_t N, uint32_t A,
+                                                          const AxcVariantParameters* __restrict__ lut_g,
+                                                          const TickParams* __restrict__ prm, int32_t dop_arg,
+                                                          ShardScratch sc, const uint4* __restrict__ comp,
+                                                          StagePlan sp) {
+  __shared__ AxcVariantParameters lut[16];
+  __shared__ uint32_t red[4][2];
+  const uint32_t tid = threadIdx.x;
+  if (tid < 64) reinterpret_cast<uint4*>(lut)[tid] = reinterpret_cast<const uint4*>(lut_g)[tid];
+  __syncthreads();
+  const uint32_t n = sc.counters[kCntSpikes] * kGsopLanes;
+  const int32_t dop = dopamine_of(prm, dop_arg);
+  uint32_t cnt_slots = 0, cnt_tgt = 0;
+  for (uint32_t q = blockIdx.x * 256 + tid; q < n; q += gridDim.x * 256) {
+    const uint32_t i = sc.spk_list[q / kGsopLanes], c = q % kGsopLanes;
+    const uint32_t F = sp.fill[i];
+    if (c * 8 < F) gsop_chunk(N, A, lut, dop, vram.soma_flags, sp, comp, i, c, F);
+    if (c == 0) {
+      cnt_slots += F;
+      cnt_tgt += F < kMaxDendrites ? F + 1 : kMaxDendrites;
+    }
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    cnt_slots += __shfl_xor(cnt_slots, off, 64);
+    cnt_tgt += __shfl_xor(cnt_tgt, off, 64);
+  }
+  if ((tid & 63u) == 0) {
+    red[tid >> 6][0] = cnt_slots;
+    red[tid >> 6][1] = cnt_tgt;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    uint64_t* part = sc.stats_part2 + (size_t)blockIdx.x * kStatsPerBlock;
+    part[kStGsopSlots] += red[0][0] + red[1][0] + red[2][0] + red[3][0];
+    part[kStGsopTargets] += red[0][1] + red[1][1] + red[2][1] + red[3][1];
+  }
 }
 
 }  // namespace axc

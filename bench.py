@@ -295,14 +295,14 @@ def main() -> int:
         batch(tb)
     eng.synchronize()
     lib.axc_kernel_timing_enable(0)
-    kms = (C.c_double * 6)()
-    kn = (C.c_uint64 * 6)()
+    kms = (C.c_double * len(TIMED_KERNELS))()
+    kn = (C.c_uint64 * len(TIMED_KERNELS))()
     lib.axc_kernel_timing_read(kms, kn, 1)
     st_t = eng.stats()
     ab_t = algorithmic_bytes(st_t, N, A, words)
     fused = 4 * st_t["gsop_targets_fused"] + 4 * st_t["gsop_slots_fused"]
     per_kernel = {TIMED_KERNELS[k]: {"avg_us": kms[k] * 1e3 / kn[k], "launches": int(kn[k])}
-                  for k in range(6) if kn[k]}
+                  for k in range(len(TIMED_KERNELS)) if kn[k]}
     nt_us = per_kernel["neuron_tick"]["avg_us"]
     nt_bytes = (ab_t["neuron_tick"] + fused) / per_kernel["neuron_tick"]["launches"]
     achieved = nt_bytes / (nt_us * 1e-6) / 1e9

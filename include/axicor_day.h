@@ -314,8 +314,9 @@ void axc_shard_targets_changed(const ShardVramPtrs *vram);
  * axc_step_day_phase_batch launches its kernels one by one instead of
  * replaying its graph (same kernels, same results) so that they can be timed.
  * axc_kernel_timing_read waits for the events and returns the summed
- * milliseconds and launch counts of 6 kernels in this order:
- * axon_tick, neuron_tick, push_apply, gsop, readout, spike_count. */
+ * milliseconds and launch counts of 7 kernels in this order:
+ * axon_tick, neuron_tick, push_apply, gsop, readout, spike_count,
+ * weights_copy (the batch's working copy of the weights, in and out). */
 void axc_kernel_timing_enable(int enable);
 int32_t axc_kernel_timing_read(double *ms_out, uint64_t *launches_out, int reset);
 

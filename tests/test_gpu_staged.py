@@ -3,7 +3,11 @@ CPU oracle and against the direct scan, on the cases that take its other
 paths: blocks whose distinct presynaptic axons overflow the LDS capacity,
 segments whose segment + propagation length passes 255 (the 16-byte
 fallback), filled slots after an empty one (ignored, as the reference's
-`break`), and targets changed between steps (the plan is rebuilt)."""
+`break`), and targets changed between steps (the plan is rebuilt). Each runs
+tick by tick and through the batch graph, where tick t's GSOP is applied in
+tick t+1's staged pass (from the staged entries of tick t+1, or from the
+previous compact snapshot for pushed inputs, network spikes and long
+reaches)."""
 import ctypes as C
 
 import numpy as np
@@ -25,7 +29,11 @@ def direct_scan():
     day_lib().axc_staging_enable(1)
 
 
-def test_overflow_blocks_far_targets(gpu):
+GRAPH = pytest.mark.parametrize("use_graph", [False, True], ids=["per_tick", "graph"])
+
+
+@GRAPH
+def test_overflow_blocks_far_targets(gpu, use_graph):
     """Targets drawn far from their somas (mean distance 60 voxels): most
     256-soma blocks have more distinct presynaptic axons than the LDS holds
     and read the reference targets; the rest are staged."""
@@ -33,12 +41,13 @@ def test_overflow_blocks_far_targets(gpu):
     spec.mean_dist = 60.0
     p = Pair(B.generate(spec), n_out=32)
     for k in range(4):
-        out_ref, _ = p.step(15, dopamine=(120 if k % 2 else -80))
+        out_ref, _ = p.step(15, dopamine=(120 if k % 2 else -80), use_graph=use_graph)
         np.testing.assert_array_equal(p.outputs(15), out_ref)
         p.check(f"far targets batch {k}")
 
 
-def test_wide_segments_use_the_compact_snapshot(gpu):
+@GRAPH
+def test_wide_segments_use_the_compact_snapshot(gpu, use_graph):
     """Axons 200-255 segments long and propagation length 60: a large share
     of slots has segment + length > 255."""
     lut = B.lut_from([B.template_variant(False, signal_propagation_length=60),
@@ -46,11 +55,12 @@ def test_wide_segments_use_the_compact_snapshot(gpu):
     spec = B.spec_small(6000, 64, 400, seed=23, axon_len=(200, 255), lut=lut)
     p = Pair(B.generate(spec), n_out=0)
     for k in range(5):
-        p.step(40, dopamine=200)
+        p.step(40, dopamine=200, use_graph=use_graph)
         p.check(f"wide segments batch {k}")
 
 
-def test_holes_after_the_first_empty_slot_are_ignored(gpu):
+@GRAPH
+def test_holes_after_the_first_empty_slot_are_ignored(gpu, use_graph):
     """Somas with filled slots after an empty one (as sprouting leaves them
     until the next sort): the scan and GSOP stop at the first empty target."""
     b = B.generate(B.spec_small(4000, 80, 200, seed=25))
@@ -61,15 +71,16 @@ def test_holes_after_the_first_empty_slot_are_ignored(gpu):
     v.targets[hole, somas] = 0
     p = Pair(b, n_out=0)
     for k in range(3):
-        p.step(20, dopamine=300)
+        p.step(20, dopamine=300, use_graph=use_graph)
         p.check(f"holes batch {k}")
 
 
-def test_targets_uploaded_between_steps_rebuild_the_plan(gpu):
+@GRAPH
+def test_targets_uploaded_between_steps_rebuild_the_plan(gpu, use_graph):
     """A host upload of the targets (as the Night's sprouting upload does)
     between two batches: the next batch sees the new targets."""
     p = Pair(B.generate(B.spec_small(5000, 64, 250, seed=27)), n_out=0)
-    p.step(20)
+    p.step(20, use_graph=use_graph)
     p.check("before upload")
     v = StateView(p.ref_state, p.b.padded_n)
     rng = np.random.default_rng(5)
@@ -80,11 +91,12 @@ def test_targets_uploaded_between_steps_rebuild_the_plan(gpu):
     tg = np.ascontiguousarray(v.targets)
     assert day_lib().gpu_memcpy_host_to_device(p.vram.ptrs.dendrite_targets, tg.ctypes.data, tg.nbytes)
     for k in range(3):
-        p.step(20)
+        p.step(20, use_graph=use_graph)
         p.check(f"after upload batch {k}")
 
 
-def test_staged_and_direct_scans_agree(gpu):
+@GRAPH
+def test_staged_and_direct_scans_agree(gpu, use_graph):
     """The same brain stepped with the staged scan and with the direct one
     ends in the same state (and both equal the oracle)."""
     spec = B.spec_small(9000, 100, 450, seed=29, axon_len=(1, 255), lut=B.lut_c2(),
@@ -92,14 +104,14 @@ def test_staged_and_direct_scans_agree(gpu):
     b = B.generate(spec)
     a = Pair(b.copy(), n_out=24)
     for k in range(3):
-        a.step(30, dopamine=(200 - 150 * k))
+        a.step(30, dopamine=(200 - 150 * k), use_graph=use_graph)
         a.check(f"staged batch {k}")
     lib = day_lib()
     lib.axc_staging_enable(0)
     try:
         d = Pair(b.copy(), n_out=24)
         for k in range(3):
-            d.step(30, dopamine=(200 - 150 * k))
+            d.step(30, dopamine=(200 - 150 * k), use_graph=use_graph)
             d.check(f"direct batch {k}")
     finally:
         lib.axc_staging_enable(1)
@@ -114,3 +126,40 @@ def test_direct_scan_per_tick(gpu, direct_scan):
         out_ref, _ = p.step(1, dopamine=50)
         np.testing.assert_array_equal(p.outputs(1), out_ref)
         p.check(f"direct tick {t}")
+
+
+def test_graph_with_spike_schedule_and_inputs(gpu):
+    """Batches with input pushes every tick and network spikes (K3) on some
+    ticks: tick t's GSOP in tick t+1's pass reads the previous compact
+    snapshot for every slot of a tick with network spikes and for the slots
+    of pushed inputs."""
+    b = B.generate(B.spec_small(6000, 80, 1200, seed=33, axon_len=(1, 120)))
+    p = Pair(b, max_spikes=256, max_batch=16)
+    rng = np.random.default_rng(7)
+    for k in range(4):
+        T = 16
+        sched = rng.integers(0, b.total_axons, size=(T, 256)).astype(np.uint32)
+        counts = np.where(rng.random(T) < 0.4, rng.integers(1, 256, T), 0).astype(np.uint32)
+        p.step(T, spikes=sched.reshape(-1), counts=counts, dopamine=(150 if k % 2 else -60), use_graph=True)
+        p.check(f"spike schedule batch {k}")
+
+
+def test_graph_heads_near_the_wrap_and_sentinel(gpu):
+    """Heads close to the u32 wrap, at and next to the sentinel, and large
+    heads, with v_seg = 3, through the batch graph: the fused GSOP's
+    distances from this tick's byte entries at segment + v_seg equal those of
+    the previous post-push heads."""
+    spec = B.spec_small(4000, 64, 300, seed=35, axon_len=(1, 255))
+    spec.v_seg = 3
+    b = B.generate(spec)
+    h = b.heads.reshape(-1, 8)
+    rng = np.random.default_rng(9)
+    rows = rng.choice(b.total_axons, 400, replace=False)
+    patterns = np.array([[0xFFFFFFF0, 0x80000001, 0x7FFFFFFF, 0, 3, 0x80000000, 0xFFFFFFFF, 17],
+                         [0xFFFFFFFE, 0xFFFFFFFD, 250, 251, 252, 253, 254, 255],
+                         [0xFFFFFFFC, 1, 2, 0xFFFFFFFF, 0xFFFFFFFF, 0xFFFFFFFF, 0xFFFFFFFF, 0xFFFFFFFF]], np.uint32)
+    h[rows] = patterns[np.arange(rows.size) % 3]
+    p = Pair(b, n_out=0)
+    for k in range(4):
+        p.step(12, dopamine=400, v_seg=3, use_graph=True)
+        p.check(f"wrap heads batch {k}")
