@@ -75,6 +75,7 @@ __global__ __launch_bounds__(256) void stage_build_kernel(const uint32_t* __rest
       if (probe == kHashMaxProbe) overflow = 1;
     }
     sp.fill[i] = (uint8_t)F;
+    for (uint32_t s = F; s & 7; s++) sp.tgt[((size_t)(s >> 3) * N + i) * 8 + (s & 7)] = 0;
   }
   __syncthreads();
   // Compact the table (lane tid owns positions [64 tid, 64 tid + 64)).
@@ -136,6 +137,12 @@ __global__ __launch_bounds__(256) void stage_build_kernel(const uint32_t* __rest
     const size_t c = ((size_t)(s >> 3) * N + i) * 8 + (s & 7);  // chunk-interleaved
     sp.code[c] = (uint16_t)lo;
     sp.seg[c] = (uint8_t)(t >> 24);
+  }
+  // The rest of the last chunk: the inert entry, segment 0 (never a hit).
+  for (uint32_t s = F; s & 7; s++) {
+    const size_t c = ((size_t)(s >> 3) * N + i) * 8 + (s & 7);
+    sp.code[c] = (uint16_t)kStageCap;
+    sp.seg[c] = 0;
   }
 }
 
@@ -203,62 +210,126 @@ struct Chunk {
   }
 };
 
+typedef short axc_s2 __attribute__((ext_vector_type(2)));
+
+// Distances of the eight heads of byte entry e from segment s, as u16 lanes
+// (h - s; heads below s wrap to >= 65281), reduced to two lanes. S = s in
+// both halves.
+__device__ __forceinline__ uint32_t dist_pair_lanes(const uint2& e, uint32_t S) {
+  const axc_us2 s = __builtin_bit_cast(axc_us2, S);
+  const axc_us2 d0 = __builtin_bit_cast(axc_us2, __builtin_amdgcn_perm(0u, e.x, 0x0c010c00u)) - s;
+  const axc_us2 d1 = __builtin_bit_cast(axc_us2, __builtin_amdgcn_perm(0u, e.x, 0x0c030c02u)) - s;
+  const axc_us2 d2 = __builtin_bit_cast(axc_us2, __builtin_amdgcn_perm(0u, e.y, 0x0c010c00u)) - s;
+  const axc_us2 d3 = __builtin_bit_cast(axc_us2, __builtin_amdgcn_perm(0u, e.y, 0x0c030c02u)) - s;
+  return __builtin_bit_cast(uint32_t, __builtin_elementwise_min(__builtin_elementwise_min(d0, d1),
+                                                                __builtin_elementwise_min(d2, d3)));
+}
+// [min of ru, min of rv] from two dist_pair_lanes results.
+__device__ __forceinline__ uint32_t pair_min(uint32_t ru, uint32_t rv) {
+  const axc_us2 lo = __builtin_bit_cast(axc_us2, __builtin_amdgcn_perm(rv, ru, 0x05040100u));
+  const axc_us2 hi = __builtin_bit_cast(axc_us2, __builtin_amdgcn_perm(rv, ru, 0x07060302u));
+  return __builtin_bit_cast(uint32_t, __builtin_elementwise_min(lo, hi));
+}
+// Largest of the eight bytes of (x, y).
+__device__ __forceinline__ uint32_t max_byte8(uint32_t x, uint32_t y) {
+  const axc_us2 a = __builtin_bit_cast(axc_us2, x & 0x00FF00FFu), b = __builtin_bit_cast(axc_us2, (x >> 8) & 0x00FF00FFu);
+  const axc_us2 c = __builtin_bit_cast(axc_us2, y & 0x00FF00FFu), d = __builtin_bit_cast(axc_us2, (y >> 8) & 0x00FF00FFu);
+  const axc_us2 m = __builtin_elementwise_max(__builtin_elementwise_max(a, b), __builtin_elementwise_max(c, d));
+  return max((uint32_t)m.x, (uint32_t)m.y);
+}
+
+// Hit test and weighted sum of one loaded chunk c (slots 8c .. 8c + 7) of
+// soma i. The plan gives the slots of the last chunk at or past the fill the
+// inert entry and segment 0, so every slot of a chunk is tested alike.
+// Slots are taken in pairs: the two minimum distances in one register, the
+// hit flags (distance < length) as two u16 lanes, and their weights added
+// with one 2-lane dot product. hit_cnt counts the hits in its two lanes.
+template <bool STAGED, bool PW>
+__device__ __forceinline__ void hit_chunk(const Chunk<STAGED, PW>& now, uint32_t c, uint32_t N, uint32_t A,
+                                          const StagePlan& sp, const uint2* ent, const uint4* __restrict__ cur,
+                                          uint32_t i, uint32_t prop, int32_t& i_in, axc_us2& hit_cnt) {
+  constexpr int U = 8;
+  const axc_us2 P = {(unsigned short)prop, (unsigned short)prop};
+  const axc_us2 kHalf = {0x7FFF, 0x7FFF}, k15 = {15, 15};
+  // Segment u of the chunk in both u16 halves.
+  uint32_t S[U];
+#pragma unroll
+  for (int u = 0; u < U; u++) {
+    if (STAGED) S[u] = __builtin_amdgcn_perm(0u, now.k[4 + (u >> 2)], 0x0c000c00u | ((u & 3) * 0x00010001u));
+    else S[u] = __builtin_amdgcn_perm(0u, now.k[u], 0x0c030c03u);
+  }
+  uint32_t M[U / 2];
+  constexpr int H = 4;  // entries read in two halves of four
+#pragma unroll
+  for (int h0 = 0; h0 < U; h0 += H) {
+    uint2 e[H];
+#pragma unroll
+    for (int u = 0; u < H; u++) {
+      if (STAGED) e[u] = ent[now.entry(h0 + u, A)];
+      else e[u] = sp.comp8[now.entry(h0 + u, A)];
+    }
+#pragma unroll
+    for (int u = 0; u < H; u += 2)
+      M[(h0 + u) / 2] = pair_min(dist_pair_lanes(e[u], S[h0 + u]), dist_pair_lanes(e[u + 1], S[h0 + u + 1]));
+  }
+  // Segment + length beyond the byte form (rare): the slot's exact distance
+  // from its target and the 16-byte compact entry.
+  uint32_t mx;
+  if (STAGED) {
+    mx = max_byte8(now.k[4], now.k[5]);
+  } else {
+    mx = 0;
+#pragma unroll
+    for (int u = 0; u < U; u++) mx = max(mx, now.k[u] >> 24);
+  }
+  if (mx + prop > 255u) {
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const uint32_t sg = S[u] & 0xFFu;
+      if (sg + prop > 255u) {
+        uint32_t t;
+        if constexpr (STAGED)
+          t = *reinterpret_cast<const uint32_t*>(reinterpret_cast<const char*>(sp.tgt + (size_t)c * N * 8) +
+                                                 i * 32u + u * 4u);
+        else
+          t = now.k[u];
+        // A slot at or past the fill has the empty target: no hit.
+        const uint32_t d = t ? min_dist16(snap_at(cur, snap_index(t, A)), sg) : 0xFFFFu;
+        const uint32_t sh = 16 * (u & 1);
+        M[u / 2] = (M[u / 2] & ~(0xFFFFu << sh)) | (d << sh);
+      }
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < U / 2; j++) {
+    // 1 in the u16 lanes whose distance is below the length.
+    const axc_us2 gap = __builtin_elementwise_sub_sat(P, __builtin_bit_cast(axc_us2, M[j]));
+    const axc_us2 h = (gap + kHalf) >> k15;
+    i_in = __builtin_amdgcn_sdot2(__builtin_bit_cast(axc_s2, h), __builtin_bit_cast(axc_s2, now.w2[j]), i_in, false);
+    hit_cnt += h;
+  }
+}
+
 // The hit sum of soma i over the slots of group gu: chunks of 8 slots
-// c = (gu + 3) % 4 + 4k below the soma's filled prefix F.
+// c = (gu + 3) % 4 + 4k below the soma's filled prefix F, one at a time.
+// (Loading each lane's first chunk before the staging gathers, or all of a
+// lane's chunks at once in a one-workgroup-per-CU pipelined pass with two
+// LDS entry buffers, measured slower.)
 template <bool STAGED, bool PW>
 __device__ __forceinline__ void scan_slots(const ShardVramPtrs& vram, uint32_t N, uint32_t A, const StagePlan& sp,
                                            const uint2* ent, const uint4* __restrict__ cur, uint32_t gu, uint32_t i,
                                            uint32_t F, uint32_t prop, int32_t& i_in, uint32_t& hits) {
-  constexpr int U = 8;
   // Group 0, which finishes the somas afterwards, takes chunks 3, 7, 11, ...
   // so that a partly filled last chunk lands on another group.
   const uint32_t cg = (gu + kStageGroups - 1) % kStageGroups;
+  axc_us2 hit_cnt = {0, 0};
 #pragma unroll 1
-  for (uint32_t c = cg; c * U < F; c += kStageGroups) {
-    const uint32_t s0 = c * U;
-    const uint32_t nv = F - s0 < (uint32_t)U ? F - s0 : (uint32_t)U;
+  for (uint32_t c = cg; c * 8 < F; c += kStageGroups) {
     Chunk<STAGED, PW> now;
     now.load(vram, sp, N, c, i);
-    uint32_t hit_mask = 0, wide = 0;
-    constexpr int H = 4;  // entries read in two halves of four
-#pragma unroll
-    for (int h0 = 0; h0 < U; h0 += H) {
-      uint2 e[H];
-#pragma unroll
-      for (int u = 0; u < H; u++) {
-        const bool live = (uint32_t)(h0 + u) < nv;
-        if (STAGED) e[u] = ent[live ? now.entry(h0 + u, A) : kStageCap];
-        else e[u] = sp.comp8[live ? now.entry(h0 + u, A) : A];
-      }
-#pragma unroll
-      for (int u = 0; u < H; u++) {
-        const bool live = (uint32_t)(h0 + u) < nv;
-        const uint32_t sgv = now.seg(h0 + u);
-        hit_mask |= (live && min_dist8(e[u], sgv) < prop) ? (1u << (h0 + u)) : 0u;
-        wide |= (live && sgv + prop > 255u) ? (1u << (h0 + u)) : 0u;
-      }
-    }
-    if (wide) {
-      // Segment + length beyond the byte form (rare): the slot's target and
-      // the 16-byte compact entry (scalar bases, lane offsets).
-#pragma unroll
-      for (int u = 0; u < U; u++) {
-        if (wide & (1u << u)) {
-          uint32_t t;
-          if constexpr (STAGED)
-            t = *reinterpret_cast<const uint32_t*>(reinterpret_cast<const char*>(sp.tgt + (size_t)c * N * 8) +
-                                                   i * 32u + u * 4u);
-          else
-            t = now.k[u];
-          const bool h = min_dist16(snap_at(cur, snap_index(t, A)), t >> 24) < prop;
-          hit_mask = h ? (hit_mask | (1u << u)) : (hit_mask & ~(1u << u));
-        }
-      }
-    }
-#pragma unroll
-    for (int u = 0; u < U; u++) i_in = wadd(i_in, (hit_mask >> u) & 1u ? now.weight(u) : 0);
-    hits += __builtin_popcount(hit_mask);
+    hit_chunk<STAGED, PW>(now, c, N, A, sp, ent, cur, i, prop, i_in, hit_cnt);
   }
+  hits += (uint32_t)hit_cnt.x + (uint32_t)hit_cnt.y;
 }
 
 // Loads the block's byte snapshot entries into ent[0, cnt); ent[kStageCap]
@@ -271,8 +342,11 @@ __device__ __forceinline__ void stage_entries(uint2* ent, const uint32_t* __rest
     uint2 v[SU];
 #pragma unroll
     for (uint32_t k = 0; k < SU; k++) {
+      // Unconditional loads (entry 0 stands in past the end), so that all
+      // eight are in flight at once.
       const uint32_t j = j0 + k * kStageThreads + tid;
-      ix[k] = j < cnt ? bset[j] : A;
+      const uint32_t a = bset[j < cnt ? j : 0u];
+      ix[k] = j < cnt ? a : A;
     }
 #pragma unroll
     for (uint32_t k = 0; k < SU; k++) v[k] = comp8[ix[k]];
@@ -285,55 +359,32 @@ __device__ __forceinline__ void stage_entries(uint2* ent, const uint32_t* __rest
   if (tid == 0) ent[kStageCap] = make_uint2(0xFFFFFFFFu, 0xFFFFFFFFu);
 }
 
-// K1 + K5 of one tick for soma block lb (256 somas) of one shard, by a
-// workgroup of kStageThreads lanes (see the file comment). `lut` is the
-// shard's table, already in LDS (visible after the first barrier here).
-// Counters go to part[] (this workgroup's own partials) or, with
-// atomic_stats, are added atomically to a shared per-shard array;
-// type_spikes (may be null) counts spikes per variant type. The tick's GSOP
-// (K6) is left to gsop_kernel, after the pushes; count_gsop adds its slot
-// and target counts here (for callers whose GSOP kernel does not count).
-// PW: the weights are read from the plan's working copy.
-template <bool PW>
-__device__ __forceinline__ void staged_block(const ShardVramPtrs& vram, uint32_t N, uint32_t A,
-                                             const AxcVariantParameters* lut, uint32_t tick, const ShardScratch& sc,
-                                             const uint4* __restrict__ cur, const StagePlan& sp, uint32_t lb,
-                                             uint64_t* part, bool atomic_stats, bool count_tick,
-                                             uint64_t* type_spikes, bool count_gsop = false) {
-  extern __shared__ uint2 ent[];  // kStageCap + 1 entries
-  __shared__ int32_t red_in[kStageGroups][256];
-  __shared__ uint16_t red_hits[kStageGroups][256];
-  __shared__ uint32_t wave_cnt[2][4];
-  __shared__ uint32_t blk_base[2];
-  __shared__ uint32_t red[4][8];
-  const uint32_t tid = threadIdx.x;
-  const uint32_t cnt = sp.cnt[lb];
-  const bool staged = cnt != kStageOverflow;
-  const uint32_t* __restrict__ bset = sp.set + (size_t)lb * kStageCap;
-  if (staged) stage_entries(ent, bset, cnt, sp.comp8, A, tid);
-  __syncthreads();
+// Workgroup scratch of one soma block's finish.
+struct BlockLds {
+  int32_t red_in[kStageGroups][256];
+  uint16_t red_hits[kStageGroups][256];
+  uint32_t wave_cnt[2][4];
+  uint32_t blk_base[2];
+  uint32_t red[4][8];
+};
 
+// The end of a block's tick, after the scan (every lane of the workgroup
+// takes part: it holds barriers): the groups' hit sums are added, group 0
+// finishes its 256 somas (GLIF update, flags) and compacts the spikers and
+// first owners into the shard's lists. Counters go to part[] (this
+// workgroup's own partials) or, with atomic_stats, are added atomically to
+// a shared per-shard array; type_spikes (may be null) counts spikes per
+// variant type; count_gsop adds the slot and target counts of the tick's
+// GSOP (for callers whose GSOP kernel does not count).
+__device__ __forceinline__ void finish_block(BlockLds& L, const ShardVramPtrs& vram, uint32_t N, uint32_t A,
+                                             const AxcVariantParameters& p, uint32_t tick, const ShardScratch& sc,
+                                             uint32_t tid, uint32_t i, uint32_t F, uint8_t flags_in, uint8_t timer,
+                                             bool refr, int32_t i_in, uint32_t hits, uint64_t* part,
+                                             bool atomic_stats, bool count_tick, uint64_t* type_spikes,
+                                             bool count_gsop) {
   const uint32_t g = tid >> 8, li = tid & 255u;
-  const uint32_t i = lb * 256 + li;
-  int32_t i_in = 0;
-  uint32_t hits = 0, F = 0;
-  uint8_t flags_in = 0, timer = 0;
-  bool refr = true;
-  if (i < N) {
-    flags_in = vram.soma_flags[i];
-    timer = vram.timers[i];
-    refr = timer > 0;
-    F = sp.fill[i];
-  }
-  const AxcVariantParameters& p = lut[(flags_in >> 4) & 0x0F];
-  // The group index is the same for the 64 lanes of a wave: scalar.
-  const uint32_t gu = __builtin_amdgcn_readfirstlane(g);
-  if (i < N && !refr) {
-    if (staged) scan_slots<true, PW>(vram, N, A, sp, ent, cur, gu, i, F, p.signal_propagation_length, i_in, hits);
-    else scan_slots<false, PW>(vram, N, A, sp, ent, cur, gu, i, F, p.signal_propagation_length, i_in, hits);
-  }
-  red_in[g][li] = i_in;
-  red_hits[g][li] = (uint16_t)hits;
+  L.red_in[g][li] = i_in;
+  L.red_hits[g][li] = (uint16_t)hits;
   __syncthreads();
 
   // Group 0 finishes its 256 somas; the other groups only join the barriers.
@@ -345,8 +396,8 @@ __device__ __forceinline__ void staged_block(const ShardVramPtrs& vram, uint32_t
     uint32_t hsum = 0;
 #pragma unroll
     for (uint32_t k = 0; k < kStageGroups; k++) {
-      sum = wadd(sum, red_in[k][li]);
-      hsum += red_hits[k][li];
+      sum = wadd(sum, L.red_in[k][li]);
+      hsum += L.red_hits[k][li];
     }
     if (!refr) {
       n_scan = F;
@@ -378,27 +429,27 @@ __device__ __forceinline__ void staged_block(const ShardVramPtrs& vram, uint32_t
     for (int off = 32; off > 0; off >>= 1) r[k] += __shfl_xor(r[k], off, 64);
   }
   if (g == 0 && lane == 0) {
-    wave_cnt[0][wv] = (uint32_t)__popcll(bs);
-    wave_cnt[1][wv] = (uint32_t)__popcll(bo);
+    L.wave_cnt[0][wv] = (uint32_t)__popcll(bs);
+    L.wave_cnt[1][wv] = (uint32_t)__popcll(bo);
 #pragma unroll
-    for (int k = 0; k < 6; k++) red[wv][k] = r[k];
+    for (int k = 0; k < 6; k++) L.red[wv][k] = r[k];
   }
   __syncthreads();
   if (tid == 0) {
-    const uint32_t cs = wave_cnt[0][0] + wave_cnt[0][1] + wave_cnt[0][2] + wave_cnt[0][3];
-    const uint32_t co = wave_cnt[1][0] + wave_cnt[1][1] + wave_cnt[1][2] + wave_cnt[1][3];
-    blk_base[0] = cs ? atomicAdd(&sc.counters[kCntSpikes], cs) : 0u;
-    blk_base[1] = co ? atomicAdd(&sc.counters[kCntOwners], co) : 0u;
+    const uint32_t cs = L.wave_cnt[0][0] + L.wave_cnt[0][1] + L.wave_cnt[0][2] + L.wave_cnt[0][3];
+    const uint32_t co = L.wave_cnt[1][0] + L.wave_cnt[1][1] + L.wave_cnt[1][2] + L.wave_cnt[1][3];
+    L.blk_base[0] = cs ? atomicAdd(&sc.counters[kCntSpikes], cs) : 0u;
+    L.blk_base[1] = co ? atomicAdd(&sc.counters[kCntOwners], co) : 0u;
     const uint64_t v[kStatsPerBlock] = {};
     uint64_t add[kStatsPerBlock];
 #pragma unroll
     for (uint32_t k = 0; k < kStatsPerBlock; k++) add[k] = v[k];
-    add[kStIntegrated] = red[0][0] + red[1][0] + red[2][0] + red[3][0];
-    add[kStScanned] = red[0][1] + red[1][1] + red[2][1] + red[3][1];
-    add[kStTargets] = red[0][2] + red[1][2] + red[2][2] + red[3][2];
-    add[kStHits] = red[0][3] + red[1][3] + red[2][3] + red[3][3];
-    add[kStGsopSlots] = red[0][4] + red[1][4] + red[2][4] + red[3][4];
-    add[kStGsopTargets] = red[0][5] + red[1][5] + red[2][5] + red[3][5];
+    add[kStIntegrated] = L.red[0][0] + L.red[1][0] + L.red[2][0] + L.red[3][0];
+    add[kStScanned] = L.red[0][1] + L.red[1][1] + L.red[2][1] + L.red[3][1];
+    add[kStTargets] = L.red[0][2] + L.red[1][2] + L.red[2][2] + L.red[3][2];
+    add[kStHits] = L.red[0][3] + L.red[1][3] + L.red[2][3] + L.red[3][3];
+    add[kStGsopSlots] = L.red[0][4] + L.red[1][4] + L.red[2][4] + L.red[3][4];
+    add[kStGsopTargets] = L.red[0][5] + L.red[1][5] + L.red[2][5] + L.red[3][5];
     add[kStSpikes] = cs;
     add[kStTicks] = count_tick ? 1 : 0;
 #pragma unroll
@@ -410,14 +461,60 @@ __device__ __forceinline__ void staged_block(const ShardVramPtrs& vram, uint32_t
   }
   __syncthreads();
   if (g == 0) {
-    uint32_t pre_s = blk_base[0], pre_o = blk_base[1];
+    uint32_t pre_s = L.blk_base[0], pre_o = L.blk_base[1];
     for (uint32_t w = 0; w < wv; w++) {
-      pre_s += wave_cnt[0][w];
-      pre_o += wave_cnt[1][w];
+      pre_s += L.wave_cnt[0][w];
+      pre_o += L.wave_cnt[1][w];
     }
     if (fin) sc.spk_list[pre_s + (uint32_t)__popcll(bs & below)] = i;
     if (own_first) sc.own_list[pre_o + (uint32_t)__popcll(bo & below)] = my_axon;
   }
+}
+
+// K1 + K5 of one tick for soma block lb (256 somas) of one shard, by a
+// workgroup of kStageThreads lanes (see the file comment). `lut` is the
+// shard's table, already in LDS (visible after the first barrier here).
+// The tick's GSOP (K6) is left to gsop_kernel, after the pushes. PW: the
+// weights are read from the plan's working copy. See finish_block for the
+// counters.
+template <bool PW>
+__device__ __forceinline__ void staged_block(const ShardVramPtrs& vram, uint32_t N, uint32_t A,
+                                             const AxcVariantParameters* lut, uint32_t tick, const ShardScratch& sc,
+                                             const uint4* __restrict__ cur, const StagePlan& sp, uint32_t lb,
+                                             uint64_t* part, bool atomic_stats, bool count_tick,
+                                             uint64_t* type_spikes, bool count_gsop = false) {
+  extern __shared__ uint2 ent[];  // kStageCap + 1 entries
+  __shared__ BlockLds L;
+  const uint32_t tid = threadIdx.x;
+  const uint32_t cnt = sp.cnt[lb];
+  const bool staged = cnt != kStageOverflow;
+  const uint32_t* __restrict__ bset = sp.set + (size_t)lb * kStageCap;
+  const uint32_t g = tid >> 8, li = tid & 255u;
+  const uint32_t i = lb * 256 + li;
+  // The somas' bytes are requested before the staging gathers, which hide
+  // their latency.
+  uint32_t F = 0;
+  uint8_t flags_in = 0, timer = 0;
+  if (i < N) {
+    flags_in = vram.soma_flags[i];
+    timer = vram.timers[i];
+    F = sp.fill[i];
+  }
+  if (staged) stage_entries(ent, bset, cnt, sp.comp8, A, tid);
+  __syncthreads();
+
+  int32_t i_in = 0;
+  uint32_t hits = 0;
+  const bool refr = i >= N || timer > 0;
+  const AxcVariantParameters& p = lut[(flags_in >> 4) & 0x0F];
+  // The group index is the same for the 64 lanes of a wave: scalar.
+  const uint32_t gu = __builtin_amdgcn_readfirstlane(g);
+  if (!refr) {
+    if (staged) scan_slots<true, PW>(vram, N, A, sp, ent, cur, gu, i, F, p.signal_propagation_length, i_in, hits);
+    else scan_slots<false, PW>(vram, N, A, sp, ent, cur, gu, i, F, p.signal_propagation_length, i_in, hits);
+  }
+  finish_block(L, vram, N, A, p, tick, sc, tid, i, F, flags_in, timer, refr, i_in, hits, part, atomic_stats,
+               count_tick, type_spikes, count_gsop);
 }
 
 // The single-shard soma pass: one workgroup per soma block, blocks dealt to
