@@ -1084,8 +1084,8 @@ hipError_t batch_capture(Batch* b, uint32_t T) {
       for (uint32_t t = 0; t < T; t++) {
         batch_axon_kernel<<<b->tune.axon_blocks, kThreads, 0, cap>>>(b->d_brains, b->d_axon_items, b->n_axon_items,
                                                                      b->d_tick, t, t);
-        batch_staged_kernel<<<b->tune.soma_blocks, kStageThreads, lds, cap>>>(b->d_brains, b->d_soma_items,
-                                                                             b->n_soma_items, b->d_tick, t, t);
+        batch_staged_kernel<<<(b->n_soma_items + kNumXcd - 1) / kNumXcd * kNumXcd, kStageThreads, lds, cap>>>(
+            b->d_brains, b->d_soma_items, b->n_soma_items, b->d_tick, t, t);
         batch_spike_prefix_kernel<<<1, 1024, 0, cap>>>(b->d_brains, n, b->d_prefix + n + 1, kCntOwners);
         batch_push_flat_kernel<<<b->tune.list_blocks, kThreads, 0, cap>>>(b->d_brains, n, b->d_prefix + n + 1, t);
         batch_spike_prefix_kernel<<<1, 1024, 0, cap>>>(b->d_brains, n, b->d_prefix, kCntSpikes);
