@@ -72,7 +72,8 @@ def cpu_baseline(brain, inputs_host: np.ndarray, seconds: float, max_ticks: int)
     return state, heads, ticks, time.perf_counter() - t0, tot
 
 
-def bench_batched(n_brains: int, ticks: int, input_ticks: int, batch_ticks: int, seed: int) -> dict:
+def bench_batched(n_brains: int, ticks: int, input_ticks: int, batch_ticks: int, seed: int,
+                  per_tick_ticks: int = 200) -> dict:
     """Batched brains (128 brains of 8k-250k neurons, slots in {32,64,128},
     their own parameter vectors and input rates) stepped together, against the
     same brains stepped one after another with the plain engine. Every brain's
@@ -139,6 +140,28 @@ def bench_batched(n_brains: int, ticks: int, input_ticks: int, batch_ticks: int,
     identical = all(np.array_equal(rb[0], m["vram"].download_state()) and
                     np.array_equal(rb[1], m["vram"].download_axons()) for rb, m in zip(res_b, mems))
     log(f"batched vs serial: {'bit-identical' if identical else 'MISMATCH'} for all {n_brains} brains")
+    # Serial per-tick launches, the reference's structure (one cu_step_day_phase
+    # call per tick, compute/shard.rs:75-120), timed on the first pt_ticks
+    # ticks of every brain and scaled to the full tick count.
+    pt_ticks = min(ticks, per_tick_ticks)
+    for (b, _, _), m in zip(members, mems):
+        m["vram"].upload_state(b.state)
+        m["vram"].upload_axon_heads(b.heads)
+    w0 = time.perf_counter()
+    for (b, dop, _), m in zip(members, mems):
+        upload_constant_memory(b.lut)
+        v = m["vram"]
+        for t in range(pt_ticks):
+            row = t % input_ticks
+            err = lib.cu_step_day_phase(C.byref(v.ptrs), v.padded_n, v.total_axons, 1, t,
+                                        m["d_inputs"].ptr + row * b.input_words * 4, b.virtual_offset, b.n_inputs,
+                                        None, 0, None, None, 0, dop, st)
+            if err:
+                raise RuntimeError(f"per-tick step failed {err}")
+        lib.gpu_stream_synchronize(st)
+    t_per_tick = (time.perf_counter() - w0) * ticks / pt_ticks
+    log(f"serial per-tick launches: {pt_ticks} ticks x {n_brains} brains in {time.perf_counter() - w0:.2f}s "
+        f"({t_per_tick:.1f}s scaled to {ticks} ticks)")
     lib.gpu_stream_destroy(st)
     rates = [s[0]["spikes"] / max(1, s[0]["ticks"] * b.padded_n) for s, (b, _, _) in zip(stats, members)]
     return {
@@ -147,6 +170,9 @@ def bench_batched(n_brains: int, ticks: int, input_ticks: int, batch_ticks: int,
         "speedup": t_serial / t_batched, "bit_identical_all_brains": identical, "tuning": tune,
         "batched_seconds": t_batched, "serial_seconds": t_serial,
         "serial": "each brain alone through axc_step_day_phase_batch (HIP-graph batches), one after another",
+        "serial_per_tick_brains_per_s": n_brains / t_per_tick, "speedup_vs_per_tick": t_per_tick / t_batched,
+        "serial_per_tick": f"each brain alone, one cu_step_day_phase call per tick (the reference's launch "
+                           f"structure), timed over its first {pt_ticks} ticks and scaled to {ticks}",
         "firing_rate_median": float(np.median(rates)),
     }
 
