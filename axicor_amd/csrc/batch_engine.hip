@@ -141,13 +141,13 @@ __global__ __launch_bounds__(256) void batch_gsop_kernel(const BrainDev* __restr
 
 __global__ void batch_set_tick_kernel(BatchTick* __restrict__ bt, uint32_t tick_base) { bt->tick_base = tick_base; }
 
-// Staged soma phase over all brains: one workgroup per work item of 256
-// somas (staged_block, weights from each brain's working copy), the items of
+// Staged soma phase over all brains: one workgroup per work item of
+// kBlockSomas somas (staged_block, weights from each brain's working copy), the items of
 // one XCD a contiguous eighth of the list (grid: the item count rounded up to
 // a multiple of 8). GSOP counts are taken here, for batch_gsop_chunks_kernel.
 // (A persistent grid walking the items measured slower: the values the
 // compiler keeps across items spill.)
-__global__ __launch_bounds__(kStageThreads) __attribute__((amdgpu_waves_per_eu(2 * kStageGroups, 2 * kStageGroups)))
+__global__ __launch_bounds__(kStageThreads) __attribute__((amdgpu_waves_per_eu(8, 8)))
 void batch_staged_kernel(const BrainDev* __restrict__ brains, const WorkItem* __restrict__ items, uint32_t n_items,
                          const BatchTick* __restrict__ bt, uint32_t t_off, uint32_t snap) {
   __shared__ AxcVariantParameters lut[16];
@@ -158,8 +158,8 @@ void batch_staged_kernel(const BrainDev* __restrict__ brains, const WorkItem* __
   const BrainDev& b = brains[w.brain];
   if (threadIdx.x < 64)
     reinterpret_cast<uint4*>(lut)[threadIdx.x] = reinterpret_cast<const uint4*>(b.lut)[threadIdx.x];
-  staged_block<true>(b.vram, b.N, b.A, lut, bt->tick_base + t_off, b.sc, b.sc.comp[snap & 1], b.sp, w.start >> 8,
-                     b.stats, true, w.start == 0, b.type_spikes, true);
+  staged_block<true>(b.vram, b.N, b.A, lut, bt->tick_base + t_off, b.sc, b.sc.comp[snap & 1], b.sp,
+                     w.start / kBlockSomas, b.stats, true, w.start == 0, b.type_spikes, true);
 }
 
 // Exclusive prefix of the brains' list counts of this tick (one workgroup):

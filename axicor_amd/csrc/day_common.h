@@ -85,7 +85,10 @@ struct ShardScratch {
 // (somas [256b, 256b+256)) reads the byte snapshot entries of the distinct
 // presynaptic axons of its filled slots from LDS: set[b*kStageCap + j] is the
 // axon of block-local entry j (A for targets outside [0, A)).
-constexpr uint32_t kStageCap = 8704;             // LDS entries per block (8 B each)
+#ifndef AXC_STAGE_CAP
+#define AXC_STAGE_CAP 8704
+#endif
+constexpr uint32_t kStageCap = AXC_STAGE_CAP;    // LDS entries per block (8 B each)
 constexpr uint32_t kStageOverflow = 0xFFFFFFFFu; // cnt of a block whose set does not fit
 struct StagePlan {
   uint16_t* code;  // [16][N][8] block-local entry of each filled slot's axon: slot s of soma i at

@@ -46,6 +46,10 @@ constexpr uint32_t kThreads = 256;
 inline uint32_t blocks_for(uint64_t n, uint32_t t = kThreads) { return (uint32_t)((n + t - 1) / t); }
 // Soma blocks of neuron_tick, padded to a multiple of the XCD count.
 inline uint32_t neuron_blocks(uint32_t N) { return (blocks_for(N) + kNumXcd - 1) / kNumXcd * kNumXcd; }
+// Blocks of the staged soma pass (kBlockSomas somas each), and that count
+// padded to a multiple of the XCD count (its grid).
+inline uint32_t stage_blocks(uint32_t N) { return blocks_for(N, kBlockSomas); }
+inline uint32_t staged_grid(uint32_t N) { return (stage_blocks(N) + kNumXcd - 1) / kNumXcd * kNumXcd; }
 
 // ---- per-device variant table (the reference's __constant__ VARIANT_LUT[16],
 // physics.cu:82; process-global per device like constant memory) -----------
@@ -174,7 +178,7 @@ bool ensure_stage(Shard* s, const ShardVramPtrs& v, hipStream_t st, hipError_t* 
   *err = hipSuccess;
   if (!g_staging.load() || s->N == 0) return false;
   const size_t N = s->N, A = s->A;
-  const size_t blocks = neuron_blocks((uint32_t)N);
+  const size_t blocks = staged_grid((uint32_t)N);
   if (!s->sp.comp8) {
     static std::once_flag attr_once;
     std::call_once(attr_once, [] {
@@ -207,7 +211,8 @@ bool ensure_stage(Shard* s, const ShardVramPtrs& v, hipStream_t st, hipError_t* 
     s->stage_dirty = true;
   }
   if (s->stage_dirty.exchange(false)) {
-    stage_build_kernel<<<blocks_for(N), kThreads, 0, st>>>(v.dendrite_targets, (uint32_t)N, (uint32_t)A, s->sp);
+    stage_build_kernel<<<stage_blocks((uint32_t)N), kBlockSomas, 0, st>>>(v.dendrite_targets, (uint32_t)N,
+                                                                          (uint32_t)A, s->sp);
     *err = hipGetLastError();
     if (*err != hipSuccess) {
       s->stage_dirty = true;
@@ -251,7 +256,7 @@ uint64_t targets_version_of(const void* state_base) {
 
 hipError_t alloc_scratch(Shard* s) {
   const size_t N = s->N ? s->N : 1, A = s->A ? s->A : 1;
-  s->stats_blocks = neuron_blocks((uint32_t)N);
+  s->stats_blocks = std::max(neuron_blocks((uint32_t)N), staged_grid((uint32_t)N));
   hipError_t e;
   if ((e = hipMalloc((void**)&s->sc.spk_list, N * 4)) != hipSuccess) return e;
   if ((e = hipMalloc((void**)&s->sc.own_list, A * 4)) != hipSuccess) return e;
@@ -348,10 +353,10 @@ hipError_t enqueue_tick(const ShardVramPtrs& v, Shard* s, uint32_t v_seg, uint32
     TimedScope ts(kKNeuron, st, timed);
     const size_t lds = (kStageCap + 1) * sizeof(uint2);
     if (pw)
-      neuron_staged_kernel<true><<<neuron_blocks(N), kStageThreads, lds, st>>>(v, N, A, lut, prm, tick, s->sc, cur,
+      neuron_staged_kernel<true><<<staged_grid(N), kStageThreads, lds, st>>>(v, N, A, lut, prm, tick, s->sc, cur,
                                                                                s->sp);
     else
-      neuron_staged_kernel<false><<<neuron_blocks(N), kStageThreads, lds, st>>>(v, N, A, lut, prm, tick, s->sc, cur,
+      neuron_staged_kernel<false><<<staged_grid(N), kStageThreads, lds, st>>>(v, N, A, lut, prm, tick, s->sc, cur,
                                                                                 s->sp);
   } else if (N > 0) {
     TimedScope ts(kKNeuron, st, timed);
@@ -1050,7 +1055,8 @@ struct Batch {
   int16_t* d_prune = nullptr;          // [n] prune thresholds of the last axc_batch_synapse_stats
   WorkItem* d_axon_items = nullptr;
   WorkItem* d_soma_items = nullptr;
-  uint32_t n_axon_items = 0, n_soma_items = 0;
+  WorkItem* d_stage_items = nullptr;  // (brain, first soma) of each kBlockSomas block, for the staged pass
+  uint32_t n_axon_items = 0, n_soma_items = 0, n_stage_items = 0;
   BatchTick* d_tick = nullptr;
   AxcBatchTuning tune{0, 0, 0};
   hipGraphExec_t gexec = nullptr;
@@ -1084,8 +1090,8 @@ hipError_t batch_capture(Batch* b, uint32_t T) {
       for (uint32_t t = 0; t < T; t++) {
         batch_axon_kernel<<<b->tune.axon_blocks, kThreads, 0, cap>>>(b->d_brains, b->d_axon_items, b->n_axon_items,
                                                                      b->d_tick, t, t);
-        batch_staged_kernel<<<(b->n_soma_items + kNumXcd - 1) / kNumXcd * kNumXcd, kStageThreads, lds, cap>>>(
-            b->d_brains, b->d_soma_items, b->n_soma_items, b->d_tick, t, t);
+        batch_staged_kernel<<<(b->n_stage_items + kNumXcd - 1) / kNumXcd * kNumXcd, kStageThreads, lds, cap>>>(
+            b->d_brains, b->d_stage_items, b->n_stage_items, b->d_tick, t, t);
         batch_spike_prefix_kernel<<<1, 1024, 0, cap>>>(b->d_brains, n, b->d_prefix + n + 1, kCntOwners);
         batch_push_flat_kernel<<<b->tune.list_blocks, kThreads, 0, cap>>>(b->d_brains, n, b->d_prefix + n + 1, t);
         batch_spike_prefix_kernel<<<1, 1024, 0, cap>>>(b->d_brains, n, b->d_prefix, kCntSpikes);
@@ -1151,6 +1157,7 @@ void batch_free(Batch* b) {
   (void)hipFree(b->d_prune);
   (void)hipFree(b->d_axon_items);
   (void)hipFree(b->d_soma_items);
+  (void)hipFree(b->d_stage_items);
   (void)hipFree(b->d_prefix);
   (void)hipFree(b->d_tick);
   delete b;
@@ -1166,7 +1173,7 @@ int32_t axc_batch_create(const AxcBrainDesc* brains, uint32_t n, void** out) {
   (void)hipGetDevice(&b->device);
   b->desc.assign(brains, brains + n);
   hipError_t e = hipSuccess;
-  std::vector<WorkItem> ax, so;
+  std::vector<WorkItem> ax, so, sg;
   for (uint32_t k = 0; k < n && e == hipSuccess; k++) {
     const AxcBrainDesc& d = brains[k];
     if (d.num_virtual_axons && (uint64_t)d.virtual_offset + d.num_virtual_axons > d.total_axons) {
@@ -1192,6 +1199,7 @@ int32_t axc_batch_create(const AxcBrainDesc* brains, uint32_t n, void** out) {
     b->built_version.push_back(targets_version_of(d.vram.soma_voltage));
     for (uint32_t a = 0; a < d.total_axons; a += kThreads) ax.push_back(WorkItem{k, a});
     for (uint32_t i = 0; i < d.padded_n; i += kThreads) so.push_back(WorkItem{k, i});
+    for (uint32_t i = 0; i < d.padded_n; i += kBlockSomas) sg.push_back(WorkItem{k, i});
   }
   if (e == hipSuccess) e = hipMalloc((void**)&b->d_luts, (size_t)n * 16 * sizeof(AxcVariantParameters));
   if (e == hipSuccess) e = hipMalloc((void**)&b->d_stats, (size_t)n * kStatsPerBlock * 8);
@@ -1208,6 +1216,10 @@ int32_t axc_batch_create(const AxcBrainDesc* brains, uint32_t n, void** out) {
     if (e == hipSuccess) e = hipMalloc((void**)&b->d_soma_items, so.size() * sizeof(WorkItem));
     if (e == hipSuccess) e = hipMemcpy(b->d_axon_items, ax.data(), ax.size() * sizeof(WorkItem), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(b->d_soma_items, so.data(), so.size() * sizeof(WorkItem), hipMemcpyHostToDevice);
+    b->n_stage_items = (uint32_t)sg.size();
+    if (e == hipSuccess) e = hipMalloc((void**)&b->d_stage_items, sg.size() * sizeof(WorkItem));
+    if (e == hipSuccess)
+      e = hipMemcpy(b->d_stage_items, sg.data(), sg.size() * sizeof(WorkItem), hipMemcpyHostToDevice);
   }
   if (e == hipSuccess) {
     b->host_dev.resize(n);
@@ -1259,7 +1271,7 @@ int32_t axc_batch_step(void* h, uint32_t tick_base, uint32_t ticks, void* stream
       const AxcBrainDesc& d = b->desc[k];
       const uint64_t v = targets_version_of(d.vram.soma_voltage);
       if (v == b->built_version[k] || d.padded_n == 0) continue;
-      stage_build_kernel<<<blocks_for(d.padded_n), kThreads, 0, st>>>(d.vram.dendrite_targets, d.padded_n,
+      stage_build_kernel<<<stage_blocks(d.padded_n), kBlockSomas, 0, st>>>(d.vram.dendrite_targets, d.padded_n,
                                                                       d.total_axons, b->shards[k]->sp);
       hipError_t e = hipGetLastError();
       if (e != hipSuccess) return (int32_t)e;

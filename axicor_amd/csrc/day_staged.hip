@@ -36,7 +36,17 @@ namespace axc {
 // 8 slots. (Two groups at half the occupancy, with the next chunk's loads
 // issued early, measured slower.)
 constexpr uint32_t kStageGroups = 4;
-constexpr uint32_t kStageThreads = 256 * kStageGroups;
+// Somas per staged block (the block's lanes are kStageGroups groups of this
+// many). 128-soma blocks (3 or 4 workgroups per CU, kStageCap 5248 or 4544)
+// measured 25-30% slower at 1.5M neurons: their staging costs more per soma
+// (32 entries per soma against 25) than the extra workgroups recover.
+#ifndef AXC_BLOCK_SOMAS
+#define AXC_BLOCK_SOMAS 256
+#endif
+constexpr uint32_t kBlockSomas = AXC_BLOCK_SOMAS;
+static_assert(kBlockSomas == 128 || kBlockSomas == 256, "staged blocks of 128 or 256 somas");
+constexpr uint32_t kStageThreads = kBlockSomas * kStageGroups;
+constexpr uint32_t kGroup0Waves = kBlockSomas / 64;  // the waves of group 0
 constexpr uint32_t kHashSize = 16384;     // LDS hash set of the build kernel
 constexpr uint32_t kHashEmpty = 0xFFFFFFFFu;
 constexpr uint32_t kHashMaxProbe = 256;
@@ -48,15 +58,15 @@ __device__ __forceinline__ uint32_t hash_slot(uint32_t key) { return (key * 2654
 // that the set (and the LDS entries) are in axon order: consecutive lanes of
 // the per-tick staging loop then gather neighbouring snapshot entries, which
 // share cache lines. A slot's block-local entry is the rank of its axon.
-__global__ __launch_bounds__(256) void stage_build_kernel(const uint32_t* __restrict__ targets, uint32_t N,
+__global__ __launch_bounds__(kBlockSomas) void stage_build_kernel(const uint32_t* __restrict__ targets, uint32_t N,
                                                           uint32_t A, StagePlan sp) {
   __shared__ uint32_t keys[kHashSize];
   __shared__ uint32_t sorted[kHashSize];
-  __shared__ uint32_t scan[256];
+  __shared__ uint32_t scan[kBlockSomas];
   __shared__ uint32_t overflow;
   const uint32_t tid = threadIdx.x, b = blockIdx.x;
-  const uint32_t i = b * 256 + tid;
-  for (uint32_t j = tid; j < kHashSize; j += 256) keys[j] = kHashEmpty;
+  const uint32_t i = b * kBlockSomas + tid;
+  for (uint32_t j = tid; j < kHashSize; j += kBlockSomas) keys[j] = kHashEmpty;
   if (tid == 0) overflow = 0;
   __syncthreads();
   uint32_t F = 0;
@@ -78,19 +88,19 @@ __global__ __launch_bounds__(256) void stage_build_kernel(const uint32_t* __rest
     for (uint32_t s = F; s & 7; s++) sp.tgt[((size_t)(s >> 3) * N + i) * 8 + (s & 7)] = 0;
   }
   __syncthreads();
-  // Compact the table (lane tid owns positions [64 tid, 64 tid + 64)).
-  constexpr uint32_t kPer = kHashSize / 256;
+  // Compact the table (lane tid owns positions [kPer tid, kPer tid + kPer)).
+  constexpr uint32_t kPer = kHashSize / kBlockSomas;
   uint32_t mine = 0;
   for (uint32_t j = 0; j < kPer; j++) mine += keys[tid * kPer + j] != kHashEmpty;
   scan[tid] = mine;
   __syncthreads();
-  for (uint32_t off = 1; off < 256; off <<= 1) {
+  for (uint32_t off = 1; off < kBlockSomas; off <<= 1) {
     const uint32_t v = tid >= off ? scan[tid - off] : 0u;
     __syncthreads();
     scan[tid] += v;
     __syncthreads();
   }
-  const uint32_t total = scan[255];
+  const uint32_t total = scan[kBlockSomas - 1];
   const bool fits = overflow == 0 && total <= kStageCap;
   if (tid == 0) sp.cnt[b] = fits ? total : kStageOverflow;
   if (!fits) return;  // block-uniform
@@ -103,11 +113,11 @@ __global__ __launch_bounds__(256) void stage_build_kernel(const uint32_t* __rest
   // padded with kHashEmpty (which sorts last).
   uint32_t S = 1;
   while (S < total) S <<= 1;
-  for (uint32_t j = total + tid; j < S; j += 256) sorted[j] = kHashEmpty;
+  for (uint32_t j = total + tid; j < S; j += kBlockSomas) sorted[j] = kHashEmpty;
   __syncthreads();
   for (uint32_t k = 2; k <= S; k <<= 1) {
     for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-      for (uint32_t x = tid; x < S; x += 256) {
+      for (uint32_t x = tid; x < S; x += kBlockSomas) {
         const uint32_t y = x ^ j;
         if (y > x) {
           const uint32_t a = sorted[x], c = sorted[y];
@@ -122,7 +132,7 @@ __global__ __launch_bounds__(256) void stage_build_kernel(const uint32_t* __rest
     }
   }
   __syncthreads();  // also when S == 1 and the loops above did not run
-  for (uint32_t j = tid; j < total; j += 256) sp.set[(size_t)b * kStageCap + j] = sorted[j];
+  for (uint32_t j = tid; j < total; j += kBlockSomas) sp.set[(size_t)b * kStageCap + j] = sorted[j];
   if (i >= N) return;
   for (uint32_t s = 0; s < F; s++) {
     const size_t o = (size_t)s * N + i;
@@ -359,10 +369,18 @@ __device__ __forceinline__ void stage_entries(uint2* ent, const uint32_t* __rest
   if (tid == 0) ent[kStageCap] = make_uint2(0xFFFFFFFFu, 0xFFFFFFFFu);
 }
 
+// Sum of column k over the waves of group 0.
+__device__ __forceinline__ uint32_t wave_sum(const uint32_t (&red)[4][8], int k) {
+  uint32_t v = 0;
+#pragma unroll
+  for (uint32_t w = 0; w < kGroup0Waves; w++) v += red[w][k];
+  return v;
+}
+
 // Workgroup scratch of one soma block's finish.
 struct BlockLds {
-  int32_t red_in[kStageGroups][256];
-  uint16_t red_hits[kStageGroups][256];
+  int32_t red_in[kStageGroups][kBlockSomas];
+  uint16_t red_hits[kStageGroups][kBlockSomas];
   uint32_t wave_cnt[2][4];
   uint32_t blk_base[2];
   uint32_t red[4][8];
@@ -382,7 +400,7 @@ __device__ __forceinline__ void finish_block(BlockLds& L, const ShardVramPtrs& v
                                              bool refr, int32_t i_in, uint32_t hits, uint64_t* part,
                                              bool atomic_stats, bool count_tick, uint64_t* type_spikes,
                                              bool count_gsop) {
-  const uint32_t g = tid >> 8, li = tid & 255u;
+  const uint32_t g = tid / kBlockSomas, li = tid % kBlockSomas;
   L.red_in[g][li] = i_in;
   L.red_hits[g][li] = (uint16_t)hits;
   __syncthreads();
@@ -436,20 +454,24 @@ __device__ __forceinline__ void finish_block(BlockLds& L, const ShardVramPtrs& v
   }
   __syncthreads();
   if (tid == 0) {
-    const uint32_t cs = L.wave_cnt[0][0] + L.wave_cnt[0][1] + L.wave_cnt[0][2] + L.wave_cnt[0][3];
-    const uint32_t co = L.wave_cnt[1][0] + L.wave_cnt[1][1] + L.wave_cnt[1][2] + L.wave_cnt[1][3];
+    uint32_t cs = 0, co = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < kGroup0Waves; w++) {
+      cs += L.wave_cnt[0][w];
+      co += L.wave_cnt[1][w];
+    }
     L.blk_base[0] = cs ? atomicAdd(&sc.counters[kCntSpikes], cs) : 0u;
     L.blk_base[1] = co ? atomicAdd(&sc.counters[kCntOwners], co) : 0u;
     const uint64_t v[kStatsPerBlock] = {};
     uint64_t add[kStatsPerBlock];
 #pragma unroll
     for (uint32_t k = 0; k < kStatsPerBlock; k++) add[k] = v[k];
-    add[kStIntegrated] = L.red[0][0] + L.red[1][0] + L.red[2][0] + L.red[3][0];
-    add[kStScanned] = L.red[0][1] + L.red[1][1] + L.red[2][1] + L.red[3][1];
-    add[kStTargets] = L.red[0][2] + L.red[1][2] + L.red[2][2] + L.red[3][2];
-    add[kStHits] = L.red[0][3] + L.red[1][3] + L.red[2][3] + L.red[3][3];
-    add[kStGsopSlots] = L.red[0][4] + L.red[1][4] + L.red[2][4] + L.red[3][4];
-    add[kStGsopTargets] = L.red[0][5] + L.red[1][5] + L.red[2][5] + L.red[3][5];
+    add[kStIntegrated] = wave_sum(L.red, 0);
+    add[kStScanned] = wave_sum(L.red, 1);
+    add[kStTargets] = wave_sum(L.red, 2);
+    add[kStHits] = wave_sum(L.red, 3);
+    add[kStGsopSlots] = wave_sum(L.red, 4);
+    add[kStGsopTargets] = wave_sum(L.red, 5);
     add[kStSpikes] = cs;
     add[kStTicks] = count_tick ? 1 : 0;
 #pragma unroll
@@ -489,8 +511,8 @@ __device__ __forceinline__ void staged_block(const ShardVramPtrs& vram, uint32_t
   const uint32_t cnt = sp.cnt[lb];
   const bool staged = cnt != kStageOverflow;
   const uint32_t* __restrict__ bset = sp.set + (size_t)lb * kStageCap;
-  const uint32_t g = tid >> 8, li = tid & 255u;
-  const uint32_t i = lb * 256 + li;
+  const uint32_t g = tid / kBlockSomas, li = tid % kBlockSomas;
+  const uint32_t i = lb * kBlockSomas + li;
   // The somas' bytes are requested before the staging gathers, which hide
   // their latency.
   uint32_t F = 0;
@@ -518,10 +540,10 @@ __device__ __forceinline__ void staged_block(const ShardVramPtrs& vram, uint32_t
 }
 
 // The single-shard soma pass: one workgroup per soma block, blocks dealt to
-// XCDs in contiguous eighths (see neuron_tick_kernel). Two workgroups per CU:
-// 2 * kStageThreads / 64 waves over 4 SIMDs.
+// XCDs in contiguous eighths (see neuron_tick_kernel). Eight waves per SIMD:
+// 2048 / kStageThreads workgroups per CU.
 template <bool PW>
-__global__ __launch_bounds__(kStageThreads) __attribute__((amdgpu_waves_per_eu(2 * kStageGroups, 2 * kStageGroups)))
+__global__ __launch_bounds__(kStageThreads) __attribute__((amdgpu_waves_per_eu(8, 8)))
 void neuron_staged_kernel(ShardVramPtrs vram, uint32_t N, uint32_t A, const AxcVariantParameters* __restrict__ lut_g,
                           const TickParams* __restrict__ prm, uint32_t tick_arg, ShardScratch sc,
                           const uint4* __restrict__ cur, StagePlan sp) {
