@@ -73,7 +73,7 @@ def cpu_baseline(brain, inputs_host: np.ndarray, seconds: float, max_ticks: int)
 
 
 def bench_batched(n_brains: int, ticks: int, input_ticks: int, batch_ticks: int, seed: int,
-                  per_tick_ticks: int = 200) -> dict:
+                  per_tick_ticks: int = 200, serial: bool = True) -> dict:
     """Batched brains (128 brains of 8k-250k neurons, slots in {32,64,128},
     their own parameter vectors and input rates) stepped together, against the
     same brains stepped one after another with the plain engine. Every brain's
@@ -118,6 +118,10 @@ def bench_batched(n_brains: int, ticks: int, input_ticks: int, batch_ticks: int,
     be.destroy()
     res_b = [(m["vram"].download_state(), m["vram"].download_axons()) for m in mems]
     log(f"batched: {ticks} ticks in {t_batched:.2f}s")
+    if not serial:
+        lib.gpu_stream_destroy(st)
+        return {"brains": n_brains, "ticks_per_brain": ticks, "neurons": neurons, "synapses": synapses,
+                "batched_brains_per_s": n_brains / t_batched, "batched_seconds": t_batched, "tuning": tune}
     # Serial: the same brains from their initial state, one after another.
     for (b, _, _), m in zip(members, mems):
         m["vram"].upload_state(b.state)
@@ -192,6 +196,9 @@ def main() -> int:
                     help="ticks per batch of the two batches whose kernels are timed one by one with HIP events")
     ap.add_argument("--batched-brains", type=int, default=128, help="0 skips the batched-brains leg")
     ap.add_argument("--batched-ticks", type=int, default=5000)
+    ap.add_argument("--batched-only", action="store_true",
+                    help="run only the batched-brains leg (for profiling); prints its JSON object")
+    ap.add_argument("--batched-no-serial", action="store_true", help="skip the serial runs of the batched leg")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="also run one device-to-device copy of this size (reference for HBM byte counters)")
     args = ap.parse_args()
@@ -207,6 +214,11 @@ def main() -> int:
 
     lib = day_lib()
     lib.gpu_set_device(0)
+
+    if args.batched_only:
+        print(json.dumps({"batched": bench_batched(args.batched_brains, args.batched_ticks, 1000, 50, 20261203,
+                                                   serial=not args.batched_no_serial)}), flush=True)
+        return 0
 
     spec = {"c2": B.spec_c2, "c1": B.spec_c1, "c2-linear": lambda: B.spec_c2(brick=0)}[args.workload]()
     t0 = time.perf_counter()
@@ -389,7 +401,8 @@ def main() -> int:
     }
     if args.batched_brains:
         del vram, eng, d_in
-        out["batched"] = bench_batched(args.batched_brains, args.batched_ticks, 1000, 50, 20261203)
+        out["batched"] = bench_batched(args.batched_brains, args.batched_ticks, 1000, 50, 20261203,
+                                       serial=not args.batched_no_serial)
     if cpu is not None:
         out["cpu_baseline"] = {
             "value": cpu["stats"]["slots_scanned"] / cpu["elapsed"], "unit": "synaptic events/s",
