@@ -14,7 +14,8 @@ def bf16_bits(x):
 
 
 @pytest.mark.parametrize("ta,tb", [(False, False), (True, False), (False, True), (True, True)])
-@pytest.mark.parametrize("M_,N_,K_", [(64, 64, 32), (104, 72, 40), (256, 200, 296), (8, 16, 8), (128, 8, 520)])
+@pytest.mark.parametrize("M_,N_,K_", [(64, 64, 32), (104, 72, 40), (256, 200, 296), (8, 16, 8), (128, 8, 520),
+                                      (384, 256, 512), (520, 136, 64), (128, 128, 24)])
 def test_gemm_all_operand_layouts(gpu, ta, tb, M_, N_, K_):
     from axicor_amd.surrogate import gemm
     rng = np.random.default_rng(M_ * 7 + N_ * 3 + K_ + 2 * ta + tb)
@@ -31,12 +32,13 @@ def test_gemm_all_operand_layouts(gpu, ta, tb, M_, N_, K_):
 
 
 def test_gemm_ragged_rows(gpu):
-    """M not a multiple of 8 or of the 64-row tile (row-major A only)."""
+    """M not a multiple of 8 or of the 64- and 128-row tiles (row-major A
+    only), with N below and above 128 (the two tile sizes)."""
     from axicor_amd.surrogate import gemm
     rng = np.random.default_rng(11)
-    for M_ in (1, 37, 65, 130):
+    for M_, N_ in ((1, 24), (37, 24), (65, 24), (130, 24), (129, 256), (300, 136), (1000, 512)):
         a = M.bf16_round(rng.standard_normal((M_, 48)).astype(np.float32)).astype(np.float64)
-        b = M.bf16_round(rng.standard_normal((24, 48)).astype(np.float32)).astype(np.float64)
+        b = M.bf16_round(rng.standard_normal((N_, 48)).astype(np.float32)).astype(np.float64)
         c = gemm(bf16_bits(a.astype(np.float32)), bf16_bits(b.astype(np.float32)), False, True)
         np.testing.assert_allclose(c, a @ b.T, rtol=0, atol=1e-4 * (np.abs(a) @ np.abs(b.T)).max())
 
