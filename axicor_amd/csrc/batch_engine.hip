@@ -198,15 +198,17 @@ __device__ __forceinline__ uint32_t brain_of(const uint32_t* __restrict__ prefix
   return lo;
 }
 
-// Push phase over the flattened owner lists of all brains (prefix of the
-// owner counts from batch_spike_prefix_kernel).
+// Push phase over the flattened spike lists of all brains (prefix of the
+// spike counts from batch_spike_prefix_kernel): each spiker's own axon, as
+// push_spikers_kernel.
 __global__ __launch_bounds__(256) void batch_push_flat_kernel(const BrainDev* __restrict__ brains, uint32_t n_brains,
                                                               const uint32_t* __restrict__ prefix, uint32_t snap) {
   const uint32_t total = prefix[n_brains];
   for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < total; k += gridDim.x * blockDim.x) {
     const uint32_t bi = brain_of(prefix, n_brains, k);
     const BrainDev& b = brains[bi];
-    push_apply_one(b.vram.axon_heads, b.sc, b.sc.comp[snap & 1], k - prefix[bi], b.sp.comp8);
+    push_spiker_one(b.vram.axon_heads, b.vram.soma_to_axon, b.A, b.sc, b.sc.comp[snap & 1], k - prefix[bi],
+                    b.sp.comp8);
   }
 }
 

@@ -369,7 +369,11 @@ hipError_t enqueue_tick(const ShardVramPtrs& v, Shard* s, uint32_t v_seg, uint32
   }
   {
     TimedScope ts(kKPush, st, timed);
-    push_apply_kernel<<<kListBlocks / 4, kThreads, 0, st>>>(v.axon_heads, s->sc, cur, staged ? s->sp.comp8 : nullptr);
+    if (staged)
+      push_spikers_kernel<<<kListBlocks / 4, kThreads, 0, st>>>(v.axon_heads, v.soma_to_axon, A, s->sc, cur,
+                                                                s->sp.comp8);
+    else
+      push_apply_kernel<<<kListBlocks / 4, kThreads, 0, st>>>(v.axon_heads, s->sc, cur, nullptr);
   }
   if (gsop_now) {
     TimedScope ts(kKGsop, st, timed);
@@ -1092,9 +1096,8 @@ hipError_t batch_capture(Batch* b, uint32_t T) {
                                                                      b->d_tick, t, t);
         batch_staged_kernel<<<(b->n_stage_items + kNumXcd - 1) / kNumXcd * kNumXcd, kStageThreads, lds, cap>>>(
             b->d_brains, b->d_stage_items, b->n_stage_items, b->d_tick, t, t);
-        batch_spike_prefix_kernel<<<1, 1024, 0, cap>>>(b->d_brains, n, b->d_prefix + n + 1, kCntOwners);
-        batch_push_flat_kernel<<<b->tune.list_blocks, kThreads, 0, cap>>>(b->d_brains, n, b->d_prefix + n + 1, t);
         batch_spike_prefix_kernel<<<1, 1024, 0, cap>>>(b->d_brains, n, b->d_prefix, kCntSpikes);
+        batch_push_flat_kernel<<<b->tune.list_blocks, kThreads, 0, cap>>>(b->d_brains, n, b->d_prefix, t);
         batch_gsop_chunks_kernel<<<b->tune.list_blocks * 8, kThreads, 0, cap>>>(b->d_brains, n, b->d_prefix, t);
       }
       batch_copy_weights_kernel<false><<<b->tune.soma_blocks, kThreads, 0, cap>>>(b->d_brains, b->d_soma_items,

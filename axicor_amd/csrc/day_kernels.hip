@@ -249,17 +249,19 @@ __global__ __launch_bounds__(256) void spike_count_kernel(const uint32_t* __rest
 // dendritic input i_in: refractory countdown, or homeostasis decay, adaptive
 // leak, leak toward rest, threshold test, heartbeat, resets, burst count and
 // flags. `flags` already has K1 and the spike bit cleared. Sets fin (spike)
-// and, for a spike, the soma's own axon to push.
-__device__ __forceinline__ void soma_tick(const ShardVramPtrs& vram, uint32_t i, uint32_t A,
-                                          const AxcVariantParameters& p, uint8_t flags, uint8_t timer, bool refr,
-                                          uint32_t tick, int32_t i_in, bool& fin, bool& push_own,
-                                          uint32_t& my_axon) {
+// and, for a spike, the soma's own axon to push. thr_in and v_in are the
+// soma's threshold offset and voltage as read before the tick (used only
+// when it integrates).
+__device__ __forceinline__ void soma_tick_with(const ShardVramPtrs& vram, uint32_t i, uint32_t A,
+                                               const AxcVariantParameters& p, uint8_t flags, uint8_t timer, bool refr,
+                                               uint32_t tick, int32_t i_in, int32_t thr_in, int32_t v_in, bool& fin,
+                                               bool& push_own, uint32_t& my_axon) {
   if (refr) {
     vram.timers[i] = (uint8_t)(timer - 1);
     vram.soma_flags[i] = flags;
     return;
   }
-  int32_t thr = vram.threshold_offset[i];
+  int32_t thr = thr_in;
   int32_t decayed = wsub(thr, (int32_t)p.homeostasis_decay);
   thr = decayed & ~(decayed >> 31);
   int32_t leak = p.leak_rate;
@@ -269,7 +271,7 @@ __device__ __forceinline__ void soma_tick(const ShardVramPtrs& vram, uint32_t i,
     int32_t over = wsub(leak, p.adaptive_leak_max);
     leak = wsub(leak, over & ~(over >> 31));
   }
-  int32_t v = wadd(vram.soma_voltage[i], i_in);
+  int32_t v = wadd(v_in, i_in);
   int32_t diff = wsub(v, p.rest_potential);
   int32_t sign = (diff > 0) - (diff < 0);
   int32_t abs_diff = wmul(diff, sign);
@@ -299,6 +301,15 @@ __device__ __forceinline__ void soma_tick(const ShardVramPtrs& vram, uint32_t i,
   vram.soma_flags[i] = (uint8_t)((flags & 0xF0) | (bc << 1) | (fin ? 1 : 0));
   vram.threshold_offset[i] = thr;
   vram.timers[i] = new_timer;
+}
+
+__device__ __forceinline__ void soma_tick(const ShardVramPtrs& vram, uint32_t i, uint32_t A,
+                                          const AxcVariantParameters& p, uint8_t flags, uint8_t timer, bool refr,
+                                          uint32_t tick, int32_t i_in, bool& fin, bool& push_own,
+                                          uint32_t& my_axon) {
+  const int32_t thr = refr ? 0 : vram.threshold_offset[i];
+  const int32_t v = refr ? 0 : vram.soma_voltage[i];
+  soma_tick_with(vram, i, A, p, flags, timer, refr, tick, i_in, thr, v, fin, push_own, my_axon);
 }
 
 // ---------------------------------------------------------------------------
@@ -544,6 +555,33 @@ __device__ __forceinline__ void push_apply_one(AxcBurstHeads8* __restrict__ head
   store_heads(heads, a, h);
   if (comp) comp[a] = compact_heads(h);
   if (comp8) comp8[a] = byte_heads(h);
+}
+
+// The pushes of spiker j's own axon (the staged pass counted them in
+// push_cnt without an owner list): the count is taken with an exchange, so
+// somas sharing an axon push it once with the full count.
+__device__ __forceinline__ void push_spiker_one(AxcBurstHeads8* __restrict__ heads,
+                                                const uint32_t* __restrict__ soma_to_axon, uint32_t A,
+                                                const ShardScratch& sc, uint4* __restrict__ comp, uint32_t j,
+                                                uint2* __restrict__ comp8) {
+  const uint32_t a = soma_to_axon[sc.spk_list[j]];
+  if (a == kNoAxon || a >= A) return;
+  const uint32_t c = atomicExch(&sc.push_cnt[a], 0u);
+  if (c == 0) return;
+  Heads8 h = load_heads(heads, a);
+  push_heads(h, c > 8 ? 8 : c);
+  store_heads(heads, a, h);
+  if (comp) comp[a] = compact_heads(h);
+  if (comp8) comp8[a] = byte_heads(h);
+}
+
+__global__ __launch_bounds__(256) void push_spikers_kernel(AxcBurstHeads8* __restrict__ heads,
+                                                           const uint32_t* __restrict__ soma_to_axon, uint32_t A,
+                                                           ShardScratch sc, uint4* __restrict__ comp,
+                                                           uint2* __restrict__ comp8) {
+  const uint32_t n = sc.counters[kCntSpikes];
+  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x)
+    push_spiker_one(heads, soma_to_axon, A, sc, comp, j, comp8);
 }
 
 __global__ __launch_bounds__(256) void push_apply_kernel(AxcBurstHeads8* __restrict__ heads, ShardScratch sc,

@@ -379,6 +379,7 @@ __device__ __forceinline__ uint32_t wave_sum(const uint32_t (&red)[4][8], int k)
 
 // Workgroup scratch of one soma block's finish.
 struct BlockLds {
+  int32_t pre_v[kBlockSomas], pre_thr[kBlockSomas];  // group 0's somas' voltage and threshold offset
   int32_t red_in[kStageGroups][kBlockSomas];
   uint16_t red_hits[kStageGroups][kBlockSomas];
   uint32_t wave_cnt[2][4];
@@ -424,7 +425,7 @@ __device__ __forceinline__ void finish_block(BlockLds& L, const ShardVramPtrs& v
     }
     uint8_t flags = flags_in & 0xF1;  // K1
     flags &= (uint8_t)~0x01;
-    soma_tick(vram, i, A, p, flags, timer, refr, tick, sum, fin, push_own, my_axon);
+    soma_tick_with(vram, i, A, p, flags, timer, refr, tick, sum, L.pre_thr[li], L.pre_v[li], fin, push_own, my_axon);
     n_int = refr ? 0u : 1u;
     if (fin && type_spikes) atomicAdd((unsigned long long*)&type_spikes[flags_in >> 4], 1ull);
     if (fin && count_gsop) {
@@ -432,14 +433,14 @@ __device__ __forceinline__ void finish_block(BlockLds& L, const ShardVramPtrs& v
       n_gtgt = F < kMaxDendrites ? F + 1 : kMaxDendrites;
     }
   }
-  bool own_first = false;
-  if (push_own) own_first = atomicAdd(&sc.push_cnt[my_axon], 1u) == 0u;
+  // The push is counted here and applied by push_spikers_kernel, which walks
+  // the spike list (no owner list: nothing waits for this atomic).
+  if (push_own) atomicAdd(&sc.push_cnt[my_axon], 1u);
 
-  // Compaction of group 0's spikers and first owners, as in neuron_block.
+  // Compaction of group 0's spikers.
   const uint32_t lane = tid & 63u, wv = tid >> 6;
   const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   const uint64_t bs = __ballot(fin);
-  const uint64_t bo = __ballot(own_first);
   uint32_t r[6] = {n_int, n_scan, n_tread, n_hits, n_gslots, n_gtgt};
 #pragma unroll
   for (int k = 0; k < 6; k++) {
@@ -448,20 +449,15 @@ __device__ __forceinline__ void finish_block(BlockLds& L, const ShardVramPtrs& v
   }
   if (g == 0 && lane == 0) {
     L.wave_cnt[0][wv] = (uint32_t)__popcll(bs);
-    L.wave_cnt[1][wv] = (uint32_t)__popcll(bo);
 #pragma unroll
     for (int k = 0; k < 6; k++) L.red[wv][k] = r[k];
   }
   __syncthreads();
   if (tid == 0) {
-    uint32_t cs = 0, co = 0;
+    uint32_t cs = 0;
 #pragma unroll
-    for (uint32_t w = 0; w < kGroup0Waves; w++) {
-      cs += L.wave_cnt[0][w];
-      co += L.wave_cnt[1][w];
-    }
+    for (uint32_t w = 0; w < kGroup0Waves; w++) cs += L.wave_cnt[0][w];
     L.blk_base[0] = cs ? atomicAdd(&sc.counters[kCntSpikes], cs) : 0u;
-    L.blk_base[1] = co ? atomicAdd(&sc.counters[kCntOwners], co) : 0u;
     const uint64_t v[kStatsPerBlock] = {};
     uint64_t add[kStatsPerBlock];
 #pragma unroll
@@ -483,13 +479,9 @@ __device__ __forceinline__ void finish_block(BlockLds& L, const ShardVramPtrs& v
   }
   __syncthreads();
   if (g == 0) {
-    uint32_t pre_s = L.blk_base[0], pre_o = L.blk_base[1];
-    for (uint32_t w = 0; w < wv; w++) {
-      pre_s += L.wave_cnt[0][w];
-      pre_o += L.wave_cnt[1][w];
-    }
+    uint32_t pre_s = L.blk_base[0];
+    for (uint32_t w = 0; w < wv; w++) pre_s += L.wave_cnt[0][w];
     if (fin) sc.spk_list[pre_s + (uint32_t)__popcll(bs & below)] = i;
-    if (own_first) sc.own_list[pre_o + (uint32_t)__popcll(bo & below)] = my_axon;
   }
 }
 
@@ -517,12 +509,21 @@ __device__ __forceinline__ void staged_block(const ShardVramPtrs& vram, uint32_t
   // their latency.
   uint32_t F = 0;
   uint8_t flags_in = 0, timer = 0;
+  int32_t v_in = 0, thr_in = 0;
   if (i < N) {
     flags_in = vram.soma_flags[i];
     timer = vram.timers[i];
     F = sp.fill[i];
+    if (g == 0) {  // group 0 finishes the somas: their voltage and threshold, parked in LDS over the scan
+      v_in = vram.soma_voltage[i];
+      thr_in = vram.threshold_offset[i];
+    }
   }
   if (staged) stage_entries(ent, bset, cnt, sp.comp8, A, tid);
+  if (g == 0) {
+    L.pre_v[li] = v_in;
+    L.pre_thr[li] = thr_in;
+  }
   __syncthreads();
 
   int32_t i_in = 0;
