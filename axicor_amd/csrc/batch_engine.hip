@@ -159,7 +159,7 @@ void batch_staged_kernel(const BrainDev* __restrict__ brains, const WorkItem* __
   if (threadIdx.x < 64)
     reinterpret_cast<uint4*>(lut)[threadIdx.x] = reinterpret_cast<const uint4*>(b.lut)[threadIdx.x];
   staged_block<true>(b.vram, b.N, b.A, lut, bt->tick_base + t_off, b.sc, b.sc.comp[snap & 1], b.sp,
-                     w.start / kBlockSomas, b.stats, true, w.start == 0, b.type_spikes, true);
+                     w.start / kBlockSomas, b.stats, w.start == 0, b.type_spikes, true);
 }
 
 // Exclusive prefix of the brains' list counts of this tick (one workgroup):

@@ -249,13 +249,13 @@ __global__ __launch_bounds__(256) void spike_count_kernel(const uint32_t* __rest
 // dendritic input i_in: refractory countdown, or homeostasis decay, adaptive
 // leak, leak toward rest, threshold test, heartbeat, resets, burst count and
 // flags. `flags` already has K1 and the spike bit cleared. Sets fin (spike)
-// and, for a spike, the soma's own axon to push. thr_in and v_in are the
-// soma's threshold offset and voltage as read before the tick (used only
-// when it integrates).
+// and, for a spike, the soma's own axon to push. thr_in, v_in and axon_in
+// are the soma's threshold offset, voltage and axon as read before the tick
+// (used only when it integrates).
 __device__ __forceinline__ void soma_tick_with(const ShardVramPtrs& vram, uint32_t i, uint32_t A,
                                                const AxcVariantParameters& p, uint8_t flags, uint8_t timer, bool refr,
-                                               uint32_t tick, int32_t i_in, int32_t thr_in, int32_t v_in, bool& fin,
-                                               bool& push_own, uint32_t& my_axon) {
+                                               uint32_t tick, int32_t i_in, int32_t thr_in, int32_t v_in,
+                                               uint32_t axon_in, bool& fin, bool& push_own, uint32_t& my_axon) {
   if (refr) {
     vram.timers[i] = (uint8_t)(timer - 1);
     vram.soma_flags[i] = flags;
@@ -290,7 +290,7 @@ __device__ __forceinline__ void soma_tick_with(const ShardVramPtrs& vram, uint32
   uint8_t new_timer = glif ? p.refractory_period : (uint8_t)0;
   if (fin) {
     flags = (uint8_t)((flags & 0xFC) | 0x03);
-    my_axon = vram.soma_to_axon[i];
+    my_axon = axon_in;
     push_own = (my_axon != kNoAxon) && (my_axon < A);
   } else {
     flags = (uint8_t)(flags & ~0x01);
@@ -309,7 +309,8 @@ __device__ __forceinline__ void soma_tick(const ShardVramPtrs& vram, uint32_t i,
                                           uint32_t& my_axon) {
   const int32_t thr = refr ? 0 : vram.threshold_offset[i];
   const int32_t v = refr ? 0 : vram.soma_voltage[i];
-  soma_tick_with(vram, i, A, p, flags, timer, refr, tick, i_in, thr, v, fin, push_own, my_axon);
+  const uint32_t ax = refr ? 0u : vram.soma_to_axon[i];
+  soma_tick_with(vram, i, A, p, flags, timer, refr, tick, i_in, thr, v, ax, fin, push_own, my_axon);
 }
 
 // ---------------------------------------------------------------------------

@@ -369,48 +369,39 @@ __device__ __forceinline__ void stage_entries(uint2* ent, const uint32_t* __rest
   if (tid == 0) ent[kStageCap] = make_uint2(0xFFFFFFFFu, 0xFFFFFFFFu);
 }
 
-// Sum of column k over the waves of group 0.
-__device__ __forceinline__ uint32_t wave_sum(const uint32_t (&red)[4][8], int k) {
-  uint32_t v = 0;
-#pragma unroll
-  for (uint32_t w = 0; w < kGroup0Waves; w++) v += red[w][k];
-  return v;
-}
-
 // Workgroup scratch of one soma block's finish.
 struct BlockLds {
-  int32_t pre_v[kBlockSomas], pre_thr[kBlockSomas];  // group 0's somas' voltage and threshold offset
+  int32_t pre_v[kBlockSomas], pre_thr[kBlockSomas];  // group 0's somas: voltage, threshold offset
+  uint32_t pre_ax[kBlockSomas];                      // and axon, read before the scan
   int32_t red_in[kStageGroups][kBlockSomas];
   uint16_t red_hits[kStageGroups][kBlockSomas];
-  uint32_t wave_cnt[2][4];
-  uint32_t blk_base[2];
-  uint32_t red[4][8];
+  uint32_t red[kGroup0Waves][8];  // group 0's waves' counters
+  uint32_t done;                  // group 0's waves that have written theirs
 };
 
-// The end of a block's tick, after the scan (every lane of the workgroup
-// takes part: it holds barriers): the groups' hit sums are added, group 0
-// finishes its 256 somas (GLIF update, flags) and compacts the spikers and
-// first owners into the shard's lists. Counters go to part[] (this
-// workgroup's own partials) or, with atomic_stats, are added atomically to
-// a shared per-shard array; type_spikes (may be null) counts spikes per
-// variant type; count_gsop adds the slot and target counts of the tick's
-// GSOP (for callers whose GSOP kernel does not count).
+// The end of a block's tick, after the scan: the groups' hit sums are
+// added and group 0 finishes its somas (GLIF update, flags), each of its
+// waves then compacting its own spikers into the shard's list (one atomic
+// per wave with spikers); the block's counters go to part[] with atomics
+// whose results nobody waits for. The other groups leave after the barrier.
+// type_spikes (may be null) counts spikes per variant type; count_gsop adds
+// the slot and target counts of the tick's GSOP (for callers whose GSOP
+// kernel does not count).
 __device__ __forceinline__ void finish_block(BlockLds& L, const ShardVramPtrs& vram, uint32_t N, uint32_t A,
                                              const AxcVariantParameters& p, uint32_t tick, const ShardScratch& sc,
                                              uint32_t tid, uint32_t i, uint32_t F, uint8_t flags_in, uint8_t timer,
                                              bool refr, int32_t i_in, uint32_t hits, uint64_t* part,
-                                             bool atomic_stats, bool count_tick, uint64_t* type_spikes,
-                                             bool count_gsop) {
+                                             bool count_tick, uint64_t* type_spikes, bool count_gsop) {
   const uint32_t g = tid / kBlockSomas, li = tid % kBlockSomas;
   L.red_in[g][li] = i_in;
   L.red_hits[g][li] = (uint16_t)hits;
   __syncthreads();
+  if (g != 0) return;  // whole waves
 
-  // Group 0 finishes its 256 somas; the other groups only join the barriers.
   bool fin = false, push_own = false;
   uint32_t my_axon = 0;
   uint32_t n_int = 0, n_scan = 0, n_tread = 0, n_hits = 0, n_gslots = 0, n_gtgt = 0;
-  if (g == 0 && i < N) {
+  if (i < N) {
     int32_t sum = 0;
     uint32_t hsum = 0;
 #pragma unroll
@@ -425,7 +416,8 @@ __device__ __forceinline__ void finish_block(BlockLds& L, const ShardVramPtrs& v
     }
     uint8_t flags = flags_in & 0xF1;  // K1
     flags &= (uint8_t)~0x01;
-    soma_tick_with(vram, i, A, p, flags, timer, refr, tick, sum, L.pre_thr[li], L.pre_v[li], fin, push_own, my_axon);
+    soma_tick_with(vram, i, A, p, flags, timer, refr, tick, sum, L.pre_thr[li], L.pre_v[li], L.pre_ax[li], fin,
+                   push_own, my_axon);
     n_int = refr ? 0u : 1u;
     if (fin && type_spikes) atomicAdd((unsigned long long*)&type_spikes[flags_in >> 4], 1ull);
     if (fin && count_gsop) {
@@ -437,51 +429,43 @@ __device__ __forceinline__ void finish_block(BlockLds& L, const ShardVramPtrs& v
   // the spike list (no owner list: nothing waits for this atomic).
   if (push_own) atomicAdd(&sc.push_cnt[my_axon], 1u);
 
-  // Compaction of group 0's spikers.
-  const uint32_t lane = tid & 63u, wv = tid >> 6;
+  // This wave's spikers, appended to the list.
+  const uint32_t lane = tid & 63u;
   const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   const uint64_t bs = __ballot(fin);
-  uint32_t r[6] = {n_int, n_scan, n_tread, n_hits, n_gslots, n_gtgt};
+  const uint32_t cs = (uint32_t)__popcll(bs);
+  if (cs) {
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(&sc.counters[kCntSpikes], cs);
+    base = __shfl(base, 0, 64);
+    if (fin) sc.spk_list[base + (uint32_t)__popcll(bs & below)] = i;
+  }
+  // This wave's counters; the last of group 0's waves to get here adds the
+  // block's to part[] (one set of atomics per block, no barrier).
+  uint32_t r[7] = {n_int, n_scan, n_tread, n_hits, n_gslots, n_gtgt, cs};
 #pragma unroll
   for (int k = 0; k < 6; k++) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) r[k] += __shfl_xor(r[k], off, 64);
   }
-  if (g == 0 && lane == 0) {
-    L.wave_cnt[0][wv] = (uint32_t)__popcll(bs);
+  if (lane == 0) {
+    const uint32_t wv = tid >> 6;
 #pragma unroll
-    for (int k = 0; k < 6; k++) L.red[wv][k] = r[k];
-  }
-  __syncthreads();
-  if (tid == 0) {
-    uint32_t cs = 0;
+    for (int k = 0; k < 7; k++) L.red[wv][k] = r[k];
+    const uint32_t before =
+        __hip_atomic_fetch_add(&L.done, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (before == kGroup0Waves - 1) {
+      const uint32_t slot[7] = {kStIntegrated, kStScanned, kStTargets, kStHits, kStGsopSlots, kStGsopTargets,
+                                kStSpikes};
 #pragma unroll
-    for (uint32_t w = 0; w < kGroup0Waves; w++) cs += L.wave_cnt[0][w];
-    L.blk_base[0] = cs ? atomicAdd(&sc.counters[kCntSpikes], cs) : 0u;
-    const uint64_t v[kStatsPerBlock] = {};
-    uint64_t add[kStatsPerBlock];
+      for (int k = 0; k < 7; k++) {
+        uint32_t v = 0;
 #pragma unroll
-    for (uint32_t k = 0; k < kStatsPerBlock; k++) add[k] = v[k];
-    add[kStIntegrated] = wave_sum(L.red, 0);
-    add[kStScanned] = wave_sum(L.red, 1);
-    add[kStTargets] = wave_sum(L.red, 2);
-    add[kStHits] = wave_sum(L.red, 3);
-    add[kStGsopSlots] = wave_sum(L.red, 4);
-    add[kStGsopTargets] = wave_sum(L.red, 5);
-    add[kStSpikes] = cs;
-    add[kStTicks] = count_tick ? 1 : 0;
-#pragma unroll
-    for (uint32_t k = 0; k < kStatsPerBlock; k++) {
-      if (!add[k]) continue;
-      if (atomic_stats) atomicAdd((unsigned long long*)&part[k], (unsigned long long)add[k]);
-      else part[k] += add[k];
+        for (uint32_t w = 0; w < kGroup0Waves; w++) v += L.red[w][k];
+        if (v) atomicAdd((unsigned long long*)&part[slot[k]], (unsigned long long)v);
+      }
+      if (count_tick) atomicAdd((unsigned long long*)&part[kStTicks], 1ull);
     }
-  }
-  __syncthreads();
-  if (g == 0) {
-    uint32_t pre_s = L.blk_base[0];
-    for (uint32_t w = 0; w < wv; w++) pre_s += L.wave_cnt[0][w];
-    if (fin) sc.spk_list[pre_s + (uint32_t)__popcll(bs & below)] = i;
   }
 }
 
@@ -495,8 +479,8 @@ template <bool PW>
 __device__ __forceinline__ void staged_block(const ShardVramPtrs& vram, uint32_t N, uint32_t A,
                                              const AxcVariantParameters* lut, uint32_t tick, const ShardScratch& sc,
                                              const uint4* __restrict__ cur, const StagePlan& sp, uint32_t lb,
-                                             uint64_t* part, bool atomic_stats, bool count_tick,
-                                             uint64_t* type_spikes, bool count_gsop = false) {
+                                             uint64_t* part, bool count_tick, uint64_t* type_spikes,
+                                             bool count_gsop = false) {
   extern __shared__ uint2 ent[];  // kStageCap + 1 entries
   __shared__ BlockLds L;
   const uint32_t tid = threadIdx.x;
@@ -510,20 +494,24 @@ __device__ __forceinline__ void staged_block(const ShardVramPtrs& vram, uint32_t
   uint32_t F = 0;
   uint8_t flags_in = 0, timer = 0;
   int32_t v_in = 0, thr_in = 0;
+  uint32_t ax_in = 0;
   if (i < N) {
     flags_in = vram.soma_flags[i];
     timer = vram.timers[i];
     F = sp.fill[i];
-    if (g == 0) {  // group 0 finishes the somas: their voltage and threshold, parked in LDS over the scan
+    if (g == 0) {  // group 0 finishes the somas: their state, parked in LDS over the scan
       v_in = vram.soma_voltage[i];
       thr_in = vram.threshold_offset[i];
+      ax_in = vram.soma_to_axon[i];
     }
   }
   if (staged) stage_entries(ent, bset, cnt, sp.comp8, A, tid);
   if (g == 0) {
     L.pre_v[li] = v_in;
     L.pre_thr[li] = thr_in;
+    L.pre_ax[li] = ax_in;
   }
+  if (tid == 0) L.done = 0;
   __syncthreads();
 
   int32_t i_in = 0;
@@ -536,8 +524,8 @@ __device__ __forceinline__ void staged_block(const ShardVramPtrs& vram, uint32_t
     if (staged) scan_slots<true, PW>(vram, N, A, sp, ent, cur, gu, i, F, p.signal_propagation_length, i_in, hits);
     else scan_slots<false, PW>(vram, N, A, sp, ent, cur, gu, i, F, p.signal_propagation_length, i_in, hits);
   }
-  finish_block(L, vram, N, A, p, tick, sc, tid, i, F, flags_in, timer, refr, i_in, hits, part, atomic_stats,
-               count_tick, type_spikes, count_gsop);
+  finish_block(L, vram, N, A, p, tick, sc, tid, i, F, flags_in, timer, refr, i_in, hits, part, count_tick,
+               type_spikes, count_gsop);
 }
 
 // The single-shard soma pass: one workgroup per soma block, blocks dealt to
@@ -553,7 +541,7 @@ void neuron_staged_kernel(ShardVramPtrs vram, uint32_t N, uint32_t A, const AxcV
   const uint32_t lb = (blockIdx.x % kNumXcd) * per_xcd + blockIdx.x / kNumXcd;
   if (threadIdx.x < 64) reinterpret_cast<uint4*>(lut)[threadIdx.x] = reinterpret_cast<const uint4*>(lut_g)[threadIdx.x];
   staged_block<PW>(vram, N, A, lut, tick_of(prm, tick_arg), sc, cur, sp, lb,
-                   sc.stats_part + (size_t)blockIdx.x * kStatsPerBlock, false, blockIdx.x == 0, nullptr);
+                   sc.stats_part + (size_t)blockIdx.x * kStatsPerBlock, blockIdx.x == 0, nullptr);
 }
 
 // The working copy of the weights: soma i's chunks c with 8c < fill[i], to
