@@ -52,8 +52,10 @@ __global__ __launch_bounds__(256) void batch_axon_kernel(const BrainDev* __restr
     if (w.start == 0 && threadIdx.x < kNumCounters) b.sc.counters[threadIdx.x] = 0;
     if (a >= b.A) continue;
     const uint32_t* bm = b.bitmask ? b.bitmask + (size_t)(tick % b.input_ticks) * b.words : nullptr;
+    // The staged pass reads the byte snapshot (and the heads for exact
+    // fallbacks); the compact one only serves the direct pass.
     axon_step(b.vram.axon_heads, a, b.v_seg, bm, b.virt_off, bm ? b.n_virt : 0u, b.sc.push_cnt, false,
-              b.sc.comp[snap & 1], b.sp.comp8);
+              b.sp.comp8 ? nullptr : b.sc.comp[snap & 1], b.sp.comp8);
   }
 }
 
@@ -158,7 +160,7 @@ void batch_staged_kernel(const BrainDev* __restrict__ brains, const WorkItem* __
   const BrainDev& b = brains[w.brain];
   if (threadIdx.x < 64)
     reinterpret_cast<uint4*>(lut)[threadIdx.x] = reinterpret_cast<const uint4*>(b.lut)[threadIdx.x];
-  staged_block<true>(b.vram, b.N, b.A, lut, bt->tick_base + t_off, b.sc, b.sc.comp[snap & 1], b.sp,
+  staged_block<true>(b.vram, b.N, b.A, lut, bt->tick_base + t_off, b.sc, b.sp,
                      w.start / kBlockSomas, b.stats, w.start == 0, b.type_spikes, true);
 }
 
@@ -207,8 +209,7 @@ __global__ __launch_bounds__(256) void batch_push_flat_kernel(const BrainDev* __
   for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < total; k += gridDim.x * blockDim.x) {
     const uint32_t bi = brain_of(prefix, n_brains, k);
     const BrainDev& b = brains[bi];
-    push_spiker_one(b.vram.axon_heads, b.vram.soma_to_axon, b.A, b.sc, b.sc.comp[snap & 1], k - prefix[bi],
-                    b.sp.comp8);
+    push_spiker_one(b.vram.axon_heads, b.vram.soma_to_axon, b.A, b.sc, nullptr, k - prefix[bi], b.sp.comp8);
   }
 }
 
@@ -225,7 +226,7 @@ __global__ __launch_bounds__(256) void batch_gsop_chunks_kernel(const BrainDev* 
     const BrainDev& b = brains[bi];
     const uint32_t i = b.sc.spk_list[k - prefix[bi]];
     const uint32_t F = b.sp.fill[i];
-    if (c * 8 < F) gsop_chunk(b.N, b.A, b.lut, b.dop, b.vram.soma_flags, b.sp, b.sc.comp[snap & 1], i, c, F);
+    if (c * 8 < F) gsop_chunk(b.N, b.A, b.lut, b.dop, b.vram.soma_flags, b.sp, b.vram.axon_heads, i, c, F);
   }
 }
 

@@ -345,7 +345,7 @@ hipError_t enqueue_tick(const ShardVramPtrs& v, Shard* s, uint32_t v_seg, uint32
     TimedScope ts(kKAxon, st, timed);
     axon_tick_kernel<<<blocks_for(A > kNumCounters ? A : kNumCounters), kThreads, 0, st>>>(
         v.axon_heads, A, v_seg, k2 ? bitmask : nullptr, virt_off, k2 ? n_virt : 0u, s->sc.push_cnt, k3 ? 1 : 0,
-        s->sc.counters, prm, tick, words, cur, staged ? s->sp.comp8 : nullptr);
+        s->sc.counters, prm, tick, words, staged ? nullptr : cur, staged ? s->sp.comp8 : nullptr);
   }
   if (N > 0 && staged) {
     // The staged pass never fuses the previous tick's GSOP (gsop_now is set
@@ -353,10 +353,9 @@ hipError_t enqueue_tick(const ShardVramPtrs& v, Shard* s, uint32_t v_seg, uint32
     TimedScope ts(kKNeuron, st, timed);
     const size_t lds = (kStageCap + 1) * sizeof(uint2);
     if (pw)
-      neuron_staged_kernel<true><<<staged_grid(N), kStageThreads, lds, st>>>(v, N, A, lut, prm, tick, s->sc, cur,
-                                                                               s->sp);
+      neuron_staged_kernel<true><<<staged_grid(N), kStageThreads, lds, st>>>(v, N, A, lut, prm, tick, s->sc, s->sp);
     else
-      neuron_staged_kernel<false><<<staged_grid(N), kStageThreads, lds, st>>>(v, N, A, lut, prm, tick, s->sc, cur,
+      neuron_staged_kernel<false><<<staged_grid(N), kStageThreads, lds, st>>>(v, N, A, lut, prm, tick, s->sc,
                                                                                 s->sp);
   } else if (N > 0) {
     TimedScope ts(kKNeuron, st, timed);
@@ -370,7 +369,7 @@ hipError_t enqueue_tick(const ShardVramPtrs& v, Shard* s, uint32_t v_seg, uint32
   {
     TimedScope ts(kKPush, st, timed);
     if (staged)
-      push_spikers_kernel<<<kListBlocks / 4, kThreads, 0, st>>>(v.axon_heads, v.soma_to_axon, A, s->sc, cur,
+      push_spikers_kernel<<<kListBlocks / 4, kThreads, 0, st>>>(v.axon_heads, v.soma_to_axon, A, s->sc, nullptr,
                                                                 s->sp.comp8);
     else
       push_apply_kernel<<<kListBlocks / 4, kThreads, 0, st>>>(v.axon_heads, s->sc, cur, nullptr);
@@ -378,7 +377,7 @@ hipError_t enqueue_tick(const ShardVramPtrs& v, Shard* s, uint32_t v_seg, uint32
   if (gsop_now) {
     TimedScope ts(kKGsop, st, timed);
     if (pw)
-      gsop_chunks_kernel<<<kListBlocks, kThreads, 0, st>>>(v, N, A, lut, prm, dopamine, s->sc, cur, s->sp);
+      gsop_chunks_kernel<<<kListBlocks, kThreads, 0, st>>>(v, N, A, lut, prm, dopamine, s->sc, s->sp);
     else
       gsop_kernel<<<kListBlocks, kThreads, 0, st>>>(v, N, A, lut, prm, dopamine, s->sc, cur,
                                                     staged ? s->sp.fill : nullptr, staged ? s->sp.comp8 : nullptr);

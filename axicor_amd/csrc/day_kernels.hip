@@ -625,16 +625,29 @@ __device__ __forceinline__ uint32_t gsop_soma(const ShardVramPtrs& vram, uint32_
 
 // GSOP of spiking soma i with its filled prefix F known (the staged plan's
 // fill): lane l takes slots l and l + 64 at once. comp8: the byte snapshot
-// after the pushes (push_apply refreshed it), exact for segment + length
-// <= 255; other slots read the compact form.
-__device__ __forceinline__ uint32_t gsop_dist(const uint4* __restrict__ comp, const uint2* __restrict__ comp8,
-                                              uint32_t a, uint32_t seg, uint32_t len) {
-  return seg + len > 255u ? min_dist16(comp[a], seg) : min_dist8(comp8[a], seg);
+// after the pushes (the push kernel refreshed it), exact for segment +
+// length <= 255; other slots read the heads.
+// Smallest h - seg over the eight u32 heads of axon a (wrapping: heads below
+// seg count as far, as in the reference's test); the inert index A is far.
+__device__ __forceinline__ uint32_t min_dist_heads(const AxcBurstHeads8* __restrict__ heads, uint32_t A,
+                                                   uint32_t a, uint32_t seg) {
+  if (a >= A) return 0xFFFFFFFFu;
+  const Heads8 h = load_heads(heads, a);
+  uint32_t m = min(h.lo.x - seg, h.lo.y - seg);
+  m = min(m, min(h.lo.z - seg, h.lo.w - seg));
+  m = min(m, min(h.hi.x - seg, h.hi.y - seg));
+  return min(m, min(h.hi.z - seg, h.hi.w - seg));
+}
+// GSOP's distance for a slot: from the byte snapshot where it is exact
+// (segment + length <= 255), else from the heads themselves.
+__device__ __forceinline__ uint32_t gsop_dist(const AxcBurstHeads8* __restrict__ heads, uint32_t A,
+                                              const uint2* __restrict__ comp8, uint32_t a, uint32_t seg,
+                                              uint32_t len) {
+  return seg + len > 255u ? min_dist_heads(heads, A, a, seg) : min_dist8(comp8[a], seg);
 }
 __device__ __forceinline__ void gsop_soma_filled(const ShardVramPtrs& vram, uint32_t N, uint32_t A,
                                                  const AxcVariantParameters* lut, int32_t dop, uint32_t i, uint32_t F,
-                                                 const uint4* __restrict__ comp, const uint2* __restrict__ comp8,
-                                                 uint32_t lane) {
+                                                 const uint2* __restrict__ comp8, uint32_t lane) {
   const uint8_t flags = vram.soma_flags[i];
   const AxcVariantParameters& p = lut[(flags >> 4) & 0x0F];
   const GsopTerms g = gsop_terms(flags, p, dop);
@@ -642,8 +655,8 @@ __device__ __forceinline__ void gsop_soma_filled(const ShardVramPtrs& vram, uint
   const size_t c0 = (size_t)lane * N + i, c1 = (size_t)(lane + 64) * N + i;
   const uint32_t t0 = l0 ? vram.dendrite_targets[c0] : 0u, t1 = l1 ? vram.dendrite_targets[c1] : 0u;
   const int16_t w0 = l0 ? vram.dendrite_weights[c0] : (int16_t)0, w1 = l1 ? vram.dendrite_weights[c1] : (int16_t)0;
-  const uint32_t d0 = gsop_dist(comp, comp8, l0 ? snap_index(t0, A) : A, t0 >> 24, g.len);
-  const uint32_t d1 = gsop_dist(comp, comp8, l1 ? snap_index(t1, A) : A, t1 >> 24, g.len);
+  const uint32_t d0 = gsop_dist(vram.axon_heads, A, comp8, l0 ? snap_index(t0, A) : A, t0 >> 24, g.len);
+  const uint32_t d1 = gsop_dist(vram.axon_heads, A, comp8, l1 ? snap_index(t1, A) : A, t1 >> 24, g.len);
   if (l0) vram.dendrite_weights[c0] = gsop_weight(w0, d0, g.len, g.raw_pot, g.final_dep, g.bm, p.inertia_curve);
   if (l1) vram.dendrite_weights[c1] = gsop_weight(w1, d1, g.len, g.raw_pot, g.final_dep, g.bm, p.inertia_curve);
 }
@@ -674,7 +687,7 @@ __global__ __launch_bounds__(256) void gsop_kernel(ShardVramPtrs vram, uint32_t 
     uint32_t filled;
     if (fill) {
       filled = fill[i];
-      gsop_soma_filled(vram, N, A, lut, dop, i, filled, comp, comp8, lane);
+      gsop_soma_filled(vram, N, A, lut, dop, i, filled, comp8, lane);
     } else {
       filled = gsop_soma(vram, N, A, lut, dop, i, comp, lane);
     }

@@ -256,7 +256,8 @@ __device__ __forceinline__ uint32_t max_byte8(uint32_t x, uint32_t y) {
 // with one 2-lane dot product. hit_cnt counts the hits in its two lanes.
 template <bool STAGED, bool PW>
 __device__ __forceinline__ void hit_chunk(const Chunk<STAGED, PW>& now, uint32_t c, uint32_t N, uint32_t A,
-                                          const StagePlan& sp, const uint2* ent, const uint4* __restrict__ cur,
+                                          const StagePlan& sp, const uint2* ent,
+                                          const AxcBurstHeads8* __restrict__ heads,
                                           uint32_t i, uint32_t prop, int32_t& i_in, axc_us2& hit_cnt) {
   constexpr int U = 8;
   const axc_us2 P = {(unsigned short)prop, (unsigned short)prop};
@@ -304,7 +305,7 @@ __device__ __forceinline__ void hit_chunk(const Chunk<STAGED, PW>& now, uint32_t
         else
           t = now.k[u];
         // A slot at or past the fill has the empty target: no hit.
-        const uint32_t d = t ? min_dist16(snap_at(cur, snap_index(t, A)), sg) : 0xFFFFu;
+        const uint32_t d = min(t ? min_dist_heads(heads, A, snap_index(t, A), sg) : 0xFFFFu, 0xFFFFu);
         const uint32_t sh = 16 * (u & 1);
         M[u / 2] = (M[u / 2] & ~(0xFFFFu << sh)) | (d << sh);
       }
@@ -327,7 +328,7 @@ __device__ __forceinline__ void hit_chunk(const Chunk<STAGED, PW>& now, uint32_t
 // LDS entry buffers, measured slower.)
 template <bool STAGED, bool PW>
 __device__ __forceinline__ void scan_slots(const ShardVramPtrs& vram, uint32_t N, uint32_t A, const StagePlan& sp,
-                                           const uint2* ent, const uint4* __restrict__ cur, uint32_t gu, uint32_t i,
+                                           const uint2* ent, uint32_t gu, uint32_t i,
                                            uint32_t F, uint32_t prop, int32_t& i_in, uint32_t& hits) {
   // Group 0, which finishes the somas afterwards, takes chunks 3, 7, 11, ...
   // so that a partly filled last chunk lands on another group.
@@ -337,7 +338,7 @@ __device__ __forceinline__ void scan_slots(const ShardVramPtrs& vram, uint32_t N
   for (uint32_t c = cg; c * 8 < F; c += kStageGroups) {
     Chunk<STAGED, PW> now;
     now.load(vram, sp, N, c, i);
-    hit_chunk<STAGED, PW>(now, c, N, A, sp, ent, cur, i, prop, i_in, hit_cnt);
+    hit_chunk<STAGED, PW>(now, c, N, A, sp, ent, vram.axon_heads, i, prop, i_in, hit_cnt);
   }
   hits += (uint32_t)hit_cnt.x + (uint32_t)hit_cnt.y;
 }
@@ -478,7 +479,7 @@ __device__ __forceinline__ void finish_block(BlockLds& L, const ShardVramPtrs& v
 template <bool PW>
 __device__ __forceinline__ void staged_block(const ShardVramPtrs& vram, uint32_t N, uint32_t A,
                                              const AxcVariantParameters* lut, uint32_t tick, const ShardScratch& sc,
-                                             const uint4* __restrict__ cur, const StagePlan& sp, uint32_t lb,
+                                             const StagePlan& sp, uint32_t lb,
                                              uint64_t* part, bool count_tick, uint64_t* type_spikes,
                                              bool count_gsop = false) {
   extern __shared__ uint2 ent[];  // kStageCap + 1 entries
@@ -521,8 +522,8 @@ __device__ __forceinline__ void staged_block(const ShardVramPtrs& vram, uint32_t
   // The group index is the same for the 64 lanes of a wave: scalar.
   const uint32_t gu = __builtin_amdgcn_readfirstlane(g);
   if (!refr) {
-    if (staged) scan_slots<true, PW>(vram, N, A, sp, ent, cur, gu, i, F, p.signal_propagation_length, i_in, hits);
-    else scan_slots<false, PW>(vram, N, A, sp, ent, cur, gu, i, F, p.signal_propagation_length, i_in, hits);
+    if (staged) scan_slots<true, PW>(vram, N, A, sp, ent, gu, i, F, p.signal_propagation_length, i_in, hits);
+    else scan_slots<false, PW>(vram, N, A, sp, ent, gu, i, F, p.signal_propagation_length, i_in, hits);
   }
   finish_block(L, vram, N, A, p, tick, sc, tid, i, F, flags_in, timer, refr, i_in, hits, part, count_tick,
                type_spikes, count_gsop);
@@ -535,12 +536,12 @@ template <bool PW>
 __global__ __launch_bounds__(kStageThreads) __attribute__((amdgpu_waves_per_eu(8, 8)))
 void neuron_staged_kernel(ShardVramPtrs vram, uint32_t N, uint32_t A, const AxcVariantParameters* __restrict__ lut_g,
                           const TickParams* __restrict__ prm, uint32_t tick_arg, ShardScratch sc,
-                          const uint4* __restrict__ cur, StagePlan sp) {
+                          StagePlan sp) {
   __shared__ AxcVariantParameters lut[16];
   const uint32_t per_xcd = gridDim.x / kNumXcd;
   const uint32_t lb = (blockIdx.x % kNumXcd) * per_xcd + blockIdx.x / kNumXcd;
   if (threadIdx.x < 64) reinterpret_cast<uint4*>(lut)[threadIdx.x] = reinterpret_cast<const uint4*>(lut_g)[threadIdx.x];
-  staged_block<PW>(vram, N, A, lut, tick_of(prm, tick_arg), sc, cur, sp, lb,
+  staged_block<PW>(vram, N, A, lut, tick_of(prm, tick_arg), sc, sp, lb,
                    sc.stats_part + (size_t)blockIdx.x * kStatsPerBlock, blockIdx.x == 0, nullptr);
 }
 
@@ -585,7 +586,8 @@ __global__ __launch_bounds__(256) void copy_weights_kernel(int16_t* __restrict__
 // one (comp) where segment + length > 255. F: the soma's fill (> 8c).
 __device__ __forceinline__ void gsop_chunk(uint32_t N, uint32_t A, const AxcVariantParameters* lut, int32_t dop,
                                            const uint8_t* __restrict__ flags_all, const StagePlan& sp,
-                                           const uint4* __restrict__ comp, uint32_t i, uint32_t c, uint32_t F) {
+                                           const AxcBurstHeads8* __restrict__ heads, uint32_t i, uint32_t c,
+                                           uint32_t F) {
   const uint8_t flags = flags_all[i];
   const AxcVariantParameters& p = lut[(flags >> 4) & 0x0F];
   const GsopTerms g = gsop_terms(flags, p, dop);
@@ -598,7 +600,8 @@ __device__ __forceinline__ void gsop_chunk(uint32_t N, uint32_t A, const AxcVari
   uint32_t w2[4] = {wv.x, wv.y, wv.z, wv.w};
   uint32_t d[8];
 #pragma unroll
-  for (int u = 0; u < 8; u++) d[u] = gsop_dist(comp, sp.comp8, (uint32_t)u < nv ? snap_index(t[u], A) : A, t[u] >> 24, g.len);
+  for (int u = 0; u < 8; u++)
+    d[u] = gsop_dist(heads, A, sp.comp8, (uint32_t)u < nv ? snap_index(t[u], A) : A, t[u] >> 24, g.len);
 #pragma unroll
   for (int u = 0; u < 8; u++) {
     if ((uint32_t)u >= nv) continue;
@@ -616,8 +619,7 @@ constexpr uint32_t kGsopLanes = kMaxDendrites / 8;
 __global__ __launch_bounds__(256) void gsop_chunks_kernel(ShardVramPtrs vram, uint32_t N, uint32_t A,
                                                           const AxcVariantParameters* __restrict__ lut_g,
                                                           const TickParams* __restrict__ prm, int32_t dop_arg,
-                                                          ShardScratch sc, const uint4* __restrict__ comp,
-                                                          StagePlan sp) {
+                                                          ShardScratch sc, StagePlan sp) {
   __shared__ AxcVariantParameters lut[16];
   __shared__ uint32_t red[4][2];
   const uint32_t tid = threadIdx.x;
@@ -629,7 +631,7 @@ __global__ __launch_bounds__(256) void gsop_chunks_kernel(ShardVramPtrs vram, ui
   for (uint32_t q = blockIdx.x * 256 + tid; q < n; q += gridDim.x * 256) {
     const uint32_t i = sc.spk_list[q / kGsopLanes], c = q % kGsopLanes;
     const uint32_t F = sp.fill[i];
-    if (c * 8 < F) gsop_chunk(N, A, lut, dop, vram.soma_flags, sp, comp, i, c, F);
+    if (c * 8 < F) gsop_chunk(N, A, lut, dop, vram.soma_flags, sp, vram.axon_heads, i, c, F);
     if (c == 0) {
       cnt_slots += F;
       cnt_tgt += F < kMaxDendrites ? F + 1 : kMaxDendrites;
