@@ -10,6 +10,10 @@ variant tables and the named workloads:
   100 slots each (150M synapses), 128x128x64 grid, axons 1-255 segments,
   |w| in [64,512], 5% of neurons as inputs (75,000) at p=0.02 per tick;
   somas numbered in 8x8x8-voxel bricks.
+* ``c5`` — the scale point: 12M neurons of 6 types, 128 slots each (1.5B
+  synapses), 256x256x128 grid, axons 1-255 segments, 2% of neurons as inputs
+  (240,000) at p=0.01 per tick; ``spec_c5(200_000)`` is its CPU-sized
+  sub-shard from the same seed (about 1.2% firing per tick).
 
 Parameter values follow the reference template (_template/blueprints.toml:13-37)
 and the blueprint defaults (genesis-core/src/config/blueprints.rs:131-152).
@@ -139,6 +143,40 @@ def spec_c2(seed: int = DEFAULT_MASTER_SEED, brick: int = 8) -> BrainSpec:
     return BrainSpec(1_500_000, 75_000, (128, 128, 64), 100, (1, 255),
                      type_ppm=[400_000, 400_000, 100_000, 100_000], seed=seed, input_ppm=20_000, lut=lut_c2(),
                      brick=brick)
+
+
+def lut_c5() -> np.ndarray:
+    """Six types: excitatory/inhibitory x fast, slow and medium leak. Fast and
+    slow are the 1.5M-neuron brain's; medium is three quarters of the template
+    leak with a gentler adaptive leak (gain 32, up to 2250). Propagation 10
+    segments and a 128-tick heartbeat, as there."""
+    common = dict(signal_propagation_length=10, spontaneous_firing_period_ticks=128)
+    slow = dict(leak_rate=750, adaptive_mode=1, adaptive_leak_gain=64, adaptive_leak_max=3000)
+    medium = dict(leak_rate=1125, adaptive_mode=1, adaptive_leak_gain=32, adaptive_leak_max=2250)
+    return lut_from([
+        template_variant(False, **common),
+        template_variant(False, **common, **slow),
+        template_variant(False, **common, **medium),
+        template_variant(True, **common),
+        template_variant(True, **common, **slow),
+        template_variant(True, **common, **medium),
+    ])
+
+
+def spec_c5(n_neurons: int = 12_000_000, seed: int = DEFAULT_MASTER_SEED, brick: int = 8) -> BrainSpec:
+    """The scale point: 12M neurons x 128 slots (about 1.5B synapses) of six
+    types, axons of 1-255 segments, 2% of the neurons' count as inputs at
+    p = 0.01 per tick. The voxel grid keeps the 1.5M-neuron brain's density
+    (about 1.43 somas per voxel, x:y:z = 2:2:1, sides rounded to whole
+    bricks), so a smaller n (a sub-shard from the same seed, e.g. 200,000 on
+    a 64x64x32 grid for the CPU side) has nearly the same local
+    connectivity."""
+    vox = n_neurons / (1_500_000 / (128 * 128 * 64))
+    q = max(brick, 1)  # grid dimensions are whole bricks
+    z = max(q, int(round((vox / 4) ** (1 / 3) / q)) * q)
+    return BrainSpec(n_neurons, n_neurons // 50, (2 * z, 2 * z, z), 128, (1, 255),
+                     type_ppm=[300_000, 300_000, 200_000, 80_000, 70_000, 50_000], seed=seed, input_ppm=10_000,
+                     lut=lut_c5(), brick=brick)
 
 
 def spec_small(n: int, slots: int, n_inputs: int, seed: int, axon_len=(1, 32), input_ppm=50_000,

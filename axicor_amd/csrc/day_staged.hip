@@ -25,9 +25,11 @@
 // scan + GSOP at 1.5M neurons; inside the scan loop the registers spill).
 // The snapshot entries are the byte form of the heads (min(h, 255) x 8, see
 // byte_heads); a slot whose segment + propagation length exceeds 255 reads the
-// 16-byte compact form from global memory instead, so every hit is exact.
-// Blocks whose distinct axons exceed kStageCap read the reference targets and
-// gather the byte snapshot from global memory.
+// u32 heads from global memory instead, so every hit is exact.
+// Blocks whose distinct axons exceed kStageCap stage the first kStageCap of
+// them; the slots of the others take the exact path from the u32 heads. Only
+// blocks whose hash set overflows read the plan's targets and gather the
+// byte snapshot from global memory.
 #include "day_common.h"
 
 namespace axc {
@@ -101,8 +103,12 @@ __global__ __launch_bounds__(kBlockSomas) void stage_build_kernel(const uint32_t
     __syncthreads();
   }
   const uint32_t total = scan[kBlockSomas - 1];
-  const bool fits = overflow == 0 && total <= kStageCap;
-  if (tid == 0) sp.cnt[b] = fits ? total : kStageOverflow;
+  // Only a failed hash insert sends the block to the unstaged scan. Past
+  // kStageCap distinct axons the first kStageCap (in axon order) are staged
+  // and the slots of the rest are spilled (see below).
+  const bool fits = overflow == 0;
+  const uint32_t staged_n = total < kStageCap ? total : kStageCap;
+  if (tid == 0) sp.cnt[b] = fits ? staged_n : kStageOverflow;
   if (!fits) return;  // block-uniform
   uint32_t next = scan[tid] - mine;
   for (uint32_t j = 0; j < kPer; j++) {
@@ -132,7 +138,7 @@ __global__ __launch_bounds__(kBlockSomas) void stage_build_kernel(const uint32_t
     }
   }
   __syncthreads();  // also when S == 1 and the loops above did not run
-  for (uint32_t j = tid; j < total; j += kBlockSomas) sp.set[(size_t)b * kStageCap + j] = sorted[j];
+  for (uint32_t j = tid; j < staged_n; j += kBlockSomas) sp.set[(size_t)b * kStageCap + j] = sorted[j];
   if (i >= N) return;
   for (uint32_t s = 0; s < F; s++) {
     const size_t o = (size_t)s * N + i;
@@ -145,8 +151,12 @@ __global__ __launch_bounds__(kBlockSomas) void stage_build_kernel(const uint32_t
       else hi = mid;
     }
     const size_t c = ((size_t)(s >> 3) * N + i) * 8 + (s & 7);  // chunk-interleaved
-    sp.code[c] = (uint16_t)lo;
-    sp.seg[c] = (uint8_t)(t >> 24);
+    // A spilled slot (its axon past the staged ones) gets the inert entry
+    // and segment 255, which sends it down the scan's exact path (segment
+    // + length > 255), where the target and the u32 heads decide it.
+    const bool spilled = lo >= kStageCap;
+    sp.code[c] = (uint16_t)(spilled ? kStageCap : lo);
+    sp.seg[c] = (uint8_t)(spilled ? 255u : (t >> 24));
   }
   // The rest of the last chunk: the inert entry, segment 0 (never a hit).
   for (uint32_t s = F; s & 7; s++) {
@@ -283,8 +293,8 @@ __device__ __forceinline__ void hit_chunk(const Chunk<STAGED, PW>& now, uint32_t
     for (int u = 0; u < H; u += 2)
       M[(h0 + u) / 2] = pair_min(dist_pair_lanes(e[u], S[h0 + u]), dist_pair_lanes(e[u + 1], S[h0 + u + 1]));
   }
-  // Segment + length beyond the byte form (rare): the slot's exact distance
-  // from its target and the 16-byte compact entry.
+  // Segment + length beyond the byte form (rare), or a spilled slot: the
+  // slot's exact distance from its target (segment included) and the heads.
   uint32_t mx;
   if (STAGED) {
     mx = max_byte8(now.k[4], now.k[5]);
@@ -305,7 +315,7 @@ __device__ __forceinline__ void hit_chunk(const Chunk<STAGED, PW>& now, uint32_t
         else
           t = now.k[u];
         // A slot at or past the fill has the empty target: no hit.
-        const uint32_t d = min(t ? min_dist_heads(heads, A, snap_index(t, A), sg) : 0xFFFFu, 0xFFFFu);
+        const uint32_t d = min(t ? min_dist_heads(heads, A, snap_index(t, A), t >> 24) : 0xFFFFu, 0xFFFFu);
         const uint32_t sh = 16 * (u & 1);
         M[u / 2] = (M[u / 2] & ~(0xFFFFu << sh)) | (d << sh);
       }

@@ -187,8 +187,9 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--ticks", type=int, default=50, help="ticks per step (one BSP batch)")
-    ap.add_argument("--workload", default="c2", choices=["c2", "c1", "c2-linear"],
-                    help="c2-linear: the c2 brain with somas numbered x-fastest instead of in bricks")
+    ap.add_argument("--workload", default="c2", choices=["c2", "c1", "c2-linear", "c5", "c5-sub"],
+                    help="c2-linear: the c2 brain with somas numbered x-fastest instead of in bricks; "
+                         "c5: the 12M-neuron x 128-slot scale point; c5-sub: its 200k-neuron sub-shard")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--cpu-max-ticks", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
@@ -220,7 +221,8 @@ def main() -> int:
                                                    serial=not args.batched_no_serial)}), flush=True)
         return 0
 
-    spec = {"c2": B.spec_c2, "c1": B.spec_c1, "c2-linear": lambda: B.spec_c2(brick=0)}[args.workload]()
+    spec = {"c2": B.spec_c2, "c1": B.spec_c1, "c2-linear": lambda: B.spec_c2(brick=0), "c5": B.spec_c5,
+            "c5-sub": lambda: B.spec_c5(200_000)}[args.workload]()
     t0 = time.perf_counter()
     brain = B.generate(spec)
     N, A = brain.padded_n, brain.total_axons
@@ -356,7 +358,7 @@ def main() -> int:
     log("per-kernel: " + json.dumps(per_kernel))
 
     traffic = None
-    tf = os.path.join(REPO, "profiles", "neuron_tick_c2_traffic.json")
+    tf = os.path.join(REPO, "profiles", f"neuron_tick_{args.workload}_traffic.json")
     if os.path.exists(tf):
         try:
             traffic = json.load(open(tf)).get("hbm_bytes_per_launch")

@@ -68,3 +68,24 @@ def test_inputs_bernoulli_rate_and_determinism():
     assert abs(bits.mean() - 0.05) < 0.003
     assert np.array_equal(x, B.gen_inputs(5, 10_000, 50_000, 0, 50))
     assert np.array_equal(x[10:], B.gen_inputs(5, 10_000, 50_000, 10, 40))
+
+
+def test_c5_scale_point_and_its_subshard():
+    full = B.spec_c5()
+    assert (full.n_neurons, full.slots_filled, full.grid) == (12_000_000, 128, (256, 256, 128))
+    assert full.n_inputs == 240_000 and full.input_ppm == 10_000 and full.axon_len == (1, 255)
+    assert len(full.type_ppm) == 6 and sum(full.type_ppm) == 1_000_000
+    assert (full.lut["threshold"][:6] > 0).all() and (full.lut["threshold"][6:] == 0).all()
+    assert B.sizes(full) == (12_000_000, 12_000_000 + 240_000)
+    sub = B.spec_c5(200_000)
+    assert sub.grid == (64, 64, 32) and sub.n_inputs == 4_000 and sub.seed == full.seed
+    b = B.generate(sub)
+    v = b.view()
+    n = 200_000
+    assert (v.targets[:128, :n] != 0).all()
+    types = v.flags[:n] >> 4
+    assert types.max() == 5
+    assert abs((types >= 3).mean() - 0.2) < 0.01  # 8% + 7% + 5% inhibitory
+    axon = (v.targets[:128, :n] & 0xFFFFFF) - 1
+    assert (axon < b.total_axons).all()
+    assert ((v.targets[:128, :n] >> 24) < b.axon_len[axon]).all()

@@ -1,13 +1,12 @@
 """The staged dendrite scan (per-block presynaptic entries in LDS) against the
 CPU oracle and against the direct scan, on the cases that take its other
-paths: blocks whose distinct presynaptic axons overflow the LDS capacity,
-segments whose segment + propagation length passes 255 (the 16-byte
-fallback), filled slots after an empty one (ignored, as the reference's
-`break`), and targets changed between steps (the plan is rebuilt). Each runs
-tick by tick and through the batch graph, where tick t's GSOP is applied in
-tick t+1's staged pass (from the staged entries of tick t+1, or from the
-previous compact snapshot for pushed inputs, network spikes and long
-reaches)."""
+paths: blocks whose distinct presynaptic axons exceed the LDS capacity (the
+slots of the axons past it spilled to the exact path) or the build's hash set
+(the unstaged scan), segments whose segment + propagation length passes 255
+(the exact path from the u32 heads), filled slots after an empty one
+(ignored, as the reference's `break`), and targets changed between steps
+(the plan is rebuilt). Each runs tick by tick and through the batch graph, where the staged pass and GSOP use
+the plan's working copy of the weights."""
 import ctypes as C
 
 import numpy as np
@@ -33,17 +32,32 @@ GRAPH = pytest.mark.parametrize("use_graph", [False, True], ids=["per_tick", "gr
 
 
 @GRAPH
-def test_overflow_blocks_far_targets(gpu, use_graph):
-    """Targets drawn far from their somas (mean distance 60 voxels): most
-    256-soma blocks have more distinct presynaptic axons than the LDS holds
-    and read the reference targets; the rest are staged."""
-    spec = B.spec_small(12000, 120, 600, seed=21, axon_len=(1, 64))
-    spec.mean_dist = 60.0
+def test_spilled_slots_past_the_lds_capacity(gpu, use_graph):
+    """40k neurons x 128 slots on a 100x100x4 grid with targets 12 voxels
+    away on average: 80% of the 256-soma blocks have more than 8,704
+    distinct presynaptic axons (median 10,700). They stage the first 8,704
+    and the slots of the others take the exact path."""
+    spec = B.spec_small(40000, 128, 2000, seed=37, axon_len=(1, 64), grid=(100, 100, 4))
+    spec.mean_dist = 12.0
     p = Pair(B.generate(spec), n_out=32)
-    for k in range(4):
-        out_ref, _ = p.step(15, dopamine=(120 if k % 2 else -80), use_graph=use_graph)
-        np.testing.assert_array_equal(p.outputs(15), out_ref)
-        p.check(f"far targets batch {k}")
+    for k in range(3):
+        out_ref, _ = p.step(12, dopamine=(120 if k % 2 else -80), use_graph=use_graph)
+        np.testing.assert_array_equal(p.outputs(12), out_ref)
+        p.check(f"spilled slots batch {k}")
+
+
+@GRAPH
+def test_unstaged_blocks_hash_overflow(gpu, use_graph):
+    """65,536 neurons x 128 slots on a flat 256x256 grid with targets 60
+    voxels away on average: over half of the 256-soma blocks have more
+    distinct presynaptic axons than the build's 16,384-entry hash set and
+    take the unstaged scan; the others stage 8,704 and spill the rest."""
+    spec = B.spec_small(65536, 128, 3276, seed=37, axon_len=(1, 64), grid=(256, 256, 1))
+    spec.mean_dist = 60.0
+    p = Pair(B.generate(spec), n_out=16)
+    for k in range(3):
+        p.step(8, dopamine=(90 if k % 2 else -60), use_graph=use_graph)
+        p.check(f"hash overflow batch {k}")
 
 
 @GRAPH
