@@ -5,12 +5,17 @@ Night boundaries: day batches through the batch graph, a Night (flags before
 the GPU sort & prune, host sprouting, upload; shard_thread.rs:269-378), then
 more day batches. At this density 256-soma blocks hold up to ~9,300
 distinct presynaptic axons, so the staged scan's spill path runs in every
-tick. The full 12M-neuron brain is measured by `bench.py --workload c5`."""
+tick. The full 12M-neuron brain is measured by `bench.py --workload c5`.
+
+Also the full 1.5M-neuron C2 brain: the spike raster of every soma compared
+with the oracle's every tick for the first 100 ticks, the whole state at the
+batch boundaries and after a Night's sort & prune."""
 import numpy as np
 import pytest
 
 from axicor_amd import brain as B
 from axicor_amd import night as NT
+from axicor_amd.engine import DeviceBuffer
 from axicor_amd.layout import StateView
 from helpers import Pair
 from oracle import oracle as O
@@ -41,3 +46,22 @@ def test_c5_subshard_bit_exact_at_night_boundaries(gpu):
     out_ref, _ = p.step(50, dopamine=30, use_graph=True)
     np.testing.assert_array_equal(p.outputs(50), out_ref)
     p.check("c5 sub-shard day after the night")
+
+
+def test_c2_every_tick_for_100_ticks_and_a_night(gpu):
+    b = B.generate(B.spec_c2())
+    N = b.padded_n
+    p = Pair(b, n_out=N, max_batch=50)
+    p.mapped = np.arange(N, dtype=np.uint32)  # every soma's spike bit, every tick
+    p.d_mapped = DeviceBuffer.from_array(p.mapped)
+    for k in range(2):
+        out_ref, _ = p.step(50, dopamine=(40 if k else -25), use_graph=True)
+        got = p.outputs(50)
+        for t in range(50):
+            np.testing.assert_array_equal(got[t], out_ref[t], err_msg=f"C2 raster tick {50 * k + t}")
+        p.check(f"C2 tick {p.tick}")
+    assert 0.002 < out_ref.mean() < 0.05  # the brain is firing (about 1% per tick)
+    p.night(15)
+    p.check("C2 after the Night's sort & prune")
+    p.step(10, use_graph=True)
+    p.check("C2 day after the Night")
