@@ -7,9 +7,16 @@
 // GEMMs run on v_mfma_f32_16x16x32_bf16. A workgroup of 4 waves computes a
 // 64x64 tile of C, each wave a 32x32 quarter as 2x2 MFMA tiles; K advances in
 // steps of 32 staged through LDS with 16-byte global loads. One kernel serves
-// the forward product (X W), the input gradient (dZ W^T) and the weight
-// gradient (X^T dZ) through the transpose flags, with the elementwise step of
-// each (bias, GELU, GELU') fused into its epilogue.
+// the forward product (X W) and the input gradient (dZ W^T) in any operand
+// layout through the transpose flags, with the elementwise step of each
+// (bias, GELU, GELU') fused into its epilogue. Products with many rows whose
+// operands are both contiguous in K (the forward product reads a transposed
+// bf16 copy of W for this) take gemm_big_kernel instead: 128x128 tiles, each
+// wave 64x64 as 4x4 MFMA tiles, and the next K step's global loads in flight
+// while the current one is multiplied (two LDS buffers, one barrier per
+// step). The weight gradient (X^T dZ, a sum over the batch) transposes X and
+// dZ to rows contiguous in the batch and runs the big kernel split over the
+// batch, adding the partials in a fixed order.
 //
 // MFMA 16x16x32 operand layout, lane l of 64: the A fragment is 8 bf16 of row
 // l%16, columns 8*(l/16)..+7; the B fragment is 8 bf16 of column l%16, rows
@@ -29,6 +36,7 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 constexpr int BM = 64, BN = 64, BK = 32;
 constexpr int LDSK = BK + 8;      // row pitch of the LDS tiles in bf16 (80 B): spreads rows over banks
 constexpr int kColsumChunks = 32; // row chunks of the two-pass column sum
+constexpr int kWgSplits = 16;     // batch splits of the weight gradient (split K)
 
 enum Epi { EPI_BIAS_GELU = 0, EPI_BIAS = 1, EPI_GELU_GRAD = 2, EPI_STORE = 3, EPI_BIAS_GELU_NOZ = 4 };
 
@@ -73,6 +81,34 @@ __device__ __forceinline__ void stage_tile(__bf16 (*T)[LDSK], const __bf16* __re
   }
 }
 
+// The epilogue for one 16x16 accumulator tile at (r0, c0): accumulator r of
+// lane l is row r0 + 4(l/16) + r, column c0 + l%16.
+template <int EPI>
+__device__ __forceinline__ void store_tile(const f32x4& acc, int r0, int c0, int lane, int M, int N,
+                                           const float* __restrict__ bias, const float* __restrict__ aux, int ldaux,
+                                           float* __restrict__ C32, __bf16* __restrict__ C16, int ldc) {
+  const int col = c0 + (lane & 15);
+  if (col >= N) return;
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    const int row = r0 + 4 * (lane >> 4) + r;
+    if (row >= M) continue;
+    float v = acc[r];
+    const size_t o = (size_t)row * ldc + col;
+    if (EPI == EPI_BIAS_GELU || EPI == EPI_BIAS_GELU_NOZ) {
+      v += bias[col];
+      if (EPI == EPI_BIAS_GELU) C32[o] = v;  // pre-activation, kept for the backward pass
+      C16[o] = (__bf16)gelu_f(v);
+    } else if (EPI == EPI_BIAS) {
+      C32[o] = v + bias[col];
+    } else if (EPI == EPI_GELU_GRAD) {
+      C16[o] = (__bf16)(v * gelu_grad_f(aux[(size_t)row * ldaux + col]));
+    } else {
+      C32[o] = v;
+    }
+  }
+}
+
 // C[M,N] = op(A)[M,K] op(B)[K,N], then the epilogue. op(A)(m,k) is A[m*lda+k],
 // or A[k*lda+m] when TA; op(B)(k,n) is B[k*ldb+n], or B[n*ldb+k] when TB.
 // Leading dimensions are multiples of 8 elements; M, N, K are arbitrary.
@@ -105,29 +141,95 @@ __global__ __launch_bounds__(256) void gemm_kernel(int M, int N, int K, const __
     __syncthreads();
   }
 
-  for (int i = 0; i < 2; i++) {
-    for (int j = 0; j < 2; j++) {
-      const int col = n0 + wn * 32 + j * 16 + (lane & 15);
-      if (col >= N) continue;
-      for (int r = 0; r < 4; r++) {
-        const int row = m0 + wm * 32 + i * 16 + 4 * (lane >> 4) + r;
-        if (row >= M) continue;
-        float v = acc[i][j][r];
-        const size_t o = (size_t)row * ldc + col;
-        if (EPI == EPI_BIAS_GELU || EPI == EPI_BIAS_GELU_NOZ) {
-          v += bias[col];
-          if (EPI == EPI_BIAS_GELU) C32[o] = v;  // pre-activation, kept for the backward pass
-          C16[o] = (__bf16)gelu_f(v);
-        } else if (EPI == EPI_BIAS) {
-          C32[o] = v + bias[col];
-        } else if (EPI == EPI_GELU_GRAD) {
-          C16[o] = (__bf16)(v * gelu_grad_f(aux[(size_t)row * ldaux + col]));
-        } else {
-          C32[o] = v;
-        }
-      }
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++)
+      store_tile<EPI>(acc[i][j], m0 + wm * 32 + i * 16, n0 + wn * 32 + j * 16, lane, M, N, bias, aux, ldaux, C32,
+                      C16, ldc);
+}
+
+// gemm_big_kernel: C[M,N] = A[M,K] B[N,K]^T with A[m*lda+k] and B[n*ldb+k]
+// (both contiguous in K), then the epilogue. K, lda and ldb are multiples of
+// 8; M and N are arbitrary (rows past them read as zero and are not stored).
+// Split K (ksplit > 0, a multiple of 32, EPI_STORE only): workgroup z sums
+// k in [z ksplit, (z+1) ksplit) into its own partial C32 + z M ldc.
+// 256 threads: wave w computes rows 64(w/2).., columns 64(w%2).. of the
+// 128x128 tile as 4x4 MFMA 16x16x32 tiles. Each K step of 32 is two 128x32
+// operand blocks, 512 pieces of 16 bytes each, two pieces per thread; the
+// pieces of step s+1 are loaded into registers before step s is multiplied
+// and stored to the other LDS buffer after it.
+constexpr int BT = 128;
+__device__ __forceinline__ uint4 load_piece(const __bf16* __restrict__ P, int ld, int r, int R, int k, int K) {
+  return (r < R && k < K) ? *reinterpret_cast<const uint4*>(P + (size_t)r * ld + k) : make_uint4(0u, 0u, 0u, 0u);
+}
+template <int EPI>
+__global__ __launch_bounds__(256) void gemm_big_kernel(int M, int N, int K, const __bf16* __restrict__ A, int lda,
+                                                       const __bf16* __restrict__ B, int ldb,
+                                                       const float* __restrict__ bias, const float* __restrict__ aux,
+                                                       int ldaux, float* __restrict__ C32, __bf16* __restrict__ C16,
+                                                       int ldc, int ksplit) {
+  __shared__ __attribute__((aligned(16))) __bf16 As[2][BT][LDSK];
+  __shared__ __attribute__((aligned(16))) __bf16 Bs[2][BT][LDSK];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int wm = w >> 1, wn = w & 1;
+  const int m0 = blockIdx.y * BT, n0 = blockIdx.x * BT;
+  const int kb = ksplit ? (int)blockIdx.z * ksplit : 0;
+  const int ke = ksplit ? min(K, kb + ksplit) : K;
+  if (ksplit) C32 += (size_t)blockIdx.z * M * ldc;
+  // This thread's two pieces of each operand block: rows pr, pr + 64, column pk.
+  const int pr = tid >> 2, pk = (tid & 3) * 8;
+  f32x4 acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+  uint4 ra0 = load_piece(A, lda, m0 + pr, M, kb + pk, ke), ra1 = load_piece(A, lda, m0 + pr + 64, M, kb + pk, ke);
+  uint4 rb0 = load_piece(B, ldb, n0 + pr, N, kb + pk, ke), rb1 = load_piece(B, ldb, n0 + pr + 64, N, kb + pk, ke);
+  *reinterpret_cast<uint4*>(&As[0][pr][pk]) = ra0;
+  *reinterpret_cast<uint4*>(&As[0][pr + 64][pk]) = ra1;
+  *reinterpret_cast<uint4*>(&Bs[0][pr][pk]) = rb0;
+  *reinterpret_cast<uint4*>(&Bs[0][pr + 64][pk]) = rb1;
+  __syncthreads();
+  int cur = 0;
+  for (int k0 = kb; k0 < ke; k0 += BK) {
+    const bool more = k0 + BK < ke;
+    if (more) {
+      ra0 = load_piece(A, lda, m0 + pr, M, k0 + BK + pk, ke);
+      ra1 = load_piece(A, lda, m0 + pr + 64, M, k0 + BK + pk, ke);
+      rb0 = load_piece(B, ldb, n0 + pr, N, k0 + BK + pk, ke);
+      rb1 = load_piece(B, ldb, n0 + pr + 64, N, k0 + BK + pk, ke);
     }
+    bf16x8 a[4], b[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+      a[i] = *reinterpret_cast<const bf16x8*>(&As[cur][wm * 64 + i * 16 + (lane & 15)][8 * (lane >> 4)]);
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+      b[j] = *reinterpret_cast<const bf16x8*>(&Bs[cur][wn * 64 + j * 16 + (lane & 15)][8 * (lane >> 4)]);
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+      for (int j = 0; j < 4; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+    if (more) {
+      *reinterpret_cast<uint4*>(&As[cur ^ 1][pr][pk]) = ra0;
+      *reinterpret_cast<uint4*>(&As[cur ^ 1][pr + 64][pk]) = ra1;
+      *reinterpret_cast<uint4*>(&Bs[cur ^ 1][pr][pk]) = rb0;
+      *reinterpret_cast<uint4*>(&Bs[cur ^ 1][pr + 64][pk]) = rb1;
+    }
+    // Buffer cur^1 is complete for the next step, and every wave is done
+    // reading buffer cur before the next step overwrites it.
+    __syncthreads();
+    cur ^= 1;
   }
+  // Unrolled, like every loop over acc: the tiles stay in registers.
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+      store_tile<EPI>(acc[i][j], m0 + wm * 64 + i * 16, n0 + wn * 64 + j * 16, lane, M, N, bias, aux, ldaux, C32,
+                      C16, ldc);
 }
 
 // Gradient of the mean squared error over the first n_out columns:
@@ -194,12 +296,45 @@ __global__ __launch_bounds__(256) void cast_rows_kernel(const float* __restrict_
   Y[idx] = (__bf16)(c < cols ? X[(size_t)r * ldx + c] : 0.0f);
 }
 
+// dst[c][r] = src[r][c] for src [rows][cols] (row pitch lds) into dst with
+// row pitch ldd >= rows; dst columns [rows, ldd) are zeroed. 64x64 tiles
+// through LDS, 256 threads, both sides read and written along rows.
+__global__ __launch_bounds__(256) void transpose_bf16_kernel(const __bf16* __restrict__ src, int rows, int cols,
+                                                             int lds, __bf16* __restrict__ dst, int ldd) {
+  __shared__ __bf16 T[64][65];
+  const int r0 = blockIdx.y * 64, c0 = blockIdx.x * 64, tid = threadIdx.x;
+  const __bf16 zero = (__bf16)0.0f;
+#pragma unroll
+  for (int q = 0; q < 16; q++) {
+    const int r = q * 4 + (tid >> 6), c = tid & 63;
+    T[r][c] = (r0 + r < rows && c0 + c < cols) ? src[(size_t)(r0 + r) * lds + c0 + c] : zero;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < 16; q++) {
+    const int c = q * 4 + (tid >> 6), r = tid & 63;  // dst row c0 + c, column r0 + r
+    if (c0 + c < cols && r0 + r < ldd) dst[(size_t)(c0 + c) * ldd + r0 + r] = T[r][c];
+  }
+}
+
+// out[i] = sum over z < parts of part[z n + i], in order z = 0, 1, ...
+// (the split-K partials of the weight gradient: deterministic).
+__global__ __launch_bounds__(256) void sum_parts_kernel(const float* __restrict__ part, int parts, int n,
+                                                        float* __restrict__ out) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  float acc = 0.f;
+  for (int z = 0; z < parts; z++) acc += part[(size_t)z * n + i];
+  out[i] = acc;
+}
+
 // Adam on the fp32 master parameters (bias-corrected, c1 = 1 - b1^t,
 // c2 = 1 - b2^t); refreshes the bf16 copy the GEMMs read. Entries with
 // mask 0 (padding) are left untouched.
 __global__ __launch_bounds__(256) void adam_kernel(float* __restrict__ p, const float* __restrict__ g,
                                                    float* __restrict__ m, float* __restrict__ v, __bf16* __restrict__ p16,
-                                                   int n, float lr, float b1, float b2, float eps, float c1, float c2,
+                                                   __bf16* __restrict__ p16t, int cols, int n, float lr, float b1,
+                                                   float b2, float eps, float c1, float c2,
                                                    const uint8_t* __restrict__ mask) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= n || !mask[i]) return;
@@ -211,11 +346,17 @@ __global__ __launch_bounds__(256) void adam_kernel(float* __restrict__ p, const 
   const float pi = p[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
   p[i] = pi;
   if (p16) p16[i] = (__bf16)pi;
+  if (p16t) p16t[(size_t)(i % cols) * (n / cols) + i / cols] = (__bf16)pi;  // [cols][rows]
 }
 
-__global__ __launch_bounds__(256) void to_bf16_kernel(const float* __restrict__ x, __bf16* __restrict__ y, int n) {
+// y = bf16(x) for x[rows][cols], and yt[cols][rows] its transpose.
+__global__ __launch_bounds__(256) void to_bf16_kernel(const float* __restrict__ x, __bf16* __restrict__ y,
+                                                      __bf16* __restrict__ yt, int rows, int cols) {
   const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i < n) y[i] = (__bf16)x[i];
+  if (i >= rows * cols) return;
+  const __bf16 b = (__bf16)x[i];
+  y[i] = b;
+  yt[(size_t)(i % cols) * rows + i / cols] = b;
 }
 
 }  // namespace axcmlp
@@ -228,6 +369,7 @@ struct Layer {
   int in, out;             // padded sizes (in: multiple of 32, out: multiple of 16)
   float *W, *b;            // fp32 master parameters, W[in][out] row-major
   __bf16* W16;             // bf16 copy read by the GEMMs
+  __bf16* W16T;            // its transpose [out][in]: the forward product's operand, contiguous in K
   float *gW, *gb, *mW, *vW, *mb, *vb;
   uint8_t *maskW, *maskb;  // 1 on real entries, 0 on padding
 };
@@ -241,6 +383,9 @@ struct Mlp {
   std::vector<__bf16*> dZ;    // [max_batch][out_l] bf16: gradient wrt Z[l]
   float* part = nullptr;      // loss partial sums
   float* cpart = nullptr;     // column-sum partials [kColsumChunks][max out]
+  __bf16* hT = nullptr;       // [max in][pad(max_batch, 8)]: H_l transposed for the weight gradient
+  __bf16* dzT = nullptr;      // [max out][pad(max_batch, 8)]: dZ_l transposed
+  float* wpart = nullptr;     // [kWgSplits][max in x out]: split-K partials of the weight gradient
   int step = 0;
   int max_out = 0;
 };
@@ -248,11 +393,38 @@ struct Mlp {
 inline int pad(int x, int m) { return (x + m - 1) / m * m; }
 inline unsigned grid1(size_t n) { return (unsigned)((n + 255) / 256); }
 
+// 128x128 tiles once they fill the chip (at least one per CU); below that
+// the 64x64 kernel's four times as many workgroups win.
+constexpr int kBigTilesMin = 256;
 template <bool TA, bool TB, int EPI>
 void gemm(hipStream_t st, int M, int N, int K, const __bf16* A, int lda, const __bf16* B, int ldb, const float* bias,
           const float* aux, int ldaux, float* C32, __bf16* C16, int ldc) {
+  const long tiles = (long)((M + BT - 1) / BT) * ((N + BT - 1) / BT);
+  if (!TA && TB && tiles >= kBigTilesMin && K % 8 == 0 && lda % 8 == 0 && ldb % 8 == 0) {
+    dim3 grid((N + BT - 1) / BT, (M + BT - 1) / BT);
+    gemm_big_kernel<EPI><<<grid, 256, 0, st>>>(M, N, K, A, lda, B, ldb, bias, aux, ldaux, C32, C16, ldc, 0);
+    return;
+  }
   dim3 grid((N + BN - 1) / BN, (M + BM - 1) / BM);
   gemm_kernel<TA, TB, EPI><<<grid, 256, 0, st>>>(M, N, K, A, lda, B, ldb, bias, aux, ldaux, C32, C16, ldc);
+}
+
+// dW_l = H_l^T dZ_l: H_l and dZ_l transposed to [feature][batch] (rows
+// contiguous in the batch, zero-padded to a multiple of 8), then the big
+// kernel with the batch split over kWgSplits workgroup layers, and the
+// partials added in a fixed order.
+void weight_grad(Mlp* m, int l, int Bn, hipStream_t st) {
+  Layer& y = m->L[l];
+  const int ldT = pad(Bn, 8);
+  transpose_bf16_kernel<<<dim3((y.in + 63) / 64, (ldT + 63) / 64), 256, 0, st>>>(m->H[l], Bn, y.in, y.in, m->hT, ldT);
+  transpose_bf16_kernel<<<dim3((y.out + 63) / 64, (ldT + 63) / 64), 256, 0, st>>>(m->dZ[l], Bn, y.out, y.out, m->dzT,
+                                                                                  ldT);
+  const int ksplit = pad((ldT + kWgSplits - 1) / kWgSplits, BK);
+  const int parts = (ldT + ksplit - 1) / ksplit;
+  dim3 grid((y.out + BT - 1) / BT, (y.in + BT - 1) / BT, parts);
+  gemm_big_kernel<EPI_STORE><<<grid, 256, 0, st>>>(y.in, y.out, ldT, m->hT, ldT, m->dzT, ldT, nullptr, nullptr, 0,
+                                                   m->wpart, nullptr, y.out, ksplit);
+  sum_parts_kernel<<<grid1((size_t)y.in * y.out), 256, 0, st>>>(m->wpart, parts, y.in * y.out, y.gW);
 }
 
 void forward(Mlp* m, int Bn, hipStream_t st, bool keep_z) {
@@ -261,13 +433,13 @@ void forward(Mlp* m, int Bn, hipStream_t st, bool keep_z) {
     Layer& y = m->L[l];
     if (l + 1 < nl) {
       if (keep_z)
-        gemm<false, false, EPI_BIAS_GELU>(st, Bn, y.out, y.in, m->H[l], y.in, y.W16, y.out, y.b, nullptr, 0, m->Z[l],
+        gemm<false, true, EPI_BIAS_GELU>(st, Bn, y.out, y.in, m->H[l], y.in, y.W16T, y.in, y.b, nullptr, 0, m->Z[l],
                                           m->H[l + 1], y.out);
       else
-        gemm<false, false, EPI_BIAS_GELU_NOZ>(st, Bn, y.out, y.in, m->H[l], y.in, y.W16, y.out, y.b, nullptr, 0,
+        gemm<false, true, EPI_BIAS_GELU_NOZ>(st, Bn, y.out, y.in, m->H[l], y.in, y.W16T, y.in, y.b, nullptr, 0,
                                               nullptr, m->H[l + 1], y.out);
     } else {
-      gemm<false, false, EPI_BIAS>(st, Bn, y.out, y.in, m->H[l], y.in, y.W16, y.out, y.b, nullptr, 0, m->Z[l],
+      gemm<false, true, EPI_BIAS>(st, Bn, y.out, y.in, m->H[l], y.in, y.W16T, y.in, y.b, nullptr, 0, m->Z[l],
                                    nullptr, y.out);
     }
   }
@@ -282,8 +454,7 @@ void backward(Mlp* m, const float* d_t, int Bn, hipStream_t st) {
   for (int l = nl - 1; l >= 0; l--) {
     Layer& y = m->L[l];
     // dW_l = H_l^T dZ_l (in x out); db_l = column sums of dZ_l.
-    gemm<true, false, EPI_STORE>(st, y.in, y.out, Bn, m->H[l], y.in, m->dZ[l], y.out, nullptr, nullptr, 0, y.gW,
-                                 nullptr, y.out);
+    weight_grad(m, l, Bn, st);
     colsum_part_kernel<<<dim3((y.out + 63) / 64, kColsumChunks), 256, 0, st>>>(m->dZ[l], Bn, y.out, y.out, m->cpart);
     colsum_final_kernel<<<grid1(y.out), 256, 0, st>>>(m->cpart, y.out, y.gb);
     if (l > 0) {
@@ -308,14 +479,14 @@ uint64_t splitmix(uint64_t& s) {
 
 void release(Mlp* m) {
   for (Layer& y : m->L)
-    for (void* p : {(void*)y.W, (void*)y.b, (void*)y.W16, (void*)y.gW, (void*)y.gb, (void*)y.mW, (void*)y.vW,
-                    (void*)y.mb, (void*)y.vb, (void*)y.maskW, (void*)y.maskb})
+    for (void* p : {(void*)y.W, (void*)y.b, (void*)y.W16, (void*)y.W16T, (void*)y.gW, (void*)y.gb, (void*)y.mW,
+                    (void*)y.vW, (void*)y.mb, (void*)y.vb, (void*)y.maskW, (void*)y.maskb})
       if (p) (void)hipFree(p);
   for (auto p : m->H) if (p) (void)hipFree(p);
   for (auto p : m->Z) if (p) (void)hipFree(p);
   for (auto p : m->dZ) if (p) (void)hipFree(p);
-  if (m->part) (void)hipFree(m->part);
-  if (m->cpart) (void)hipFree(m->cpart);
+  for (void* p : {(void*)m->part, (void*)m->cpart, (void*)m->hT, (void*)m->dzT, (void*)m->wpart})
+    if (p) (void)hipFree(p);
   delete m;
 }
 
@@ -364,6 +535,7 @@ int32_t axc_mlp_create(int d_in, int hidden, int depth, int d_out, int max_batch
     for (float** p : {&y.b, &y.gb, &y.mb, &y.vb})
       if (e == hipSuccess) e = hipMalloc((void**)p, (size_t)y.out * 4);
     if (e == hipSuccess) e = hipMalloc((void**)&y.W16, nw * 2);
+    if (e == hipSuccess) e = hipMalloc((void**)&y.W16T, nw * 2);
     if (e == hipSuccess) e = hipMalloc((void**)&y.maskW, nw);
     if (e == hipSuccess) e = hipMalloc((void**)&y.maskb, y.out);
     if (e == hipSuccess) e = hipMemcpy(y.W, hw.data(), nw * 4, hipMemcpyHostToDevice);
@@ -375,7 +547,7 @@ int32_t axc_mlp_create(int d_in, int hidden, int depth, int d_out, int max_batch
     for (float* p : {y.gb, y.mb, y.vb})
       if (e == hipSuccess) e = hipMemset(p, 0, (size_t)y.out * 4);
     if (e == hipSuccess) {
-      to_bf16_kernel<<<grid1(nw), 256>>>(y.W, y.W16, (int)nw);
+      to_bf16_kernel<<<grid1(nw), 256>>>(y.W, y.W16, y.W16T, y.in, y.out);
       e = hipGetLastError();
     }
   }
@@ -392,6 +564,18 @@ int32_t axc_mlp_create(int d_in, int hidden, int depth, int d_out, int max_batch
   }
   if (e == hipSuccess) e = hipMalloc((void**)&m->part, (size_t)grid1(B * dims.back()) * 4);
   if (e == hipSuccess) e = hipMalloc((void**)&m->cpart, (size_t)kColsumChunks * m->max_out * 4);
+  {
+    int max_in = 0;
+    size_t max_w = 0;
+    for (const Layer& y : m->L) {
+      max_in = std::max(max_in, y.in);
+      max_w = std::max(max_w, (size_t)y.in * y.out);
+    }
+    const size_t ldT = (size_t)pad(max_batch, 8);
+    if (e == hipSuccess) e = hipMalloc((void**)&m->hT, (size_t)max_in * ldT * 2);
+    if (e == hipSuccess) e = hipMalloc((void**)&m->dzT, (size_t)m->max_out * ldT * 2);
+    if (e == hipSuccess) e = hipMalloc((void**)&m->wpart, (size_t)kWgSplits * max_w * 4);
+  }
   if (e == hipSuccess) e = hipDeviceSynchronize();
   if (e != hipSuccess) {
     release(m);
@@ -429,8 +613,10 @@ int32_t axc_mlp_train_step(void* h, const float* d_x, const float* d_t, int n, f
   const float c1 = 1.0f - powf(b1, (float)m->step), c2 = 1.0f - powf(b2, (float)m->step);
   for (Layer& y : m->L) {
     const int nw = y.in * y.out;
-    adam_kernel<<<grid1(nw), 256, 0, st>>>(y.W, y.gW, y.mW, y.vW, y.W16, nw, lr, b1, b2, eps, c1, c2, y.maskW);
-    adam_kernel<<<grid1(y.out), 256, 0, st>>>(y.b, y.gb, y.mb, y.vb, nullptr, y.out, lr, b1, b2, eps, c1, c2, y.maskb);
+    adam_kernel<<<grid1(nw), 256, 0, st>>>(y.W, y.gW, y.mW, y.vW, y.W16, y.W16T, y.out, nw, lr, b1, b2, eps, c1, c2,
+                                           y.maskW);
+    adam_kernel<<<grid1(y.out), 256, 0, st>>>(y.b, y.gb, y.mb, y.vb, nullptr, nullptr, y.out, y.out, lr, b1, b2, eps,
+                                              c1, c2, y.maskb);
   }
   hipError_t e = hipGetLastError();
   if (e == hipSuccess && loss) {
@@ -511,7 +697,7 @@ int32_t axc_mlp_set_params(void* h, const float* host) {
     const size_t nw = (size_t)y.in * y.out;
     if (e == hipSuccess) e = hipMemcpy(y.W, host + off, nw * 4, hipMemcpyHostToDevice);
     if (e == hipSuccess) {
-      to_bf16_kernel<<<grid1(nw), 256>>>(y.W, y.W16, (int)nw);
+      to_bf16_kernel<<<grid1(nw), 256>>>(y.W, y.W16, y.W16T, y.in, y.out);
       e = hipGetLastError();
     }
     off += nw;

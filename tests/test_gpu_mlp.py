@@ -43,6 +43,22 @@ def test_gemm_ragged_rows(gpu):
         np.testing.assert_allclose(c, a @ b.T, rtol=0, atol=1e-4 * (np.abs(a) @ np.abs(b.T)).max())
 
 
+@pytest.mark.parametrize("M_,N_,K_", [(8300, 520, 200), (32768, 128, 512), (16385, 264, 24)])
+def test_gemm_big_tiles(gpu, M_, N_, K_):
+    """Products with at least 256 tiles of 128x128 and both operands
+    contiguous in K take the double-buffered 128x128 kernel: ragged M and N,
+    K a multiple of 8 but not of the 32-deep step, and K below one step."""
+    from axicor_amd.surrogate import gemm
+    rng = np.random.default_rng(M_ + N_ + K_)
+    a = M.bf16_round(rng.standard_normal((M_, K_)).astype(np.float32)).astype(np.float64)
+    b = M.bf16_round(rng.standard_normal((N_, K_)).astype(np.float32)).astype(np.float64)
+    c = gemm(bf16_bits(a.astype(np.float32)), bf16_bits(b.astype(np.float32)), False, True)
+    ref = a @ b.T
+    bound = np.abs(a) @ np.abs(b.T)
+    err = np.abs(c - ref)
+    assert np.all(err <= 2 * K_ * 2.0 ** -24 * bound + 1e-30), float((err / (bound + 1e-30)).max())
+
+
 def _net(seed=1, max_batch=512):
     from axicor_amd.surrogate import Mlp
     net = Mlp(16, 512, 3, 8, max_batch=max_batch, seed=seed)
@@ -83,6 +99,22 @@ def test_forward_matches_oracle(gpu):
     Y2 = net.predict(X2)
     assert np.array_equal(Y2[:300], net.predict(X2[:300]))
     assert np.array_equal(Y2[1024:], net.predict(X2[1024:]))
+    net.destroy()
+
+
+def test_forward_with_big_tiles_matches_oracle(gpu):
+    """A batch of 40,000 rows: every layer's product has at least 313 tiles
+    of 128x128 and takes the double-buffered kernel."""
+    net, lay = _net(seed=3, max_batch=40000)
+    rng = np.random.default_rng(4)
+    params = lay.random_params(rng).astype(np.float32)
+    net.set_params(params)
+    X = rng.standard_normal((40000, 16)).astype(np.float32)
+    Y = net.predict(X)
+    Yo, _ = M.forward(lay, params.astype(np.float64), X, emulate=True)
+    rel = np.linalg.norm(Y - Yo) / np.linalg.norm(Yo)
+    print(f"forward (big tiles): rel error vs bf16-emulating oracle {rel:.2e}")
+    assert rel < 5e-3
     net.destroy()
 
 
