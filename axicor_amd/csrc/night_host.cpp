@@ -19,13 +19,12 @@
 //      the presynaptic type;
 //   6. ghost axons left without synapses are reported for pruning.
 // Somas are processed in index order because a sprout changes the weights
-// that later power indices read. Two exact shortcuts keep the cost down: the
-// duplicate test uses a per-soma set of presynaptic axons, and power indices
-// are cached per soma and dropped when that soma sprouts.
+// that later power indices read; the candidate scan of each empty slot is
+// split over threads with a merge that keeps the sequential choice (step 5).
 #include <cstdint>
 #include <cstring>
 #include <vector>
-#include <unordered_set>
+#include <omp.h>
 #include <algorithm>
 
 namespace {
@@ -195,10 +194,17 @@ int axc_sprout_pass(uint32_t* targets, int16_t* weights, const uint8_t* flags, c
   for (size_t a = ghost_start; a < ghost_end; a++) nudge(a);
 
   // 4. Segment grid, cells of 2 voxels, axons in reverse order, dense CSR
-  // with the insertion order of the reference's per-cell vectors.
+  // with the insertion order of the reference's per-cell vectors. Built in
+  // parallel: thread t takes the t-th contiguous range of axons in reverse
+  // order and places its segments after those of threads t' < t in every
+  // cell, which is the sequential insertion order. Only the first segment of
+  // an axon in a cell is kept: the others follow it directly in that cell
+  // and score the same (the score depends on the axon and its type only), so
+  // the scan's strict `>` never picks them.
   const uint32_t cell = 2;
   uint32_t gx = 1, gy = 1, gz = 1;
-  for (size_t a = 0; a < A; a++) {
+#pragma omp parallel for schedule(static) reduction(max : gx, gy, gz)
+  for (int64_t a = 0; a < (int64_t)A; a++) {
     for (uint32_t s = 0; s < lengths[a]; s++) {
       uint32_t p = paths[a * 256 + s];
       if (!p) continue;
@@ -208,105 +214,194 @@ int axc_sprout_pass(uint32_t* targets, int16_t* weights, const uint8_t* flags, c
     }
   }
   const size_t ncell = (size_t)gx * gy * gz;
-  std::vector<uint32_t> start(ncell + 1, 0);
   auto cid = [&](uint32_t p) { return ((size_t)(pz(p) / cell) * gy + py(p) / cell) * gx + px(p) / cell; };
-  for (size_t ai = A; ai-- > 0;)
-    for (uint32_t s = 0; s < lengths[ai]; s++) {
-      uint32_t p = paths[ai * 256 + s];
-      if (p) start[cid(p) + 1]++;
+  const int nth = omp_get_max_threads();
+  // Axon range of thread t in reverse order: axons [A - hi(t), A - lo(t)).
+  auto lo_of = [&](int t) { return (size_t)((A * (size_t)t) / (size_t)nth); };
+  std::vector<uint32_t> cnt((size_t)nth * ncell, 0);
+  std::vector<uint32_t> start(ncell + 1, 0);
+  std::vector<SegRef> refs;
+  {
+    // Per (thread, cell) counts of all segments.
+#pragma omp parallel num_threads(nth)
+    {
+      const int t = omp_get_thread_num();
+      uint32_t* my = cnt.data() + (size_t)t * ncell;
+      for (size_t k = lo_of(t); k < lo_of(t + 1); k++) {
+        const size_t ai = A - 1 - k;
+        for (uint32_t s = 0; s < lengths[ai]; s++) {
+          uint32_t p = paths[ai * 256 + s];
+          if (p) my[cid(p)]++;
+        }
+      }
     }
-  for (size_t c = 0; c < ncell; c++) start[c + 1] += start[c];
-  std::vector<SegRef> refs(start[ncell]);
-  std::vector<uint32_t> fill(start.begin(), start.end() - 1);
-  for (size_t ai = A; ai-- > 0;)
-    for (uint32_t s = 0; s < lengths[ai]; s++) {
-      uint32_t p = paths[ai * 256 + s];
-      if (p) refs[fill[cid(p)]++] = SegRef{(uint32_t)ai, (uint16_t)s, (uint8_t)ptype(p)};
+    // Exclusive offsets: cell-major, thread order within a cell.
+    size_t run = 0;
+    for (size_t c = 0; c < ncell; c++) {
+      start[c] = (uint32_t)run;
+      for (int t = 0; t < nth; t++) {
+        const uint32_t k = cnt[(size_t)t * ncell + c];
+        cnt[(size_t)t * ncell + c] = (uint32_t)run;
+        run += k;
+      }
     }
+    start[ncell] = (uint32_t)run;
+    std::vector<SegRef> all(run);
+#pragma omp parallel num_threads(nth)
+    {
+      const int t = omp_get_thread_num();
+      uint32_t* pos = cnt.data() + (size_t)t * ncell;
+      for (size_t k = lo_of(t); k < lo_of(t + 1); k++) {
+        const size_t ai = A - 1 - k;
+        for (uint32_t s = 0; s < lengths[ai]; s++) {
+          uint32_t p = paths[ai * 256 + s];
+          if (p) all[pos[cid(p)]++] = SegRef{(uint32_t)ai, (uint16_t)s, (uint8_t)ptype(p)};
+        }
+      }
+    }
+    // Keep the first of each run of one axon (and type) within a cell.
+    std::vector<uint32_t> kept(ncell + 1, 0);
+#pragma omp parallel for schedule(static)
+    for (int64_t c = 0; c < (int64_t)ncell; c++) {
+      uint32_t k = 0;
+      for (uint32_t r = start[c]; r < start[c + 1]; r++)
+        k += (r == start[c] || all[r].axon_id != all[r - 1].axon_id || all[r].type_idx != all[r - 1].type_idx);
+      kept[c + 1] = k;
+    }
+    for (size_t c = 0; c < ncell; c++) kept[c + 1] += kept[c];
+    refs.resize(kept[ncell]);
+#pragma omp parallel for schedule(static)
+    for (int64_t c = 0; c < (int64_t)ncell; c++) {
+      uint32_t o = kept[c];
+      for (uint32_t r = start[c]; r < start[c + 1]; r++)
+        if (r == start[c] || all[r].axon_id != all[r - 1].axon_id || all[r].type_idx != all[r - 1].type_idx)
+          refs[o++] = all[r];
+    }
+    start.swap(kept);
+  }
 
-  // 5. Synaptogenesis.
-  std::vector<float> power_cache(N, -1.0f);
-  auto power_of = [&](uint32_t soma) -> float {  // sprouting.rs:32-40
-    float& c = power_cache[soma];
-    if (c >= 0.0f) return c;
+  // 5. Synaptogenesis. Three exact rewrites of the reference loop keep its
+  // results bit for bit:
+  //  - the power index of every soma (sprouting.rs:32-40) is kept in an
+  //    array and recomputed when that soma sprouts (its weights change only
+  //    then; the reference computes it at first use, from the same weights);
+  //  - random_f32 of each axon for this epoch (the explore term) is computed
+  //    once per axon instead of once per candidate segment;
+  //  - a soma's presynaptic axons are marked in an array (mark[a] == i + 1)
+  //    instead of a hash set.
+  // The candidate scan of one empty slot (the 5x5x5 cells in z, y, x order,
+  // each cell's segments in grid order) is split over threads by cells; each
+  // keeps its first best score, and the merge takes the highest score, ties
+  // to the earliest position: the strict `>` of the sequential scan.
+  auto power_index = [&](uint32_t soma) -> float {
     uint32_t pw = 0;
     for (uint32_t s = 0; s < kMaxD; s++) {
       int32_t w = weights[(size_t)s * N + soma];
       pw += (uint32_t)(w < 0 ? -w : w);
     }
-    c = (float)pw / ((float)kMaxD * 32767.0f);
-    return c;
+    return (float)pw / ((float)kMaxD * 32767.0f);
+  };
+  std::vector<float> power(N);
+#pragma omp parallel for schedule(static)
+  for (int64_t s = 0; s < (int64_t)N; s++) power[s] = power_index((uint32_t)s);
+  std::vector<float> noise(types ? A : 0);
+  if (types) {
+#pragma omp parallel for schedule(static)
+    for (int64_t a = 0; a < (int64_t)A; a++) noise[a] = random_f32(master_seed + (uint64_t)a + epoch);
+  }
+  std::vector<uint32_t> mark(A, 0);
+  struct Best {  // a thread's first best candidate
+    float score;
+    SegRef ref;
+    bool found;
   };
   uint32_t new_syn = 0;
-  std::unordered_set<uint32_t> present;
+  std::vector<size_t> cells;
+  cells.reserve(125);
   for (size_t i = 0; i < N; i++) {
     const uint32_t my_pos = soma_positions[i];
     if (my_pos == 0) continue;
     if ((flags[i] & 0x0E) == 0) continue;
     const uint32_t my_type = ptype(my_pos);
     const AxcSproutType* cfg = (types && my_type < n_types) ? &types[my_type] : nullptr;
-    present.clear();
+    const uint32_t stamp = (uint32_t)i + 1;
     for (uint32_t s = 0; s < kMaxD; s++) {
       uint32_t t = targets[(size_t)s * N + i];
-      if (t) present.insert(unpack_axon(t));
+      if (t) {
+        const uint32_t a = unpack_axon(t);
+        if (a < A) mark[a] = stamp;  // axons >= A match no segment of the grid
+      }
     }
-    uint32_t sprouts = 0;
+    const uint32_t my_axon = soma_to_axon[i];
     const int32_t cx = (int32_t)(px(my_pos) / cell), cy = (int32_t)(py(my_pos) / cell),
                   cz = (int32_t)(pz(my_pos) / cell);
+    cells.clear();
+    for (int32_t z = cz - 2; z <= cz + 2; z++) {
+      if (z < 0 || z >= (int32_t)gz) continue;
+      for (int32_t y = cy - 2; y <= cy + 2; y++) {
+        if (y < 0 || y >= (int32_t)gy) continue;
+        for (int32_t x = cx - 2; x <= cx + 2; x++) {
+          if (x < 0 || x >= (int32_t)gx) continue;
+          cells.push_back(((size_t)z * gy + (size_t)y) * gx + (size_t)x);
+        }
+      }
+    }
+    const int ncells = (int)cells.size();
+    uint32_t sprouts = 0;
     for (int32_t slot = (int32_t)kMaxD - 1; slot >= 0; slot--) {
       const size_t col = (size_t)slot * N + i;
       if (targets[col] != 0) break;
-      bool found = false;
-      SegRef best{};
-      float best_score = -1.0f;
-      for (int32_t z = cz - 2; z <= cz + 2; z++) {
-        if (z < 0 || z >= (int32_t)gz) continue;
-        for (int32_t y = cy - 2; y <= cy + 2; y++) {
-          if (y < 0 || y >= (int32_t)gy) continue;
-          for (int32_t x = cx - 2; x <= cx + 2; x++) {
-            if (x < 0 || x >= (int32_t)gx) continue;
-            const size_t c = ((size_t)z * gy + (size_t)y) * gx + (size_t)x;
-            for (uint32_t r = start[c]; r < start[c + 1]; r++) {
-              const SegRef& sr = refs[r];
-              if (soma_to_axon[i] == sr.axon_id) continue;
-              if (present.count(sr.axon_id)) continue;
-              float score = 1.0f;
-              if (cfg) {
-                const float noise = random_f32(master_seed + (uint64_t)sr.axon_id + epoch);
-                const uint32_t owner = axon_to_soma[sr.axon_id];
-                const float power = owner == 0xFFFFFFFFu ? 1.0f : power_of(owner);
-                const float same = (my_type == sr.type_idx) ? 1.0f : 0.0f;
-                // Left to right, products first, no fused multiply-add
-                // (the library is built with -ffp-contract=off).
-                float acc = cfg->w_distance * 1.0f;
-                acc = acc + cfg->w_explore * noise;
-                acc = acc + cfg->w_type * same;
-                acc = acc + cfg->w_power * power;
-                score = acc;
-              }
-              if (score > best_score) {
-                best_score = score;
-                best = sr;
-                found = true;
-              }
+      std::vector<Best> part;
+#pragma omp parallel
+      {
+#pragma omp single
+        part.assign((size_t)omp_get_num_threads(), Best{-1.0f, SegRef{}, false});
+        Best b{-1.0f, SegRef{}, false};
+        // Contiguous ranges of cells per thread, in thread order.
+#pragma omp for schedule(static)
+        for (int k = 0; k < ncells; k++) {
+          const size_t c = cells[(size_t)k];
+          for (uint32_t r = start[c]; r < start[c + 1]; r++) {
+            const SegRef& sr = refs[r];
+            const uint32_t a = sr.axon_id;
+            if (a == my_axon || mark[a] == stamp) continue;
+            float score = 1.0f;
+            if (cfg) {
+              const uint32_t owner = axon_to_soma[a];
+              const float pw = owner == 0xFFFFFFFFu ? 1.0f : power[owner];
+              const float same = (my_type == sr.type_idx) ? 1.0f : 0.0f;
+              // Left to right, products first, no fused multiply-add
+              // (the library is built with -ffp-contract=off).
+              float acc = cfg->w_distance * 1.0f;
+              acc = acc + cfg->w_explore * noise[a];
+              acc = acc + cfg->w_type * same;
+              acc = acc + cfg->w_power * pw;
+              score = acc;
             }
+            if (score > b.score) b = Best{score, sr, true};
           }
         }
+        part[(size_t)omp_get_thread_num()] = b;
       }
-      if (!found) break;
-      targets[col] = ((uint32_t)best.seg_idx << 24) | ((best.axon_id + 1u) & 0x00FFFFFFu);
+      // Threads hold increasing ranges of cells: the first strictly better
+      // one in thread order is the sequential scan's choice.
+      Best best{-1.0f, SegRef{}, false};
+      for (const Best& b : part)
+        if (b.found && b.score > best.score) best = b;
+      if (!best.found) break;
+      targets[col] = ((uint32_t)best.ref.seg_idx << 24) | ((best.ref.axon_id + 1u) & 0x00FFFFFFu);
       bool inhib = false;
       int16_t w0 = 74;
       if (types) {
-        if (best.type_idx < n_types) {
-          inhib = types[best.type_idx].is_inhibitory != 0;
-          w0 = (int16_t)types[best.type_idx].initial_synapse_weight;
+        if (best.ref.type_idx < n_types) {
+          inhib = types[best.ref.type_idx].is_inhibitory != 0;
+          w0 = (int16_t)types[best.ref.type_idx].initial_synapse_weight;
         }
       }
       weights[col] = inhib ? (int16_t)-w0 : w0;
-      present.insert(best.axon_id);
-      active[best.axon_id] = 1;
-      power_cache[i] = -1.0f;  // this soma's weights changed
+      mark[best.ref.axon_id] = stamp;
+      active[best.ref.axon_id] = 1;
+      power[i] = power_index((uint32_t)i);  // this soma's weights changed
       new_syn++;
       sprouts++;
       if (sprouts >= max_sprouts) break;

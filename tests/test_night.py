@@ -142,3 +142,43 @@ def test_sprout_invariants_c1():
             src_type = int(b.paths[a, int(t1[s, i]) >> 24]) >> 28
             want = int(types[src_type]["initial_synapse_weight"])
             assert w1[s, i] == (-want if types[src_type]["is_inhibitory"] else want)
+
+
+_SPROUT_CHILD = r"""
+import sys, hashlib
+import numpy as np
+sys.path.insert(0, sys.argv[1])
+from axicor_amd import brain as B, night as NT
+spec = B.spec_small(6000, 48, 300, seed=17, axon_len=(1, 40))
+b = B.generate(spec, with_paths=True)
+v = b.view()
+rng = np.random.default_rng(17)
+for i in range(spec.n_neurons):
+    k = int(rng.integers(24, 49))
+    v.targets[k:, i] = 0
+    v.weights[k:, i] = 0
+v.flags[:spec.n_neurons] = (v.flags[:spec.n_neurons] & 0xF0) | np.where(rng.random(spec.n_neurons) < 0.3, 0x05, 0)
+t, w = v.targets.copy(), v.weights.copy()
+res = NT.sprout_pass(t, w, v.flags.copy(), v.soma_to_axon.copy(), b.padded_n, NT.geometry_of(b),
+                     NT.sprout_types_from_lut(b.lut), 5, b.spec.seed)
+print(res.new_synapses, hashlib.sha256(t.tobytes() + w.tobytes()).hexdigest())
+"""
+
+
+def test_sprout_same_result_for_any_thread_count():
+    """The segment grid is built and each slot's candidate scan is split over
+    OpenMP threads; the merge keeps the sequential choice, so 1, 3 and 8
+    threads give the same targets and weights (6,000 somas, about 30% of
+    them bursting, with empty tails)."""
+    import os
+    import subprocess
+    import sys
+    repo = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    outs = []
+    for n in (1, 3, 8):
+        env = dict(os.environ, OMP_NUM_THREADS=str(n))
+        r = subprocess.run([sys.executable, "-c", _SPROUT_CHILD, repo], env=env, capture_output=True, text=True,
+                           timeout=300, check=True)
+        outs.append(r.stdout.strip())
+    assert int(outs[0].split()[0]) > 0
+    assert outs[0] == outs[1] == outs[2], outs
