@@ -320,3 +320,37 @@ def test_sort_and_prune_invariants_and_idempotence():
     s2 = state.copy()
     O.sort_and_prune_blob(state, pn, 0)
     np.testing.assert_array_equal(state, s2)
+
+
+# ---- Multi-zone helpers (oracle/zone_oracle.py), hand-derived from
+# bindings.cu:341-418 and ring_buffer.rs:35-54.
+def test_zone_extract_known_answers():
+    from oracle import zone_oracle as Z
+    heads = np.full((3, 8), 0x80000000, np.uint32)
+    heads[0] = [0, 1, 49, 50, 0x6FFFFFFF, 0x70000000, 0x80000000, 7]
+    heads[1] = [2, 0x80000000, 0x80000000, 0x80000000, 0x80000000, 0x80000000, 0x80000000, 0x80000000]
+    # v_seg 1, T 50: heads 0, 1, 49, 7 spiked 0, 1, 49, 7 ticks ago -> offsets 49, 48, 0, 42;
+    # 50 is one tick before the batch; 0x6FFFFFFF is too old; 0x70000000 and the sentinel are dead.
+    ev = Z.extract_outgoing_spikes(heads, [0, 0x80000000, 0xFFFFFFFF, 1], [900, 901, 902, 903], 50, 1)
+    assert ev == [(900, 49), (900, 48), (900, 0), (900, 42), (903, 47)]
+    # v_seg 3: ticks since the spike are head // 3 (0, 0, 16, 16, -, -, -, 2).
+    ev3 = Z.extract_outgoing_spikes(heads, [0], [5], 50, 3)
+    assert ev3 == [(5, 49), (5, 49), (5, 33), (5, 33), (5, 47)]
+    # A batch of one tick keeps only the heads pushed in it.
+    assert Z.extract_outgoing_spikes(heads, [0, 1], [1, 2], 1, 1) == [(1, 0)]
+
+
+def test_zone_schedule_known_answers():
+    from oracle import zone_oracle as Z
+    ids, cnt = Z.schedule_from_events([(5, 0), (6, 0), (7, 49), (8, 50), (9, 0), (10, 3)], 50, max_spikes=2)
+    assert cnt.tolist()[:4] == [2, 0, 0, 1] and cnt[49] == 1 and cnt.sum() == 4
+    assert ids[0, :2].tolist() == [5, 6] and ids[49, 0] == 7 and ids[3, 0] == 10  # (9, 0) over the cap, (8, 50) past T
+
+
+def test_zone_ghost_sync_skips_poisoned_sources():
+    from oracle import zone_oracle as Z
+    src = np.arange(4 * 8, dtype=np.uint32).reshape(4, 8)
+    dst = np.full((5, 8), 7, np.uint32)
+    Z.ghost_sync(src, dst, [2, 0x80000000, 0xFFFFFFFF, 0], [4, 0, 1, 2])
+    assert dst[4].tolist() == list(range(16, 24)) and dst[2].tolist() == list(range(0, 8))
+    assert (dst[[0, 1, 3]] == 7).all()
