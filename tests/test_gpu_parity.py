@@ -214,3 +214,34 @@ def test_night_phase_with_sprouting(gpu):
         p.check(f"after night {night}")
     p.step(50, use_graph=True)
     p.check("day after nights")
+
+
+def test_night_through_the_shm_contract(gpu):
+    """The Night as genesis-node runs it with genesis-baker-daemon: the shard's
+    side copies the flags, the GPU-pruned matrices and the request into the
+    shared region (axicor_amd/shm.py), the daemon's side sprouts on the
+    region, the shard uploads. It ends in the same state as the twin shard
+    whose Night ran through execute_night_phase, and stays equal after more
+    day ticks."""
+    from axicor_amd import night as NT
+    from axicor_amd import shm as S
+    from axicor_amd.layout import StateView
+    b = B.generate(B.spec_c1(), with_paths=True)
+    p, q = Pair(b.copy()), Pair(b.copy())
+    types = NT.sprout_types_from_lut(b.lut)
+    for _ in range(4):
+        p.step(50, use_graph=True)
+        q.step(50, use_graph=True)
+    NT.execute_night_phase(q.vram, 15, NT.geometry_of(b), types, epoch=0, master_seed=b.spec.seed)
+    region = S.ShmRegion.create(0x5151, b.padded_n, b.total_axons)
+    req = S.runtime_night_begin(p.vram, region, 15, max_sprouts=4)
+    reply = S.daemon_night(region, req, NT.geometry_of(b), b.view().soma_to_axon, types, b.spec.seed)
+    acks = S.runtime_night_end(p.vram, region, reply)
+    assert len(acks) == 0 and int(region.header["state"]) == S.ShmState.IDLE
+    sp, sq = p.vram.download_state(), q.vram.download_state()
+    assert np.array_equal(sp, sq)
+    assert not np.array_equal(StateView(sp, b.padded_n).targets, b.view().targets)  # pruned and sprouted
+    p.step(50, use_graph=True)
+    q.step(50, use_graph=True)
+    assert np.array_equal(p.vram.download_state(), q.vram.download_state())
+    assert np.array_equal(p.vram.download_axons(), q.vram.download_axons())
