@@ -245,3 +245,28 @@ def test_night_through_the_shm_contract(gpu):
     q.step(50, use_graph=True)
     assert np.array_equal(p.vram.download_state(), q.vram.download_state())
     assert np.array_equal(p.vram.download_axons(), q.vram.download_axons())
+
+
+def test_hot_checkpoint_resume(gpu, tmp_path):
+    """save_hot_checkpoint (shard_thread.rs:229-263) after 50 ticks: the files
+    hold the oracle's state and heads byte for byte, and a fresh shard
+    resumed from them continues exactly as the uninterrupted one."""
+    from axicor_amd import baked as K
+    b = B.generate(B.spec_c1())
+    p = Pair(b.copy(), n_out=32)
+    p.step(50, use_graph=True, dopamine=60)
+    s_path, a_path = K.save_checkpoint(p.vram, str(tmp_path))
+    assert open(s_path, "rb").read() == p.ref_state.tobytes()
+    assert open(a_path, "rb").read() == p.ref_heads.tobytes()
+    q = Pair(b.copy(), n_out=32)
+    K.load_checkpoint(q.vram, str(tmp_path))
+    q.ref_state[:] = p.ref_state
+    q.ref_heads[:] = p.ref_heads
+    q.tick = p.tick
+    for k in range(2):
+        out_p, _ = p.step(50, use_graph=True, dopamine=60)
+        out_q, _ = q.step(50, use_graph=True, dopamine=60)
+        np.testing.assert_array_equal(p.outputs(50), q.outputs(50))
+        assert np.array_equal(p.vram.download_state(), q.vram.download_state())
+        assert np.array_equal(p.vram.download_axons(), q.vram.download_axons())
+        q.check(f"resumed batch {k}")
