@@ -72,6 +72,36 @@ def cpu_baseline(brain, inputs_host: np.ndarray, seconds: float, max_ticks: int)
     return state, heads, ticks, time.perf_counter() - t0, tot
 
 
+def cpu_single_thread(brain, inputs_host: np.ndarray, ticks: int) -> dict:
+    """The same oracle on one thread (OpenMP's thread count set to 1 for the
+    run, then restored) over the first `ticks` ticks of a fresh copy."""
+    import ctypes as C
+    from oracle import oracle as O
+    O.lib()  # loads the oracle and its OpenMP runtime
+    gomp = C.CDLL("libgomp.so.1")
+    before = gomp.omp_get_max_threads()
+    gomp.omp_set_num_threads(1)
+    try:
+        state, heads = brain.state.copy(), brain.heads.copy()
+        t0 = time.perf_counter()
+        _, st = O.step_batch(state, brain.padded_n, heads, brain.total_axons, brain.lut, brain.spec.v_seg, 0, ticks,
+                             inputs_host[:ticks], brain.virtual_offset, brain.n_inputs)
+        el = time.perf_counter() - t0
+    finally:
+        gomp.omp_set_num_threads(before)
+    return {"value": st["slots_scanned"] / el, "ticks": ticks, "seconds": el}
+
+
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def bench_batched(n_brains: int, ticks: int, input_ticks: int, batch_ticks: int, seed: int,
                   per_tick_ticks: int = 200, serial: bool = True) -> dict:
     """Batched brains (128 brains of 8k-250k neurons, slots in {32,64,128},
@@ -192,6 +222,7 @@ def main() -> int:
                          "c5: the 12M-neuron x 128-slot scale point; c5-sub: its 200k-neuron sub-shard")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--cpu-max-ticks", type=int, default=20)
+    ap.add_argument("--cpu-single-ticks", type=int, default=3, help="ticks of the one-thread CPU oracle run")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--timing-ticks", type=int, default=10,
                     help="ticks per batch of the two batches whose kernels are timed one by one with HIP events")
@@ -247,6 +278,9 @@ def main() -> int:
         cpu = {"state": c_state, "heads": c_heads, "ticks": c_ticks, "elapsed": c_el, "stats": c_st,
                "threads": threads}
         log(f"cpu oracle: {c_ticks} ticks in {c_el:.2f}s, {c_st['slots_scanned'] / c_el:.3e} events/s")
+        cpu["single"] = cpu_single_thread(brain, inputs, max(1, min(args.cpu_single_ticks, n_in_ticks)))
+        log(f"cpu oracle, one thread: {cpu['single']['ticks']} ticks in {cpu['single']['seconds']:.2f}s, "
+            f"{cpu['single']['value']:.3e} events/s")
 
     # ---- GPU ----------------------------------------------------------------
     upload_constant_memory(brain.lut)
@@ -411,6 +445,10 @@ def main() -> int:
             "cores": cpu["threads"], "kind": "port",
             "sample": f"first {cpu['ticks']} ticks of the same workload, CPU oracle (C, OpenMP over somas)",
             "ticks_per_s": cpu["ticks"] / cpu["elapsed"],
+            "single_thread": {"value": cpu["single"]["value"], "unit": "synaptic events/s", "cores": 1,
+                              "sample": f"first {cpu['single']['ticks']} ticks of the same workload, one thread"},
+            "cpu_model": cpu_model(),
+            "nproc": os.cpu_count(),
         }
     print(json.dumps(out), flush=True)
     return 0
