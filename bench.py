@@ -92,6 +92,52 @@ def cpu_single_thread(brain, inputs_host: np.ndarray, ticks: int) -> dict:
     return {"value": st["slots_scanned"] / el, "ticks": ticks, "seconds": el}
 
 
+def bench_nights(lib, brain, vram, eng, batch, nights: int, prune: int) -> dict:
+    """Nights of shard_thread.rs:269-378 on the workload brain, each after one
+    more Day batch: flags download, GPU sort & prune, weights and targets
+    download, host sprouting (csrc/night_host.cpp), upload. Wall-clock per
+    phase, averaged."""
+    import ctypes as C
+    from axicor_amd import night as NT
+    from axicor_amd.layout import MAX_DENDRITES
+    from axicor_amd._ffi import ptr
+    N = brain.padded_n
+    types = NT.sprout_types_from_lut(brain.lut)
+    geo = NT.geometry_of(brain)
+    ph = {"prune_s": 0.0, "download_s": 0.0, "sprout_s": 0.0, "upload_s": 0.0}
+    new_syn = 0
+    flags = np.empty(N, np.uint8)
+    w = np.empty((MAX_DENDRITES, N), np.int16)
+    t = np.empty((MAX_DENDRITES, N), np.uint32)
+    for k in range(nights):
+        batch(50)
+        eng.synchronize()
+        t0 = time.perf_counter()
+        lib.gpu_memcpy_device_to_host(ptr(flags), vram.ptrs.soma_flags, N)
+        lib.launch_sort_and_prune(C.byref(vram.ptrs), N, prune)
+        lib.gpu_device_synchronize()
+        t1 = time.perf_counter()
+        lib.gpu_memcpy_device_to_host(ptr(w), vram.ptrs.dendrite_weights, w.nbytes)
+        lib.gpu_memcpy_device_to_host(ptr(t), vram.ptrs.dendrite_targets, t.nbytes)
+        t2 = time.perf_counter()
+        res = NT.sprout_pass(t, w, flags, np.ascontiguousarray(brain.view().soma_to_axon), N, geo, types, k,
+                             brain.spec.seed)
+        t3 = time.perf_counter()
+        lib.gpu_memcpy_host_to_device(vram.ptrs.dendrite_targets, ptr(t), t.nbytes)
+        lib.gpu_memcpy_host_to_device(vram.ptrs.dendrite_weights, ptr(w), w.nbytes)
+        lib.gpu_device_synchronize()
+        t4 = time.perf_counter()
+        for key, dt in (("prune_s", t1 - t0), ("download_s", t2 - t1), ("sprout_s", t3 - t2), ("upload_s", t4 - t3)):
+            ph[key] += dt
+        new_syn += res.new_synapses
+        log(f"night {k}: prune {t1 - t0:.3f}s, download {t2 - t1:.2f}s, sprout {t3 - t2:.2f}s "
+            f"({res.new_synapses} new synapses), upload {t4 - t3:.2f}s")
+    out = {key: v / nights for key, v in ph.items()}
+    out.update({"nights": nights, "new_synapses_per_night": new_syn / nights,
+                "sprout_threads": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))})
+    return out
+
+
 def cpu_model() -> str:
     try:
         for line in open("/proc/cpuinfo"):
@@ -223,6 +269,10 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--cpu-max-ticks", type=int, default=20)
     ap.add_argument("--cpu-single-ticks", type=int, default=3, help="ticks of the one-thread CPU oracle run")
+    ap.add_argument("--nights", type=int, default=0,
+                    help="after the timed steps, this many Nights (GPU sort & prune, host sprouting), "
+                         "each after one more batch of --ticks ticks, timed phase by phase")
+    ap.add_argument("--prune", type=int, default=15, help="prune threshold of the Nights")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--timing-ticks", type=int, default=10,
                     help="ticks per batch of the two batches whose kernels are timed one by one with HIP events")
@@ -255,7 +305,7 @@ def main() -> int:
     spec = {"c2": B.spec_c2, "c1": B.spec_c1, "c2-linear": lambda: B.spec_c2(brick=0), "c5": B.spec_c5,
             "c5-sub": lambda: B.spec_c5(200_000)}[args.workload]()
     t0 = time.perf_counter()
-    brain = B.generate(spec)
+    brain = B.generate(spec, with_paths=args.nights > 0)
     N, A = brain.padded_n, brain.total_axons
     log(f"generated {args.workload}: padded_n={N} axons={A} state={brain.state.nbytes / 1e9:.2f} GB "
         f"in {time.perf_counter() - t0:.1f}s")
@@ -435,6 +485,13 @@ def main() -> int:
         "per_kernel": per_kernel,
         "parity": parity,
     }
+    if args.nights:
+        def night_batch(ticks: int):
+            nonlocal tick
+            if tick + ticks > n_in_ticks:
+                tick = 0  # reuse resident inputs from the start; the state just keeps running
+            batch(ticks)
+        out["nights"] = bench_nights(lib, brain, vram, eng, night_batch, args.nights, args.prune)
     if args.batched_brains:
         del vram, eng, d_in
         out["batched"] = bench_batched(args.batched_brains, args.batched_ticks, 1000, 50, 20261203,
