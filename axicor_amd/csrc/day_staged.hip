@@ -15,14 +15,16 @@
 //                       chunks 8g..8g+7, 8(g+4).. of the block's somas (the
 //                       slots before the first empty target, as the reference
 //                       loop's `break`), and group 0 finishes each soma with
-//                       the GLIF update and the spike compaction;
-// Each tick's GSOP (K6) runs after the pushes in gsop_kernel, one wave per
-// spiking soma. Applying it inside the next tick's staged pass (as the
-// direct pass does) measured slower even from the staged byte entries (this
-// tick's heads at segment + v_seg, no dependent gathers): at 1% firing most
-// waves hold one or two somas that spiked, whose slot pass runs with the
-// other lanes idle (a pass before the scan: 575 us against 306 + 136 us for
-// scan + GSOP at 1.5M neurons; inside the scan loop the registers spill).
+//                       the GLIF update and the spike compaction.
+// Each tick's GSOP (K6) runs after the pushes: gsop_chunks_kernel on the
+// batch graphs' working copy of the weights (16 lanes per spiking soma, one
+// chunk of 8 slots each), gsop_kernel on the reference rows for single
+// ticks. Applying it inside the next tick's staged pass (as the direct pass
+// does) measured slower even from the staged byte entries (this tick's heads
+// at segment + v_seg, no dependent gathers): at 1% firing most waves hold one
+// or two somas that spiked, whose slot pass runs with the other lanes idle (a
+// pass before the scan: 575 us against 306 + 136 us for scan + GSOP at 1.5M
+// neurons; inside the scan loop the registers spill).
 // The snapshot entries are the byte form of the heads (min(h, 255) x 8, see
 // byte_heads); a slot whose segment + propagation length exceeds 255 reads the
 // u32 heads from global memory instead, so every hit is exact.
