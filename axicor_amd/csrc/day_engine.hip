@@ -77,13 +77,14 @@ struct GraphKey {
   uint32_t T, words, virt_off, n_virt, max_spk, n_out;
   bool has_bitmask, has_spikes, has_out;
   const void* mapped;
+  const void* heads;  // the captured kernels hold the arrays' addresses
   uint32_t v_seg;
   bool staged;
   std::vector<uint32_t> spike_counts;
   bool operator==(const GraphKey& o) const {
     return T == o.T && words == o.words && virt_off == o.virt_off && n_virt == o.n_virt && staged == o.staged &&
            max_spk == o.max_spk && n_out == o.n_out && has_bitmask == o.has_bitmask &&
-           has_spikes == o.has_spikes && has_out == o.has_out && mapped == o.mapped && v_seg == o.v_seg &&
+           has_spikes == o.has_spikes && has_out == o.has_out && mapped == o.mapped && heads == o.heads && v_seg == o.v_seg &&
            spike_counts == o.spike_counts;
   }
 };
@@ -294,7 +295,12 @@ Shard* shard_for(const ShardVramPtrs* v, uint32_t N, uint32_t A, hipError_t* err
     free_scratch(s);
     s->N = N;
     s->A = A;
-    if ((*err = alloc_scratch(s)) != hipSuccess) return nullptr;
+    if ((*err = alloc_scratch(s)) != hipSuccess) {  // unregister rather than keep a half-built workspace
+      free_scratch(s);
+      g_shards.erase(it);
+      delete s;
+      return nullptr;
+    }
     return s;
   }
   Shard* s = new Shard();
@@ -427,10 +433,14 @@ int32_t cu_allocate_shard(uint32_t padded_n, uint32_t total_axons, ShardVramPtrs
     return (int32_t)e;
   }
   if (total_axons) init_sentinels_kernel<<<blocks_for(total_axons), kThreads>>>(out->axon_heads, total_axons);
-  if ((e = hipDeviceSynchronize()) != hipSuccess) return (int32_t)e;
-  hipError_t se;
-  Shard* s = shard_for(out, padded_n, total_axons, &se);
-  if (!s) return (int32_t)se;
+  e = hipDeviceSynchronize();
+  Shard* s = e == hipSuccess ? shard_for(out, padded_n, total_axons, &e) : nullptr;
+  if (!s) {  // nothing is left allocated on failure
+    (void)hipFree(out->axon_heads);
+    (void)hipFree(base);
+    *out = ShardVramPtrs{};
+    return (int32_t)e;
+  }
   s->owned_by_library = true;
   return 0;
 }
@@ -670,11 +680,17 @@ int32_t cu_allocate_io_buffers(uint32_t input_words, uint32_t schedule_capacity,
   *d_input_bitmask = nullptr;
   *d_incoming_spikes = nullptr;
   *d_output_history = nullptr;
-  if (input_words > 0 && hipMalloc((void**)d_input_bitmask, (size_t)input_words * 4) != hipSuccess) return -1;
-  if (schedule_capacity > 0 && hipMalloc((void**)d_incoming_spikes, (size_t)schedule_capacity * 4) != hipSuccess)
-    return -1;
-  if (output_capacity > 0 && hipMalloc((void**)d_output_history, (size_t)output_capacity) != hipSuccess) return -1;
-  return 0;
+  bool ok = input_words == 0 || hipMalloc((void**)d_input_bitmask, (size_t)input_words * 4) == hipSuccess;
+  ok = ok && (schedule_capacity == 0 ||
+              hipMalloc((void**)d_incoming_spikes, (size_t)schedule_capacity * 4) == hipSuccess);
+  ok = ok && (output_capacity == 0 || hipMalloc((void**)d_output_history, (size_t)output_capacity) == hipSuccess);
+  if (ok) return 0;
+  // On failure nothing stays allocated and every output is null.
+  for (void** q : {(void**)d_input_bitmask, (void**)d_incoming_spikes, (void**)d_output_history}) {
+    if (*q) (void)hipFree(*q);
+    *q = nullptr;
+  }
+  return -1;
 }
 
 int32_t cu_dma_h2d_io(uint32_t* d_in, const uint32_t* h_in, uint32_t words, uint32_t* d_spk, const uint32_t* h_spk,
@@ -858,6 +874,7 @@ int32_t axc_step_day_phase_batch(const ShardVramPtrs* vram, uint32_t padded_n, u
   key.has_spikes = d_spikes != nullptr;
   key.has_out = d_out != nullptr && d_mapped != nullptr && n_out > 0;
   key.mapped = d_mapped;
+  key.heads = vram->axon_heads;
   key.v_seg = v_seg;
   key.staged = ensure_stage(s, *vram, st, &e);
   if (e != hipSuccess) return (int32_t)e;
