@@ -111,11 +111,17 @@ def sprout_pass(targets, weights, flags, soma_to_axon, padded_n, lengths, paths,
             x, y, z, t = pos_xyzt(p)
             cells.setdefault(hash_cell(x // 2, y // 2, z // 2), []).append((a, si, t))
 
+    # compute_power_index (sprouting.rs:32-40) of a soma, memoised: only the
+    # soma being filled changes its weights, and its entry is dropped then.
+    power_memo = {}
+
     def power_index(soma):
-        pw = 0
-        for sl in range(MAXD):
-            pw += abs(int(weights[sl, soma]))
-        return F32(pw) / (F32(MAXD) * F32(32767.0))
+        if soma not in power_memo:
+            pw = 0
+            for sl in range(MAXD):
+                pw += abs(int(weights[sl, soma]))
+            power_memo[soma] = F32(pw) / (F32(MAXD) * F32(32767.0))
+        return power_memo[soma]
 
     new = 0
     for i in range(N):
@@ -127,6 +133,9 @@ def sprout_pass(targets, weights, flags, soma_to_axon, padded_n, lengths, paths,
         mx, my, mz, mt = pos_xyzt(mp)
         cfg = types[mt] if (types is not None and mt < len(types)) else None
         sprouts = 0
+        # The axons already on the soma (the reference's scan of all 128
+        # slots per candidate), kept as a set that grows with each sprout.
+        present = {unpack_axon_id(int(targets[e, i])) for e in range(MAXD) if int(targets[e, i]) != 0}
         for slot in range(MAXD - 1, -1, -1):
             if targets[slot, i] != 0:
                 break
@@ -144,8 +153,7 @@ def sprout_pass(targets, weights, flags, soma_to_axon, padded_n, lengths, paths,
                         for (a, si, t) in cells.get(hash_cell(x, y, z), []):
                             if int(soma_to_axon[i]) == a:
                                 continue
-                            if any(int(targets[e, i]) != 0 and unpack_axon_id(int(targets[e, i])) == a
-                                   for e in range(MAXD)):
+                            if a in present:
                                 continue
                             same = F32(1.0) if mt == t else F32(0.0)
                             noise = random_f32(master_seed + a + epoch)
@@ -168,6 +176,8 @@ def sprout_pass(targets, weights, flags, soma_to_axon, padded_n, lengths, paths,
             else:
                 inh, w0 = False, 74
             weights[slot, i] = -w0 if inh else w0
+            present.add(a)
+            power_memo.pop(i, None)
             active[a] = True
             new += 1
             sprouts += 1
