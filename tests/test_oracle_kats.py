@@ -240,6 +240,41 @@ def test_snapshot_rule_same_tick_push_invisible_to_integration():
     assert v.voltage[1] == 10000 + 5000 - 1500   # head at 1: (1-0) < 20
 
 
+@pytest.mark.parametrize("dop", [0, 100])
+def test_gsop_tick_after_a_spike(dop):
+    """Soma 0 fires at tick 0 (v = 40000) and its GSOP runs on the heads
+    after propagation and after its own push (physics.cu:281-373, template
+    variant: pot 150, dep 60, length 20, curve 128, 120, ..., d1 = d2 = 128;
+    a spiker's burst count is 2). Slots, derived by hand:
+      0: extra axon 0, seg 3, w 100, head 4 -> 5: d = 2, rank 0, cs 0
+         dop 0: dp = (150*128*2)>>7 = 300 -> 400; dop 100: pot 250, dp 500 -> 600
+      1: extra axon 1, seg 0, w -500, no head: depression
+         dop 0: dd = (60*128*2)>>7 = 120 -> -380; dop 100: dep max(0, 60-100) = 0 -> -500
+      2: extra axon 2, seg 10, w 3000, head 26 -> 27: d = 17, rank 1 (in 120), cs 1
+         dop 0: dp = (150*120*2)>>7 = 281, 281>>1 = 140, (140*128)>>8 = 70 -> 3070
+         dop 100: dp = (250*120*2)>>7 = 468, 234, (234*128)>>8 = 117 -> 3117
+      3: extra axon 3, seg 5, w 0, head 4 -> 5: d = 0; sign(0) = +1 -> 300 (dop 100: 500)
+      4: its own axon, seg 0, w 100, no head before the tick: the spike's push
+         (h = 0) comes after integration but before GSOP: d = 0 -> 400 (600)"""
+    lut = lut1()
+    state, v, heads, pn, A = tiny(1, lut, a_extra=8)
+    v.voltage[0] = 40000
+    v.soma_to_axon[0] = 0
+    x = [pn + k for k in range(4)]
+    for slot, (ax, seg, w) in enumerate([(x[0], 3, 100), (x[1], 0, -500), (x[2], 10, 3000), (x[3], 5, 0),
+                                         (0, 0, 100)]):
+        v.targets[slot, 0] = (seg << 24) | (ax + 1)
+        v.weights[slot, 0] = w
+    heads[x[0] * 8] = 4
+    heads[x[2] * 8] = 26
+    heads[x[3] * 8] = 4
+    run(state, pn, heads, A, lut, tick0=0, dop=dop)
+    assert v.flags[0] == (2 << 1) | 1 and v.voltage[0] == 10000
+    want = [400, -380, 3070, 300, 400] if dop == 0 else [600, -500, 3117, 500, 600]
+    assert [int(v.weights[s, 0]) for s in range(5)] == want
+    assert heads[0] == 0  # the push
+
+
 def test_inputs_push_virtual_axons():
     """Bit i of the mask pushes axon virt_off+i before propagation."""
     lut = lut1()
