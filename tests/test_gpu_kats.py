@@ -1,6 +1,6 @@
 """The hand-traced tick cases of tests/test_oracle_kats.py run on the engine
 instead of the oracle: the same tiny brains and the same hand-derived
-expectations (leak toward rest, a GLIF spike with its reset, refractory
+expectations (leak toward rest, the adaptive leak, a GLIF spike with its reset, refractory
 countdown and own-axon push, heartbeat spikes without a reset, the same-tick
 snapshot rule, input pushes, propagation and the sentinel, a tick's GSOP
 after a spike with and without dopamine), each case
@@ -14,7 +14,7 @@ from axicor_amd.engine import IoDeviceBuffers, ShardEngine, VramState, upload_co
 pytestmark = pytest.mark.gpu
 
 CASES = {f.__name__[len("test_"):]: f for f in (
-    K.test_leak_toward_rest_closed_form, K.test_glif_spike_reset_refractory_and_push,
+    K.test_leak_toward_rest_closed_form, K.test_adaptive_leak_mode_1, K.test_glif_spike_reset_refractory_and_push,
     K.test_heartbeat_spikes_without_reset, K.test_snapshot_rule_same_tick_push_invisible_to_integration,
     K.test_inputs_push_virtual_axons, K.test_propagation_reference_vectors)}
 CASES["gsop_tick_after_a_spike_dop0"] = lambda: K.test_gsop_tick_after_a_spike(0)

@@ -185,6 +185,27 @@ def test_leak_toward_rest_closed_form():
         assert v.voltage[1] == 10000 - max(0, 3200 - 1500 * k)
 
 
+def test_adaptive_leak_mode_1():
+    """physics.cu:209-275 with adaptive_mode 1: thr = max(0, thr - decay),
+    leak = min(leak + (thr*gain)>>8, leak_max), the product wrapping in i32.
+    Template: rest 10000, leak 1500, decay 20. Derived by hand:
+      soma 0 (gain 64): thr 5020 -> 5000, leak 1500 + 1250 = 2750, v 16000 -> 13250
+      soma 1 (gain 64): thr 20020 -> 20000, leak 1500 + 5000 -> max 4000, v 5000 -> 9000
+      soma 2 (gain 4): thr 2^30 + 20 -> 2^30, 2^30 * 4 wraps to 0, leak 1500, v 12000 -> 10500"""
+    mk = lambda gain: B.template_variant(False, spontaneous_firing_period_ticks=0, adaptive_mode=1,  # noqa: E731
+                                         adaptive_leak_gain=gain, adaptive_leak_max=4000)
+    lut = B.lut_from([mk(64), mk(4)])
+    state, v, heads, pn, A = tiny(3, lut)
+    v.soma_to_axon[:] = 0xFFFFFFFF
+    v.flags[2] = 1 << 4
+    v.voltage[:3] = [16000, 5000, 12000]
+    v.threshold_offset[:3] = [5020, 20020, (1 << 30) + 20]
+    run(state, pn, heads, A, lut, tick0=3)
+    assert list(v.threshold_offset[:3]) == [5000, 20000, 1 << 30]
+    assert list(v.voltage[:3]) == [13250, 9000, 10500]
+    assert not (v.flags[:3] & 1).any()
+
+
 def test_glif_spike_reset_refractory_and_push():
     """A soma over threshold fires: v <- rest, thr += penalty, timer <-
     refractory, flags = type<<4 | burst 2 | spike, own axon pushed (h0=0)
