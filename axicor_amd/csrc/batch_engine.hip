@@ -213,6 +213,37 @@ __global__ __launch_bounds__(256) void batch_push_flat_kernel(const BrainDev* __
   }
 }
 
+// compact_spikes_kernel for every brain of the batch: four lanes per staged
+// work item (its 256 somas' four mask words), one returning atomic per item on
+// its brain's spike counter.
+__global__ __launch_bounds__(256) void batch_compact_spikes_kernel(const BrainDev* __restrict__ brains,
+                                                                   const WorkItem* __restrict__ items,
+                                                                   uint32_t n_items) {
+  const uint32_t q = blockIdx.x * 256 + threadIdx.x, l = q & 3u;
+  const uint32_t it = q >> 2;
+  uint64_t m = 0;
+  uint32_t word = 0, bi = 0;
+  if (it < n_items) {
+    const WorkItem w = items[it];
+    bi = w.brain;
+    word = w.start / 64u + l;
+    if (w.start + l * 64u < brains[bi].N) m = brains[bi].sc.spk_mask[word];
+  }
+  const uint32_t c = (uint32_t)__popcll(m);
+  uint32_t x = c;  // inclusive prefix over the item's four lanes
+  uint32_t y = __shfl_up(x, 1, 4);
+  if (l >= 1) x += y;
+  y = __shfl_up(x, 2, 4);
+  if (l >= 2) x += y;
+  const uint32_t total = __shfl(x, 3, 4);
+  uint32_t base = 0;
+  if (l == 3 && total) base = atomicAdd(&brains[bi].sc.counters[kCntSpikes], total);
+  if (total == 0) return;  // the item's four lanes alike
+  uint32_t pos = __shfl(base, 3, 4) + x - c;
+  uint32_t* list = brains[bi].sc.spk_list;
+  for (uint64_t r = m; r; r &= r - 1) list[pos++] = word * 64u + (uint32_t)__builtin_ctzll(r);
+}
+
 // GSOP of this tick for every brain's spikers over the flattened spike lists
 // (prefix from batch_spike_prefix_kernel), kGsopLanes lanes per spiker, one
 // chunk of 8 slots each, on the brains' working copies of the weights.

@@ -72,6 +72,7 @@ struct TickParams {
 // Engine-private per-shard device buffers (no counterpart in the reference).
 struct ShardScratch {
   uint32_t* spk_list;    // [N] somas that spiked this tick
+  uint64_t* spk_mask;    // [ceil(N/256)*4] the staged pass's spikes, bit l of word w = soma 64w + l
   uint32_t* own_list;    // [A] axons receiving pushes this tick (first owner)
   uint32_t* push_cnt;    // [A] pending pushes per axon, zero between uses
   uint32_t* counters;    // [kNumCounters]

@@ -156,6 +156,7 @@ void free_scratch(Shard* s) {
   s->graph = nullptr;
   s->has_graph = false;
   (void)hipFree(s->sc.spk_list);
+  (void)hipFree(s->sc.spk_mask);
   (void)hipFree(s->sc.own_list);
   (void)hipFree(s->sc.push_cnt);
   (void)hipFree(s->sc.counters);
@@ -260,6 +261,8 @@ hipError_t alloc_scratch(Shard* s) {
   s->stats_blocks = std::max(neuron_blocks((uint32_t)N), staged_grid((uint32_t)N));
   hipError_t e;
   if ((e = hipMalloc((void**)&s->sc.spk_list, N * 4)) != hipSuccess) return e;
+  if ((e = hipMalloc((void**)&s->sc.spk_mask, (size_t)staged_grid((uint32_t)N) * (kBlockSomas / 64) * 8)) != hipSuccess)
+    return e;
   if ((e = hipMalloc((void**)&s->sc.own_list, A * 4)) != hipSuccess) return e;
   if ((e = hipMalloc((void**)&s->sc.push_cnt, A * 4)) != hipSuccess) return e;
   if ((e = hipMalloc((void**)&s->sc.counters, kNumCounters * 4)) != hipSuccess) return e;
@@ -374,11 +377,15 @@ hipError_t enqueue_tick(const ShardVramPtrs& v, Shard* s, uint32_t v_seg, uint32
   }
   {
     TimedScope ts(kKPush, st, timed);
-    if (staged)
+    if (staged) {
+      // The staged pass leaves its spikes as mask words: the spike list first.
+      const uint32_t n_words = (N + 63) / 64;
+      if (N > 0) compact_spikes_kernel<<<blocks_for(n_words), kThreads, 0, st>>>(n_words, s->sc);
       push_spikers_kernel<<<kListBlocks / 4, kThreads, 0, st>>>(v.axon_heads, v.soma_to_axon, A, s->sc, nullptr,
                                                                 s->sp.comp8);
-    else
+    } else {
       push_apply_kernel<<<kListBlocks / 4, kThreads, 0, st>>>(v.axon_heads, s->sc, cur, nullptr);
+    }
   }
   if (gsop_now) {
     TimedScope ts(kKGsop, st, timed);
@@ -1112,6 +1119,8 @@ hipError_t batch_capture(Batch* b, uint32_t T) {
                                                                      b->d_tick, t, t);
         batch_staged_kernel<<<(b->n_stage_items + kNumXcd - 1) / kNumXcd * kNumXcd, kStageThreads, lds, cap>>>(
             b->d_brains, b->d_stage_items, b->n_stage_items, b->d_tick, t, t);
+        batch_compact_spikes_kernel<<<blocks_for((uint64_t)b->n_stage_items * 4), kThreads, 0, cap>>>(
+            b->d_brains, b->d_stage_items, b->n_stage_items);
         batch_spike_prefix_kernel<<<1, 1024, 0, cap>>>(b->d_brains, n, b->d_prefix, kCntSpikes);
         batch_push_flat_kernel<<<b->tune.list_blocks, kThreads, 0, cap>>>(b->d_brains, n, b->d_prefix, t);
         batch_gsop_chunks_kernel<<<b->tune.list_blocks * 8, kThreads, 0, cap>>>(b->d_brains, n, b->d_prefix, t);

@@ -15,7 +15,8 @@
 //                       chunks 8g..8g+7, 8(g+4).. of the block's somas (the
 //                       slots before the first empty target, as the reference
 //                       loop's `break`), and group 0 finishes each soma with
-//                       the GLIF update and the spike compaction.
+//                       the GLIF update and a spike mask word per wave
+//                       (compact_spikes_kernel then builds the spike list).
 // Each tick's GSOP (K6) runs after the pushes: gsop_chunks_kernel on the
 // batch graphs' working copy of the weights (16 lanes per spiking soma, one
 // chunk of 8 slots each), gsop_kernel on the reference rows for single
@@ -394,8 +395,8 @@ struct BlockLds {
 
 // The end of a block's tick, after the scan: the groups' hit sums are
 // added and group 0 finishes its somas (GLIF update, flags), each of its
-// waves then compacting its own spikers into the shard's list (one atomic
-// per wave with spikers); the block's counters go to part[] with atomics
+// waves then storing its spikes as one mask word (sc.spk_mask, turned into
+// the spike list by compact_spikes_kernel); the block's counters go to part[] with atomics
 // whose results nobody waits for. The other groups leave after the barrier.
 // type_spikes (may be null) counts spikes per variant type; count_gsop adds
 // the slot and target counts of the tick's GSOP (for callers whose GSOP
@@ -442,17 +443,14 @@ __device__ __forceinline__ void finish_block(BlockLds& L, const ShardVramPtrs& v
   // the spike list (no owner list: nothing waits for this atomic).
   if (push_own) atomicAdd(&sc.push_cnt[my_axon], 1u);
 
-  // This wave's spikers, appended to the list.
+  // This wave's spikers, as one mask word (zero included, so that no bit of
+  // an earlier tick stays); compact_spikes_kernel turns the words into the
+  // spike list. (Appending here with a returning atomic kept the block's LDS
+  // held for the atomic's round trip: 13.7 us of a 250 us pass.)
   const uint32_t lane = tid & 63u;
-  const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   const uint64_t bs = __ballot(fin);
+  if (lane == 0) sc.spk_mask[i >> 6] = bs;
   const uint32_t cs = (uint32_t)__popcll(bs);
-  if (cs) {
-    uint32_t base = 0;
-    if (lane == 0) base = atomicAdd(&sc.counters[kCntSpikes], cs);
-    base = __shfl(base, 0, 64);
-    if (fin) sc.spk_list[base + (uint32_t)__popcll(bs & below)] = i;
-  }
   // This wave's counters; the last of group 0's waves to get here adds the
   // block's to part[] (one set of atomics per block, no barrier).
   uint32_t r[7] = {n_int, n_scan, n_tread, n_hits, n_gslots, n_gtgt, cs};
@@ -664,6 +662,27 @@ __global__ __launch_bounds__(256) void gsop_chunks_kernel(ShardVramPtrs vram, ui
     part[kStGsopSlots] += red[0][0] + red[1][0] + red[2][0] + red[3][0];
     part[kStGsopTargets] += red[0][1] + red[1][1] + red[2][1] + red[3][1];
   }
+}
+
+// The tick's spike list from the staged pass's mask words (bit l of word w:
+// soma 64w + l): per wave of 64 words one returning atomic reserves the
+// wave's range, and each lane writes its word's somas in ascending order.
+__global__ __launch_bounds__(256) void compact_spikes_kernel(uint32_t n_words, ShardScratch sc) {
+  const uint32_t w = blockIdx.x * 256 + threadIdx.x, lane = threadIdx.x & 63u;
+  const uint64_t m = w < n_words ? sc.spk_mask[w] : 0ull;
+  const uint32_t c = (uint32_t)__popcll(m);
+  uint32_t x = c;  // inclusive prefix over the wave
+#pragma unroll
+  for (uint32_t off = 1; off < 64; off <<= 1) {
+    const uint32_t y = __shfl_up(x, off, 64);
+    if (lane >= off) x += y;
+  }
+  const uint32_t total = __shfl(x, 63, 64);
+  if (total == 0) return;  // wave-uniform
+  uint32_t base = 0;
+  if (lane == 63) base = atomicAdd(&sc.counters[kCntSpikes], total);
+  uint32_t pos = __shfl(base, 63, 64) + x - c;
+  for (uint64_t r = m; r; r &= r - 1) sc.spk_list[pos++] = w * 64u + (uint32_t)__builtin_ctzll(r);
 }
 
 }  // namespace axc
