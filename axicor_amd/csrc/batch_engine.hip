@@ -214,8 +214,8 @@ __global__ __launch_bounds__(256) void batch_push_flat_kernel(const BrainDev* __
 }
 
 // compact_spikes_kernel for every brain of the batch: four lanes per staged
-// work item (its 256 somas' four mask words), one returning atomic per item on
-// its brain's spike counter.
+// work item (its 256 somas' four mask words), one returning atomic per wave
+// (or per item where a wave's items span brains) on the brain's spike counter.
 __global__ __launch_bounds__(256) void batch_compact_spikes_kernel(const BrainDev* __restrict__ brains,
                                                                    const WorkItem* __restrict__ items,
                                                                    uint32_t n_items) {
@@ -230,16 +230,24 @@ __global__ __launch_bounds__(256) void batch_compact_spikes_kernel(const BrainDe
     if (w.start + l * 64u < brains[bi].N) m = brains[bi].sc.spk_mask[word];
   }
   const uint32_t c = (uint32_t)__popcll(m);
-  uint32_t x = c;  // inclusive prefix over the item's four lanes
-  uint32_t y = __shfl_up(x, 1, 4);
-  if (l >= 1) x += y;
-  y = __shfl_up(x, 2, 4);
-  if (l >= 2) x += y;
-  const uint32_t total = __shfl(x, 3, 4);
+  // The items of a brain are consecutive, so a wave's 16 items are mostly of
+  // one brain: then one atomic reserves the whole wave's range (per item,
+  // the atomics queue on the brain's one counter).
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t b0 = __shfl(bi, 0, 64);
+  const bool one_brain = __all(it >= n_items || bi == b0);
+  const uint32_t width = one_brain ? 64u : 4u, seg = lane & (width - 1);
+  uint32_t x = c;  // inclusive prefix over the wave, or over the item's four lanes
+  for (uint32_t off = 1; off < width; off <<= 1) {
+    const uint32_t y = __shfl_up(x, off, (int)width);
+    if (seg >= off) x += y;
+  }
+  const uint32_t total = __shfl(x, (int)(width - 1), (int)width);
   uint32_t base = 0;
-  if (l == 3 && total) base = atomicAdd(&brains[bi].sc.counters[kCntSpikes], total);
-  if (total == 0) return;  // the item's four lanes alike
-  uint32_t pos = __shfl(base, 3, 4) + x - c;
+  // (lane 63 may lie past the last item: the wave's brain is lane 0's)
+  if (seg == width - 1 && total) base = atomicAdd(&brains[one_brain ? b0 : bi].sc.counters[kCntSpikes], total);
+  if (total == 0) return;  // alike over each prefix segment
+  uint32_t pos = __shfl(base, (int)(width - 1), (int)width) + x - c;
   uint32_t* list = brains[bi].sc.spk_list;
   for (uint64_t r = m; r; r &= r - 1) list[pos++] = word * 64u + (uint32_t)__builtin_ctzll(r);
 }
