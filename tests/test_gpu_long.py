@@ -1,0 +1,53 @@
+"""Long runs on the GPU against the CPU oracle, skipped unless AXC_LONG=1
+(minutes of oracle time each, so not part of the default suite):
+
+- the full 1.5M-neuron C2 brain for 2,000 ticks through the batch graph with
+  a Night every 500 ticks as shard_thread.rs:269-378 runs it (the flags
+  taken before the GPU sort & prune at 15, the host sprouting, the upload):
+  every soma's spike bit every tick, and the whole state and the heads
+  before and after every Night and at the end, bit for bit.
+
+Run: AXC_LONG=1 python -u -m pytest tests/test_gpu_long.py -s --timeout 1100"""
+import os
+
+import numpy as np
+import pytest
+
+from axicor_amd import brain as B
+from axicor_amd import night as NT
+from axicor_amd.engine import DeviceBuffer
+from axicor_amd.layout import StateView
+from helpers import Pair
+from oracle import oracle as O
+
+pytestmark = [pytest.mark.gpu,
+              pytest.mark.skipif(os.environ.get("AXC_LONG") != "1", reason="long run: set AXC_LONG=1")]
+
+
+def test_c2_2000_ticks_with_a_night_every_500(gpu):
+    b = B.generate(B.spec_c2(), with_paths=True)
+    N = b.padded_n
+    p = Pair(b, n_out=N, max_batch=50)
+    p.mapped = np.arange(N, dtype=np.uint32)  # every soma's spike bit, every tick
+    p.d_mapped = DeviceBuffer.from_array(p.mapped)
+    types = NT.sprout_types_from_lut(b.lut)
+    geo_gpu, geo_cpu = NT.geometry_of(b), NT.geometry_of(b)
+    for k in range(40):
+        out_ref, _ = p.step(50, dopamine=(40 if k % 2 else -25), use_graph=True)
+        np.testing.assert_array_equal(p.outputs(50), out_ref, err_msg=f"C2 raster, batch {k}")
+        print(f"[c2 long] ticks to {p.tick}: raster equal, {int(out_ref.sum())} spikes", flush=True)
+        if p.tick % 500 == 0 and p.tick < 2000:
+            p.check(f"C2 before the Night at tick {p.tick}")
+            epoch = p.tick // 500 - 1
+            res = NT.execute_night_phase(p.vram, 15, geo_gpu, types, epoch=epoch, master_seed=b.spec.seed)
+            v = StateView(p.ref_state, N)
+            flags = v.flags.copy()
+            O.sort_and_prune_blob(p.ref_state, N, 15)
+            t, w = v.targets.copy(), v.weights.copy()
+            ref = NT.sprout_pass(t, w, flags, v.soma_to_axon.copy(), N, geo_cpu, types, epoch, b.spec.seed)
+            v.targets[:] = t
+            v.weights[:] = w
+            assert res.new_synapses == ref.new_synapses
+            p.check(f"C2 after the Night at tick {p.tick}")
+            print(f"[c2 long] Night at {p.tick}: {ref.new_synapses} new synapses, state equal", flush=True)
+    p.check("C2 after 2,000 ticks")
