@@ -213,14 +213,17 @@ __global__ __launch_bounds__(256) void batch_push_flat_kernel(const BrainDev* __
   }
 }
 
-// compact_spikes_kernel for every brain of the batch: four lanes per staged
-// work item (its 256 somas' four mask words), one returning atomic per wave
-// (or per item where a wave's items span brains) on the brain's spike counter.
+// compact_spikes_kernel for every brain of the batch: kItemWords lanes per
+// staged work item (one per mask word of its kBlockSomas somas), one
+// returning atomic per wave (or per item where a wave's items span brains) on
+// the brain's spike counter.
+constexpr uint32_t kItemWords = kBlockSomas / 64;
+static_assert(kItemWords == 2 || kItemWords == 4, "an item's mask words are a power-of-two lane group");
 __global__ __launch_bounds__(256) void batch_compact_spikes_kernel(const BrainDev* __restrict__ brains,
                                                                    const WorkItem* __restrict__ items,
                                                                    uint32_t n_items) {
-  const uint32_t q = blockIdx.x * 256 + threadIdx.x, l = q & 3u;
-  const uint32_t it = q >> 2;
+  const uint32_t q = blockIdx.x * 256 + threadIdx.x, l = q % kItemWords;
+  const uint32_t it = q / kItemWords;
   uint64_t m = 0;
   uint32_t word = 0, bi = 0;
   if (it < n_items) {
@@ -230,14 +233,14 @@ __global__ __launch_bounds__(256) void batch_compact_spikes_kernel(const BrainDe
     if (w.start + l * 64u < brains[bi].N) m = brains[bi].sc.spk_mask[word];
   }
   const uint32_t c = (uint32_t)__popcll(m);
-  // The items of a brain are consecutive, so a wave's 16 items are mostly of
+  // The items of a brain are consecutive, so a wave's items are mostly of
   // one brain: then one atomic reserves the whole wave's range (per item,
   // the atomics queue on the brain's one counter).
   const uint32_t lane = threadIdx.x & 63u;
   const uint32_t b0 = __shfl(bi, 0, 64);
   const bool one_brain = __all(it >= n_items || bi == b0);
-  const uint32_t width = one_brain ? 64u : 4u, seg = lane & (width - 1);
-  uint32_t x = c;  // inclusive prefix over the wave, or over the item's four lanes
+  const uint32_t width = one_brain ? 64u : kItemWords, seg = lane & (width - 1);
+  uint32_t x = c;  // inclusive prefix over the wave, or over the item's lanes
   for (uint32_t off = 1; off < width; off <<= 1) {
     const uint32_t y = __shfl_up(x, off, (int)width);
     if (seg >= off) x += y;

@@ -67,6 +67,7 @@ struct TickParams {
   const uint32_t* bitmask;  // [T][words] input bitmasks, or null
   const uint32_t* spikes;   // [T][max_spikes] spike schedule, or null
   uint8_t* out;             // [T][n_out] output history, or null
+  const uint32_t* counts;   // [T] entries of each tick's schedule slice (device), or null
 };
 
 // Engine-private per-shard device buffers (no counterpart in the reference).

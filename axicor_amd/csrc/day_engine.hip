@@ -80,12 +80,12 @@ struct GraphKey {
   const void* heads;  // the captured kernels hold the arrays' addresses
   uint32_t v_seg;
   bool staged;
-  std::vector<uint32_t> spike_counts;
+  // The per-tick spike counts are not part of the key: the graph's K3
+  // kernels read them from the shard's device copy (TickParams::counts).
   bool operator==(const GraphKey& o) const {
     return T == o.T && words == o.words && virt_off == o.virt_off && n_virt == o.n_virt && staged == o.staged &&
            max_spk == o.max_spk && n_out == o.n_out && has_bitmask == o.has_bitmask &&
-           has_spikes == o.has_spikes && has_out == o.has_out && mapped == o.mapped && heads == o.heads && v_seg == o.v_seg &&
-           spike_counts == o.spike_counts;
+           has_spikes == o.has_spikes && has_out == o.has_out && mapped == o.mapped && heads == o.heads && v_seg == o.v_seg;
   }
 };
 
@@ -104,6 +104,14 @@ struct Shard {
   GraphKey gkey{};
   hipGraphExec_t gexec = nullptr;
   hipGraph_t graph = nullptr;
+  uint64_t graph_captures = 0;  // graphs recorded for this shard so far
+  // Device copy of a graph batch's per-tick spike counts ([counts_cap]).
+  uint32_t* d_counts = nullptr;
+  uint32_t counts_cap = 0;
+  // Event recorded after the last asynchronous host upload into the
+  // targets; the plan rebuild waits for it on its own stream.
+  hipEvent_t upload_ev = nullptr;
+  bool upload_pending = false;
   // Staged dendrite scan (day_staged.hip): the plan, allocated at the first
   // step, and whether the targets changed since it was built.
   StagePlan sp{};
@@ -165,13 +173,29 @@ void free_scratch(Shard* s) {
   (void)hipFree(s->sc.comp[0]);
   (void)hipFree(s->sc.comp[1]);
   (void)hipFree(s->prm);
+  (void)hipFree(s->d_counts);
   s->sc = ShardScratch{};
   s->prm = nullptr;
+  s->d_counts = nullptr;
+  s->counts_cap = 0;
   for (void* q : {(void*)s->sp.code, (void*)s->sp.seg, (void*)s->sp.fill, (void*)s->sp.set, (void*)s->sp.cnt,
                   (void*)s->sp.comp8, (void*)s->sp.tgt, (void*)s->sp.w})
     if (q) (void)hipFree(q);
   s->sp = StagePlan{};
   s->stage_dirty = true;
+}
+
+// Makes `st` wait for the last asynchronous host upload into the shard's
+// targets (gpu_memcpy_host_to_device_async on another stream), so that a plan
+// is never built from targets still in flight.
+void wait_target_upload(Shard* s, hipStream_t st) {
+  hipEvent_t ev = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (s->upload_pending) ev = s->upload_ev;
+    s->upload_pending = false;
+  }
+  if (ev) (void)hipStreamWaitEvent(st, ev, 0);
 }
 
 // Allocates the staging plan of a shard (first step) and rebuilds it on
@@ -213,6 +237,7 @@ bool ensure_stage(Shard* s, const ShardVramPtrs& v, hipStream_t st, hipError_t* 
     s->stage_dirty = true;
   }
   if (s->stage_dirty.exchange(false)) {
+    wait_target_upload(s, st);
     stage_build_kernel<<<stage_blocks((uint32_t)N), kBlockSomas, 0, st>>>(v.dendrite_targets, (uint32_t)N,
                                                                           (uint32_t)A, s->sp);
     *err = hipGetLastError();
@@ -225,7 +250,9 @@ bool ensure_stage(Shard* s, const ShardVramPtrs& v, hipStream_t st, hipError_t* 
 }
 
 // A host write of [dst, dst+size) may have changed some shard's targets.
-void note_device_write(const void* dst, size_t size) {
+// For an asynchronous write (`async`, already enqueued on `st`) an event is
+// recorded after it, which the next plan rebuild waits for.
+void note_device_write(const void* dst, size_t size, bool async = false, hipStream_t st = nullptr) {
   if (!dst || size == 0) return;
   const char* a = (const char*)dst;
   std::lock_guard<std::mutex> lk(g_mu);
@@ -234,10 +261,21 @@ void note_device_write(const void* dst, size_t size) {
     const char* t0 = (const char*)s->state_base + (size_t)s->N * 14;
     const char* t1 = t0 + (size_t)s->N * 4 * kMaxDendrites;
     if (a < t1 && a + size > t0) {
+      if (async) {
+        if (!s->upload_ev) (void)hipEventCreateWithFlags(&s->upload_ev, hipEventDisableTiming);
+        if (s->upload_ev && hipEventRecord(s->upload_ev, st) == hipSuccess) s->upload_pending = true;
+      }
       s->stage_dirty = true;
       s->targets_version++;
     }
   }
+}
+
+// Frees a shard's workspace and its bookkeeping.
+void destroy_shard(Shard* s) {
+  free_scratch(s);
+  if (s->upload_ev) (void)hipEventDestroy(s->upload_ev);
+  delete s;
 }
 
 void mark_stage_dirty(const void* state_base) {
@@ -299,9 +337,8 @@ Shard* shard_for(const ShardVramPtrs* v, uint32_t N, uint32_t A, hipError_t* err
     s->N = N;
     s->A = A;
     if ((*err = alloc_scratch(s)) != hipSuccess) {  // unregister rather than keep a half-built workspace
-      free_scratch(s);
       g_shards.erase(it);
-      delete s;
+      destroy_shard(s);
       return nullptr;
     }
     return s;
@@ -312,8 +349,7 @@ Shard* shard_for(const ShardVramPtrs* v, uint32_t N, uint32_t A, hipError_t* err
   (void)hipGetDevice(&s->device);
   s->state_base = v->soma_voltage;
   if ((*err = alloc_scratch(s)) != hipSuccess) {
-    free_scratch(s);
-    delete s;
+    destroy_shard(s);
     return nullptr;
   }
   g_shards[v->soma_voltage] = s;
@@ -343,11 +379,14 @@ hipError_t enqueue_tick(const ShardVramPtrs& v, Shard* s, uint32_t v_seg, uint32
     std::lock_guard<std::mutex> lk(g_tmu);
     timed = g_timing.enabled && prm == nullptr;  // never inside a captured graph
   }
-  const bool k3 = n_spk > 0 && spikes != nullptr;
+  // K3: per call, launched for the given count; in a graph (prm), for
+  // every tick whenever the batch has a schedule, with the count read on the
+  // device and a grid that strides over it.
+  const bool k3 = prm ? (spikes != nullptr && max_spk > 0) : (n_spk > 0 && spikes != nullptr);
   if (k3) {
     TimedScope ts(kKSpikes, st, timed);
-    spike_count_kernel<<<blocks_for(n_spk), kThreads, 0, st>>>(spikes, n_spk, A, s->sc.push_cnt, prm, tick,
-                                                                max_spk);
+    const uint32_t grid = prm ? std::min<uint32_t>(blocks_for(max_spk), 64u) : blocks_for(n_spk);
+    spike_count_kernel<<<grid, kThreads, 0, st>>>(spikes, n_spk, A, s->sc.push_cnt, prm, tick, max_spk);
   }
   const bool k2 = n_virt > 0 && bitmask != nullptr;
   {
@@ -492,8 +531,7 @@ void cu_free_shard(ShardVramPtrs* vram) {
     auto it = g_shards.find(vram->soma_voltage);
     if (it != g_shards.end()) {
       (void)hipDeviceSynchronize();
-      free_scratch(it->second);
-      delete it->second;
+      destroy_shard(it->second);
       g_shards.erase(it);
     }
   }
@@ -577,15 +615,15 @@ void gpu_host_free(void* p) {
   if (p) (void)hipHostFree(p);
 }
 void gpu_memcpy_host_to_device_async(void* dst, const void* src, size_t size, void* stream) {
-  note_device_write(dst, size);
   (void)hipMemcpyAsync(dst, src, size, hipMemcpyHostToDevice, (hipStream_t)stream);
+  note_device_write(dst, size, true, (hipStream_t)stream);
 }
 void gpu_memcpy_device_to_host_async(void* dst, const void* src, size_t size, void* stream) {
   (void)hipMemcpyAsync(dst, src, size, hipMemcpyDeviceToHost, (hipStream_t)stream);
 }
 bool gpu_memcpy_host_to_device(void* dst, const void* src, size_t size) {
-  note_device_write(dst, size);
   hipError_t e = hipMemcpy(dst, src, size, hipMemcpyHostToDevice);
+  note_device_write(dst, size);
   if (e != hipSuccess) fprintf(stderr, "[gpu_memcpy_host_to_device] %s (size %zu)\n", hipGetErrorString(e), size);
   return e == hipSuccess;
 }
@@ -706,6 +744,22 @@ int32_t cu_dma_h2d_io(uint32_t* d_in, const uint32_t* h_in, uint32_t words, uint
   if (words > 0 && d_in && h_in) (void)hipMemcpyAsync(d_in, h_in, (size_t)words * 4, hipMemcpyHostToDevice, st);
   if (cap > 0 && d_spk && h_spk) (void)hipMemcpyAsync(d_spk, h_spk, (size_t)cap * 4, hipMemcpyHostToDevice, st);
   return 0;
+}
+
+int32_t axc_dma_h2d_schedule(uint32_t* d_spikes, const uint32_t* h_spikes, uint32_t max_spk, const uint32_t* h_counts,
+                             uint32_t T, void* stream) {
+  if (T == 0) return 0;
+  if (!h_counts) return (int32_t)hipErrorInvalidValue;
+  uint32_t widest = 0;
+  for (uint32_t t = 0; t < T; t++) {
+    if (h_counts[t] > max_spk) return (int32_t)hipErrorInvalidValue;
+    widest = std::max(widest, h_counts[t]);
+  }
+  if (widest == 0) return 0;
+  if (!d_spikes || !h_spikes) return (int32_t)hipErrorInvalidValue;
+  const size_t pitch = (size_t)max_spk * 4;
+  return (int32_t)hipMemcpy2DAsync(d_spikes, pitch, h_spikes, pitch, (size_t)widest * 4, T, hipMemcpyHostToDevice,
+                                   (hipStream_t)stream);
 }
 
 int32_t cu_dma_d2h_io(uint8_t* h_out, const uint8_t* d_out, uint32_t cap, void* stream) {
@@ -863,6 +917,11 @@ int32_t axc_step_day_phase_batch(const ShardVramPtrs* vram, uint32_t padded_n, u
   if (n_virt > 0 && d_bitmask && (uint64_t)virt_off + n_virt > (uint64_t)total_axons)
     return (int32_t)hipErrorInvalidValue;
   if (T == 0) return 0;
+  // A tick's count beyond its slice (or counts without a schedule) would read
+  // past the schedule buffer: refused.
+  if (h_counts)
+    for (uint32_t t = 0; t < T; t++)
+      if (h_counts[t] > (d_spikes ? max_spk : 0u)) return (int32_t)hipErrorInvalidValue;
   hipError_t e;
   Shard* s = shard_for(vram, padded_n, total_axons, &e);
   if (!s) return (int32_t)e;
@@ -891,9 +950,8 @@ int32_t axc_step_day_phase_batch(const ShardVramPtrs* vram, uint32_t padded_n, u
   // back after the last.
   const bool fuse = !key.staged;
   const bool pw = key.staged;
-  key.spike_counts.assign(T, 0);
-  if (d_spikes && h_counts)
-    for (uint32_t t = 0; t < T; t++) key.spike_counts[t] = h_counts[t];
+  std::vector<uint32_t> counts(T, 0);
+  if (d_spikes && h_counts) counts.assign(h_counts, h_counts + T);
 
   // The per-batch burst reset of shard.rs:71 (idempotent with the per-tick one).
   reset_burst_kernel<<<blocks_for(padded_n ? padded_n : 1), kThreads, 0, st>>>(vram->soma_flags, padded_n);
@@ -912,7 +970,7 @@ int32_t axc_step_day_phase_batch(const ShardVramPtrs* vram, uint32_t padded_n, u
     for (uint32_t t = 0; t < T && e == hipSuccess; t++) {
       e = enqueue_tick(*vram, s, v_seg, tick_base + t, d_bitmask ? d_bitmask + (size_t)t * words : nullptr, words,
                        virt_off, n_virt, d_spikes ? d_spikes + (size_t)t * max_spk : nullptr, max_spk,
-                       key.spike_counts[t], d_mapped, d_out ? d_out + (size_t)t * n_out : nullptr, n_out,
+                       counts[t], d_mapped, d_out ? d_out + (size_t)t * n_out : nullptr, n_out,
                        (int32_t)dopamine, nullptr, st, lut, t, t > 0 && fuse, t + 1 == T || !fuse, key.staged, pw);
     }
     if (pw && e == hipSuccess) {
@@ -928,6 +986,22 @@ int32_t axc_step_day_phase_batch(const ShardVramPtrs* vram, uint32_t padded_n, u
   pv.bitmask = d_bitmask;
   pv.spikes = d_spikes;
   pv.out = d_out;
+  pv.counts = nullptr;
+  if (key.has_spikes) {
+    // The counts go to the shard's device copy, read by the graph's K3
+    // kernels (stream order keeps the previous replay's reads before this).
+    if (s->counts_cap < T) {
+      (void)hipStreamSynchronize(st);
+      (void)hipFree(s->d_counts);
+      s->d_counts = nullptr;
+      s->counts_cap = 0;
+      if ((e = hipMalloc((void**)&s->d_counts, (size_t)T * 4)) != hipSuccess) return (int32_t)e;
+      s->counts_cap = T;
+    }
+    if ((e = hipMemcpyAsync(s->d_counts, counts.data(), (size_t)T * 4, hipMemcpyHostToDevice, st)) != hipSuccess)
+      return (int32_t)e;
+    pv.counts = s->d_counts;
+  }
   set_params_kernel<<<1, 1, 0, st>>>(s->prm, pv);
 
   if (!(s->has_graph && s->gkey == key)) {
@@ -947,7 +1021,7 @@ int32_t axc_step_day_phase_batch(const ShardVramPtrs* vram, uint32_t padded_n, u
         // GSOP of tick t runs inside tick t+1's neuron pass; the last tick's
         // GSOP is launched on its own so the batch leaves a complete state.
         e = enqueue_tick(*vram, s, v_seg, t, d_bitmask, words, virt_off, n_virt, d_spikes, max_spk,
-                         key.spike_counts[t], d_mapped, d_out, n_out, 0, s->prm, cap, lut, t, t > 0 && fuse,
+                         0, d_mapped, d_out, n_out, 0, s->prm, cap, lut, t, t > 0 && fuse,
                          t + 1 == T || !fuse, key.staged, pw);
       }
       if (pw && e == hipSuccess)
@@ -967,6 +1041,7 @@ int32_t axc_step_day_phase_batch(const ShardVramPtrs* vram, uint32_t padded_n, u
     if (e != hipSuccess) return (int32_t)e;
     s->gkey = key;
     s->has_graph = true;
+    s->graph_captures++;
   }
   e = hipGraphLaunch(s->gexec, st);
   return (int32_t)e;
@@ -1020,6 +1095,15 @@ void axc_host_sort_and_prune(uint8_t* flags, uint32_t* targets, int16_t* weights
 }
 
 void axc_staging_enable(int enable) { g_staging = enable != 0; }
+
+int32_t axc_shard_graph_captures(const ShardVramPtrs* vram, uint64_t* out) {
+  if (!vram || !out) return (int32_t)hipErrorInvalidValue;
+  std::lock_guard<std::mutex> lk(g_mu);
+  auto it = g_shards.find(vram->soma_voltage);
+  if (it == g_shards.end()) return (int32_t)hipErrorInvalidValue;
+  *out = it->second->graph_captures;
+  return 0;
+}
 
 void axc_shard_targets_changed(const ShardVramPtrs* vram) {
   if (vram) mark_stage_dirty(vram->soma_voltage);
@@ -1119,7 +1203,7 @@ hipError_t batch_capture(Batch* b, uint32_t T) {
                                                                      b->d_tick, t, t);
         batch_staged_kernel<<<(b->n_stage_items + kNumXcd - 1) / kNumXcd * kNumXcd, kStageThreads, lds, cap>>>(
             b->d_brains, b->d_stage_items, b->n_stage_items, b->d_tick, t, t);
-        batch_compact_spikes_kernel<<<blocks_for((uint64_t)b->n_stage_items * 4), kThreads, 0, cap>>>(
+        batch_compact_spikes_kernel<<<blocks_for((uint64_t)b->n_stage_items * kItemWords), kThreads, 0, cap>>>(
             b->d_brains, b->d_stage_items, b->n_stage_items);
         batch_spike_prefix_kernel<<<1, 1024, 0, cap>>>(b->d_brains, n, b->d_prefix, kCntSpikes);
         batch_push_flat_kernel<<<b->tune.list_blocks, kThreads, 0, cap>>>(b->d_brains, n, b->d_prefix, t);
@@ -1173,10 +1257,7 @@ hipError_t batch_run(Batch* b, uint32_t tick_base, uint32_t T, hipStream_t st) {
 void batch_free(Batch* b) {
   if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
-  for (Shard* s : b->shards) {
-    free_scratch(s);
-    delete s;
-  }
+  for (Shard* s : b->shards) destroy_shard(s);
   (void)hipFree(b->d_brains);
   (void)hipFree(b->d_luts);
   (void)hipFree(b->d_stats);

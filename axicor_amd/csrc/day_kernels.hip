@@ -233,16 +233,24 @@ __global__ __launch_bounds__(256) void axon_tick_kernel(AxcBurstHeads8* __restri
   axon_step(heads, a, v_seg, bitmask, virt_off, n_virt, push_cnt, use_cnt != 0, comp, comp8);
 }
 
-// K3 (physics.cu:114-132): one count per occurrence of an id < A.
-__global__ __launch_bounds__(256) void spike_count_kernel(const uint32_t* __restrict__ ids_arg, uint32_t n,
+// K3 (physics.cu:114-132): one count per occurrence of an id < A. With
+// TickParams the tick's count is read on the device (prm->counts[t_off],
+// at most the slice length `stride`), so a captured graph serves any
+// schedule; the grid strides over the count.
+__global__ __launch_bounds__(256) void spike_count_kernel(const uint32_t* __restrict__ ids_arg, uint32_t n_arg,
                                                           uint32_t A, uint32_t* __restrict__ push_cnt,
                                                           const TickParams* __restrict__ prm, uint32_t t_off,
                                                           uint32_t stride) {
-  uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= n) return;
   const uint32_t* __restrict__ ids = prm ? prm->spikes + (size_t)t_off * stride : ids_arg;
-  uint32_t a = ids[j];
-  if (a < A) atomicAdd(&push_cnt[a], 1u);
+  uint32_t n = n_arg;
+  if (prm) {
+    n = prm->counts ? prm->counts[t_off] : 0u;
+    n = n < stride ? n : stride;
+  }
+  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+    const uint32_t a = ids[j];
+    if (a < A) atomicAdd(&push_cnt[a], 1u);
+  }
 }
 
 // The soma part of K5 (physics.cu:165-174, 209-275) for soma i after its

@@ -197,14 +197,19 @@ class ShardEngine:
         lib = day_lib()
         T = sync_batch_ticks
         words = io.input_words_per_tick * T if h_input_bitmask is not None else 0
-        cap = io.max_spikes_per_tick * T if h_incoming_spikes is not None else 0
         hin = np.ascontiguousarray(h_input_bitmask, dtype=np.uint32) if h_input_bitmask is not None else None
         hsp = np.ascontiguousarray(h_incoming_spikes, dtype=np.uint32) if h_incoming_spikes is not None else None
-        _check(lib.cu_dma_h2d_io(io.d_input_bitmask or None, ptr(hin), words, io.d_incoming_spikes or None,
-                                 ptr(hsp), cap, self.stream), "cu_dma_h2d_io")
         counts = np.ascontiguousarray(np.asarray(h_spike_counts, dtype=np.uint32).reshape(-1)[:T])
         if counts.size < T:
             counts = np.concatenate([counts, np.zeros(T - counts.size, np.uint32)])
+        # Inputs as the reference uploads them (cu_dma_h2d_io); the schedule
+        # without its full-capacity copy: only the used part of each tick's
+        # slice (axc_dma_h2d_schedule), nothing for an empty schedule.
+        _check(lib.cu_dma_h2d_io(io.d_input_bitmask or None, ptr(hin), words, None, None, 0, self.stream),
+               "cu_dma_h2d_io")
+        if hsp is not None and io.d_incoming_spikes:
+            _check(lib.axc_dma_h2d_schedule(io.d_incoming_spikes, ptr(hsp), io.max_spikes_per_tick, ptr(counts), T,
+                                            self.stream), "axc_dma_h2d_schedule")
         bm = io.d_input_bitmask if (io.d_input_bitmask and hin is not None) else None
         sp = io.d_incoming_spikes if (io.d_incoming_spikes and hsp is not None) else None
         outp = io.d_output_history or None
@@ -228,6 +233,12 @@ class ShardEngine:
 
     def synchronize(self) -> None:
         day_lib().gpu_stream_synchronize(self.stream)
+
+    def graph_captures(self) -> int:
+        """Graphs recorded for this shard by the batch entry point so far."""
+        n = C.c_uint64()
+        _check(day_lib().axc_shard_graph_captures(C.byref(self.vram.ptrs), C.byref(n)), "axc_shard_graph_captures")
+        return n.value
 
     def stats_reset(self) -> None:
         _check(day_lib().axc_shard_stats_reset(C.byref(self.vram.ptrs), self.stream), "axc_shard_stats_reset")

@@ -282,7 +282,12 @@ int32_t axc_shard_stats_read(const ShardVramPtrs *vram, AxcShardStats *out);
  * (host array, may be NULL when the schedule is NULL).
  * Results are bit-identical to sync_batch_ticks calls of cu_step_day_phase.
  * For callers that own the tick loop (the Rust ShardEngine, the benchmarks):
- * the engine captures the batch in a HIP graph and replays it. */
+ * the engine captures the batch in a HIP graph and replays it. The graph
+ * depends on the batch's shape (T, slice sizes, pointers, v_seg), not on the
+ * spike counts: those are copied to the device each call and read there, so
+ * changing schedules replay the same graph. Returns hipErrorInvalidValue
+ * when some h_spike_counts[t] exceeds max_spikes_per_tick, or is nonzero
+ * without a schedule. */
 int32_t axc_step_day_phase_batch(const ShardVramPtrs *vram, uint32_t padded_n, uint32_t total_axons,
                                  uint32_t v_seg, uint32_t tick_base, uint32_t sync_batch_ticks,
                                  const uint32_t *d_input_bitmask, uint32_t input_words_per_tick,
@@ -291,6 +296,21 @@ int32_t axc_step_day_phase_batch(const ShardVramPtrs *vram, uint32_t padded_n, u
                                  const uint32_t *h_spike_counts,
                                  const uint32_t *d_mapped_soma_ids, uint8_t *d_output_history,
                                  uint32_t num_outputs, int16_t dopamine, void *stream);
+
+/* Graphs recorded so far for the shard at vram (one per new batch shape). */
+int32_t axc_shard_graph_captures(const ShardVramPtrs *vram, uint64_t *out);
+
+/* The spike schedule's upload without the reference's full-capacity copy
+ * (compute/shard.rs:55-67 hands cu_dma_h2d_io max_spikes_per_tick * T words
+ * every batch, bindings.cu:760-764, even when the schedule is empty): one 2D
+ * copy of the first max_t(h_spike_counts[t]) ids of each tick's slice of the
+ * host schedule h_spikes ([T][max_spikes_per_tick]) into the same slices of
+ * d_spikes, nothing when every count is 0. The kernels read only the first
+ * count ids of a slice, so the result equals the full copy. Asynchronous on
+ * `stream` (pinned h_spikes), hipErrorInvalidValue for a count beyond the
+ * slice. For callers of axc_step_day_phase_batch and cu_step_day_phase. */
+int32_t axc_dma_h2d_schedule(uint32_t *d_spikes, const uint32_t *h_spikes, uint32_t max_spikes_per_tick,
+                             const uint32_t *h_spike_counts, uint32_t ticks, void *stream);
 
 /* Host-side Night Phase sort & prune on columnar host arrays, the same rule as
  * launch_sort_and_prune; for hosts that run the Night Phase on CPU cores. */

@@ -98,3 +98,27 @@ def test_surrogate_library_exports_every_declared_symbol():
     exported = set(line.split()[-1] for line in out.splitlines() if line.strip())
     for name in names:
         assert hasattr(lib, name) and name in exported, name
+
+
+def test_batch_and_schedule_dma_refuse_counts_past_the_slice():
+    """Host-side checks, made before any GPU call: a tick's spike count above
+    max_spikes_per_tick (or counts without a schedule) would read past the
+    schedule buffer, so both entry points refuse it; an all-zero schedule
+    copies nothing."""
+    import ctypes as C
+    import numpy as np
+    from axicor_amd._ffi import ShardVramPtrs
+    lib = day_lib()
+    invalid = 1  # hipErrorInvalidValue
+    v = ShardVramPtrs()
+    counts = np.array([0, 3, 9, 2], np.uint32)
+    fake = C.c_void_p(0x1000)  # never dereferenced: the call is refused first
+    rc = lib.axc_step_day_phase_batch(C.byref(v), 64, 64, 1, 0, 4, None, 0, 0, 0, fake, 8,
+                                      counts.ctypes.data, None, None, 0, 0, None)
+    assert rc == invalid
+    rc = lib.axc_step_day_phase_batch(C.byref(v), 64, 64, 1, 0, 4, None, 0, 0, 0, None, 8,
+                                      counts.ctypes.data, None, None, 0, 0, None)
+    assert rc == invalid
+    assert lib.axc_dma_h2d_schedule(fake, fake, 8, counts.ctypes.data, 4, None) == invalid
+    zeros = np.zeros(4, np.uint32)
+    assert lib.axc_dma_h2d_schedule(None, None, 8, zeros.ctypes.data, 4, None) == 0

@@ -74,6 +74,31 @@ def test_spike_schedule_duplicates_and_bad_ids(gpu):
     p.check("spike schedule")
 
 
+def test_graph_reused_across_varying_schedules(gpu):
+    """The batch graph does not depend on the spike counts: six graph batches
+    whose schedules differ (empty, full slices, ragged counts, duplicates,
+    out-of-range ids) replay one captured graph, each equal to the oracle."""
+    b = B.generate(B.spec_small(2048, 32, 128, seed=15))
+    p = Pair(b, max_spikes=48, max_batch=10, n_out=24)
+    rng = np.random.default_rng(21)
+    T = 10
+    for k in range(6):
+        sched = rng.integers(0, b.total_axons, size=(T, 48)).astype(np.uint32)
+        sched[:, 3] = sched[:, 4]            # duplicates
+        sched[::3, 7] = b.total_axons + 1    # out of range
+        if k == 0:
+            counts = np.zeros(T, np.uint32)
+        elif k == 1:
+            counts = np.full(T, 48, np.uint32)
+        else:
+            counts = rng.integers(0, 49, size=T).astype(np.uint32)
+            counts[rng.integers(0, T)] = 0
+        out_ref, _ = p.step(T, spikes=sched.reshape(-1), counts=counts, use_graph=True, dopamine=30 * k)
+        np.testing.assert_array_equal(p.outputs(T), out_ref, err_msg=f"outputs, schedule {k}")
+        p.check(f"graph batch with schedule {k}")
+    assert p.eng.graph_captures() == 1
+
+
 def test_edge_state_values(gpu):
     """Hand-placed edge cases: heads close to the u32 wrap and past the
     sentinel, targets whose axon field is 0 or >= A, two somas sharing one
