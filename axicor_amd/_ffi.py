@@ -140,6 +140,8 @@ SIGNATURES = {
     "axc_shard_targets_changed": (None, [C.POINTER(ShardVramPtrs)]),
     "axc_shard_graph_captures": (_I32, [_VP, C.POINTER(C.c_uint64)]),
     "axc_dma_h2d_schedule": (_I32, [_P, _P, _U32, _P, _U32, _P]),
+    "axc_step_day_phase_noalloc": (_I32, [_VP, _U32, _U32, _U32, _U32, _P, _U32, _U32, _P, _U32, _P, _P, _U32, _I16,
+                                          _P]),
 }
 
 TIMED_KERNELS = ["axon_tick", "neuron_tick", "push_apply", "gsop", "readout", "spike_count", "weights_copy"]

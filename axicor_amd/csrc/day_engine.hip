@@ -59,6 +59,16 @@ struct DeviceLut {
 std::mutex g_lut_mu;
 std::map<int, DeviceLut> g_luts;
 
+// The table of the current device if cu_upload_constant_memory (or another
+// entry point) has created it, else null; allocates nothing.
+const AxcVariantParameters* device_lut_if_present() {
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  std::lock_guard<std::mutex> lk(g_lut_mu);
+  auto it = g_luts.find(dev);
+  return it != g_luts.end() ? it->second.dev : nullptr;
+}
+
 AxcVariantParameters* device_lut() {
   int dev = 0;
   (void)hipGetDevice(&dev);
@@ -370,14 +380,14 @@ hipError_t enqueue_tick(const ShardVramPtrs& v, Shard* s, uint32_t v_seg, uint32
                         const uint32_t* spikes, uint32_t max_spk, uint32_t n_spk, const uint32_t* mapped,
                         uint8_t* out, uint32_t n_out, int32_t dopamine, const TickParams* prm, hipStream_t st,
                         const AxcVariantParameters* lut, uint32_t snap, bool fuse_prev, bool gsop_now,
-                        bool staged, bool pw = false) {
+                        bool staged, bool pw = false, bool allow_timing = true) {
   const uint32_t N = s->N, A = s->A;
   uint4* cur = s->sc.comp[snap & 1];
   uint4* prev = s->sc.comp[(snap & 1) ^ 1];
   bool timed;
   {
     std::lock_guard<std::mutex> lk(g_tmu);
-    timed = g_timing.enabled && prm == nullptr;  // never inside a captured graph
+    timed = allow_timing && g_timing.enabled && prm == nullptr;  // never inside a captured graph
   }
   // K3: per call, launched for the given count; in a graph (prm), for
   // every tick whenever the batch has a schedule, with the count read on the
@@ -439,6 +449,62 @@ hipError_t enqueue_tick(const ShardVramPtrs& v, Shard* s, uint32_t v_seg, uint32
     readout_kernel<<<blocks_for(n_out), kThreads, 0, st>>>(v.soma_flags, N, mapped, out, n_out, prm, tick);
   }
   return hipGetLastError();
+}
+
+// ---- the allocation-free per-tick entry point ------------------------------
+// Device scratch of axc_step_day_phase_noalloc: the direct scan's workspace
+// (list counters, the compact head snapshot, push counts, owner and spike
+// lists, the kernels' counter partials) for one shard of at most
+// kNoAllocScratchBytes, carved out per call. Every GPU the library runs on
+// holds its own copy until the process ends. Host code never passes the
+// array's name to a kernel (on the host it is a stand-in, not a device
+// address): it uses the per-device address from hipGetSymbolAddress.
+constexpr size_t kNoAllocScratchBytes = (size_t)16 << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+std::mutex g_na_mu;
+std::map<int, unsigned char*> g_na_ptr;  // device -> address of that device's scratch
+
+// Bytes of scratch a shard of (N, A) needs, each piece 256-byte aligned.
+struct NoAllocLayout {
+  size_t counters, comp, push_cnt, own_list, spk_list, stats_part, stats_part2, total;
+};
+NoAllocLayout noalloc_layout(uint32_t N, uint32_t A) {
+  auto up = [](size_t x) { return (x + 255) / 256 * 256; };
+  NoAllocLayout l{};
+  size_t o = 0;
+  l.counters = o; o += up(kNumCounters * 4);
+  l.comp = o; o += up(((size_t)A + 1) * sizeof(uint4));
+  l.push_cnt = o; o += up((size_t)(A ? A : 1) * 4);
+  l.own_list = o; o += up((size_t)(A ? A : 1) * 4);
+  l.spk_list = o; o += up((size_t)(N ? N : 1) * 4);
+  l.stats_part = o; o += up((size_t)neuron_blocks(N ? N : 1) * kStatsPerBlock * 8);
+  l.stats_part2 = o; o += up((size_t)kListBlocks * kStatsPerBlock * 8);
+  l.total = o;
+  return l;
+}
+
+// The address of the current device's scratch, looked up once per device
+// (with that device current); null on failure (*err set).
+unsigned char* noalloc_scratch_ptr(hipError_t* err) {
+  int dev = 0;
+  *err = hipGetDevice(&dev);
+  if (*err != hipSuccess) return nullptr;
+  std::lock_guard<std::mutex> lk(g_na_mu);
+  auto it = g_na_ptr.find(dev);
+  if (it != g_na_ptr.end()) return it->second;
+  void* p = nullptr;
+  *err = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (*err != hipSuccess || !p) {
+    if (*err == hipSuccess) *err = hipErrorInvalidSymbol;
+    return nullptr;
+  }
+  g_na_ptr[dev] = (unsigned char*)p;
+  return (unsigned char*)p;
+}
+
+__global__ void fill_u32_kernel(uint32_t* __restrict__ p, uint32_t n, uint32_t v) {
+  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) p[j] = v;
 }
 
 }  // namespace
@@ -580,6 +646,53 @@ int32_t cu_step_day_phase(const ShardVramPtrs* vram, uint32_t padded_n, uint32_t
                    incoming_spikes, 0, num_incoming_spikes, mapped_soma_ids, output_history, num_outputs,
                    (int32_t)dopamine, nullptr, (hipStream_t)stream, lut, s->flip, false, true, staged);
   s->flip ^= 1u;
+  return (int32_t)e;
+}
+
+int32_t axc_step_day_phase_noalloc(const ShardVramPtrs* vram, uint32_t padded_n, uint32_t total_axons,
+                                   uint32_t v_seg, uint32_t current_tick, const uint32_t* input_bitmask,
+                                   uint32_t virtual_offset, uint32_t num_virtual_axons,
+                                   const uint32_t* incoming_spikes, uint32_t num_incoming_spikes,
+                                   const uint32_t* mapped_soma_ids, uint8_t* output_history, uint32_t num_outputs,
+                                   int16_t dopamine, void* stream) {
+  if (!vram) return (int32_t)hipErrorInvalidValue;
+  if (num_virtual_axons > 0 && input_bitmask &&
+      (uint64_t)virtual_offset + num_virtual_axons > (uint64_t)total_axons)
+    return (int32_t)hipErrorInvalidValue;
+  const NoAllocLayout l = noalloc_layout(padded_n, total_axons);
+  if (l.total > kNoAllocScratchBytes) {
+    fprintf(stderr,
+            "[axc_step_day_phase_noalloc] padded_n=%u total_axons=%u needs %zu bytes of scratch; the limit is %zu "
+            "(use cu_step_day_phase)\n",
+            padded_n, total_axons, l.total, kNoAllocScratchBytes);
+    return (int32_t)hipErrorInvalidValue;
+  }
+  const AxcVariantParameters* lut = device_lut_if_present();
+  if (!lut) return (int32_t)hipErrorNotInitialized;  // cu_upload_constant_memory first
+  hipError_t e;
+  unsigned char* base = noalloc_scratch_ptr(&e);
+  if (!base) return (int32_t)e;
+  // A workspace view over the scratch; no registry entry, no staging plan.
+  Shard w;
+  w.N = padded_n;
+  w.A = total_axons;
+  w.sc.counters = (uint32_t*)(base + l.counters);
+  w.sc.comp[0] = w.sc.comp[1] = (uint4*)(base + l.comp);
+  w.sc.push_cnt = (uint32_t*)(base + l.push_cnt);
+  w.sc.own_list = (uint32_t*)(base + l.own_list);
+  w.sc.spk_list = (uint32_t*)(base + l.spk_list);
+  w.sc.stats_part = (uint64_t*)(base + l.stats_part);
+  w.sc.stats_part2 = (uint64_t*)(base + l.stats_part2);
+  w.sc.spk_mask = nullptr;  // staged pass only
+  hipStream_t st = (hipStream_t)stream;
+  // The scratch keeps the last call's contents (any shard's): the push counts
+  // must start at zero and the snapshot's entry A must be the inert axon.
+  fill_u32_kernel<<<std::min<uint32_t>(blocks_for(total_axons ? total_axons : 1), 1024u), kThreads, 0, st>>>(
+      w.sc.push_cnt, total_axons, 0u);
+  fill_u32_kernel<<<1, 64, 0, st>>>((uint32_t*)(w.sc.comp[0] + total_axons), 4u, 0xFFFFFFFFu);
+  e = enqueue_tick(*vram, &w, v_seg, current_tick, input_bitmask, 0, virtual_offset, num_virtual_axons,
+                   incoming_spikes, 0, num_incoming_spikes, mapped_soma_ids, output_history, num_outputs,
+                   (int32_t)dopamine, nullptr, st, lut, 0, false, true, false, false, false);
   return (int32_t)e;
 }
 

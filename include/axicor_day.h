@@ -297,6 +297,24 @@ int32_t axc_step_day_phase_batch(const ShardVramPtrs *vram, uint32_t padded_n, u
                                  const uint32_t *d_mapped_soma_ids, uint8_t *d_output_history,
                                  uint32_t num_outputs, int16_t dopamine, void *stream);
 
+/* The tick of cu_step_day_phase without allocating: no counterpart in the
+ * reference. For callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc). Same
+ * arguments, results and error codes as cu_step_day_phase, computed by the
+ * direct dendrite scan, with its workspace in one static device buffer of
+ * 16 MiB per GPU instead of per-shard allocations. It needs about
+ * 16 (A + 1) + 8 A + 4.25 N + 128 KiB bytes of it (N = padded_n,
+ * A = total_axons; e.g. N = A = 500,000) and returns hipErrorInvalidValue
+ * beyond that, hipErrorNotInitialized before cu_upload_constant_memory.
+ * Every call on one GPU shares that buffer: calls that overlap on one GPU
+ * (other streams, other threads) race and must be serialised by the caller. */
+int32_t axc_step_day_phase_noalloc(const ShardVramPtrs *vram, uint32_t padded_n, uint32_t total_axons,
+                                   uint32_t v_seg, uint32_t current_tick, const uint32_t *input_bitmask,
+                                   uint32_t virtual_offset, uint32_t num_virtual_axons,
+                                   const uint32_t *incoming_spikes, uint32_t num_incoming_spikes,
+                                   const uint32_t *mapped_soma_ids, uint8_t *output_history, uint32_t num_outputs,
+                                   int16_t dopamine, void *stream);
+
 /* Graphs recorded so far for the shard at vram (one per new batch shape). */
 int32_t axc_shard_graph_captures(const ShardVramPtrs *vram, uint64_t *out);
 
