@@ -1,14 +1,12 @@
 """Long runs on the GPU against the CPU oracle, skipped unless AXC_LONG=1
-(minutes of oracle time each, so not part of the default suite):
+(many minutes of oracle time, so not part of the default suite, which runs
+each configuration for fewer ticks in tests/test_gpu_configs.py):
 
-- the full 1.5M-neuron C2 brain for 2,000 ticks through the batch graph with
+- the full 1.5M-neuron brain for 2,000 ticks through the batch graph with
   a Night every 500 ticks as shard_thread.rs:269-378 runs it (the flags
   taken before the GPU sort & prune at 15, the host sprouting, the upload):
   every soma's spike bit every tick, and the whole state and the heads
-  before and after every Night and at the end, bit for bit;
-- the full 12M-neuron C5 brain (128 slots, 1.5B synapses) for 100 ticks
-  through the batch graph: 64 mapped somas' spike bits every tick and the
-  whole state and heads after every batch of 50.
+  before and after every Night and at the end, bit for bit.
 
 Run: AXC_LONG=1 python -u -m pytest tests/test_gpu_long.py -s --timeout 1100"""
 import os
@@ -54,13 +52,3 @@ def test_c2_2000_ticks_with_a_night_every_500(gpu):
             p.check(f"C2 after the Night at tick {p.tick}")
             print(f"[c2 long] Night at {p.tick}: {ref.new_synapses} new synapses, state equal", flush=True)
     p.check("C2 after 2,000 ticks")
-
-
-def test_c5_full_brain_100_ticks(gpu):
-    b = B.generate(B.spec_c5())
-    p = Pair(b, n_out=64, max_batch=50)
-    for k in range(2):
-        out_ref, _ = p.step(50, dopamine=(30 if k else -20), use_graph=True)
-        np.testing.assert_array_equal(p.outputs(50), out_ref, err_msg=f"C5 outputs, batch {k}")
-        p.check(f"C5 after {p.tick} ticks")
-        print(f"[c5 long] ticks to {p.tick}: state equal", flush=True)
