@@ -134,6 +134,8 @@ SIGNATURES = {
     "axc_batch_get_tuning": (_I32, [_P, C.POINTER(AxcBatchTuning)]),
     "axc_batch_stats": (_I32, [_P, _U32, C.POINTER(AxcShardStats), _P]),
     "axc_batch_synapse_stats": (_I32, [_P, _P, _P, _P]),
+    "axc_batch_update": (_I32, [_P, C.POINTER(AxcBrainDesc), _U32]),
+    "axc_batch_stats_all": (_I32, [_P, C.POINTER(AxcShardStats), _P]),
     "axc_batch_destroy": (None, [_P]),
     "axc_version": (C.c_char_p, []),
     "axc_staging_enable": (None, [C.c_int]),

@@ -19,12 +19,9 @@ class BatchEngine:
     """Owns nothing but the batch handle: the brains' VramStates, tables and
     input buffers stay with the caller and must outlive the batch."""
 
-    def __init__(self, members: list[dict]):
-        """members: dicts with keys vram (VramState), lut (VARIANT_DTYPE[16]),
-        d_inputs (DeviceBuffer or None), input_words, input_ticks, v_seg,
-        virtual_offset, n_inputs, dopamine."""
-        lib = _sig()
-        self._luts = [np.ascontiguousarray(m["lut"], dtype=VARIANT_DTYPE) for m in members]
+    @staticmethod
+    def _descs(members: list[dict]):
+        luts = [np.ascontiguousarray(m["lut"], dtype=VARIANT_DTYPE) for m in members]
         arr = (AxcBrainDesc * len(members))()
         for k, m in enumerate(members):
             d = arr[k]
@@ -34,11 +31,19 @@ class BatchEngine:
             d.v_seg = m.get("v_seg", 1)
             d.virtual_offset = m["virtual_offset"]
             d.num_virtual_axons = m["n_inputs"]
-            d.lut = ptr(self._luts[k])
+            d.lut = ptr(luts[k])
             d.d_input_bitmask = m["d_inputs"].ptr if m.get("d_inputs") is not None else None
             d.input_words = m.get("input_words", 0)
             d.input_ticks = m.get("input_ticks", 1)
             d.dopamine = m.get("dopamine", 0)
+        return arr, luts
+
+    def __init__(self, members: list[dict]):
+        """members: dicts with keys vram (VramState), lut (VARIANT_DTYPE[16]),
+        d_inputs (DeviceBuffer or None), input_words, input_ticks, v_seg,
+        virtual_offset, n_inputs, dopamine."""
+        lib = _sig()
+        arr, self._luts = self._descs(members)
         h = C.c_void_p()
         err = lib.axc_batch_create(arr, len(members), C.byref(h))
         if err:
@@ -71,6 +76,25 @@ class BatchEngine:
         if err:
             raise GpuError(f"axc_batch_stats failed ({err})")
         return {n: getattr(s, n) for n, _ in AxcShardStats._fields_}, ts
+
+    def update(self, members: list[dict]) -> None:
+        """New tables, dopamine and inputs for the same brains (same VramStates
+        in the same order); counters restart at zero (axc_batch_update)."""
+        arr, self._luts = self._descs(members)
+        err = day_lib().axc_batch_update(self.h, arr, len(members))
+        if err:
+            raise GpuError(f"axc_batch_update failed ({err})")
+
+    def stats_all(self) -> tuple[np.ndarray, np.ndarray]:
+        """Counters of every brain: ([n] structured AxcShardStats fields as a
+        dict of arrays, [n][16] spikes per variant type)."""
+        arr = (AxcShardStats * self.n)()
+        ts = np.zeros((self.n, 16), np.uint64)
+        err = day_lib().axc_batch_stats_all(self.h, arr, ptr(ts))
+        if err:
+            raise GpuError(f"axc_batch_stats_all failed ({err})")
+        st = {n: np.array([getattr(a, n) for a in arr], np.uint64) for n, _ in AxcShardStats._fields_}
+        return st, ts
 
     def synapse_stats(self, prune_thresholds=None, stream=None) -> np.ndarray:
         """[n][5] int64: filled slots, sum |w|, sum w^2, slots with |w| below

@@ -112,7 +112,8 @@ class BrainSpec:
     seed: int = DEFAULT_MASTER_SEED
     input_ppm: int = 50_000
     lut: np.ndarray | None = None
-    brick: int = 0  # 0: voxels numbered x-fastest; B: BxBxB bricks (a spatial sort of the somas)
+    brick: int = 0  # 0: voxels numbered x-fastest; B: BxBxB bricks (a spatial sort of the somas);
+    #                 BAKER_ORDER: z-planes in order, each plane shuffled (the baker's numbering)
 
     def to_c(self) -> GenSpec:
         g = GenSpec()
@@ -130,7 +131,7 @@ class BrainSpec:
             g.type_ppm_cdf[i] = acc
         g.mean_dist_milli = int(round(self.mean_dist * 1000))
         g.master_seed = self.seed & 0xFFFFFFFFFFFFFFFF
-        g.brick = self.brick
+        g.brick = self.brick & 0xFFFFFFFF
         return g
 
 
@@ -138,8 +139,15 @@ def spec_c1(seed: int = DEFAULT_MASTER_SEED) -> BrainSpec:
     return BrainSpec(4096, 256, (16, 16, 16), 64, (1, 32), seed=seed, input_ppm=50_000, lut=lut_c1())
 
 
+# genesis-baker/src/bake/neuron_placement.rs:60-87: the baker sorts its
+# placed somas by z only, each plane's positions drawn from a shuffled pool,
+# so a baked shard numbers its somas plane by plane in random (x, y) order.
+BAKER_ORDER = -1
+
+
 def spec_c2(seed: int = DEFAULT_MASTER_SEED, brick: int = 8) -> BrainSpec:
-    """Somas numbered brick by brick (8x8x8 voxels) unless brick=0."""
+    """Somas numbered brick by brick (8x8x8 voxels) unless brick=0
+    (x-fastest) or BAKER_ORDER (z-planes, shuffled within each)."""
     return BrainSpec(1_500_000, 75_000, (128, 128, 64), 100, (1, 255),
                      type_ppm=[400_000, 400_000, 100_000, 100_000], seed=seed, input_ppm=20_000, lut=lut_c2(),
                      brick=brick)

@@ -83,7 +83,9 @@ struct GenSpec {
   uint32_t mean_dist_milli;  // mean Chebyshev distance of targets, in 1/1000 voxel
   uint64_t master_seed;
   uint32_t brick;            // 0: voxels numbered x-fastest; B > 0: BxBxB bricks, x-fastest
-                             // inside a brick and over bricks (dims multiples of B)
+                             // inside a brick and over bricks (dims multiples of B);
+                             // kBakerOrder: z-planes in order, each plane's voxels in a
+                             // seeded shuffle (the baker's numbering)
 };
 
 // padded_n and total_axons of a spec (memory.rs:32 align_to_warp, boot.rs:65-71).
@@ -97,10 +99,26 @@ void axc_gen_sizes(const GenSpec* spec, uint32_t* padded_n, uint32_t* total_axon
 
 // Voxel numbering: rank <-> (x, y, z). With bricks, the voxels of one brick
 // get consecutive ranks, so consecutive somas are spatial neighbours in all
-// three axes.
+// three axes. In the baker's order (genesis-baker/src/bake/neuron_placement.rs
+// :60-87 sorts the placed somas by z only, each plane's positions taken from a
+// shuffled pool) consecutive ranks share a z-plane and nothing else: rank r
+// is plane r / (gx gy), at position perm[r % (gx gy) + plane offset] of that
+// plane, perm a seeded Fisher-Yates shuffle per plane (our own shuffle; the
+// baker's ChaCha8 stream is not reproduced).
+constexpr uint32_t kBakerOrder = 0xFFFFFFFFu;
 struct VoxelOrder {
   uint32_t gx, gy, gz, b;
+  const uint32_t* perm = nullptr;  // baker order: [gz][gx*gy] in-plane index of each rank
+  const uint32_t* inv = nullptr;   // its inverse
   void unrank(uint64_t r, uint32_t* x, uint32_t* y, uint32_t* z) const {
+    if (b == kBakerOrder) {
+      const uint64_t P = (uint64_t)gx * gy;
+      const uint32_t q = perm[r];
+      *x = q % gx;
+      *y = q / gx;
+      *z = (uint32_t)(r / P);
+      return;
+    }
     if (b == 0) {
       *x = (uint32_t)(r % gx);
       *y = (uint32_t)((r / gx) % gy);
@@ -116,6 +134,10 @@ struct VoxelOrder {
     *z = bz * b + (uint32_t)(off / ((uint64_t)b * b));
   }
   uint64_t rank(uint32_t x, uint32_t y, uint32_t z) const {
+    if (b == kBakerOrder) {
+      const uint64_t P = (uint64_t)gx * gy;
+      return (uint64_t)z * P + inv[(uint64_t)z * P + (uint64_t)x + (uint64_t)gx * y];
+    }
     if (b == 0) return (uint64_t)x + (uint64_t)gx * ((uint64_t)y + (uint64_t)gy * (uint64_t)z);
     const uint32_t nbx = gx / b, nby = gy / b;
     const uint64_t br = (uint64_t)(x / b) + (uint64_t)nbx * ((uint64_t)(y / b) + (uint64_t)nby * (uint64_t)(z / b));
@@ -123,6 +145,26 @@ struct VoxelOrder {
     return br * b * b * b + off;
   }
 };
+
+// The per-plane shuffles of the baker's order (empty for the other orders).
+void baker_tables(const GenSpec* spec, std::vector<uint32_t>& perm, std::vector<uint32_t>& inv) {
+  if (spec->brick != kBakerOrder) return;
+  const uint64_t P = (uint64_t)spec->gx * spec->gy;
+  perm.resize(P * spec->gz);
+  inv.resize(P * spec->gz);
+#pragma omp parallel for schedule(static)
+  for (int64_t z = 0; z < (int64_t)spec->gz; z++) {
+    uint32_t* p = perm.data() + (size_t)z * P;
+    for (uint64_t k = 0; k < P; k++) p[k] = (uint32_t)k;
+    uint64_t h = entity_seed(spec->master_seed ^ 0x706c616e65000007ull, (uint32_t)z);
+    for (uint64_t k = P - 1; k > 0; k--) {
+      h = entity_seed(h, (uint32_t)k);
+      std::swap(p[k], p[h % (k + 1)]);
+    }
+    uint32_t* v = inv.data() + (size_t)z * P;
+    for (uint64_t k = 0; k < P; k++) v[p[k]] = (uint32_t)k;
+  }
+}
 
 // Fills `state` (910*padded_n bytes), `heads` (8*total_axons u32) and
 // optionally `axon_len` (total_axons bytes, 0 = no axon) and `soma_pos`
@@ -136,8 +178,12 @@ int axc_gen_brain(const GenSpec* spec, const GenVariant* lut, uint8_t* state, ui
       spec->w_max > 32767 || spec->w_min > spec->w_max || spec->n_types == 0 || spec->n_types > 16 ||
       spec->gx == 0 || spec->gy == 0 || spec->gz == 0)
     return -1;
-  if (spec->brick && (spec->gx % spec->brick || spec->gy % spec->brick || spec->gz % spec->brick)) return -1;
-  const VoxelOrder vo{spec->gx, spec->gy, spec->gz, spec->brick};
+  if (spec->brick && spec->brick != kBakerOrder &&
+      (spec->gx % spec->brick || spec->gy % spec->brick || spec->gz % spec->brick))
+    return -1;
+  std::vector<uint32_t> bperm, binv;
+  baker_tables(spec, bperm, binv);
+  const VoxelOrder vo{spec->gx, spec->gy, spec->gz, spec->brick, bperm.data(), binv.data()};
   uint32_t pn, A;
   axc_gen_sizes(spec, &pn, &A);
   const uint32_t n = spec->n_neurons, nin = spec->n_inputs;
@@ -268,7 +314,9 @@ int axc_gen_paths(const GenSpec* spec, const uint32_t* soma_pos, const uint8_t* 
   axc_gen_sizes(spec, &pn, &A);
   const uint32_t n = spec->n_neurons, nin = spec->n_inputs;
   const uint64_t V = (uint64_t)spec->gx * spec->gy * spec->gz;
-  const VoxelOrder vo{spec->gx, spec->gy, spec->gz, spec->brick};
+  std::vector<uint32_t> bperm, binv;
+  baker_tables(spec, bperm, binv);
+  const VoxelOrder vo{spec->gx, spec->gy, spec->gz, spec->brick, bperm.data(), binv.data()};
   const uint64_t ms = spec->master_seed ^ 0x7061746800000006ull;
 #pragma omp parallel for schedule(static)
   for (int64_t ai = 0; ai < (int64_t)A; ai++) {

@@ -1633,6 +1633,70 @@ int32_t axc_batch_stats(void* h, uint32_t brain, AxcShardStats* out, uint64_t* t
   return 0;
 }
 
+int32_t axc_batch_update(void* h, const AxcBrainDesc* brains, uint32_t n) {
+  if (!h || !brains) return (int32_t)hipErrorInvalidValue;
+  Batch* b = (Batch*)h;
+  if (n != b->desc.size()) return (int32_t)hipErrorInvalidValue;
+  for (uint32_t k = 0; k < n; k++) {
+    const AxcBrainDesc& d = brains[k];
+    const AxcBrainDesc& o = b->desc[k];
+    if (d.vram.soma_voltage != o.vram.soma_voltage || d.vram.axon_heads != o.vram.axon_heads ||
+        d.padded_n != o.padded_n || d.total_axons != o.total_axons || !d.lut)
+      return (int32_t)hipErrorInvalidValue;
+    if (d.num_virtual_axons && (uint64_t)d.virtual_offset + d.num_virtual_axons > d.total_axons)
+      return (int32_t)hipErrorInvalidValue;
+    if (d.d_input_bitmask && d.input_ticks == 0) return (int32_t)hipErrorInvalidValue;
+  }
+  // The previous steps may still read the tables and descriptors.
+  hipError_t e = hipDeviceSynchronize();
+  std::vector<AxcVariantParameters> luts((size_t)n * 16);
+  for (uint32_t k = 0; k < n && e == hipSuccess; k++) {
+    const AxcBrainDesc& d = brains[k];
+    b->desc[k] = d;
+    std::copy(d.lut, d.lut + 16, luts.begin() + (size_t)k * 16);
+    BrainDev& hd = b->host_dev[k];
+    hd.v_seg = d.v_seg;
+    hd.virt_off = d.virtual_offset;
+    hd.n_virt = d.num_virtual_axons;
+    hd.words = d.input_words;
+    hd.input_ticks = d.input_ticks ? d.input_ticks : 1;
+    hd.dop = d.dopamine;
+    hd.bitmask = d.d_input_bitmask;
+  }
+  if (e == hipSuccess)
+    e = hipMemcpy(b->d_luts, luts.data(), luts.size() * sizeof(AxcVariantParameters), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(b->d_brains, b->host_dev.data(), n * sizeof(BrainDev), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemset(b->d_stats, 0, (size_t)n * kStatsPerBlock * 8);
+  if (e == hipSuccess) e = hipMemset(b->d_type_spikes, 0, (size_t)n * 16 * 8);
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  return (int32_t)e;
+}
+
+int32_t axc_batch_stats_all(void* h, AxcShardStats* out, uint64_t* type_spikes16) {
+  if (!h || !out) return (int32_t)hipErrorInvalidValue;
+  Batch* b = (Batch*)h;
+  const size_t n = b->desc.size();
+  std::vector<uint64_t> v(n * kStatsPerBlock);
+  hipError_t e = hipMemcpy(v.data(), b->d_stats, v.size() * 8, hipMemcpyDeviceToHost);
+  if (e == hipSuccess && type_spikes16) e = hipMemcpy(type_spikes16, b->d_type_spikes, n * 16 * 8, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return (int32_t)e;
+  for (size_t k = 0; k < n; k++) {
+    const uint64_t* q = &v[k * kStatsPerBlock];
+    AxcShardStats& r = out[k];
+    r.integrated = q[kStIntegrated];
+    r.slots_scanned = q[kStScanned];
+    r.targets_read = q[kStTargets];
+    r.hits = q[kStHits];
+    r.spikes = q[kStSpikes];
+    r.gsop_slots = q[kStGsopSlots];
+    r.gsop_targets = q[kStGsopTargets];
+    r.ticks = q[kStTicks];
+    r.gsop_slots_fused = 0;
+    r.gsop_targets_fused = 0;
+  }
+  return 0;
+}
+
 void axc_batch_destroy(void* h) {
   if (!h) return;
   (void)hipDeviceSynchronize();

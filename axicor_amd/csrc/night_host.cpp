@@ -53,10 +53,18 @@ inline uint32_t unpack_axon(uint32_t t) {  // layout.rs:268-271 (saturating -1)
   return a ? a - 1 : 0;
 }
 
+// One candidate segment of the grid, with what scoring it needs of its
+// axon inline: the owner soma (or ~0), the axon's explore noise and its
+// owner's power index at the start of the pass, and whether that owner
+// sprouts in this pass (so its power index may change).
 struct SegRef {
   uint32_t axon_id;
+  uint32_t owner;
+  float noise;
+  float pw;
   uint16_t seg_idx;
   uint8_t type_idx;
+  uint8_t owner_sprouts;
 };
 
 }  // namespace
@@ -113,12 +121,13 @@ int axc_sprout_pass(uint32_t* targets, int16_t* weights, const uint8_t* flags, c
   const size_t ghost_start = N, ghost_end = N + total_ghosts;
   if (ghost_end > A) return -1;
 
-  // 1. Liveness and the axon -> soma map.
+  // 1. Liveness and the axon -> soma map (every writer stores the same 1).
   std::vector<uint8_t> active(A, 0);
-  for (size_t k = 0; k < (size_t)kMaxD * N; k++) {
+#pragma omp parallel for schedule(static)
+  for (int64_t k = 0; k < (int64_t)((size_t)kMaxD * N); k++) {
     if (targets[k] != 0) {
       uint32_t a = unpack_axon(targets[k]);
-      if (a < A) active[a] = 1;
+      if (a < A) __atomic_store_n(&active[a], (uint8_t)1, __ATOMIC_RELAXED);
     }
   }
   std::vector<uint32_t> axon_to_soma(A, 0xFFFFFFFFu);
@@ -193,14 +202,36 @@ int axc_sprout_pass(uint32_t* targets, int16_t* weights, const uint8_t* flags, c
   }
   for (size_t a = ghost_start; a < ghost_end; a++) nudge(a);
 
+  // Power index of every soma (sprouting.rs:32-40), explore noise of every
+  // axon (random_f32 of seed + axon + epoch, sprouting.rs:280-282), and the
+  // somas that sprout candidates (placed, bursting, an empty last slot) in
+  // index order.
+  auto power_index = [&](uint32_t soma) -> float {
+    uint32_t pw = 0;
+    for (uint32_t s = 0; s < kMaxD; s++) {
+      int32_t w = weights[(size_t)s * N + soma];
+      pw += (uint32_t)(w < 0 ? -w : w);
+    }
+    return (float)pw / ((float)kMaxD * 32767.0f);
+  };
+  std::vector<float> power(N);
+#pragma omp parallel for schedule(static)
+  for (int64_t s = 0; s < (int64_t)N; s++) power[s] = power_index((uint32_t)s);
+  std::vector<uint32_t> sprouters;
+  for (size_t i = 0; i < N; i++)
+    if (soma_positions[i] != 0 && (flags[i] & 0x0E) != 0 && targets[(size_t)(kMaxD - 1) * N + i] == 0)
+      sprouters.push_back((uint32_t)i);
+  std::vector<uint8_t> may_change(N, 0);
+  for (uint32_t i : sprouters) may_change[i] = 1;
+
   // 4. Segment grid, cells of 2 voxels, axons in reverse order, dense CSR
   // with the insertion order of the reference's per-cell vectors. Built in
   // parallel: thread t takes the t-th contiguous range of axons in reverse
   // order and places its segments after those of threads t' < t in every
   // cell, which is the sequential insertion order. Only the first segment of
-  // an axon in a cell is kept: the others follow it directly in that cell
-  // and score the same (the score depends on the axon and its type only), so
-  // the scan's strict `>` never picks them.
+  // an axon (and type) in a cell is kept: the axon's other segments there
+  // come after it in that cell and score the same (the score depends on the
+  // axon and its type only), so the scan's strict `>` never picks them.
   const uint32_t cell = 2;
   uint32_t gx = 1, gy = 1, gz = 1;
 #pragma omp parallel for schedule(static) reduction(max : gx, gy, gz)
@@ -222,18 +253,45 @@ int axc_sprout_pass(uint32_t* targets, int16_t* weights, const uint8_t* flags, c
   std::vector<uint32_t> start(ncell + 1, 0);
   std::vector<SegRef> refs;
   {
-    // Per (thread, cell) counts of all segments.
+    // Calls f(segment, cell) for the first segment of axon ai of each (cell,
+    // type), in segment order; `seen` is the caller's table (stamped by axon).
+    constexpr uint32_t kSeen = 1024;
+    struct Seen {
+      uint32_t key[kSeen];
+      uint32_t stamp[kSeen];
+    };
+    auto firsts = [&](size_t ai, Seen& sn, auto&& f) {
+      const uint32_t st = (uint32_t)ai + 1;
+      for (uint32_t s = 0; s < lengths[ai]; s++) {
+        const uint32_t p = paths[ai * 256 + s];
+        if (!p) continue;
+        const size_t c = cid(p);
+        const uint32_t key = (uint32_t)c * 16u + ptype(p);
+        uint32_t h = (key * 2654435761u) >> 22;
+        bool dup = false;
+        while (sn.stamp[h] == st) {
+          if (sn.key[h] == key) {
+            dup = true;
+            break;
+          }
+          h = (h + 1) & (kSeen - 1);
+        }
+        if (dup) continue;
+        sn.stamp[h] = st;
+        sn.key[h] = key;
+        f(s, p, c);
+      }
+    };
+    // Per (thread, cell) counts.
 #pragma omp parallel num_threads(nth)
     {
       const int t = omp_get_thread_num();
       uint32_t* my = cnt.data() + (size_t)t * ncell;
-      for (size_t k = lo_of(t); k < lo_of(t + 1); k++) {
-        const size_t ai = A - 1 - k;
-        for (uint32_t s = 0; s < lengths[ai]; s++) {
-          uint32_t p = paths[ai * 256 + s];
-          if (p) my[cid(p)]++;
-        }
-      }
+      Seen* sn = new Seen;
+      std::fill(sn->stamp, sn->stamp + kSeen, 0u);
+      for (size_t k = lo_of(t); k < lo_of(t + 1); k++)
+        firsts(A - 1 - k, *sn, [&](uint32_t, uint32_t, size_t c) { my[c]++; });
+      delete sn;
     }
     // Exclusive offsets: cell-major, thread order within a cell.
     size_t run = 0;
@@ -246,166 +304,203 @@ int axc_sprout_pass(uint32_t* targets, int16_t* weights, const uint8_t* flags, c
       }
     }
     start[ncell] = (uint32_t)run;
-    std::vector<SegRef> all(run);
+    refs.resize(run);
 #pragma omp parallel num_threads(nth)
     {
       const int t = omp_get_thread_num();
-      uint32_t* pos = cnt.data() + (size_t)t * ncell;
+      uint32_t* posn = cnt.data() + (size_t)t * ncell;
+      Seen* sn = new Seen;
+      std::fill(sn->stamp, sn->stamp + kSeen, 0u);
       for (size_t k = lo_of(t); k < lo_of(t + 1); k++) {
         const size_t ai = A - 1 - k;
-        for (uint32_t s = 0; s < lengths[ai]; s++) {
-          uint32_t p = paths[ai * 256 + s];
-          if (p) all[pos[cid(p)]++] = SegRef{(uint32_t)ai, (uint16_t)s, (uint8_t)ptype(p)};
-        }
+        const uint32_t owner = axon_to_soma[ai];
+        const float nz = types ? random_f32(master_seed + (uint64_t)ai + epoch) : 0.0f;
+        const float pw = owner == 0xFFFFFFFFu ? 1.0f : power[owner];
+        const uint8_t ms = owner == 0xFFFFFFFFu ? 0 : may_change[owner];
+        firsts(ai, *sn, [&](uint32_t s, uint32_t p, size_t c) {
+          refs[posn[c]++] = SegRef{(uint32_t)ai, owner, nz, pw, (uint16_t)s, (uint8_t)ptype(p), ms};
+        });
       }
+      delete sn;
     }
-    // Keep the first of each run of one axon (and type) within a cell.
-    std::vector<uint32_t> kept(ncell + 1, 0);
-#pragma omp parallel for schedule(static)
-    for (int64_t c = 0; c < (int64_t)ncell; c++) {
-      uint32_t k = 0;
-      for (uint32_t r = start[c]; r < start[c + 1]; r++)
-        k += (r == start[c] || all[r].axon_id != all[r - 1].axon_id || all[r].type_idx != all[r - 1].type_idx);
-      kept[c + 1] = k;
-    }
-    for (size_t c = 0; c < ncell; c++) kept[c + 1] += kept[c];
-    refs.resize(kept[ncell]);
-#pragma omp parallel for schedule(static)
-    for (int64_t c = 0; c < (int64_t)ncell; c++) {
-      uint32_t o = kept[c];
-      for (uint32_t r = start[c]; r < start[c + 1]; r++)
-        if (r == start[c] || all[r].axon_id != all[r - 1].axon_id || all[r].type_idx != all[r - 1].type_idx)
-          refs[o++] = all[r];
-    }
-    start.swap(kept);
   }
 
-  // 5. Synaptogenesis. Three exact rewrites of the reference loop keep its
-  // results bit for bit:
-  //  - the power index of every soma (sprouting.rs:32-40) is kept in an
-  //    array and recomputed when that soma sprouts (its weights change only
-  //    then; the reference computes it at first use, from the same weights);
-  //  - random_f32 of each axon for this epoch (the explore term) is computed
-  //    once per axon instead of once per candidate segment;
-  //  - a soma's presynaptic axons are marked in an array (mark[a] == i + 1)
-  //    instead of a hash set.
-  // The candidate scan of one empty slot (the 5x5x5 cells in z, y, x order,
-  // each cell's segments in grid order) is split over threads by cells; each
-  // keeps its first best score, and the merge takes the highest score, ties
-  // to the earliest position: the strict `>` of the sequential scan.
-  auto power_index = [&](uint32_t soma) -> float {
-    uint32_t pw = 0;
-    for (uint32_t s = 0; s < kMaxD; s++) {
-      int32_t w = weights[(size_t)s * N + soma];
-      pw += (uint32_t)(w < 0 ? -w : w);
-    }
-    return (float)pw / ((float)kMaxD * 32767.0f);
-  };
-  std::vector<float> power(N);
-#pragma omp parallel for schedule(static)
-  for (int64_t s = 0; s < (int64_t)N; s++) power[s] = power_index((uint32_t)s);
-  std::vector<float> noise(types ? A : 0);
-  if (types) {
-#pragma omp parallel for schedule(static)
-    for (int64_t a = 0; a < (int64_t)A; a++) noise[a] = random_f32(master_seed + (uint64_t)a + epoch);
-  }
-  std::vector<uint32_t> mark(A, 0);
-  struct Best {  // a thread's first best candidate
-    float score;
+  // 5. Synaptogenesis. The reference visits the somas in index order, and
+  // for each empty slot scans the candidate segments of the 5x5x5 cells
+  // around the soma (z, y, x ascending, each cell's segments in grid order)
+  // keeping the first best score (strict `>`). A soma's choice depends on the
+  // other somas only through the power index of candidate owners
+  // (sprouting.rs:285-290), which changes when an owner sprouts earlier in
+  // the order (its weights gain the new synapses; scores only rise with
+  // power). Everything else it reads is its own. So the pass runs in two
+  // phases with the reference's results bit for bit:
+  //  A. every sprouting soma (in parallel, one scan each) keeps the best
+  //     K = max(1, max_sprouts) distinct axons among its candidates whose
+  //     owner cannot change before it (no owner, a later owner, or an owner
+  //     that does not sprout), ranked by (score, scan position), and lists
+  //     the other candidates (an earlier owner that sprouts) with the part of
+  //     their score before the power term;
+  //  B. in index order, each empty slot takes the best of the unpicked kept
+  //     candidates and the listed ones rescored with the owners' current
+  //     power index; after its sprouts the soma's power index is updated.
+  // The soma's present axons are found in a small hash set.
+  const uint32_t K = max_sprouts > 0 ? max_sprouts : 1u;
+  struct Cand {
+    float score;   // full score (kept candidates) or the part before the power term (listed ones)
+    uint32_t pos;  // scan position: earlier wins a tie
     SegRef ref;
-    bool found;
   };
-  uint32_t new_syn = 0;
-  std::vector<size_t> cells;
-  cells.reserve(125);
-  for (size_t i = 0; i < N; i++) {
-    const uint32_t my_pos = soma_positions[i];
-    if (my_pos == 0) continue;
-    if ((flags[i] & 0x0E) == 0) continue;
-    const uint32_t my_type = ptype(my_pos);
-    const AxcSproutType* cfg = (types && my_type < n_types) ? &types[my_type] : nullptr;
-    const uint32_t stamp = (uint32_t)i + 1;
-    for (uint32_t s = 0; s < kMaxD; s++) {
-      uint32_t t = targets[(size_t)s * N + i];
-      if (t) {
+  auto better = [](float s1, uint32_t p1, float s2, uint32_t p2) { return s1 > s2 || (s1 == s2 && p1 < p2); };
+  const size_t n_spr = sprouters.size();
+  std::vector<Cand> kept((size_t)n_spr * K);
+  std::vector<uint32_t> n_kept(n_spr, 0);
+  std::vector<std::vector<Cand>> listed(n_spr);
+
+#pragma omp parallel
+  {
+    constexpr uint32_t kHash = 512;  // the soma's present axons (<= 128)
+    uint32_t hset[kHash];
+    std::vector<size_t> cells;
+    cells.reserve(125);
+#pragma omp for schedule(dynamic, 16)
+    for (int64_t q = 0; q < (int64_t)n_spr; q++) {
+      const uint32_t i = sprouters[(size_t)q];
+      const uint32_t my_pos = soma_positions[i];
+      const uint32_t my_type = ptype(my_pos);
+      const AxcSproutType* cfg = (types && my_type < n_types) ? &types[my_type] : nullptr;
+      for (uint32_t h = 0; h < kHash; h++) hset[h] = 0xFFFFFFFFu;
+      for (uint32_t s = 0; s < kMaxD; s++) {
+        const uint32_t t = targets[(size_t)s * N + i];
+        if (!t) continue;
         const uint32_t a = unpack_axon(t);
-        if (a < A) mark[a] = stamp;  // axons >= A match no segment of the grid
+        uint32_t h = (a * 2654435761u) >> 23;
+        while (hset[h] != 0xFFFFFFFFu && hset[h] != a) h = (h + 1) & (kHash - 1);
+        hset[h] = a;
       }
-    }
-    const uint32_t my_axon = soma_to_axon[i];
-    const int32_t cx = (int32_t)(px(my_pos) / cell), cy = (int32_t)(py(my_pos) / cell),
-                  cz = (int32_t)(pz(my_pos) / cell);
-    cells.clear();
-    for (int32_t z = cz - 2; z <= cz + 2; z++) {
-      if (z < 0 || z >= (int32_t)gz) continue;
-      for (int32_t y = cy - 2; y <= cy + 2; y++) {
-        if (y < 0 || y >= (int32_t)gy) continue;
-        for (int32_t x = cx - 2; x <= cx + 2; x++) {
-          if (x < 0 || x >= (int32_t)gx) continue;
-          cells.push_back(((size_t)z * gy + (size_t)y) * gx + (size_t)x);
+      auto present = [&](uint32_t a) {
+        uint32_t h = (a * 2654435761u) >> 23;
+        while (hset[h] != 0xFFFFFFFFu) {
+          if (hset[h] == a) return true;
+          h = (h + 1) & (kHash - 1);
+        }
+        return false;
+      };
+      const uint32_t my_axon = soma_to_axon[i];
+      const int32_t cx = (int32_t)(px(my_pos) / cell), cy = (int32_t)(py(my_pos) / cell),
+                    cz = (int32_t)(pz(my_pos) / cell);
+      cells.clear();
+      for (int32_t z = cz - 2; z <= cz + 2; z++) {
+        if (z < 0 || z >= (int32_t)gz) continue;
+        for (int32_t y = cy - 2; y <= cy + 2; y++) {
+          if (y < 0 || y >= (int32_t)gy) continue;
+          for (int32_t x = cx - 2; x <= cx + 2; x++) {
+            if (x < 0 || x >= (int32_t)gx) continue;
+            cells.push_back(((size_t)z * gy + (size_t)y) * gx + (size_t)x);
+          }
         }
       }
+      Cand* top = &kept[(size_t)q * K];
+      uint32_t nt = 0;
+      std::vector<Cand>& lst = listed[(size_t)q];
+      uint32_t pos = 0;
+      for (size_t c : cells) {
+        for (uint32_t r = start[c]; r < start[c + 1]; r++, pos++) {
+          const SegRef& sr = refs[r];
+          const uint32_t a = sr.axon_id;
+          float score = 1.0f;
+          if (cfg) {
+            // Left to right, products first, no fused multiply-add (the
+            // library is built with -ffp-contract=off).
+            const float same = (my_type == sr.type_idx) ? 1.0f : 0.0f;
+            float acc = cfg->w_distance * 1.0f;
+            acc = acc + cfg->w_explore * sr.noise;
+            acc = acc + cfg->w_type * same;
+            if (sr.owner_sprouts && sr.owner < i) {  // rescored in phase B
+              if (a != my_axon && !present(a)) lst.push_back(Cand{acc, pos, sr});
+              continue;
+            }
+            acc = acc + cfg->w_power * sr.pw;
+            score = acc;
+          }
+          // The scan starts from a best score of -1: lower (or NaN) never wins;
+          // nor does a reference that cannot enter a full list (an axon
+          // already listed has a score at least the last entry's).
+          if (!(score > -1.0f)) continue;
+          if (nt == K && !better(score, pos, top[K - 1].score, top[K - 1].pos)) continue;
+          if (a == my_axon || present(a)) continue;
+          // The axon's best reference so far replaces its entry; a new axon
+          // enters if the list has room or it beats the last entry.
+          uint32_t k = 0;
+          while (k < nt && top[k].ref.axon_id != a) k++;
+          if (k < nt) {
+            if (!better(score, pos, top[k].score, top[k].pos)) continue;
+          } else if (nt < K) {
+            k = nt++;
+          } else {
+            if (!better(score, pos, top[K - 1].score, top[K - 1].pos)) continue;
+            k = K - 1;
+          }
+          top[k] = Cand{score, pos, sr};
+          while (k > 0 && better(top[k].score, top[k].pos, top[k - 1].score, top[k - 1].pos)) {
+            std::swap(top[k], top[k - 1]);
+            k--;
+          }
+        }
+      }
+      n_kept[(size_t)q] = nt;
     }
-    const int ncells = (int)cells.size();
+  }
+
+  uint32_t new_syn = 0;
+  std::vector<uint32_t> picked;
+  for (size_t q = 0; q < n_spr; q++) {
+    const uint32_t i = sprouters[q];
+    const uint32_t my_type = ptype(soma_positions[i]);
+    const AxcSproutType* cfg = (types && my_type < n_types) ? &types[my_type] : nullptr;
+    const Cand* top = &kept[q * K];
+    const std::vector<Cand>& lst = listed[q];
+    picked.clear();
+    auto is_picked = [&](uint32_t a) { return std::find(picked.begin(), picked.end(), a) != picked.end(); };
     uint32_t sprouts = 0;
     for (int32_t slot = (int32_t)kMaxD - 1; slot >= 0; slot--) {
       const size_t col = (size_t)slot * N + i;
       if (targets[col] != 0) break;
-      std::vector<Best> part;
-#pragma omp parallel
-      {
-#pragma omp single
-        part.assign((size_t)omp_get_num_threads(), Best{-1.0f, SegRef{}, false});
-        Best b{-1.0f, SegRef{}, false};
-        // Contiguous ranges of cells per thread, in thread order.
-#pragma omp for schedule(static)
-        for (int k = 0; k < ncells; k++) {
-          const size_t c = cells[(size_t)k];
-          for (uint32_t r = start[c]; r < start[c + 1]; r++) {
-            const SegRef& sr = refs[r];
-            const uint32_t a = sr.axon_id;
-            if (a == my_axon || mark[a] == stamp) continue;
-            float score = 1.0f;
-            if (cfg) {
-              const uint32_t owner = axon_to_soma[a];
-              const float pw = owner == 0xFFFFFFFFu ? 1.0f : power[owner];
-              const float same = (my_type == sr.type_idx) ? 1.0f : 0.0f;
-              // Left to right, products first, no fused multiply-add
-              // (the library is built with -ffp-contract=off).
-              float acc = cfg->w_distance * 1.0f;
-              acc = acc + cfg->w_explore * noise[a];
-              acc = acc + cfg->w_type * same;
-              acc = acc + cfg->w_power * pw;
-              score = acc;
-            }
-            if (score > b.score) b = Best{score, sr, true};
-          }
-        }
-        part[(size_t)omp_get_thread_num()] = b;
+      const Cand* best = nullptr;
+      float best_score = -1.0f;
+      for (uint32_t k = 0; k < n_kept[q]; k++) {
+        if (is_picked(top[k].ref.axon_id)) continue;
+        best = &top[k];  // the kept list is ranked: its first unpicked entry is its best
+        best_score = top[k].score;
+        break;
       }
-      // Threads hold increasing ranges of cells: the first strictly better
-      // one in thread order is the sequential scan's choice.
-      Best best{-1.0f, SegRef{}, false};
-      for (const Best& b : part)
-        if (b.found && b.score > best.score) best = b;
-      if (!best.found) break;
-      targets[col] = ((uint32_t)best.ref.seg_idx << 24) | ((best.ref.axon_id + 1u) & 0x00FFFFFFu);
+      for (const Cand& c : lst) {
+        if (is_picked(c.ref.axon_id)) continue;
+        const float sc = c.score + cfg->w_power * power[c.ref.owner];
+        if (!(sc > -1.0f)) continue;
+        if (!best || better(sc, c.pos, best_score, best->pos)) {
+          best = &c;
+          best_score = sc;
+        }
+      }
+      if (!best) break;
+      const SegRef br = best->ref;
+      targets[col] = ((uint32_t)br.seg_idx << 24) | ((br.axon_id + 1u) & 0x00FFFFFFu);
       bool inhib = false;
       int16_t w0 = 74;
       if (types) {
-        if (best.ref.type_idx < n_types) {
-          inhib = types[best.ref.type_idx].is_inhibitory != 0;
-          w0 = (int16_t)types[best.ref.type_idx].initial_synapse_weight;
+        if (br.type_idx < n_types) {
+          inhib = types[br.type_idx].is_inhibitory != 0;
+          w0 = (int16_t)types[br.type_idx].initial_synapse_weight;
         }
       }
       weights[col] = inhib ? (int16_t)-w0 : w0;
-      mark[best.ref.axon_id] = stamp;
-      active[best.ref.axon_id] = 1;
-      power[i] = power_index((uint32_t)i);  // this soma's weights changed
+      picked.push_back(br.axon_id);
+      active[br.axon_id] = 1;
       new_syn++;
       sprouts++;
       if (sprouts >= max_sprouts) break;
     }
+    if (sprouts) power[i] = power_index(i);  // this soma's weights changed
   }
 
   // 6. Orphan ghost axons.

@@ -67,6 +67,21 @@ class DeviceBuffer:
             pass
 
 
+_hip = None
+
+
+def device_copy(dst: int, src: int, nbytes: int) -> None:
+    """Synchronous device-to-device copy (hipMemcpy of the HIP runtime that
+    libaxicor_day.so links)."""
+    global _hip
+    if _hip is None:
+        day_lib()  # loads the runtime first
+        _hip = C.CDLL("libamdhip64.so.7", mode=C.RTLD_GLOBAL)
+        _hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        _hip.hipMemcpy.restype = C.c_int
+    _check(_hip.hipMemcpy(dst, src, nbytes, 3), "hipMemcpy device to device")
+
+
 def upload_constant_memory(lut: np.ndarray) -> None:
     """cu_upload_constant_memory (physics.cu:504-506): the 16-entry table."""
     lut = np.ascontiguousarray(lut, dtype=VARIANT_DTYPE)

@@ -11,8 +11,10 @@ computed from the run's activity (``score_of``).
 * ``Simulator`` runs many (parameter vector, input condition) trials as one
   batch of brains (axc_batch_*). Each trial starts from the same generated
   topology state and returns the eight outcomes of OUTCOME_NAMES.
-* ``plain_search``: uniform random proposals, every one simulated. This is
-  the method without the surrogate.
+* ``tpe_search``: the reference's own search method (Optuna's default TPE
+  sampler, restated in tpe.py), every proposal simulated. This is the plain
+  search the surrogate is measured against.
+* ``plain_search``: uniform random proposals, every one simulated (for scale).
 * ``surrogate_search``: the MLP (surrogate.Mlp, hand-written MFMA kernels)
   scores a large pool of random candidates. Only the top k go to the
   simulator, which judges every proposal. New judgements are added to the
@@ -20,6 +22,7 @@ computed from the run's activity (``score_of``).
 """
 from __future__ import annotations
 
+import ctypes as C
 import time
 from dataclasses import dataclass, field
 
@@ -28,7 +31,7 @@ import numpy as np
 from . import brain as B
 from ._ffi import day_lib
 from .batch import BatchEngine
-from .engine import DeviceBuffer, VramState
+from .engine import DeviceBuffer, VramState, device_copy
 from .layout import compute_state_offsets
 
 INPUT_FEATURES = ["input_rate", "input_modulation", "input_period_kticks", "input_driven_fraction"]
@@ -123,9 +126,11 @@ class _Ptr:
 
 
 class Simulator:
-    """Runs trials on the batched engine: topologies generated once, a pool
-    of brain slots on the device, input streams generated once per
-    condition."""
+    """Runs trials on the batched engine: topologies generated once and kept
+    on the device, a pool of brain slots with one batch object per batch
+    size (reused across trials: each trial copies its topology in on the
+    device and gives the batch its table, dopamine and input stream), input
+    streams generated once per condition."""
 
     def __init__(self, wl: Workload, conditions: list[Condition] | None = None, stream=None):
         self.wl = wl
@@ -137,18 +142,23 @@ class Simulator:
         off = compute_state_offsets(self.N)
         self.off = off
         self.types = [(b.state[off.soma_flags:off.soma_flags + self.N] >> 4).astype(np.int64) for b in self.topos]
+        self.n_types = [(max(1, int(np.count_nonzero(t == 0))), max(1, int(np.count_nonzero(t == 1))))
+                        for t in self.types]
         self.syn0 = []   # (mean |w|, std |w|) over the filled slots of each topology's initial state
         for b in self.topos:
             w = b.state[off.dendrite_weights:off.dendrite_weights + 2 * 128 * self.N].view(np.int16)
             t = b.state[off.dendrite_targets:off.dendrite_targets + 4 * 128 * self.N].view(np.uint32)
             aw = np.abs(w.astype(np.int64))[t != 0]
             self.syn0.append((float(aw.mean()), float(aw.std())))
+        self.topo_dev = [(DeviceBuffer.from_array(b.state), DeviceBuffer.from_array(b.heads)) for b in self.topos]
         rng = np.random.default_rng(wl.seed)
         self.conditions = conditions if conditions is not None else [random_condition(rng)
                                                                       for _ in range(wl.n_conditions)]
         self.cond_bufs = [DeviceBuffer.from_array(c.bits(wl.n_inputs, wl.ticks, wl.seed + 7 * j))
                           for j, c in enumerate(self.conditions)]
         self.pool = [VramState(self.N, self.A) for _ in range(wl.batch_brains)]
+        self.slot_topo = [-1] * wl.batch_brains
+        self.engines = {}
         self.stream = stream
         self.tuning = None
         self.gen_seconds = time.perf_counter() - t0
@@ -170,12 +180,18 @@ class Simulator:
         return np.concatenate([params, cf], axis=1)
 
     def _members(self, params, cond_ids, topo_ids):
+        """Copies each trial's topology into its pool slot (on the device) and
+        returns the batch members."""
         mems = []
         for k, (p, cj, tj) in enumerate(zip(params, cond_ids, topo_ids)):
             b = self.topos[int(tj)]
             v = self.pool[k]
-            v.upload_state(b.state)
-            v.upload_axon_heads(b.heads)
+            st, hd = self.topo_dev[int(tj)]
+            device_copy(v.ptrs.soma_voltage, st.ptr, st.nbytes)
+            device_copy(v.ptrs.axon_heads, hd.ptr, hd.nbytes)
+            if self.slot_topo[k] != int(tj):  # another topology's targets: the slot's plan is rebuilt
+                self.lib.axc_shard_targets_changed(C.byref(v.ptrs))
+                self.slot_topo[k] = int(tj)
             mems.append({"vram": v, "lut": B.lut_from_params(p), "d_inputs": _Ptr(self.cond_bufs[int(cj)].ptr),
                          "input_words": (self.wl.n_inputs + 31) // 32, "input_ticks": self.wl.ticks, "v_seg": 1,
                          "virtual_offset": b.virtual_offset, "n_inputs": self.wl.n_inputs,
@@ -197,45 +213,56 @@ class Simulator:
         self.trials += n
         return out
 
-    def _run_batch(self, params, cond_ids, topo_ids) -> np.ndarray:
-        wl, lib = self.wl, self.lib
-        t0 = time.perf_counter()
-        mems = self._members(params, cond_ids, topo_ids)
-        be = BatchEngine(mems)
-        t1 = time.perf_counter()
-        try:
+    def _engine(self, mems):
+        n = len(mems)
+        be = self.engines.get(n)
+        if be is None:
+            be = BatchEngine(mems)
             if self.tuning is None:
                 self.tuning = be.autotune(8, self.stream)
             else:
                 be.set_tuning(**self.tuning)
-            half = wl.ticks // 2
-            mid = []
-            for t in range(0, wl.ticks, wl.graph_ticks):
-                if t == half:
-                    lib.gpu_stream_synchronize(self.stream)
-                    mid = [be.stats(k) for k in range(len(mems))]
-                be.step(t, min(wl.graph_ticks, wl.ticks - t), self.stream)
-            lib.gpu_stream_synchronize(self.stream)
-            end = [be.stats(k) for k in range(len(mems))]
-            t2 = time.perf_counter()
-            syn = be.synapse_stats(params[:, 5], self.stream)
-        finally:
-            be.destroy()
-        res = np.stack([self._outcome(params[k], topo_ids[k], mid[k], end[k], syn[k]) for k in range(len(mems))])
+            self.engines[n] = be
+            be.update(mems)  # counters from zero
+        else:
+            be.update(mems)
+        return be
+
+    def _run_batch(self, params, cond_ids, topo_ids) -> np.ndarray:
+        wl, lib = self.wl, self.lib
+        t0 = time.perf_counter()
+        mems = self._members(params, cond_ids, topo_ids)
+        be = self._engine(mems)
+        t1 = time.perf_counter()
+        half = wl.ticks // 2
+        mid = None
+        for t in range(0, wl.ticks, wl.graph_ticks):
+            if t == half:
+                lib.gpu_stream_synchronize(self.stream)
+                mid = be.stats_all()
+            be.step(t, min(wl.graph_ticks, wl.ticks - t), self.stream)
+        lib.gpu_stream_synchronize(self.stream)
+        end = be.stats_all()
+        if mid is None:
+            mid = end
+        t2 = time.perf_counter()
+        syn = be.synapse_stats(params[:, 5], self.stream)
+        res = np.stack([self._outcome(params[k], topo_ids[k], mid[1][k], end[1][k], syn[k])
+                        for k in range(len(mems))])
         ps = self.phase_seconds
         ps["upload"] += t1 - t0
         ps["step"] += t2 - t1
         ps["outcomes"] += time.perf_counter() - t2
         return res
 
-    def _outcome(self, p, tj, mid, end, syn) -> np.ndarray:
+    def _outcome(self, p, tj, ts_mid, ts_end, syn) -> np.ndarray:
+        """Outcomes of one trial from its spikes per type at half time and at
+        the end, and its synapse statistics."""
         wl, N = self.wl, self.N
-        types = self.types[int(tj)]
         s0 = self.syn0[int(tj)]
         half = wl.ticks // 2
-        ts_mid, ts_end = mid[1].astype(np.float64), end[1].astype(np.float64)
-        n_exc = max(1, int(np.count_nonzero(types == 0)))
-        n_inh = max(1, int(np.count_nonzero(types == 1)))
+        ts_mid, ts_end = ts_mid.astype(np.float64), ts_end.astype(np.float64)
+        n_exc, n_inh = self.n_types[int(tj)]
         rate_exc = 100.0 * ts_end[0] / (n_exc * wl.ticks)
         rate_inh = 100.0 * ts_end[1] / (n_inh * wl.ticks)
         r1 = 100.0 * ts_mid.sum() / (N * half)
@@ -251,10 +278,16 @@ class Simulator:
         return np.array([rate_exc, rate_inh, r1, r2, lw, pruned, thr_off, ls], np.float64)
 
     def free(self) -> None:
+        for be in self.engines.values():
+            be.destroy()
+        self.engines = {}
         for v in self.pool:
             v.free()
         for b in self.cond_bufs:
             b.free()
+        for st, hd in self.topo_dev:
+            st.free()
+            hd.free()
 
 
 # ---------------------------------------------------------------------------
@@ -292,6 +325,27 @@ def plain_search(sim: Simulator, n_trials: int, cond_id: int, rng: np.random.Gen
         p = B.random_params(rng, min(bb, n_trials - s))
         out = sim.run(p, cond_id)
         h.add(p, score_of(out), time.perf_counter() - t0)
+    return h
+
+
+def tpe_search(sim: Simulator, n_trials: int, cond_id: int, seed: int, batch: int | None = None) -> History:
+    """The reference's search method (examples/bayesian_search.py:182-187:
+    an Optuna study with its default TPE sampler), restated in tpe.py: each
+    batch of proposals is drawn from the Parzen estimators of all trials
+    judged so far; the sampler's own time counts in the wall-clock."""
+    from .tpe import TPE
+    h = History()
+    t0 = time.perf_counter()
+    bb = batch or sim.wl.batch_brains
+    opt = TPE(B.PARAM_RANGES, seed=seed)
+    h.sampler_seconds = 0.0
+    for s in range(0, n_trials, bb):
+        ts = time.perf_counter()
+        p = opt.suggest(min(bb, n_trials - s))
+        h.sampler_seconds += time.perf_counter() - ts
+        sc = score_of(sim.run(p, cond_id))
+        opt.tell(p, sc)
+        h.add(p, sc, time.perf_counter() - t0)
     return h
 
 
@@ -423,17 +477,21 @@ def run_experiment(wl: Workload, n_data: int, n_plain: int, rounds: int, candida
     flops = 2.0 * (16 * 512 + 512 * 512 * 2 + 512 * 8) * candidates
     log(f"scoring {candidates} candidates: {t_fwd * 1e3:.2f} ms, {flops / t_fwd / 1e12:.1f} TFLOP/s")
 
-    # 3. Plain search.
-    hp = plain_search(sim, n_plain, cond, rng)
+    # 3. The plain search: TPE, the reference's method; uniform random
+    # proposals beside it for scale.
+    hp = tpe_search(sim, n_plain, cond, seed + 1)
     best_plain = hp.best()
     t_plain = hp.time_to(best_plain)
-    log(f"plain search: {n_plain} trials, best {best_plain:.4f} reached at {t_plain:.1f}s "
+    log(f"plain search (TPE): {n_plain} trials, best {best_plain:.4f} reached at {t_plain:.1f}s "
         f"(of {hp.times[-1]:.1f}s)")
+    hr = plain_search(sim, n_plain, cond, rng)
+    log(f"random search: {n_plain} trials, best {hr.best():.4f} reached at {hr.time_to(hr.best()):.1f}s")
 
     # 4. Surrogate-guided search.
     hs, info = surrogate_search(sim, net, xs, ys, cond, rng, rounds, candidates, top_k, Xtr, Ytr)
     t_surr = hs.time_to(best_plain)
-    log(f"surrogate search: {len(hs.scores)} trials, best {hs.best():.4f}; plain best reached at "
+    t_surr_random = hs.time_to(hr.best())
+    log(f"surrogate search: {len(hs.scores)} trials, best {hs.best():.4f}; TPE best reached at "
         f"{'never' if t_surr is None else f'{t_surr:.1f}s'}")
     predicted, simulated = info.pop("predicted"), info.pop("simulated")
     mae_judged = [np.abs(p - o).mean(axis=0) for p, o in zip(predicted, simulated)]
@@ -455,16 +513,22 @@ def run_experiment(wl: Workload, n_data: int, n_plain: int, rounds: int, candida
         "holdout_score_rank_corr": rank_corr,
         "judged_mae_per_round": [dict(zip(OUTCOME_NAMES, m.tolist())) for m in mae_judged],
         "scoring": {"candidates": candidates, "ms": t_fwd * 1e3, "tflops": flops / t_fwd / 1e12},
-        "plain": {"trials": n_plain, "best": best_plain, "seconds_to_best": t_plain, "seconds": hp.times[-1],
-                  "best_after": {str(n): max(hp.scores[:n]) for n in (256, 512, 1024, 2048, 4096) if n <= n_plain}},
+        "plain": {"method": "TPE (tpe.py; the reference's Optuna default sampler)", "trials": n_plain,
+                  "best": best_plain, "seconds_to_best": t_plain, "seconds": hp.times[-1],
+                  "sampler_seconds": hp.sampler_seconds,
+                  "best_after": {str(n): max(hp.scores[:n]) for n in (256, 512, 1024, 2048, 4096, 8192)
+                                 if n <= n_plain}},
+        "random": {"trials": n_plain, "best": hr.best(), "seconds_to_best": hr.time_to(hr.best()),
+                   "seconds": hr.times[-1], "mean_score": float(np.mean(hr.scores))},
         "surrogate": {"rounds": rounds, "top_k": top_k, "trials": len(hs.scores), "best": hs.best(),
                       "seconds_to_plain_best": t_surr, "seconds": hs.times[-1],
                       "round_best": [max(hs.scores[r * top_k:(r + 1) * top_k]) for r in range(rounds)],
                       "round_mean": [float(np.mean(hs.scores[r * top_k:(r + 1) * top_k])) for r in range(rounds)],
                       **info},
-        "plain_random_mean_score": float(np.mean(hp.scores)),
+        "tpe_mean_score": float(np.mean(hp.scores)),
         "speedup_search_only": (t_plain / t_surr) if t_surr else None,
         "speedup_including_data_and_training": (t_plain / (t_surr + t_data + t_train)) if t_surr else None,
+        "speedup_vs_random_search_only": (hr.time_to(hr.best()) / t_surr_random) if t_surr_random else None,
         "first_round_params": np.array(first[:4]).tolist(),
     }
     return rep

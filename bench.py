@@ -31,6 +31,12 @@ METRIC = "synaptic events/s at 1.5M neurons (% HBM peak); batched vs serial brai
 HBM_PEAK_GBS = 8000.0
 
 
+def _numbering(brick: int) -> str:
+    if brick == -1:
+        return "the baker's (z-planes, shuffled within each plane)"
+    return "bricks of %d^3 voxels" % brick if brick else "x-fastest"
+
+
 def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -263,8 +269,9 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--ticks", type=int, default=50, help="ticks per step (one BSP batch)")
-    ap.add_argument("--workload", default="c2", choices=["c2", "c1", "c2-linear", "c5", "c5-sub"],
+    ap.add_argument("--workload", default="c2", choices=["c2", "c1", "c2-linear", "c2-baker", "c5", "c5-sub"],
                     help="c2-linear: the c2 brain with somas numbered x-fastest instead of in bricks; "
+                         "c2-baker: numbered as genesis-baker numbers a shard (z-planes, shuffled in each); "
                          "c5: the 12M-neuron x 128-slot scale point; c5-sub: its 200k-neuron sub-shard")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--cpu-max-ticks", type=int, default=20)
@@ -302,7 +309,8 @@ def main() -> int:
                                                    serial=not args.batched_no_serial)}), flush=True)
         return 0
 
-    spec = {"c2": B.spec_c2, "c1": B.spec_c1, "c2-linear": lambda: B.spec_c2(brick=0), "c5": B.spec_c5,
+    spec = {"c2": B.spec_c2, "c1": B.spec_c1, "c2-linear": lambda: B.spec_c2(brick=0),
+            "c2-baker": lambda: B.spec_c2(brick=B.BAKER_ORDER), "c5": B.spec_c5,
             "c5-sub": lambda: B.spec_c5(200_000)}[args.workload]()
     t0 = time.perf_counter()
     brain = B.generate(spec, with_paths=args.nights > 0)
@@ -466,7 +474,7 @@ def main() -> int:
             "workload": (f"{args.workload}: {spec.n_neurons} GLIF neurons x {spec.slots_filled} slots, "
                          f"{spec.n_inputs} inputs at {spec.input_ppm / 1e6:g}/tick, axons "
                          f"{spec.axon_len[0]}-{spec.axon_len[1]} segments, GSOP on, "
-                         f"soma numbering {'bricks of %d^3 voxels' % spec.brick if spec.brick else 'x-fastest'}"),
+                         f"soma numbering {_numbering(spec.brick)}"),
             "padded_n": N, "total_axons": A, "ticks_per_step": T, "parallelism": "single GPU",
         },
         "roofline": {

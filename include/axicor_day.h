@@ -405,6 +405,16 @@ typedef struct AxcSynapseStats {
     int64_t sum_threshold_offset;
 } AxcSynapseStats;
 int32_t axc_batch_synapse_stats(void *batch, const int16_t *prune_thresholds, AxcSynapseStats *out, void *stream);
+/* New per-brain tables, dopamine and inputs for the same brains (each
+ * brains[k].vram, padded_n and total_axons must be those given at create),
+ * and counters reset to zero: a batch object reused across trials without
+ * reallocating its workspaces. A brain whose targets changed (by the
+ * library's uploads or axc_shard_targets_changed) gets its plan rebuilt at
+ * the next step. */
+int32_t axc_batch_update(void *batch, const AxcBrainDesc *brains, uint32_t n_brains);
+/* axc_batch_stats of every brain in one call: out[n_brains],
+ * type_spikes16 [n_brains][16] (may be NULL). */
+int32_t axc_batch_stats_all(void *batch, AxcShardStats *out, uint64_t *type_spikes16);
 void axc_batch_destroy(void *batch);
 
 /* Library/version string. */

@@ -177,3 +177,20 @@ def test_graph_heads_near_the_wrap_and_sentinel(gpu):
     for k in range(4):
         p.step(12, dopamine=400, v_seg=3, use_graph=True)
         p.check(f"wrap heads batch {k}")
+
+
+@GRAPH
+def test_baker_numbering(gpu, use_graph):
+    """Somas numbered as genesis-baker numbers a shard (neuron_placement.rs
+    :60-87: z-planes in order, each plane's somas shuffled): 60k neurons x
+    100 slots on a 48x48x32 grid, 1.5M-neuron brain's four types, against
+    the oracle."""
+    spec = B.spec_small(60000, 100, 3000, seed=41, axon_len=(1, 255), grid=(48, 48, 32), lut=B.lut_c2(),
+                        type_ppm=[400_000, 400_000, 100_000, 100_000])
+    spec.brick = B.BAKER_ORDER
+    spec.input_ppm = 20_000
+    p = Pair(B.generate(spec), n_out=64)
+    for k in range(3):
+        out_ref, _ = p.step(20, dopamine=(60 if k % 2 else -40), use_graph=use_graph)
+        np.testing.assert_array_equal(p.outputs(20), out_ref)
+        p.check(f"baker numbering batch {k}")

@@ -182,3 +182,20 @@ def test_sprout_same_result_for_any_thread_count():
         outs.append(r.stdout.strip())
     assert int(outs[0].split()[0]) > 0
     assert outs[0] == outs[1] == outs[2], outs
+
+
+def test_sprout_matches_oracle_over_seeds_and_settings():
+    """Random scenes (a third of the somas bursting, empty tails) with the
+    power term on: the host pass, whose candidates with an earlier bursting
+    owner are rescored in index order, equals the oracle's sequential scan
+    for max_sprouts 0 (one sprout), 1, 3 and 7, with negative and zero
+    score weights among the settings."""
+    settings = [dict(w_distance=0.5, w_power=0.4, w_explore=0.1, w_type=0.0),
+                dict(w_distance=0.0, w_power=1.0, w_explore=0.0, w_type=0.0),
+                dict(w_distance=0.1, w_power=-0.7, w_explore=0.3, w_type=0.2),
+                dict(w_distance=-2.0, w_power=3.0, w_explore=0.5, w_type=-0.5)]
+    for seed in range(3):
+        b = small_brain(40 + seed, n=240, slots=24)
+        for k, st in enumerate(settings):
+            types = NT.sprout_types_from_lut(b.lut, **st)
+            run_both(b, types, epoch=seed * 7 + k, max_sprouts=(0, 1, 3, 7)[(seed + k) % 4])
