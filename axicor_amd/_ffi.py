@@ -141,6 +141,7 @@ SIGNATURES = {
     "axc_staging_enable": (None, [C.c_int]),
     "axc_shard_targets_changed": (None, [C.POINTER(ShardVramPtrs)]),
     "axc_shard_graph_captures": (_I32, [_VP, C.POINTER(C.c_uint64)]),
+    "axc_shard_set_soma_positions": (_I32, [_VP, _P, _U32]),
     "axc_dma_h2d_schedule": (_I32, [_P, _P, _U32, _P, _U32, _P]),
     "axc_step_day_phase_noalloc": (_I32, [_VP, _U32, _U32, _U32, _U32, _P, _U32, _U32, _P, _U32, _P, _P, _U32, _I16,
                                           _P]),

@@ -281,7 +281,7 @@ __global__ __launch_bounds__(256) void batch_copy_weights_kernel(const BrainDev*
     const WorkItem w = items[it];
     const BrainDev& b = brains[w.brain];
     const uint32_t i = w.start + threadIdx.x;
-    if (i < b.N) copy_weights_soma<IN>(b.vram.dendrite_weights, b.N, b.sp, i);
+    if (i < b.N) copy_weights_soma<IN>(b.vram.dendrite_weights, b.N, b.sp, i, i);
   }
 }
 

@@ -146,6 +146,16 @@ struct VoxelOrder {
   }
 };
 
+// The voxel order of the input axons' positions: the somas' order, except in
+// the baker's order, whose ranks are not spatial within a plane: then 8x8x8
+// bricks (x-fastest if the grid is not made of whole bricks), so that the
+// inputs a soma draws from its target voxel are near it.
+VoxelOrder input_order(const GenSpec* spec, const VoxelOrder& vo) {
+  if (spec->brick != kBakerOrder) return vo;
+  const bool bricks = spec->gx % 8 == 0 && spec->gy % 8 == 0 && spec->gz % 8 == 0;
+  return VoxelOrder{spec->gx, spec->gy, spec->gz, bricks ? 8u : 0u};
+}
+
 // The per-plane shuffles of the baker's order (empty for the other orders).
 void baker_tables(const GenSpec* spec, std::vector<uint32_t>& perm, std::vector<uint32_t>& inv) {
   if (spec->brick != kBakerOrder) return;
@@ -184,6 +194,7 @@ int axc_gen_brain(const GenSpec* spec, const GenVariant* lut, uint8_t* state, ui
   std::vector<uint32_t> bperm, binv;
   baker_tables(spec, bperm, binv);
   const VoxelOrder vo{spec->gx, spec->gy, spec->gz, spec->brick, bperm.data(), binv.data()};
+  const VoxelOrder vin = input_order(spec, vo);
   uint32_t pn, A;
   axc_gen_sizes(spec, &pn, &A);
   const uint32_t n = spec->n_neurons, nin = spec->n_inputs;
@@ -271,11 +282,31 @@ int axc_gen_brain(const GenSpec* spec, const GenVariant* lut, uint8_t* state, ui
       bool from_input = nin > 0 && uniform(h6, (uint32_t)total_src) >= n;
       uint64_t h7 = entity_seed(ds, s * 8u + 7u);
       if (from_input) {
-        uint64_t k = ((unsigned __int128)tv * nin + uniform(h7, nin)) / V;
+        const uint64_t tvi = vin.b == vo.b ? tv : vin.rank((uint32_t)tx, (uint32_t)ty, (uint32_t)tz);
+        uint64_t k = ((unsigned __int128)tvi * nin + uniform(h7, nin)) / V;
         if (k >= nin) k = nin - 1;
         axon = pn + (uint32_t)k;
       } else {
         uint64_t j = ((unsigned __int128)tv * n + uniform(h7, n)) / V;
+        if (vo.b == kBakerOrder) {
+          // Ranks are not spatial within a plane here, so take a soma of
+          // voxel tv itself (the formula above can land on rank tv - 1),
+          // other than soma i: another in the voxel, else one in the voxel
+          // next to it along x.
+          auto somas_of = [&](uint64_t r, uint64_t* a, uint64_t* b) {
+            *a = ((unsigned __int128)r * n + V - 1) / V;
+            *b = ((unsigned __int128)(r + 1) * n + V - 1) / V;
+          };
+          uint64_t lo, hi;
+          somas_of(tv, &lo, &hi);
+          if (hi > lo) j = lo + uniform(h7, (uint32_t)(hi - lo));
+          if (j == i && hi - lo > 1) j = lo + (j - lo + 1) % (hi - lo);
+          if (j == i) {
+            const uint32_t nx = tx + 1 < (int64_t)gx ? (uint32_t)tx + 1 : (tx > 0 ? (uint32_t)tx - 1 : (uint32_t)tx);
+            somas_of(vo.rank(nx, (uint32_t)ty, (uint32_t)tz), &lo, &hi);
+            if (hi > lo && lo != i) j = lo;
+          }
+        }
         if (j >= n) j = n - 1;
         if (j == i) j = (j + 1) % n;
         axon = (uint32_t)j;
@@ -317,6 +348,7 @@ int axc_gen_paths(const GenSpec* spec, const uint32_t* soma_pos, const uint8_t* 
   std::vector<uint32_t> bperm, binv;
   baker_tables(spec, bperm, binv);
   const VoxelOrder vo{spec->gx, spec->gy, spec->gz, spec->brick, bperm.data(), binv.data()};
+  const VoxelOrder vin = input_order(spec, vo);
   const uint64_t ms = spec->master_seed ^ 0x7061746800000006ull;
 #pragma omp parallel for schedule(static)
   for (int64_t ai = 0; ai < (int64_t)A; ai++) {
@@ -338,7 +370,7 @@ int axc_gen_paths(const GenSpec* spec, const uint32_t* soma_pos, const uint8_t* 
     } else {
       const uint32_t k = a - pn;
       uint32_t ux, uy, uz;
-      vo.unrank((uint64_t)k * V / (nin ? nin : 1), &ux, &uy, &uz);
+      vin.unrank((uint64_t)k * V / (nin ? nin : 1), &ux, &uy, &uz);
       x = (int32_t)ux;
       y = (int32_t)uy;
       z = (int32_t)uz;

@@ -104,6 +104,12 @@ struct StagePlan {
   int16_t* w;      // [16][N][8] weights of the filled slots' chunks, same order: the working copy of
                    // the batch graphs, loaded from the reference rows at a batch's start and stored
                    // back at its end
+  // The engine's own soma order (null: the caller's). Soma i of the plan is
+  // the caller's soma perm[i]; axon a of the caller is the plan's axon
+  // axon_map[a] (its targets and sets are in those ids).
+  const uint32_t* perm;
+  const uint32_t* perm_inv;
+  const uint32_t* axon_map;
 };
 
 }  // namespace axc

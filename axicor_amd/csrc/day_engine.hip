@@ -90,12 +90,14 @@ struct GraphKey {
   const void* heads;  // the captured kernels hold the arrays' addresses
   uint32_t v_seg;
   bool staged;
+  bool ordered;  // in the engine's soma order (the graph holds the private arrays)
   // The per-tick spike counts are not part of the key: the graph's K3
   // kernels read them from the shard's device copy (TickParams::counts).
   bool operator==(const GraphKey& o) const {
     return T == o.T && words == o.words && virt_off == o.virt_off && n_virt == o.n_virt && staged == o.staged &&
            max_spk == o.max_spk && n_out == o.n_out && has_bitmask == o.has_bitmask &&
-           has_spikes == o.has_spikes && has_out == o.has_out && mapped == o.mapped && heads == o.heads && v_seg == o.v_seg;
+           has_spikes == o.has_spikes && has_out == o.has_out && mapped == o.mapped && heads == o.heads &&
+           v_seg == o.v_seg && ordered == o.ordered;
   }
 };
 
@@ -129,6 +131,19 @@ struct Shard {
   // Bumped whenever the targets may have changed (for plans kept elsewhere:
   // the batches that include this shard's arrays).
   std::atomic<uint64_t> targets_version{1};
+  // The engine's own soma order (axc_shard_set_soma_positions): engine soma
+  // e is the caller's soma order[e]; empty for the caller's order. The plan
+  // and the private soma arrays and heads (shadow) of a step follow it; the
+  // caller's arrays are copied in at the start of a step and out at its end.
+  std::vector<uint32_t> order;
+  uint32_t* d_perm = nullptr;      // [N] order
+  uint32_t* d_perm_inv = nullptr;  // [N] its inverse
+  uint32_t* d_axon_map = nullptr;  // [A] the engine's id of each of the caller's axons
+  uint32_t* d_axon_inv = nullptr;  // [A] its inverse
+  uint32_t order_virt_off = 0, order_n_virt = 0;  // the virtual input range the axon map orders
+  std::vector<uint32_t> owned;     // the axons owned by somas, ascending (moved by the order)
+  ShardVramPtrs shadow{};          // private voltage, flags, threshold, timers, soma_to_axon, heads
+  bool ordered() const { return !order.empty(); }
 };
 
 // Staged scan on/off for the whole process (on by default).
@@ -167,6 +182,17 @@ struct TimedScope {
   }
 };
 
+// The device arrays of a shard's soma order (the host order stays).
+void free_order_arrays(Shard* s) {
+  for (void* q : {(void*)s->d_perm, (void*)s->d_perm_inv, (void*)s->d_axon_map, (void*)s->d_axon_inv,
+                  (void*)s->shadow.soma_voltage,
+                  (void*)s->shadow.soma_flags, (void*)s->shadow.threshold_offset, (void*)s->shadow.timers,
+                  (void*)s->shadow.soma_to_axon, (void*)s->shadow.axon_heads})
+    if (q) (void)hipFree(q);
+  s->d_perm = s->d_perm_inv = s->d_axon_map = s->d_axon_inv = nullptr;
+  s->shadow = ShardVramPtrs{};
+}
+
 void free_scratch(Shard* s) {
   if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
   if (s->graph) (void)hipGraphDestroy(s->graph);
@@ -193,6 +219,7 @@ void free_scratch(Shard* s) {
     if (q) (void)hipFree(q);
   s->sp = StagePlan{};
   s->stage_dirty = true;
+  free_order_arrays(s);
 }
 
 // Makes `st` wait for the last asynchronous host upload into the shard's
@@ -208,12 +235,136 @@ void wait_target_upload(Shard* s, hipStream_t st) {
   if (ev) (void)hipStreamWaitEvent(st, ev, 0);
 }
 
+// Whether some axon that the shard's order moves is one of the step's
+// virtual inputs (whose bitmask index is their axon index): such a step
+// cannot run in the engine's order.
+bool order_moves_inputs(const Shard* s, uint32_t virt_off, uint32_t n_virt) {
+  if (!s->ordered() || n_virt == 0) return false;
+  auto it = std::lower_bound(s->owned.begin(), s->owned.end(), virt_off);
+  return it != s->owned.end() && (uint64_t)*it < (uint64_t)virt_off + n_virt;
+}
+
+// The device side of a shard's soma order, at a plan rebuild: the order and
+// its inverse, the axon map from the current soma_to_axon (the k-th axon
+// owned by a soma in the engine's order takes the k-th smallest owned id),
+// the virtual inputs [virt_off, virt_off + n_virt) ordered among themselves
+// by the first soma they reach in the engine's order (other axons keep their
+// ids), the private arrays; the plan is then built in that order. Without an
+// order the plan is in the caller's.
+hipError_t setup_order(Shard* s, const ShardVramPtrs& v, hipStream_t st, uint32_t virt_off, uint32_t n_virt) {
+  s->sp.perm = nullptr;
+  s->sp.perm_inv = nullptr;
+  s->sp.axon_map = nullptr;
+  if (!s->ordered()) {
+    free_order_arrays(s);
+    s->owned.clear();
+    return hipSuccess;
+  }
+  const size_t N = s->N, A = s->A;
+  hipError_t e = hipStreamSynchronize(st);
+  std::vector<uint32_t> s2a(N);
+  if (e == hipSuccess) e = hipMemcpy(s2a.data(), v.soma_to_axon, N * 4, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return e;
+  std::vector<uint8_t> seen(A, 0);
+  std::vector<uint32_t> by_order;
+  for (size_t k = 0; k < N; k++) {
+    const uint32_t a = s2a[s->order[k]];
+    if (a < A && !seen[a]) {
+      seen[a] = 1;
+      by_order.push_back(a);
+    }
+  }
+  s->owned = by_order;
+  std::sort(s->owned.begin(), s->owned.end());
+  std::vector<uint32_t> amap(A), inv(N);
+  for (size_t a = 0; a < A; a++) amap[a] = (uint32_t)a;
+  for (size_t k = 0; k < by_order.size(); k++) amap[by_order[k]] = s->owned[k];
+  for (size_t k = 0; k < N; k++) inv[s->order[k]] = (uint32_t)k;
+  if (!s->d_perm) {
+    ShardVramPtrs& h = s->shadow;
+    e = hipMalloc((void**)&s->d_perm, N * 4);
+    if (e == hipSuccess) e = hipMalloc((void**)&s->d_perm_inv, N * 4);
+    if (e == hipSuccess) e = hipMalloc((void**)&s->d_axon_map, (A ? A : 1) * 4);
+    if (e == hipSuccess) e = hipMalloc((void**)&s->d_axon_inv, (A ? A : 1) * 4);
+    if (e == hipSuccess) e = hipMalloc((void**)&h.soma_voltage, N * 4);
+    if (e == hipSuccess) e = hipMalloc((void**)&h.soma_flags, N);
+    if (e == hipSuccess) e = hipMalloc((void**)&h.threshold_offset, N * 4);
+    if (e == hipSuccess) e = hipMalloc((void**)&h.timers, N);
+    if (e == hipSuccess) e = hipMalloc((void**)&h.soma_to_axon, N * 4);
+    if (e == hipSuccess) e = hipMalloc((void**)&h.axon_heads, (A ? A : 1) * sizeof(AxcBurstHeads8));
+    if (e != hipSuccess) {
+      free_order_arrays(s);
+      return e;
+    }
+  }
+  e = hipMemcpy(s->d_perm, s->order.data(), N * 4, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(s->d_perm_inv, inv.data(), N * 4, hipMemcpyHostToDevice);
+  if (e != hipSuccess) return e;
+  // Virtual inputs (when none of them is owned by a soma): ordered by the
+  // first soma they reach, those reaching none after the others, ties by id.
+  s->order_virt_off = virt_off;
+  s->order_n_virt = n_virt;
+  const bool inputs_free = n_virt > 0 && (uint64_t)virt_off + n_virt <= A && !order_moves_inputs(s, virt_off, n_virt);
+  if (inputs_free) {
+    std::vector<uint32_t> key(n_virt, 0xFFFFFFFFu);
+    uint32_t* d_key = s->sc.spk_list;  // scratch: N entries, rewritten by the next tick
+    if (n_virt > N) {
+      d_key = nullptr;
+      if ((e = hipMalloc((void**)&d_key, (size_t)n_virt * 4)) != hipSuccess) return e;
+    }
+    e = hipMemcpy(d_key, key.data(), (size_t)n_virt * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+      input_key_kernel<<<blocks_for(N), kThreads>>>(v.dendrite_targets, (uint32_t)N, s->d_perm_inv, virt_off, n_virt,
+                                                     d_key);
+      e = hipMemcpy(key.data(), d_key, (size_t)n_virt * 4, hipMemcpyDeviceToHost);
+    }
+    if (d_key != s->sc.spk_list) (void)hipFree(d_key);
+    if (e != hipSuccess) return e;
+    std::vector<uint32_t> idx(n_virt);
+    for (uint32_t k = 0; k < n_virt; k++) idx[k] = k;
+    std::stable_sort(idx.begin(), idx.end(), [&](uint32_t x, uint32_t y) { return key[x] < key[y]; });
+    for (uint32_t k = 0; k < n_virt; k++) amap[virt_off + idx[k]] = virt_off + k;
+  }
+  std::vector<uint32_t> ainv(A);
+  for (size_t a = 0; a < A; a++) ainv[amap[a]] = (uint32_t)a;
+  if (A) e = hipMemcpy(s->d_axon_map, amap.data(), A * 4, hipMemcpyHostToDevice);
+  if (e == hipSuccess && A) e = hipMemcpy(s->d_axon_inv, ainv.data(), A * 4, hipMemcpyHostToDevice);
+  if (e != hipSuccess) return e;
+  // The dendrite arrays stay the caller's (the plan reads them through the order).
+  s->shadow.dendrite_targets = v.dendrite_targets;
+  s->shadow.dendrite_weights = v.dendrite_weights;
+  s->shadow.dendrite_timers = v.dendrite_timers;
+  s->sp.perm = s->d_perm;
+  s->sp.perm_inv = s->d_perm_inv;
+  s->sp.axon_map = s->d_axon_map;
+  return hipSuccess;
+}
+
+// The caller's state into the engine's order (soma arrays, heads) or back.
+void order_copy(Shard* s, const ShardVramPtrs& ref, bool in, hipStream_t st) {
+  const uint32_t N = s->N, A = s->A;
+  if (in) {
+    order_somas_kernel<true><<<blocks_for(N), kThreads, 0, st>>>(ref, s->shadow, N, A, s->d_perm, s->d_axon_map);
+    if (A) order_heads_kernel<true><<<blocks_for(A), kThreads, 0, st>>>(ref.axon_heads, s->shadow.axon_heads, A,
+                                                                        s->d_axon_map);
+  } else {
+    if (A) order_heads_kernel<false><<<blocks_for(A), kThreads, 0, st>>>(ref.axon_heads, s->shadow.axon_heads, A,
+                                                                         s->d_axon_map);
+    order_somas_kernel<false><<<blocks_for(N), kThreads, 0, st>>>(ref, s->shadow, N, A, s->d_perm, s->d_axon_map);
+  }
+}
+
 // Allocates the staging plan of a shard (first step) and rebuilds it on
 // `st` when the targets changed. Returns whether the staged scan is used.
-bool ensure_stage(Shard* s, const ShardVramPtrs& v, hipStream_t st, hipError_t* err) {
+bool ensure_stage(Shard* s, const ShardVramPtrs& v, hipStream_t st, hipError_t* err, uint32_t virt_off = 0,
+                  uint32_t n_virt = 0) {
   *err = hipSuccess;
   if (!g_staging.load() || s->N == 0) return false;
   const size_t N = s->N, A = s->A;
+  // An ordered plan orders the step's virtual inputs: another input range
+  // rebuilds it.
+  if (s->ordered() && n_virt > 0 && (s->order_virt_off != virt_off || s->order_n_virt != n_virt))
+    s->stage_dirty = true;
   const size_t blocks = staged_grid((uint32_t)N);
   if (!s->sp.comp8) {
     static std::once_flag attr_once;
@@ -248,6 +399,10 @@ bool ensure_stage(Shard* s, const ShardVramPtrs& v, hipStream_t st, hipError_t* 
   }
   if (s->stage_dirty.exchange(false)) {
     wait_target_upload(s, st);
+    if ((*err = setup_order(s, v, st, virt_off, n_virt)) != hipSuccess) {
+      s->stage_dirty = true;
+      return false;
+    }
     stage_build_kernel<<<stage_blocks((uint32_t)N), kBlockSomas, 0, st>>>(v.dendrite_targets, (uint32_t)N,
                                                                           (uint32_t)A, s->sp);
     *err = hipGetLastError();
@@ -396,14 +551,16 @@ hipError_t enqueue_tick(const ShardVramPtrs& v, Shard* s, uint32_t v_seg, uint32
   if (k3) {
     TimedScope ts(kKSpikes, st, timed);
     const uint32_t grid = prm ? std::min<uint32_t>(blocks_for(max_spk), 64u) : blocks_for(n_spk);
-    spike_count_kernel<<<grid, kThreads, 0, st>>>(spikes, n_spk, A, s->sc.push_cnt, prm, tick, max_spk);
+    spike_count_kernel<<<grid, kThreads, 0, st>>>(spikes, n_spk, A, s->sc.push_cnt, prm, tick, max_spk,
+                                                  staged ? s->sp.axon_map : nullptr);
   }
   const bool k2 = n_virt > 0 && bitmask != nullptr;
   {
     TimedScope ts(kKAxon, st, timed);
     axon_tick_kernel<<<blocks_for(A > kNumCounters ? A : kNumCounters), kThreads, 0, st>>>(
         v.axon_heads, A, v_seg, k2 ? bitmask : nullptr, virt_off, k2 ? n_virt : 0u, s->sc.push_cnt, k3 ? 1 : 0,
-        s->sc.counters, prm, tick, words, staged ? nullptr : cur, staged ? s->sp.comp8 : nullptr);
+        s->sc.counters, prm, tick, words, staged ? nullptr : cur, staged ? s->sp.comp8 : nullptr,
+        (staged && s->sp.perm) ? s->d_axon_inv : nullptr);
   }
   if (N > 0 && staged) {
     // The staged pass never fuses the previous tick's GSOP (gsop_now is set
@@ -446,7 +603,8 @@ hipError_t enqueue_tick(const ShardVramPtrs& v, Shard* s, uint32_t v_seg, uint32
   }
   if (n_out > 0 && mapped != nullptr && out != nullptr) {
     TimedScope ts(kKReadout, st, timed);
-    readout_kernel<<<blocks_for(n_out), kThreads, 0, st>>>(v.soma_flags, N, mapped, out, n_out, prm, tick);
+    readout_kernel<<<blocks_for(n_out), kThreads, 0, st>>>(v.soma_flags, N, mapped, out, n_out, prm, tick,
+                                                           (staged && s->sp.perm) ? s->d_perm_inv : nullptr);
   }
   return hipGetLastError();
 }
@@ -640,11 +798,28 @@ int32_t cu_step_day_phase(const ShardVramPtrs* vram, uint32_t padded_n, uint32_t
   if (!s) return (int32_t)e;
   const AxcVariantParameters* lut = device_lut();
   if (!lut) return (int32_t)hipErrorOutOfMemory;
-  const bool staged = ensure_stage(s, *vram, (hipStream_t)stream, &e);
+  const bool staged = ensure_stage(s, *vram, (hipStream_t)stream, &e, virtual_offset,
+                                   input_bitmask ? num_virtual_axons : 0u);
   if (e != hipSuccess) return (int32_t)e;
+  hipStream_t st = (hipStream_t)stream;
+  if (staged && s->ordered()) {
+    // In the engine's order: the caller's state in, one tick on the private
+    // arrays and the working copy of the weights, the state back out.
+    if (order_moves_inputs(s, virtual_offset, input_bitmask ? num_virtual_axons : 0u))
+      return (int32_t)hipErrorInvalidValue;
+    order_copy(s, *vram, true, st);
+    copy_weights_kernel<true><<<blocks_for(padded_n), kThreads, 0, st>>>(vram->dendrite_weights, padded_n, s->sp);
+    e = enqueue_tick(s->shadow, s, v_seg, current_tick, input_bitmask, 0, virtual_offset, num_virtual_axons,
+                     incoming_spikes, 0, num_incoming_spikes, mapped_soma_ids, output_history, num_outputs,
+                     (int32_t)dopamine, nullptr, st, lut, s->flip, false, true, true, true);
+    copy_weights_kernel<false><<<blocks_for(padded_n), kThreads, 0, st>>>(vram->dendrite_weights, padded_n, s->sp);
+    order_copy(s, *vram, false, st);
+    if (e == hipSuccess) e = hipGetLastError();
+    return (int32_t)e;
+  }
   e = enqueue_tick(*vram, s, v_seg, current_tick, input_bitmask, 0, virtual_offset, num_virtual_axons,
                    incoming_spikes, 0, num_incoming_spikes, mapped_soma_ids, output_history, num_outputs,
-                   (int32_t)dopamine, nullptr, (hipStream_t)stream, lut, s->flip, false, true, staged);
+                   (int32_t)dopamine, nullptr, st, lut, s->flip, false, true, staged);
   s->flip ^= 1u;
   return (int32_t)e;
 }
@@ -1055,8 +1230,13 @@ int32_t axc_step_day_phase_batch(const ShardVramPtrs* vram, uint32_t padded_n, u
   key.mapped = d_mapped;
   key.heads = vram->axon_heads;
   key.v_seg = v_seg;
-  key.staged = ensure_stage(s, *vram, st, &e);
+  key.staged = ensure_stage(s, *vram, st, &e, virt_off, d_bitmask ? n_virt : 0u);
   if (e != hipSuccess) return (int32_t)e;
+  // In the engine's soma order the ticks run on the private arrays, copied
+  // in from the caller's at the start and out at the end.
+  key.ordered = key.staged && s->ordered();
+  if (key.ordered && order_moves_inputs(s, virt_off, d_bitmask ? n_virt : 0u)) return (int32_t)hipErrorInvalidValue;
+  const ShardVramPtrs& V = key.ordered ? s->shadow : *vram;
   // The direct pass applies tick t's GSOP inside tick t+1's soma pass; with
   // the staged pass every tick ends with the GSOP kernel, on the plan's
   // working copy of the weights, loaded before the first tick and stored
@@ -1076,12 +1256,13 @@ int32_t axc_step_day_phase_batch(const ShardVramPtrs* vram, uint32_t padded_n, u
   if (timed) {
     // Kernel by kernel, so that each launch can be timed: the same kernels
     // and tick structure as the graph below, arguments passed directly.
+    if (key.ordered) order_copy(s, *vram, true, st);
     if (pw) {
       TimedScope ts(kKWeights, st, true);
       copy_weights_kernel<true><<<blocks_for(padded_n), kThreads, 0, st>>>(vram->dendrite_weights, padded_n, s->sp);
     }
     for (uint32_t t = 0; t < T && e == hipSuccess; t++) {
-      e = enqueue_tick(*vram, s, v_seg, tick_base + t, d_bitmask ? d_bitmask + (size_t)t * words : nullptr, words,
+      e = enqueue_tick(V, s, v_seg, tick_base + t, d_bitmask ? d_bitmask + (size_t)t * words : nullptr, words,
                        virt_off, n_virt, d_spikes ? d_spikes + (size_t)t * max_spk : nullptr, max_spk,
                        counts[t], d_mapped, d_out ? d_out + (size_t)t * n_out : nullptr, n_out,
                        (int32_t)dopamine, nullptr, st, lut, t, t > 0 && fuse, t + 1 == T || !fuse, key.staged, pw);
@@ -1091,6 +1272,8 @@ int32_t axc_step_day_phase_batch(const ShardVramPtrs* vram, uint32_t padded_n, u
       copy_weights_kernel<false><<<blocks_for(padded_n), kThreads, 0, st>>>(vram->dendrite_weights, padded_n, s->sp);
       e = hipGetLastError();
     }
+    if (key.ordered) order_copy(s, *vram, false, st);
+    if (e == hipSuccess) e = hipGetLastError();
     return (int32_t)e;
   }
   TickParams pv;
@@ -1127,19 +1310,21 @@ int32_t axc_step_day_phase_batch(const ShardVramPtrs* vram, uint32_t padded_n, u
     if ((e = hipStreamCreateWithFlags(&cap, hipStreamNonBlocking)) != hipSuccess) return (int32_t)e;
     e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
     if (e == hipSuccess) {
+      if (key.ordered) order_copy(s, *vram, true, cap);
       if (pw)
         copy_weights_kernel<true><<<blocks_for(padded_n), kThreads, 0, cap>>>(vram->dendrite_weights, padded_n,
                                                                                s->sp);
       for (uint32_t t = 0; t < T && e == hipSuccess; t++) {
         // GSOP of tick t runs inside tick t+1's neuron pass; the last tick's
         // GSOP is launched on its own so the batch leaves a complete state.
-        e = enqueue_tick(*vram, s, v_seg, t, d_bitmask, words, virt_off, n_virt, d_spikes, max_spk,
+        e = enqueue_tick(V, s, v_seg, t, d_bitmask, words, virt_off, n_virt, d_spikes, max_spk,
                          0, d_mapped, d_out, n_out, 0, s->prm, cap, lut, t, t > 0 && fuse,
                          t + 1 == T || !fuse, key.staged, pw);
       }
       if (pw && e == hipSuccess)
         copy_weights_kernel<false><<<blocks_for(padded_n), kThreads, 0, cap>>>(vram->dendrite_weights, padded_n,
                                                                                 s->sp);
+      if (key.ordered && e == hipSuccess) order_copy(s, *vram, false, cap);
       hipGraph_t g = nullptr;
       hipError_t e2 = hipStreamEndCapture(cap, &g);
       if (e == hipSuccess) e = e2;
@@ -1208,6 +1393,46 @@ void axc_host_sort_and_prune(uint8_t* flags, uint32_t* targets, int16_t* weights
 }
 
 void axc_staging_enable(int enable) { g_staging = enable != 0; }
+
+int32_t axc_shard_set_soma_positions(const ShardVramPtrs* vram, const uint32_t* positions, uint32_t n) {
+  if (!vram) return (int32_t)hipErrorInvalidValue;
+  std::lock_guard<std::mutex> lk(g_mu);
+  auto it = g_shards.find(vram->soma_voltage);
+  if (it == g_shards.end()) return (int32_t)hipErrorInvalidValue;
+  Shard* s = it->second;
+  if (positions && n != s->N) return (int32_t)hipErrorInvalidValue;
+  (void)hipDeviceSynchronize();  // no step of the shard may be in flight with the old order
+  s->order.clear();
+  if (positions) {
+    // 8x8x8-voxel bricks, z-major over bricks and x-fastest inside one (the
+    // caller's index breaks ties); somas at position 0 (padding) last.
+    std::vector<std::pair<uint64_t, uint32_t>> key(n);
+    for (uint32_t i = 0; i < n; i++) {
+      const uint32_t p = positions[i];
+      const uint64_t x = p & 0x3FF, y = (p >> 10) & 0x3FF, z = (p >> 20) & 0xFF;
+      const uint64_t brick = ((z >> 3) << 14) | ((y >> 3) << 7) | (x >> 3);
+      const uint64_t inner = ((z & 7) << 6) | ((y & 7) << 3) | (x & 7);
+      key[i] = {p ? (brick << 9) | inner : ~0ull, i};
+    }
+    std::stable_sort(key.begin(), key.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
+    s->order.resize(n);
+    bool identity = true;
+    for (uint32_t k = 0; k < n; k++) {
+      s->order[k] = key[k].second;
+      identity = identity && key[k].second == k;
+    }
+    if (identity) s->order.clear();
+  }
+  // The plan (and a cached graph, whose kernels hold the arrays) follow the new order.
+  s->stage_dirty = true;
+  s->targets_version++;
+  if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
+  if (s->graph) (void)hipGraphDestroy(s->graph);
+  s->gexec = nullptr;
+  s->graph = nullptr;
+  s->has_graph = false;
+  return 0;
+}
 
 int32_t axc_shard_graph_captures(const ShardVramPtrs* vram, uint64_t* out) {
   if (!vram || !out) return (int32_t)hipErrorInvalidValue;

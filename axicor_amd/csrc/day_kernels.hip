@@ -184,14 +184,17 @@ __device__ __forceinline__ GsopTerms gsop_terms(uint8_t flags, const AxcVariantP
 // k pushes an axon receives this tick are applied at once, before propagation.
 // Also writes the compact snapshot of the propagated heads.
 // ---------------------------------------------------------------------------
+// axon_inv (may be null): the caller's id of each of the engine's axons,
+// which sets the axon's bit in the input bitmask.
 __device__ __forceinline__ void axon_step(AxcBurstHeads8* __restrict__ heads, uint32_t a, uint32_t v_seg,
                                           const uint32_t* __restrict__ bitmask, uint32_t virt_off, uint32_t n_virt,
                                           uint32_t* __restrict__ push_cnt, bool use_cnt, uint4* __restrict__ comp,
-                                          uint2* __restrict__ comp8 = nullptr) {
+                                          uint2* __restrict__ comp8 = nullptr,
+                                          const uint32_t* __restrict__ axon_inv = nullptr) {
   Heads8 h = load_heads(heads, a);
   uint32_t k = 0;
   if (bitmask) {
-    uint32_t i = a - virt_off;  // wraps for a < virt_off
+    uint32_t i = (axon_inv ? axon_inv[a] : a) - virt_off;  // wraps for ids below virt_off
     if (i < n_virt) k += (bitmask[i >> 5] >> (i & 31)) & 1u;
   }
   if (use_cnt) {
@@ -222,7 +225,8 @@ __global__ __launch_bounds__(256) void axon_tick_kernel(AxcBurstHeads8* __restri
                                                         uint32_t* __restrict__ counters,
                                                         const TickParams* __restrict__ prm, uint32_t t_off,
                                                         uint32_t words, uint4* __restrict__ comp,
-                                                        uint2* __restrict__ comp8) {
+                                                        uint2* __restrict__ comp8,
+                                                        const uint32_t* __restrict__ axon_inv = nullptr) {
   uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t* __restrict__ bitmask =
       prm ? (prm->bitmask ? prm->bitmask + (size_t)t_off * words : nullptr) : bitmask_arg;
@@ -230,17 +234,18 @@ __global__ __launch_bounds__(256) void axon_tick_kernel(AxcBurstHeads8* __restri
   // push_apply launches; this tick's neuron_tick fills them again.
   if (a < kNumCounters) counters[a] = 0;
   if (a >= A) return;
-  axon_step(heads, a, v_seg, bitmask, virt_off, n_virt, push_cnt, use_cnt != 0, comp, comp8);
+  axon_step(heads, a, v_seg, bitmask, virt_off, n_virt, push_cnt, use_cnt != 0, comp, comp8, axon_inv);
 }
 
 // K3 (physics.cu:114-132): one count per occurrence of an id < A. With
 // TickParams the tick's count is read on the device (prm->counts[t_off],
 // at most the slice length `stride`), so a captured graph serves any
 // schedule; the grid strides over the count.
+// axon_map (may be null): the engine's axon id of each of the caller's.
 __global__ __launch_bounds__(256) void spike_count_kernel(const uint32_t* __restrict__ ids_arg, uint32_t n_arg,
                                                           uint32_t A, uint32_t* __restrict__ push_cnt,
                                                           const TickParams* __restrict__ prm, uint32_t t_off,
-                                                          uint32_t stride) {
+                                                          uint32_t stride, const uint32_t* __restrict__ axon_map = nullptr) {
   const uint32_t* __restrict__ ids = prm ? prm->spikes + (size_t)t_off * stride : ids_arg;
   uint32_t n = n_arg;
   if (prm) {
@@ -249,7 +254,7 @@ __global__ __launch_bounds__(256) void spike_count_kernel(const uint32_t* __rest
   }
   for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
     const uint32_t a = ids[j];
-    if (a < A) atomicAdd(&push_cnt[a], 1u);
+    if (a < A) atomicAdd(&push_cnt[axon_map ? axon_map[a] : a], 1u);
   }
 }
 
@@ -260,10 +265,13 @@ __global__ __launch_bounds__(256) void spike_count_kernel(const uint32_t* __rest
 // and, for a spike, the soma's own axon to push. thr_in, v_in and axon_in
 // are the soma's threshold offset, voltage and axon as read before the tick
 // (used only when it integrates).
+// hb_i: the soma's index in the caller's numbering, which sets its heartbeat
+// phase (equal to i unless the step runs in the engine's own soma order).
 __device__ __forceinline__ void soma_tick_with(const ShardVramPtrs& vram, uint32_t i, uint32_t A,
                                                const AxcVariantParameters& p, uint8_t flags, uint8_t timer, bool refr,
                                                uint32_t tick, int32_t i_in, int32_t thr_in, int32_t v_in,
-                                               uint32_t axon_in, bool& fin, bool& push_own, uint32_t& my_axon) {
+                                               uint32_t axon_in, bool& fin, bool& push_own, uint32_t& my_axon,
+                                               uint32_t hb_i) {
   if (refr) {
     vram.timers[i] = (uint8_t)(timer - 1);
     vram.soma_flags[i] = flags;
@@ -289,7 +297,7 @@ __device__ __forceinline__ void soma_tick_with(const ShardVramPtrs& vram, uint32
   int32_t eff = wadd(p.threshold, thr);
   bool glif = v >= eff;
   uint32_t period = p.spontaneous_firing_period_ticks;
-  bool hb = period > 0 && ((tick + i * 104729u) % period) == 0;
+  bool hb = period > 0 && ((tick + hb_i * 104729u) % period) == 0;
   fin = glif || hb;
   if (glif) {
     v = p.rest_potential;
@@ -318,7 +326,7 @@ __device__ __forceinline__ void soma_tick(const ShardVramPtrs& vram, uint32_t i,
   const int32_t thr = refr ? 0 : vram.threshold_offset[i];
   const int32_t v = refr ? 0 : vram.soma_voltage[i];
   const uint32_t ax = refr ? 0u : vram.soma_to_axon[i];
-  soma_tick_with(vram, i, A, p, flags, timer, refr, tick, i_in, thr, v, ax, fin, push_own, my_axon);
+  soma_tick_with(vram, i, A, p, flags, timer, refr, tick, i_in, thr, v, ax, fin, push_own, my_axon, i);
 }
 
 // ---------------------------------------------------------------------------
@@ -722,16 +730,18 @@ __global__ __launch_bounds__(256) void compact_all_kernel(const AxcBurstHeads8* 
   else if (a == A) comp[a] = make_uint4(0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu);
 }
 
-// K7 (physics.cu:378-395); ids outside [0, N) read 0.
+// K7 (physics.cu:378-395); ids outside [0, N) read 0. perm_inv (may be
+// null): the engine's soma of each of the caller's (flags in engine order).
 __global__ __launch_bounds__(256) void readout_kernel(const uint8_t* __restrict__ flags, uint32_t N,
                                                       const uint32_t* __restrict__ mapped,
                                                       uint8_t* __restrict__ out_arg, uint32_t n_out,
-                                                      const TickParams* __restrict__ prm, uint32_t t_off) {
+                                                      const TickParams* __restrict__ prm, uint32_t t_off,
+                                                      const uint32_t* __restrict__ perm_inv = nullptr) {
   uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n_out) return;
   uint8_t* __restrict__ out = prm ? prm->out + (size_t)t_off * n_out : out_arg;
   uint32_t id = mapped[j];
-  out[j] = (id != kNoAxon && id < N) ? (flags[id] & 1u) : 0u;
+  out[j] = (id != kNoAxon && id < N) ? (flags[perm_inv ? perm_inv[id] : id] & 1u) : 0u;
 }
 
 // K1 alone (physics.cu:25-30).

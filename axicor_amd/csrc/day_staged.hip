@@ -58,6 +58,14 @@ constexpr uint32_t kHashMaxProbe = 256;
 
 __device__ __forceinline__ uint32_t hash_slot(uint32_t key) { return (key * 2654435761u) >> (32 - 14); }
 
+// A caller's target in the plan's axon ids (axons outside [0, A) and the
+// empty target unchanged).
+__device__ __forceinline__ uint32_t plan_target(uint32_t t, uint32_t A, const uint32_t* __restrict__ axon_map) {
+  if (!axon_map) return t;
+  const uint32_t a = (t & 0x00FFFFFFu) - 1u;
+  return a < A ? (t & 0xFF000000u) | ((axon_map[a] + 1u) & 0x00FFFFFFu) : t;
+}
+
 // One workgroup of 256 lanes per soma block (grid = ceil(N/256)). The
 // block's distinct axons are gathered in an LDS hash set, then sorted, so
 // that the set (and the LDS entries) are in axon order: consecutive lanes of
@@ -71,13 +79,14 @@ __global__ __launch_bounds__(kBlockSomas) void stage_build_kernel(const uint32_t
   __shared__ uint32_t overflow;
   const uint32_t tid = threadIdx.x, b = blockIdx.x;
   const uint32_t i = b * kBlockSomas + tid;
+  const uint32_t r = (sp.perm && i < N) ? sp.perm[i] : i;  // the caller's soma
   for (uint32_t j = tid; j < kHashSize; j += kBlockSomas) keys[j] = kHashEmpty;
   if (tid == 0) overflow = 0;
   __syncthreads();
   uint32_t F = 0;
   if (i < N) {
     for (; F < kMaxDendrites; F++) {
-      const uint32_t t = targets[(size_t)F * N + i];
+      const uint32_t t = plan_target(targets[(size_t)F * N + r], A, sp.axon_map);
       if (t == 0) break;
       sp.tgt[((size_t)(F >> 3) * N + i) * 8 + (F & 7)] = t;
       const uint32_t key = snap_index(t, A);
@@ -144,8 +153,7 @@ __global__ __launch_bounds__(kBlockSomas) void stage_build_kernel(const uint32_t
   for (uint32_t j = tid; j < staged_n; j += kBlockSomas) sp.set[(size_t)b * kStageCap + j] = sorted[j];
   if (i >= N) return;
   for (uint32_t s = 0; s < F; s++) {
-    const size_t o = (size_t)s * N + i;
-    const uint32_t t = targets[o];
+    const uint32_t t = plan_target(targets[(size_t)s * N + r], A, sp.axon_map);
     const uint32_t key = snap_index(t, A);
     uint32_t lo = 0, hi = total;  // the key is present: sorted[lo] == key at the end
     while (hi - lo > 1) {
@@ -405,7 +413,8 @@ __device__ __forceinline__ void finish_block(BlockLds& L, const ShardVramPtrs& v
                                              const AxcVariantParameters& p, uint32_t tick, const ShardScratch& sc,
                                              uint32_t tid, uint32_t i, uint32_t F, uint8_t flags_in, uint8_t timer,
                                              bool refr, int32_t i_in, uint32_t hits, uint64_t* part,
-                                             bool count_tick, uint64_t* type_spikes, bool count_gsop) {
+                                             bool count_tick, uint64_t* type_spikes, bool count_gsop,
+                                             uint32_t hb_i) {
   const uint32_t g = tid / kBlockSomas, li = tid % kBlockSomas;
   L.red_in[g][li] = i_in;
   L.red_hits[g][li] = (uint16_t)hits;
@@ -431,7 +440,7 @@ __device__ __forceinline__ void finish_block(BlockLds& L, const ShardVramPtrs& v
     uint8_t flags = flags_in & 0xF1;  // K1
     flags &= (uint8_t)~0x01;
     soma_tick_with(vram, i, A, p, flags, timer, refr, tick, sum, L.pre_thr[li], L.pre_v[li], L.pre_ax[li], fin,
-                   push_own, my_axon);
+                   push_own, my_axon, hb_i);
     n_int = refr ? 0u : 1u;
     if (fin && type_spikes) atomicAdd((unsigned long long*)&type_spikes[flags_in >> 4], 1ull);
     if (fin && count_gsop) {
@@ -505,7 +514,7 @@ __device__ __forceinline__ void staged_block(const ShardVramPtrs& vram, uint32_t
   uint32_t F = 0;
   uint8_t flags_in = 0, timer = 0;
   int32_t v_in = 0, thr_in = 0;
-  uint32_t ax_in = 0;
+  uint32_t ax_in = 0, hb_i = i;
   if (i < N) {
     flags_in = vram.soma_flags[i];
     timer = vram.timers[i];
@@ -514,6 +523,7 @@ __device__ __forceinline__ void staged_block(const ShardVramPtrs& vram, uint32_t
       v_in = vram.soma_voltage[i];
       thr_in = vram.threshold_offset[i];
       ax_in = vram.soma_to_axon[i];
+      if (sp.perm) hb_i = sp.perm[i];  // the caller's index (heartbeat phase)
     }
   }
   if (staged) stage_entries(ent, bset, cnt, sp.comp8, A, tid);
@@ -536,7 +546,7 @@ __device__ __forceinline__ void staged_block(const ShardVramPtrs& vram, uint32_t
     else scan_slots<false, PW>(vram, N, A, sp, ent, gu, i, F, p.signal_propagation_length, i_in, hits);
   }
   finish_block(L, vram, N, A, p, tick, sc, tid, i, F, flags_in, timer, refr, i_in, hits, part, count_tick,
-               type_spikes, count_gsop);
+               type_spikes, count_gsop, hb_i);
 }
 
 // The single-shard soma pass: one workgroup per soma block, blocks dealt to
@@ -559,9 +569,10 @@ void neuron_staged_kernel(ShardVramPtrs vram, uint32_t N, uint32_t A, const AxcV
 // (IN) or from the reference rows. Slots of such a chunk at or past the fill
 // are copied as they are (no pass writes them). One lane per soma, chunks in
 // turn: each reference row is read or written by consecutive lanes.
+// i: the plan's soma, r: the same soma in the caller's numbering (its rows).
 template <bool IN>
 __device__ __forceinline__ void copy_weights_soma(int16_t* __restrict__ rows, uint32_t N, const StagePlan& sp,
-                                                  uint32_t i) {
+                                                  uint32_t i, uint32_t r) {
   const uint32_t F = sp.fill[i];
   for (uint32_t c = 0; c * 8 < F; c++) {
     v4u* q = reinterpret_cast<v4u*>(sp.w + chunk_at(N, c, i));
@@ -569,25 +580,45 @@ __device__ __forceinline__ void copy_weights_soma(int16_t* __restrict__ rows, ui
       uint32_t w2[4];
 #pragma unroll
       for (int j = 0; j < 4; j++)
-        w2[j] = (uint32_t)(uint16_t)rows[(size_t)(c * 8 + 2 * j) * N + i] |
-                ((uint32_t)(uint16_t)rows[(size_t)(c * 8 + 2 * j + 1) * N + i] << 16);
+        w2[j] = (uint32_t)(uint16_t)rows[(size_t)(c * 8 + 2 * j) * N + r] |
+                ((uint32_t)(uint16_t)rows[(size_t)(c * 8 + 2 * j + 1) * N + r] << 16);
       *q = v4u{w2[0], w2[1], w2[2], w2[3]};
     } else {
       const v4u v = *q;
       const uint32_t w2[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
       for (int j = 0; j < 4; j++) {
-        rows[(size_t)(c * 8 + 2 * j) * N + i] = (int16_t)(uint16_t)(w2[j] & 0xFFFFu);
-        rows[(size_t)(c * 8 + 2 * j + 1) * N + i] = (int16_t)(uint16_t)(w2[j] >> 16);
+        rows[(size_t)(c * 8 + 2 * j) * N + r] = (int16_t)(uint16_t)(w2[j] & 0xFFFFu);
+        rows[(size_t)(c * 8 + 2 * j + 1) * N + r] = (int16_t)(uint16_t)(w2[j] >> 16);
       }
     }
   }
 }
 
+// One lane per soma of the caller's numbering, so that the rows are read
+// (IN) or written coalesced; in the engine's order each lane's chunks are
+// those of the plan's soma perm_inv[r].
 template <bool IN>
 __global__ __launch_bounds__(256) void copy_weights_kernel(int16_t* __restrict__ rows, uint32_t N, StagePlan sp) {
-  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
-  if (i < N) copy_weights_soma<IN>(rows, N, sp, i);
+  const uint32_t r = blockIdx.x * 256 + threadIdx.x;
+  if (r < N) copy_weights_soma<IN>(rows, N, sp, sp.perm_inv ? sp.perm_inv[r] : r, r);
+}
+
+// Per virtual input axon (ids [virt_off, virt_off + n_virt)) the first of
+// its postsynaptic somas in the engine's order, for ordering the inputs like
+// the somas they reach (key[k] starts at ~0).
+__global__ __launch_bounds__(256) void input_key_kernel(const uint32_t* __restrict__ targets, uint32_t N,
+                                                        const uint32_t* __restrict__ perm_inv, uint32_t virt_off,
+                                                        uint32_t n_virt, uint32_t* __restrict__ key) {
+  const uint32_t r = blockIdx.x * 256 + threadIdx.x;
+  if (r >= N) return;
+  const uint32_t e = perm_inv[r];
+  for (uint32_t s = 0; s < kMaxDendrites; s++) {
+    const uint32_t t = targets[(size_t)s * N + r];
+    if (t == 0) break;
+    const uint32_t k = (t & 0x00FFFFFFu) - 1u - virt_off;
+    if (k < n_virt) atomicMin(&key[k], e);
+  }
 }
 
 // GSOP (K6, physics.cu:281-373) of one chunk c of spiking soma i on the
@@ -683,6 +714,46 @@ __global__ __launch_bounds__(256) void compact_spikes_kernel(uint32_t n_words, S
   if (lane == 63) base = atomicAdd(&sc.counters[kCntSpikes], total);
   uint32_t pos = __shfl(base, 63, 64) + x - c;
   for (uint64_t r = m; r; r &= r - 1) sc.spk_list[pos++] = w * 64u + (uint32_t)__builtin_ctzll(r);
+}
+
+// The engine's soma order in and out of its private soma arrays (sh) from
+// and to the caller's (ref): soma i of the engine is the caller's perm[i].
+// IN also derives the private soma_to_axon in the engine's axon ids.
+template <bool IN>
+__global__ __launch_bounds__(256) void order_somas_kernel(ShardVramPtrs ref, ShardVramPtrs sh, uint32_t N, uint32_t A,
+                                                          const uint32_t* __restrict__ perm,
+                                                          const uint32_t* __restrict__ axon_map) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= N) return;
+  const uint32_t r = perm[i];
+  if (IN) {
+    sh.soma_voltage[i] = ref.soma_voltage[r];
+    sh.soma_flags[i] = ref.soma_flags[r];
+    sh.threshold_offset[i] = ref.threshold_offset[r];
+    sh.timers[i] = ref.timers[r];
+    const uint32_t a = ref.soma_to_axon[r];
+    sh.soma_to_axon[i] = a < A ? axon_map[a] : a;
+  } else {
+    ref.soma_voltage[r] = sh.soma_voltage[i];
+    ref.soma_flags[r] = sh.soma_flags[i];
+    ref.threshold_offset[r] = sh.threshold_offset[i];
+    ref.timers[r] = sh.timers[i];
+  }
+}
+
+// The heads in and out of the engine's axon order: the caller's axon a is
+// the engine's axon_map[a].
+template <bool IN>
+__global__ __launch_bounds__(256) void order_heads_kernel(AxcBurstHeads8* __restrict__ ref, AxcBurstHeads8* __restrict__ sh,
+                                                          uint32_t A, const uint32_t* __restrict__ axon_map) {
+  const uint32_t a = blockIdx.x * 256 + threadIdx.x;
+  if (a >= A) return;
+  const uint32_t e = axon_map[a];
+  const uint4* src = reinterpret_cast<const uint4*>(IN ? ref + a : sh + e);
+  uint4* dst = reinterpret_cast<uint4*>(IN ? sh + e : ref + a);
+  const uint4 x0 = src[0], x1 = src[1];
+  dst[0] = x0;
+  dst[1] = x1;
 }
 
 }  // namespace axc

@@ -123,6 +123,15 @@ class VramState:
             return
         _check(day_lib().cu_upload_axons_blob(C.byref(self.ptrs), ptr(blob), blob.nbytes), "cu_upload_axons_blob")
 
+    def set_soma_positions(self, positions: np.ndarray | None) -> None:
+        """axc_shard_set_soma_positions: the steps run in the engine's own
+        soma order (bricks of these positions); None returns to the caller's."""
+        if positions is None:
+            _check(day_lib().axc_shard_set_soma_positions(C.byref(self.ptrs), None, 0), "set_soma_positions")
+            return
+        pos = np.ascontiguousarray(positions, np.uint32)
+        _check(day_lib().axc_shard_set_soma_positions(C.byref(self.ptrs), ptr(pos), pos.size), "set_soma_positions")
+
     def download_state(self) -> np.ndarray:
         """The `.state` bytes, as save_hot_checkpoint reads them (shard_thread.rs:231-265)."""
         _, nbytes = calculate_state_blob_size(self.padded_n)

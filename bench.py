@@ -273,6 +273,9 @@ def main() -> int:
                     help="c2-linear: the c2 brain with somas numbered x-fastest instead of in bricks; "
                          "c2-baker: numbered as genesis-baker numbers a shard (z-planes, shuffled in each); "
                          "c5: the 12M-neuron x 128-slot scale point; c5-sub: its 200k-neuron sub-shard")
+    ap.add_argument("--engine-order", action="store_true",
+                    help="give the engine the soma positions (axc_shard_set_soma_positions) so that it steps "
+                         "in its own brick order whatever the workload's numbering")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--cpu-max-ticks", type=int, default=20)
     ap.add_argument("--cpu-single-ticks", type=int, default=3, help="ticks of the one-thread CPU oracle run")
@@ -345,6 +348,9 @@ def main() -> int:
     vram = VramState(N, A)
     vram.upload_state(brain.state)
     vram.upload_axon_heads(brain.heads)
+    if args.engine_order:
+        vram.set_soma_positions(brain.soma_pos)
+        log("engine order: steps run in the engine's brick order of the soma positions")
     eng = ShardEngine(vram)
     d_in = DeviceBuffer.from_array(inputs)
     ptrs = C.byref(vram.ptrs)
@@ -476,6 +482,7 @@ def main() -> int:
                          f"{spec.axon_len[0]}-{spec.axon_len[1]} segments, GSOP on, "
                          f"soma numbering {_numbering(spec.brick)}"),
             "padded_n": N, "total_axons": A, "ticks_per_step": T, "parallelism": "single GPU",
+            "engine_order": bool(args.engine_order),
         },
         "roofline": {
             "bound": "hbm", "kernel": "neuron_tick (K1 + K5: staged dendrite scan, GLIF soma update)",

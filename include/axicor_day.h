@@ -315,6 +315,19 @@ int32_t axc_step_day_phase_noalloc(const ShardVramPtrs *vram, uint32_t padded_n,
                                    const uint32_t *mapped_soma_ids, uint8_t *output_history, uint32_t num_outputs,
                                    int16_t dopamine, void *stream);
 
+/* Soma positions of the shard (PackedPosition u32 per soma, as the baker's
+ * shard.pos holds them, genesis-core/src/types.rs:16-46; n = padded_n; NULL
+ * returns to the caller's order). The staged steps then run in the engine's
+ * own soma order: somas by 8x8x8-voxel bricks, axons owned by somas in the
+ * same order, so that a block's presynaptic axons are few and close whatever
+ * the caller's numbering (the baker numbers somas by z-plane only,
+ * genesis-baker/src/bake/neuron_placement.rs:60-87). The caller's arrays keep
+ * the caller's order between calls: each step copies the soma arrays and
+ * heads into private ones and back. Results are identical in any order. A
+ * step whose virtual input axons are owned by somas is refused
+ * (hipErrorInvalidValue). Engine extension. */
+int32_t axc_shard_set_soma_positions(const ShardVramPtrs *vram, const uint32_t *positions, uint32_t n);
+
 /* Graphs recorded so far for the shard at vram (one per new batch shape). */
 int32_t axc_shard_graph_captures(const ShardVramPtrs *vram, uint64_t *out);
 

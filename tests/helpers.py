@@ -39,7 +39,10 @@ def compare_state(got_state, got_heads, want_state, want_heads, padded_n, where)
 class Pair:
     """A brain loaded on the engine next to its oracle copy."""
 
-    def __init__(self, brain: B.Brain, n_out: int = 0, max_batch: int = 64, max_spikes: int = 0, seed: int = 1):
+    def __init__(self, brain: B.Brain, n_out: int = 0, max_batch: int = 64, max_spikes: int = 0, seed: int = 1,
+                 positions: np.ndarray | None = None):
+        """positions: soma positions for the engine's own order (None: the
+        brain's numbering)."""
         self.b = brain
         self.ref_state = brain.state.copy()
         self.ref_heads = brain.heads.copy()
@@ -47,6 +50,8 @@ class Pair:
         self.vram = VramState(brain.padded_n, brain.total_axons)
         self.vram.upload_state(brain.state)
         self.vram.upload_axon_heads(brain.heads)
+        if positions is not None:
+            self.vram.set_soma_positions(positions)
         self.eng = ShardEngine(self.vram)
         self.max_batch = max_batch
         self.max_spikes = max_spikes

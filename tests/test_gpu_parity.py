@@ -13,6 +13,15 @@ from helpers import Pair
 
 pytestmark = pytest.mark.gpu
 
+ORDERS = pytest.mark.parametrize("engine_order", [False, True], ids=["caller_order", "engine_order"])
+
+
+def random_positions(n, seed):
+    """Soma positions that put the somas in a random order of bricks."""
+    rng = np.random.default_rng(seed)
+    x, y, z = rng.integers(0, 64, n), rng.integers(0, 64, n), rng.integers(0, 32, n)
+    return (x | (y << 10) | (z << 20) | (1 << 28)).astype(np.uint32)
+
 
 def test_c1_per_tick_then_batches(gpu):
     """C1 workload: 4,096 neurons x 64 slots, 256 inputs at p=0.05; every tick
@@ -74,12 +83,15 @@ def test_spike_schedule_duplicates_and_bad_ids(gpu):
     p.check("spike schedule")
 
 
-def test_graph_reused_across_varying_schedules(gpu):
+@ORDERS
+def test_graph_reused_across_varying_schedules(gpu, engine_order):
     """The batch graph does not depend on the spike counts: six graph batches
     whose schedules differ (empty, full slices, ragged counts, duplicates,
-    out-of-range ids) replay one captured graph, each equal to the oracle."""
+    out-of-range ids) replay one captured graph, each equal to the oracle;
+    in the caller's soma order and in the engine's (random positions)."""
     b = B.generate(B.spec_small(2048, 32, 128, seed=15))
-    p = Pair(b, max_spikes=48, max_batch=10, n_out=24)
+    p = Pair(b, max_spikes=48, max_batch=10, n_out=24,
+             positions=random_positions(b.padded_n, 5) if engine_order else None)
     rng = np.random.default_rng(21)
     T = 10
     for k in range(6):
@@ -99,10 +111,12 @@ def test_graph_reused_across_varying_schedules(gpu):
     assert p.eng.graph_captures() == 1
 
 
-def test_edge_state_values(gpu):
+@ORDERS
+def test_edge_state_values(gpu, engine_order):
     """Hand-placed edge cases: heads close to the u32 wrap and past the
     sentinel, targets whose axon field is 0 or >= A, two somas sharing one
-    axon, somas without an axon, extreme voltages/thresholds/weights."""
+    axon, somas without an axon, extreme voltages/thresholds/weights; in the
+    caller's soma order and in the engine's (random positions), per tick."""
     b = B.generate(B.spec_small(1024, 64, 64, seed=9))
     v = b.view()
     n, A = b.padded_n, b.total_axons
@@ -123,7 +137,7 @@ def test_edge_state_values(gpu):
     v.weights[:, 41] = 32767
     v.weights[:, 42] = 0
     v.timers[50] = 255
-    p = Pair(b, n_out=16)
+    p = Pair(b, n_out=16, positions=random_positions(b.padded_n, 3) if engine_order else None)
     for k in range(12):
         p.step(5, dopamine=(-1000 if k % 3 == 0 else 1000))
         p.check(f"edge step {k}")
@@ -211,16 +225,18 @@ def test_graph_and_per_tick_paths_agree_on_counters(gpu):
     p.check("graph counters")
 
 
-def test_night_phase_with_sprouting(gpu):
+@ORDERS
+def test_night_phase_with_sprouting(gpu, engine_order):
     """The Night sequence of shard_thread.rs:269-378 on the GPU shard (flags
     before pruning, GPU sort&prune, host sprouting, upload) against the CPU
     path (oracle sort&prune, the same sprouting on host arrays); the Day Phase
-    continues afterwards on both."""
+    continues afterwards on both (in the engine's soma order: the plan is
+    rebuilt in it after each upload)."""
     from axicor_amd import night as NT
     from axicor_amd.layout import StateView
     from oracle import oracle as O
     b = B.generate(B.spec_c1(), with_paths=True)
-    p = Pair(b)
+    p = Pair(b, positions=random_positions(b.padded_n, 7) if engine_order else None)
     types = NT.sprout_types_from_lut(b.lut)
     geo_gpu, geo_cpu = NT.geometry_of(b), NT.geometry_of(b)
     for night in range(2):

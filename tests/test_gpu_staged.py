@@ -180,16 +180,19 @@ def test_graph_heads_near_the_wrap_and_sentinel(gpu):
 
 
 @GRAPH
-def test_baker_numbering(gpu, use_graph):
+@pytest.mark.parametrize("engine_order", [False, True], ids=["caller_order", "engine_order"])
+def test_baker_numbering(gpu, use_graph, engine_order):
     """Somas numbered as genesis-baker numbers a shard (neuron_placement.rs
     :60-87: z-planes in order, each plane's somas shuffled): 60k neurons x
     100 slots on a 48x48x32 grid, 1.5M-neuron brain's four types, against
-    the oracle."""
+    the oracle; in the caller's order and in the engine's own (the soma
+    positions given: bricks)."""
     spec = B.spec_small(60000, 100, 3000, seed=41, axon_len=(1, 255), grid=(48, 48, 32), lut=B.lut_c2(),
                         type_ppm=[400_000, 400_000, 100_000, 100_000])
     spec.brick = B.BAKER_ORDER
     spec.input_ppm = 20_000
-    p = Pair(B.generate(spec), n_out=64)
+    b = B.generate(spec)
+    p = Pair(b, n_out=64, positions=b.soma_pos if engine_order else None)
     for k in range(3):
         out_ref, _ = p.step(20, dopamine=(60 if k % 2 else -40), use_graph=use_graph)
         np.testing.assert_array_equal(p.outputs(20), out_ref)
