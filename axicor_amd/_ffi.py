@@ -139,6 +139,9 @@ SIGNATURES = {
     "axc_batch_destroy": (None, [_P]),
     "axc_version": (C.c_char_p, []),
     "axc_staging_enable": (None, [C.c_int]),
+    "axc_set_window_ticks": (_I32, [_U32]),
+    "axc_window_ticks": (_U32, []),
+    "axc_shard_window_stats": (_I32, [_VP, C.POINTER(C.c_uint64)]),
     "axc_shard_targets_changed": (None, [C.POINTER(ShardVramPtrs)]),
     "axc_shard_graph_captures": (_I32, [_VP, C.POINTER(C.c_uint64)]),
     "axc_shard_set_soma_positions": (_I32, [_VP, _P, _U32]),
@@ -147,7 +150,8 @@ SIGNATURES = {
                                           _P]),
 }
 
-TIMED_KERNELS = ["axon_tick", "neuron_tick", "push_apply", "gsop", "readout", "spike_count", "weights_copy"]
+TIMED_KERNELS = ["axon_tick", "neuron_tick", "push_apply", "gsop", "readout", "spike_count", "weights_copy",
+                 "window_start", "window_pass", "window_correct", "window_soma"]
 
 SURROGATE_SIGNATURES = {
     "axc_mlp_create": (_I32, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint64, C.POINTER(C.c_void_p)]),

@@ -112,4 +112,47 @@ struct StagePlan {
   const uint32_t* axon_map;
 };
 
+// Windows of ticks of the batch graphs (day_window.hip): the slots are
+// streamed once per window of up to kWin ticks.
+#ifndef AXC_WIN_TICKS
+#define AXC_WIN_TICKS 4
+#endif
+constexpr uint32_t kWin = AXC_WIN_TICKS;
+
+// Axons pushed since the window started: flag[a] == ctl[1] (the window's
+// stamp), npush[a] their pushes so far (at most 8: the start heads shifted
+// out). Each tick's corrections are pieces of up to 16 reverse entries,
+// start | axon << 32 | (count - 1) << 56, kept in kShards parts by axon
+// group ((a / 64) % kShards, so that no counter takes many atomics): part s
+// at pieces[shard_off[s] + k], k < shard_cnt[(tick & 1) * kShards + s]; its
+// capacity is the most pieces its axons can make.
+constexpr uint32_t kShards = 256;
+struct DirtyMarks {
+  uint32_t* flag;       // [A]
+  uint8_t* npush;       // [A]
+  uint32_t* ctl;        // [2]: (unused), stamp
+  uint64_t* pieces;     // [shard_off[kShards]]
+  uint32_t* shard_off;  // [kShards + 1]
+  uint32_t* shard_cnt;  // [2][kShards] pieces of even and odd ticks
+};
+
+struct WinState {
+  int64_t* acc;               // [kWin][N] per soma and tick of the window: hits << 32 plus the hit weights' sum
+  uint2* comp8_w0;            // [A+1] byte snapshot of the window's start heads; entry A inert
+  AxcBurstHeads8* heads_w0;   // [A] the window's start heads
+  uint8_t* hazard;            // [A] a head wraps past 2^32 within the window
+  uint32_t* rev_off;          // [A+1] the reverse index: axon a's slots at rev[rev_off[a], rev_off[a+1]),
+                              // sorted by segment
+  uint64_t* rev;              // soma | slot << 32 | segment << 40
+  uint4* rev_cut;             // [A] per segment s < kRevCuts the end (u16, relative to rev_off[a]) of axon
+                              // a's entries with a segment <= s; 0xFFFF past the u16 range
+  uint64_t* cnt;              // [kWinCounters] work counters, accumulated across batches
+  uint64_t* cpart;            // [kListBlocks][2] the correction kernel's per-workgroup pieces and entries
+};
+// Window work counters: windows, somas scanned by the window passes, their
+// filled slots (then, from cpart: correction pieces and reverse entries
+// tested, summed over ticks).
+constexpr uint32_t kWinCounters = 5;
+constexpr uint32_t kRevCuts = 8;
+
 }  // namespace axc

@@ -20,6 +20,7 @@
 #include "day_common.h"
 #include "day_kernels.hip"
 #include "day_staged.hip"
+#include "day_window.hip"
 #include "batch_engine.hip"
 #include <atomic>
 
@@ -91,13 +92,15 @@ struct GraphKey {
   uint32_t v_seg;
   bool staged;
   bool ordered;  // in the engine's soma order (the graph holds the private arrays)
+  uint32_t win;      // ticks per window (1: the per-tick staged pass)
+  const void* rev;   // the window's reverse index (the graph holds its address)
   // The per-tick spike counts are not part of the key: the graph's K3
   // kernels read them from the shard's device copy (TickParams::counts).
   bool operator==(const GraphKey& o) const {
     return T == o.T && words == o.words && virt_off == o.virt_off && n_virt == o.n_virt && staged == o.staged &&
            max_spk == o.max_spk && n_out == o.n_out && has_bitmask == o.has_bitmask &&
            has_spikes == o.has_spikes && has_out == o.has_out && mapped == o.mapped && heads == o.heads &&
-           v_seg == o.v_seg && ordered == o.ordered;
+           v_seg == o.v_seg && ordered == o.ordered && win == o.win && rev == o.rev;
   }
 };
 
@@ -144,17 +147,30 @@ struct Shard {
   std::vector<uint32_t> owned;     // the axons owned by somas, ascending (moved by the order)
   ShardVramPtrs shadow{};          // private voltage, flags, threshold, timers, soma_to_axon, heads
   bool ordered() const { return !order.empty(); }
+  // Windows of ticks (day_window.hip): allocated at the first windowed
+  // batch; the reverse index follows the plan (rebuilt with it).
+  WinState win{};
+  DirtyMarks dirty{};
+  size_t rev_cap = 0;
+  bool rev_ready = false;
+  uint64_t rev_version = 0;  // targets_version the reverse index was built at
 };
 
 // Staged scan on/off for the whole process (on by default).
 std::atomic<bool> g_staging{true};
+// Ticks per window of the batch graphs' staged pass (day_window.hip); 1 is
+// the per-tick pass.
+std::atomic<uint32_t> g_window{1};
 
 std::mutex g_mu;
 std::unordered_map<const void*, Shard*> g_shards;
 
 // Optional per-kernel timing of the per-tick path: HIP events recorded on the
 // caller's stream around each launch, summed by axc_kernel_timing_read.
-enum TimedKernel { kKAxon = 0, kKNeuron, kKPush, kKGsop, kKReadout, kKSpikes, kKWeights, kNumTimed };
+enum TimedKernel {
+  kKAxon = 0, kKNeuron, kKPush, kKGsop, kKReadout, kKSpikes, kKWeights,
+  kKWinStart, kKWinPass, kKWinCorrect, kKWinSoma, kNumTimed
+};
 struct TimingState {
   bool enabled = false;
   std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending;
@@ -193,6 +209,18 @@ void free_order_arrays(Shard* s) {
   s->shadow = ShardVramPtrs{};
 }
 
+void free_window(Shard* s) {
+  for (void* q : {(void*)s->win.acc, (void*)s->win.comp8_w0, (void*)s->win.heads_w0, (void*)s->win.hazard,
+                  (void*)s->win.rev_off, (void*)s->win.rev, (void*)s->win.rev_cut, (void*)s->win.cnt,
+                  (void*)s->win.cpart, (void*)s->dirty.flag, (void*)s->dirty.npush, (void*)s->dirty.ctl,
+                  (void*)s->dirty.pieces, (void*)s->dirty.shard_off, (void*)s->dirty.shard_cnt})
+    if (q) (void)hipFree(q);
+  s->win = WinState{};
+  s->dirty = DirtyMarks{};
+  s->rev_cap = 0;
+  s->rev_ready = false;
+}
+
 void free_scratch(Shard* s) {
   if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
   if (s->graph) (void)hipGraphDestroy(s->graph);
@@ -220,6 +248,7 @@ void free_scratch(Shard* s) {
   s->sp = StagePlan{};
   s->stage_dirty = true;
   free_order_arrays(s);
+  free_window(s);
 }
 
 // Makes `st` wait for the last asynchronous host upload into the shard's
@@ -371,7 +400,9 @@ bool ensure_stage(Shard* s, const ShardVramPtrs& v, hipStream_t st, hipError_t* 
     std::call_once(attr_once, [] {
       const int bytes = (int)((kStageCap + 1) * sizeof(uint2));
       for (const void* f : {(const void*)neuron_staged_kernel<false>, (const void*)neuron_staged_kernel<true>,
-                            (const void*)batch_staged_kernel})
+                            (const void*)batch_staged_kernel, (const void*)win_pass_kernel<1>,
+                            (const void*)win_pass_kernel<2>, (const void*)win_pass_kernel<3>,
+                            (const void*)win_pass_kernel<4>})
         (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
     });
     StagePlan p{};
@@ -405,6 +436,7 @@ bool ensure_stage(Shard* s, const ShardVramPtrs& v, hipStream_t st, hipError_t* 
     }
     stage_build_kernel<<<stage_blocks((uint32_t)N), kBlockSomas, 0, st>>>(v.dendrite_targets, (uint32_t)N,
                                                                           (uint32_t)A, s->sp);
+    s->rev_ready = false;
     *err = hipGetLastError();
     if (*err != hipSuccess) {
       s->stage_dirty = true;
@@ -605,6 +637,177 @@ hipError_t enqueue_tick(const ShardVramPtrs& v, Shard* s, uint32_t v_seg, uint32
     TimedScope ts(kKReadout, st, timed);
     readout_kernel<<<blocks_for(n_out), kThreads, 0, st>>>(v.soma_flags, N, mapped, out, n_out, prm, tick,
                                                            (staged && s->sp.perm) ? s->d_perm_inv : nullptr);
+  }
+  return hipGetLastError();
+}
+
+// ---- windows of ticks (day_window.hip) -------------------------------------
+// The window length a batch uses: g_window, or 1 where the byte form of the
+// shifted segments could wrap (K v_seg must stay far below 2^16).
+uint32_t window_for(uint32_t v_seg) {
+  const uint32_t k = g_window.load();
+  if (k < 2) return 1;
+  return (uint64_t)k * v_seg < 0x8000u ? k : 1;
+}
+
+// Allocates the window buffers (first windowed batch) and builds the
+// reverse index of the current plan when it is missing or stale. Outside
+// graph capture: it synchronizes `st` once to size the index (a larger
+// index moves, and the batch graph holding its address is recorded again).
+hipError_t ensure_window(Shard* s, hipStream_t st) {
+  const size_t N = s->N, A = s->A;
+  hipError_t e = hipSuccess;
+  if (!s->win.acc) {
+    WinState w{};
+    DirtyMarks d{};
+    e = hipMalloc((void**)&w.acc, (size_t)kWin * (N ? N : 1) * sizeof(int64_t));
+    if (e == hipSuccess) e = hipMalloc((void**)&w.comp8_w0, (A + 1) * sizeof(uint2));
+    if (e == hipSuccess) e = hipMalloc((void**)&w.heads_w0, (A ? A : 1) * sizeof(AxcBurstHeads8));
+    if (e == hipSuccess) e = hipMalloc((void**)&w.hazard, A ? A : 1);
+    if (e == hipSuccess) e = hipMalloc((void**)&w.rev_off, (A + 1) * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc((void**)&w.rev_cut, (A ? A : 1) * sizeof(uint4));
+    if (e == hipSuccess) e = hipMalloc((void**)&d.flag, (A ? A : 1) * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc((void**)&d.npush, A ? A : 1);
+    if (e == hipSuccess) e = hipMalloc((void**)&d.ctl, 2 * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc((void**)&d.shard_off, (kShards + 1) * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc((void**)&d.shard_cnt, 2 * kShards * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMemsetAsync(d.shard_cnt, 0, 2 * kShards * sizeof(uint32_t), st);
+    if (e == hipSuccess) e = hipMalloc((void**)&w.cnt, kWinCounters * sizeof(uint64_t));
+    if (e == hipSuccess) e = hipMalloc((void**)&w.cpart, (size_t)kListBlocks * 2 * sizeof(uint64_t));
+    if (e == hipSuccess) e = hipMemsetAsync(w.cnt, 0, kWinCounters * sizeof(uint64_t), st);
+    if (e == hipSuccess) e = hipMemsetAsync(w.cpart, 0, (size_t)kListBlocks * 2 * sizeof(uint64_t), st);
+    if (e == hipSuccess) e = hipMemsetAsync(d.npush, 0, A ? A : 1, st);
+    if (e == hipSuccess) e = hipMemsetAsync(d.flag, 0, (A ? A : 1) * sizeof(uint32_t), st);
+    if (e == hipSuccess) e = hipMemsetAsync(d.ctl, 0, 2 * sizeof(uint32_t), st);
+    s->win = w;
+    s->dirty = d;
+    if (e != hipSuccess) {
+      free_window(s);
+      return e;
+    }
+  }
+  const uint64_t ver = s->targets_version.load();
+  if (s->rev_ready && s->rev_version == ver) return hipSuccess;
+  // Counts per axon into rev_off, their exclusive prefix in place, the
+  // total read back to size rev, then the entries (at a cursor per axon).
+  if ((e = hipMemsetAsync(s->win.rev_off, 0, (A + 1) * sizeof(uint32_t), st)) != hipSuccess) return e;
+  if (N) rev_count_kernel<<<blocks_for(N), kThreads, 0, st>>>((uint32_t)N, (uint32_t)A, s->sp, s->win.rev_off);
+  rev_scan_kernel<<<1, 1024, 0, st>>>(s->win.rev_off, (uint32_t)A, s->win.rev_off);
+  uint32_t total = 0;
+  if ((e = hipMemcpyAsync(&total, s->win.rev_off + A, 4, hipMemcpyDeviceToHost, st)) != hipSuccess) return e;
+  if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
+  if (total > s->rev_cap || !s->win.rev) {
+    // The index, and the correction pieces of a tick: an axon's range of
+    // c entries makes ceil(c / 16) pieces, so at most total / 16 + A.
+    (void)hipFree(s->win.rev);
+    (void)hipFree(s->dirty.pieces);
+    s->win.rev = nullptr;
+    s->dirty.pieces = nullptr;
+    s->rev_cap = 0;
+    const size_t cap = (size_t)total + total / 8 + 1024;
+    if ((e = hipMalloc((void**)&s->win.rev, cap * sizeof(uint64_t))) != hipSuccess) return e;
+    if ((e = hipMalloc((void**)&s->dirty.pieces, (cap / 16 + A + 16) * sizeof(uint64_t))) != hipSuccess) return e;
+    s->rev_cap = cap;
+  }
+  // The entries into a temporary array, then sorted by segment into rev.
+  uint64_t* tmp = nullptr;
+  uint32_t* cursor = nullptr;
+  e = hipMalloc((void**)&tmp, ((size_t)total + 1) * sizeof(uint64_t));
+  if (e == hipSuccess) e = hipMalloc((void**)&cursor, (A + 1) * sizeof(uint32_t));
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(cursor, s->win.rev_off, (A + 1) * sizeof(uint32_t), hipMemcpyDeviceToDevice, st);
+  if (e != hipSuccess) {
+    (void)hipFree(tmp);
+    (void)hipFree(cursor);
+    return e;
+  }
+  if (N) rev_fill_kernel<<<blocks_for(N), kThreads, 0, st>>>((uint32_t)N, (uint32_t)A, s->sp, cursor, tmp);
+  if (A) rev_sort_kernel<<<std::min<uint32_t>(blocks_for(A, 4), 8192u), kThreads, 0, st>>>((uint32_t)A,
+                                                                                           s->win.rev_off, tmp,
+                                                                                           s->win.rev,
+                                                                                           s->win.rev_cut);
+  // The correction parts' capacities and offsets (cursor as scratch).
+  if (e == hipSuccess) e = hipMemsetAsync(cursor, 0, kShards * sizeof(uint32_t), st);
+  if (e == hipSuccess && A)
+    shard_cap_kernel<<<blocks_for(A), kThreads, 0, st>>>((uint32_t)A, s->win.rev_off, cursor);
+  if (e == hipSuccess) rev_scan_kernel<<<1, 1024, 0, st>>>(cursor, kShards, s->dirty.shard_off);
+  e = hipGetLastError();
+  const hipError_t e2 = hipStreamSynchronize(st);
+  (void)hipFree(tmp);
+  (void)hipFree(cursor);
+  if (e == hipSuccess) e = e2;
+  if (e != hipSuccess) return e;
+  s->rev_ready = true;
+  s->rev_version = ver;
+  return hipSuccess;
+}
+
+// The T ticks of a batch in windows of `win` ticks, on the device
+// parameters prm (tick_base, schedule counts, slices), as the batch graphs
+// run them; `timed` times each launch (never while capturing).
+hipError_t enqueue_windows(const ShardVramPtrs& v, Shard* s, uint32_t v_seg, uint32_t T, uint32_t win, uint32_t words,
+                           uint32_t virt_off, uint32_t n_virt, bool has_bitmask, bool has_spikes, uint32_t max_spk,
+                           const uint32_t* mapped, bool has_out, uint32_t n_out, const TickParams* prm,
+                           hipStream_t st, const AxcVariantParameters* lut, bool timed) {
+  const uint32_t N = s->N, A = s->A;
+  const bool k2 = has_bitmask && n_virt > 0;
+  const bool k3 = has_spikes && max_spk > 0;
+  const size_t lds = (kStageCap + 1) * sizeof(uint2);
+  const uint32_t* axon_inv = s->sp.perm ? s->d_axon_inv : nullptr;
+  for (uint32_t t0 = 0; t0 < T; t0 += win) {
+    const uint32_t K = std::min(win, T - t0);
+    const uint32_t lim = v_seg ? (uint32_t)((1ull << 32) - (uint64_t)(K + 1) * v_seg) : 0u;
+    {
+      TimedScope ts(kKWinStart, st, timed);
+      win_start_kernel<<<blocks_for((uint64_t)A + 1), kThreads, 0, st>>>(v.axon_heads, A, lim, s->win, s->dirty);
+    }
+    if (N > 0) {
+      TimedScope ts(kKWinPass, st, timed);
+      const uint32_t g = staged_grid(N);
+      if (K == 1) win_pass_kernel<1><<<g, kStageThreads, lds, st>>>(v, N, A, lut, s->sp, s->win, v_seg, 0x00010001u);
+      else if (K == 2) win_pass_kernel<2><<<g, kStageThreads, lds, st>>>(v, N, A, lut, s->sp, s->win, v_seg, 0x00010001u);
+      else if (K == 3) win_pass_kernel<3><<<g, kStageThreads, lds, st>>>(v, N, A, lut, s->sp, s->win, v_seg, 0x00010001u);
+      else win_pass_kernel<4><<<g, kStageThreads, lds, st>>>(v, N, A, lut, s->sp, s->win, v_seg, 0x00010001u);
+    }
+    for (uint32_t j = 0; j < K; j++) {
+      const uint32_t t = t0 + j;
+      if (k3) {
+        TimedScope ts(kKSpikes, st, timed);
+        spike_count_kernel<<<std::min<uint32_t>(blocks_for(max_spk), 64u), kThreads, 0, st>>>(
+            nullptr, 0, A, s->sc.push_cnt, prm, t, max_spk, s->sp.axon_map);
+      }
+      {
+        TimedScope ts(kKAxon, st, timed);
+        win_axon_kernel<<<blocks_for(A > kNumCounters ? A : kNumCounters), kThreads, 0, st>>>(
+            v.axon_heads, A, v_seg, virt_off, k2 ? n_virt : 0u, s->sc.push_cnt, k3 ? 1 : 0, s->sc.counters, prm, t,
+            words, s->sp.comp8, axon_inv, lut, s->win, s->dirty, j);
+      }
+      if (N > 0) {
+        {
+          TimedScope ts(kKWinCorrect, st, timed);
+          win_correct_kernel<<<kListBlocks, kThreads, 0, st>>>(v.axon_heads, v.soma_flags, N, A, lut, s->sp, s->win,
+                                                               s->dirty, v_seg, j);
+        }
+        TimedScope ts(kKWinSoma, st, timed);
+        win_soma_kernel<<<blocks_for(N), kThreads, 0, st>>>(v, N, A, lut, prm, t, s->sc, s->sp, s->win, j);
+      }
+      {
+        TimedScope ts(kKPush, st, timed);
+        const uint32_t n_words = (N + 63) / 64;
+        if (N > 0) compact_spikes_kernel<<<blocks_for(n_words), kThreads, 0, st>>>(n_words, s->sc);
+        win_push_kernel<<<kListBlocks / 4, kThreads, 0, st>>>(v.axon_heads, v.soma_to_axon, A, s->sc, s->sp.comp8,
+                                                             s->dirty);
+      }
+      {
+        TimedScope ts(kKGsop, st, timed);
+        gsop_win_kernel<<<kListBlocks, kThreads, 0, st>>>(v, N, A, lut, prm, 0, s->sc, s->sp, s->win, v_seg, j, K);
+      }
+      if (has_out && n_out > 0) {
+        TimedScope ts(kKReadout, st, timed);
+        readout_kernel<<<blocks_for(n_out), kThreads, 0, st>>>(v.soma_flags, N, mapped, nullptr, n_out, prm, t,
+                                                               s->sp.perm ? s->d_perm_inv : nullptr);
+      }
+    }
   }
   return hipGetLastError();
 }
@@ -1156,7 +1359,32 @@ int32_t axc_shard_stats_reset(const ShardVramPtrs* vram, void* stream) {
   hipStream_t st = (hipStream_t)stream;
   hipError_t e = hipMemsetAsync(s->sc.stats_part, 0, (size_t)s->stats_blocks * kStatsPerBlock * 8, st);
   if (e == hipSuccess) e = hipMemsetAsync(s->sc.stats_part2, 0, (size_t)kListBlocks * kStatsPerBlock * 8, st);
+  if (e == hipSuccess && s->win.cnt) e = hipMemsetAsync(s->win.cnt, 0, kWinCounters * sizeof(uint64_t), st);
+  if (e == hipSuccess && s->win.cpart)
+    e = hipMemsetAsync(s->win.cpart, 0, (size_t)kListBlocks * 2 * sizeof(uint64_t), st);
   return (int32_t)e;
+}
+
+int32_t axc_shard_window_stats(const ShardVramPtrs* vram, uint64_t* out) {
+  if (!vram || !out) return (int32_t)hipErrorInvalidValue;
+  Shard* s = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = g_shards.find(vram->soma_voltage);
+    if (it != g_shards.end()) s = it->second;
+  }
+  if (!s) return (int32_t)hipErrorInvalidValue;
+  for (uint32_t k = 0; k < kWinCounters; k++) out[k] = 0;
+  if (!s->win.cnt) return 0;
+  hipError_t e = hipMemcpy(out, s->win.cnt, 3 * sizeof(uint64_t), hipMemcpyDeviceToHost);
+  std::vector<uint64_t> part((size_t)kListBlocks * 2);
+  if (e == hipSuccess) e = hipMemcpy(part.data(), s->win.cpart, part.size() * 8, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return (int32_t)e;
+  for (size_t k = 0; k < kListBlocks; k++) {
+    out[3] += part[2 * k];
+    out[4] += part[2 * k + 1];
+  }
+  return 0;
 }
 
 int32_t axc_shard_stats_read(const ShardVramPtrs* vram, AxcShardStats* out) {
@@ -1243,6 +1471,13 @@ int32_t axc_step_day_phase_batch(const ShardVramPtrs* vram, uint32_t padded_n, u
   // back after the last.
   const bool fuse = !key.staged;
   const bool pw = key.staged;
+  // Staged batches stream the slots once per window of key.win ticks.
+  key.win = key.staged ? window_for(v_seg) : 1u;
+  key.rev = nullptr;
+  if (key.win > 1) {
+    if ((e = ensure_window(s, st)) != hipSuccess) return (int32_t)e;
+    key.rev = s->win.rev;
+  }
   std::vector<uint32_t> counts(T, 0);
   if (d_spikes && h_counts) counts.assign(h_counts, h_counts + T);
 
@@ -1253,19 +1488,54 @@ int32_t axc_step_day_phase_batch(const ShardVramPtrs* vram, uint32_t padded_n, u
     std::lock_guard<std::mutex> lk(g_tmu);
     timed = g_timing.enabled;
   }
+  // The batch's device parameters (the graphs, and windowed batches also
+  // when timed): tick base, dopamine, slices, per-tick schedule counts.
+  auto upload_params = [&]() -> hipError_t {
+    TickParams pv;
+    pv.tick_base = tick_base;
+    pv.dopamine = (int32_t)dopamine;
+    pv.bitmask = d_bitmask;
+    pv.spikes = d_spikes;
+    pv.out = d_out;
+    pv.counts = nullptr;
+    hipError_t r;
+    if (key.has_spikes) {
+      // The counts go to the shard's device copy, read by the graph's K3
+      // kernels (stream order keeps the previous replay's reads before this).
+      if (s->counts_cap < T) {
+        (void)hipStreamSynchronize(st);
+        (void)hipFree(s->d_counts);
+        s->d_counts = nullptr;
+        s->counts_cap = 0;
+        if ((r = hipMalloc((void**)&s->d_counts, (size_t)T * 4)) != hipSuccess) return r;
+        s->counts_cap = T;
+      }
+      if ((r = hipMemcpyAsync(s->d_counts, counts.data(), (size_t)T * 4, hipMemcpyHostToDevice, st)) != hipSuccess)
+        return r;
+      pv.counts = s->d_counts;
+    }
+    set_params_kernel<<<1, 1, 0, st>>>(s->prm, pv);
+    return hipGetLastError();
+  };
   if (timed) {
     // Kernel by kernel, so that each launch can be timed: the same kernels
-    // and tick structure as the graph below, arguments passed directly.
+    // and tick structure as the graph below.
+    if (key.win > 1 && (e = upload_params()) != hipSuccess) return (int32_t)e;
     if (key.ordered) order_copy(s, *vram, true, st);
     if (pw) {
       TimedScope ts(kKWeights, st, true);
       copy_weights_kernel<true><<<blocks_for(padded_n), kThreads, 0, st>>>(vram->dendrite_weights, padded_n, s->sp);
     }
-    for (uint32_t t = 0; t < T && e == hipSuccess; t++) {
-      e = enqueue_tick(V, s, v_seg, tick_base + t, d_bitmask ? d_bitmask + (size_t)t * words : nullptr, words,
-                       virt_off, n_virt, d_spikes ? d_spikes + (size_t)t * max_spk : nullptr, max_spk,
-                       counts[t], d_mapped, d_out ? d_out + (size_t)t * n_out : nullptr, n_out,
-                       (int32_t)dopamine, nullptr, st, lut, t, t > 0 && fuse, t + 1 == T || !fuse, key.staged, pw);
+    if (key.win > 1) {
+      e = enqueue_windows(V, s, v_seg, T, key.win, words, virt_off, n_virt, key.has_bitmask, key.has_spikes, max_spk,
+                          d_mapped, key.has_out, n_out, s->prm, st, lut, true);
+    } else {
+      for (uint32_t t = 0; t < T && e == hipSuccess; t++) {
+        e = enqueue_tick(V, s, v_seg, tick_base + t, d_bitmask ? d_bitmask + (size_t)t * words : nullptr, words,
+                         virt_off, n_virt, d_spikes ? d_spikes + (size_t)t * max_spk : nullptr, max_spk,
+                         counts[t], d_mapped, d_out ? d_out + (size_t)t * n_out : nullptr, n_out,
+                         (int32_t)dopamine, nullptr, st, lut, t, t > 0 && fuse, t + 1 == T || !fuse, key.staged, pw);
+      }
     }
     if (pw && e == hipSuccess) {
       TimedScope ts(kKWeights, st, true);
@@ -1276,29 +1546,7 @@ int32_t axc_step_day_phase_batch(const ShardVramPtrs* vram, uint32_t padded_n, u
     if (e == hipSuccess) e = hipGetLastError();
     return (int32_t)e;
   }
-  TickParams pv;
-  pv.tick_base = tick_base;
-  pv.dopamine = (int32_t)dopamine;
-  pv.bitmask = d_bitmask;
-  pv.spikes = d_spikes;
-  pv.out = d_out;
-  pv.counts = nullptr;
-  if (key.has_spikes) {
-    // The counts go to the shard's device copy, read by the graph's K3
-    // kernels (stream order keeps the previous replay's reads before this).
-    if (s->counts_cap < T) {
-      (void)hipStreamSynchronize(st);
-      (void)hipFree(s->d_counts);
-      s->d_counts = nullptr;
-      s->counts_cap = 0;
-      if ((e = hipMalloc((void**)&s->d_counts, (size_t)T * 4)) != hipSuccess) return (int32_t)e;
-      s->counts_cap = T;
-    }
-    if ((e = hipMemcpyAsync(s->d_counts, counts.data(), (size_t)T * 4, hipMemcpyHostToDevice, st)) != hipSuccess)
-      return (int32_t)e;
-    pv.counts = s->d_counts;
-  }
-  set_params_kernel<<<1, 1, 0, st>>>(s->prm, pv);
+  if ((e = upload_params()) != hipSuccess) return (int32_t)e;
 
   if (!(s->has_graph && s->gkey == key)) {
     if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
@@ -1314,12 +1562,17 @@ int32_t axc_step_day_phase_batch(const ShardVramPtrs* vram, uint32_t padded_n, u
       if (pw)
         copy_weights_kernel<true><<<blocks_for(padded_n), kThreads, 0, cap>>>(vram->dendrite_weights, padded_n,
                                                                                s->sp);
-      for (uint32_t t = 0; t < T && e == hipSuccess; t++) {
-        // GSOP of tick t runs inside tick t+1's neuron pass; the last tick's
-        // GSOP is launched on its own so the batch leaves a complete state.
-        e = enqueue_tick(V, s, v_seg, t, d_bitmask, words, virt_off, n_virt, d_spikes, max_spk,
-                         0, d_mapped, d_out, n_out, 0, s->prm, cap, lut, t, t > 0 && fuse,
-                         t + 1 == T || !fuse, key.staged, pw);
+      if (key.win > 1) {
+        e = enqueue_windows(V, s, v_seg, T, key.win, words, virt_off, n_virt, key.has_bitmask, key.has_spikes,
+                            max_spk, d_mapped, key.has_out, n_out, s->prm, cap, lut, false);
+      } else {
+        for (uint32_t t = 0; t < T && e == hipSuccess; t++) {
+          // GSOP of tick t runs inside tick t+1's neuron pass; the last tick's
+          // GSOP is launched on its own so the batch leaves a complete state.
+          e = enqueue_tick(V, s, v_seg, t, d_bitmask, words, virt_off, n_virt, d_spikes, max_spk,
+                           0, d_mapped, d_out, n_out, 0, s->prm, cap, lut, t, t > 0 && fuse,
+                           t + 1 == T || !fuse, key.staged, pw);
+        }
       }
       if (pw && e == hipSuccess)
         copy_weights_kernel<false><<<blocks_for(padded_n), kThreads, 0, cap>>>(vram->dendrite_weights, padded_n,
@@ -1393,6 +1646,13 @@ void axc_host_sort_and_prune(uint8_t* flags, uint32_t* targets, int16_t* weights
 }
 
 void axc_staging_enable(int enable) { g_staging = enable != 0; }
+
+int32_t axc_set_window_ticks(uint32_t ticks) {
+  if (ticks > kWin) return (int32_t)hipErrorInvalidValue;
+  g_window = ticks < 1 ? 1u : ticks;
+  return 0;
+}
+uint32_t axc_window_ticks(void) { return g_window.load(); }
 
 int32_t axc_shard_set_soma_positions(const ShardVramPtrs* vram, const uint32_t* positions, uint32_t n) {
   if (!vram) return (int32_t)hipErrorInvalidValue;

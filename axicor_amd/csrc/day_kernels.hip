@@ -185,12 +185,14 @@ __device__ __forceinline__ GsopTerms gsop_terms(uint8_t flags, const AxcVariantP
 // Also writes the compact snapshot of the propagated heads.
 // ---------------------------------------------------------------------------
 // axon_inv (may be null): the caller's id of each of the engine's axons,
-// which sets the axon's bit in the input bitmask.
-__device__ __forceinline__ void axon_step(AxcBurstHeads8* __restrict__ heads, uint32_t a, uint32_t v_seg,
+// which sets the axon's bit in the input bitmask. Returns the pushes applied;
+// out (may be null) receives the new heads.
+__device__ __forceinline__ uint32_t axon_step(AxcBurstHeads8* __restrict__ heads, uint32_t a, uint32_t v_seg,
                                           const uint32_t* __restrict__ bitmask, uint32_t virt_off, uint32_t n_virt,
                                           uint32_t* __restrict__ push_cnt, bool use_cnt, uint4* __restrict__ comp,
                                           uint2* __restrict__ comp8 = nullptr,
-                                          const uint32_t* __restrict__ axon_inv = nullptr) {
+                                          const uint32_t* __restrict__ axon_inv = nullptr,
+                                          Heads8* out = nullptr) {
   Heads8 h = load_heads(heads, a);
   uint32_t k = 0;
   if (bitmask) {
@@ -216,6 +218,8 @@ __device__ __forceinline__ void axon_step(AxcBurstHeads8* __restrict__ heads, ui
   store_heads(heads, a, h);
   if (comp) comp[a] = compact_heads(h);
   if (comp8) comp8[a] = byte_heads(h);
+  if (out) *out = h;
+  return k;
 }
 
 __global__ __launch_bounds__(256) void axon_tick_kernel(AxcBurstHeads8* __restrict__ heads, uint32_t A,
@@ -576,20 +580,22 @@ __device__ __forceinline__ void push_apply_one(AxcBurstHeads8* __restrict__ head
 
 // The pushes of spiker j's own axon (the staged pass counted them in
 // push_cnt without an owner list): the count is taken with an exchange, so
-// somas sharing an axon push it once with the full count.
-__device__ __forceinline__ void push_spiker_one(AxcBurstHeads8* __restrict__ heads,
+// somas sharing an axon push it once with the full count. Returns the axon
+// pushed, or kNoAxon.
+__device__ __forceinline__ uint32_t push_spiker_one(AxcBurstHeads8* __restrict__ heads,
                                                 const uint32_t* __restrict__ soma_to_axon, uint32_t A,
                                                 const ShardScratch& sc, uint4* __restrict__ comp, uint32_t j,
                                                 uint2* __restrict__ comp8) {
   const uint32_t a = soma_to_axon[sc.spk_list[j]];
-  if (a == kNoAxon || a >= A) return;
+  if (a == kNoAxon || a >= A) return kNoAxon;
   const uint32_t c = atomicExch(&sc.push_cnt[a], 0u);
-  if (c == 0) return;
+  if (c == 0) return kNoAxon;
   Heads8 h = load_heads(heads, a);
   push_heads(h, c > 8 ? 8 : c);
   store_heads(heads, a, h);
   if (comp) comp[a] = compact_heads(h);
   if (comp8) comp8[a] = byte_heads(h);
+  return a;
 }
 
 __global__ __launch_bounds__(256) void push_spikers_kernel(AxcBurstHeads8* __restrict__ heads,

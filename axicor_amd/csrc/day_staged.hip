@@ -696,10 +696,12 @@ __global__ __launch_bounds__(256) void gsop_chunks_kernel(ShardVramPtrs vram, ui
 }
 
 // The tick's spike list from the staged pass's mask words (bit l of word w:
-// soma 64w + l): per wave of 64 words one returning atomic reserves the
-// wave's range, and each lane writes its word's somas in ascending order.
+// soma 64w + l): one returning atomic per workgroup of 256 words reserves
+// its range, and each lane writes its word's somas in ascending order.
+// (One atomic per wave, all on one counter, measured 6.5 us at 1.5M somas.)
 __global__ __launch_bounds__(256) void compact_spikes_kernel(uint32_t n_words, ShardScratch sc) {
-  const uint32_t w = blockIdx.x * 256 + threadIdx.x, lane = threadIdx.x & 63u;
+  __shared__ uint32_t wsum[4], base;
+  const uint32_t w = blockIdx.x * 256 + threadIdx.x, lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
   const uint64_t m = w < n_words ? sc.spk_mask[w] : 0ull;
   const uint32_t c = (uint32_t)__popcll(m);
   uint32_t x = c;  // inclusive prefix over the wave
@@ -708,11 +710,15 @@ __global__ __launch_bounds__(256) void compact_spikes_kernel(uint32_t n_words, S
     const uint32_t y = __shfl_up(x, off, 64);
     if (lane >= off) x += y;
   }
-  const uint32_t total = __shfl(x, 63, 64);
-  if (total == 0) return;  // wave-uniform
-  uint32_t base = 0;
-  if (lane == 63) base = atomicAdd(&sc.counters[kCntSpikes], total);
-  uint32_t pos = __shfl(base, 63, 64) + x - c;
+  if (lane == 63) wsum[wv] = x;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const uint32_t total = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+    base = total ? atomicAdd(&sc.counters[kCntSpikes], total) : 0u;
+  }
+  __syncthreads();
+  uint32_t pos = base + x - c;
+  for (uint32_t k = 0; k < wv; k++) pos += wsum[k];
   for (uint64_t r = m; r; r &= r - 1) sc.spk_list[pos++] = w * 64u + (uint32_t)__builtin_ctzll(r);
 }
 

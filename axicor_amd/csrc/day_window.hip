@@ -1,0 +1,760 @@
+// day_window.hip — the batch graphs' soma pass over windows of ticks.
+//
+// The staged pass (day_staged.hip) streams every filled slot once per tick.
+// But an axon's heads only advance by v_seg per tick until something pushes
+// it (physics.cu:137-154 advances every live head; pushes come from K2
+// inputs, the K3 schedule and the axon's own somas), so for the axons that
+// nothing pushes, the hits of their slots over the next K ticks follow from
+// the heads at the start of those ticks. The batch graphs therefore stream
+// the slots once per window of K <= kWin ticks:
+//   win_start_kernel    the window's start: u32 and byte copies of the heads,
+//                       a new stamp, and the axons with a head that wraps
+//                       past 2^32 within the window;
+//   win_pass_kernel     per soma and tick j of the window, the hits that the
+//                       start heads predict for tick j and the sum of their
+//                       weights (the staged scan, K tests per slot);
+// and at each tick j:
+//   win_axon_kernel     K2/K3 + K4 (axon_step); for every axon pushed since
+//                       the window started (or near the wrap), the range of
+//                       the slots it feeds (the reverse index, built with the
+//                       plan) whose prediction can be wrong, as pieces of 16;
+//   win_correct_kernel  for those slots, where the actual hit differs from
+//                       the predicted one, +-(1 hit, the slot's current
+//                       weight) into the soma's sum of tick j;
+//   win_soma_kernel     K1 + the soma update of K5 from the corrected sums;
+//   win_push_kernel     the spikers' own-axon pushes (counted per axon);
+//   gsop_win_kernel     K6 on the working copy, plus (new - old weight) for
+//                       every predicted hit at the window's later ticks.
+// So soma i's input at tick j is the sum over its slots of w_j * hit_j
+// exactly: predicted hits times the start weights, plus the GSOP changes on
+// predicted hits, plus the corrections (current weights) on the slots of
+// pushed axons, for which alone a prediction can be wrong. Integer sums do
+// not depend on the order of the atomics.
+//
+// A prediction is one rule everywhere (pass, corrections, GSOP changes):
+// for a slot of segment s and propagation length L the test of tick j is the
+// reference's test (physics.cu:198-201) at segment s - (j+1) v_seg: on the
+// window's byte snapshot (min_dist8) when s + L <= 255, on its u32 heads
+// otherwise. For an axon that is neither pushed nor near the wrap both are
+// the actual test of tick j.
+//
+// A soma's hit count and weight sum share one 64-bit word, count << 32 plus
+// the signed sum (|sum| <= 128 * 32768 < 2^31), so one atomic adds both.
+#include "day_common.h"
+
+namespace axc {
+
+__device__ __forceinline__ int64_t win_word(int32_t w, int32_t n) { return ((int64_t)n << 32) + (int64_t)w; }
+__device__ __forceinline__ void win_unpack(int64_t V, int32_t& sum, uint32_t& cnt) {
+  const int64_t c = (V + ((int64_t)1 << 31)) >> 32;
+  cnt = (uint32_t)c;
+  sum = (int32_t)(V - (c << 32));
+}
+
+// The predicted hits of a slot (axon a in the snapshot's ids, segment seg,
+// length L) at ticks 0..K-1 of the window, as bits.
+__device__ __forceinline__ uint32_t win_pred_mask(const WinState& w, uint32_t A, uint32_t a, uint32_t seg,
+                                                  uint32_t L, uint32_t v_seg, uint32_t K) {
+  uint32_t m = 0;
+  if (seg + L > 255u) {
+    if (a >= A) return 0;
+#pragma unroll
+    for (uint32_t j = 0; j < kWin; j++)
+      if (j < K && min_dist_heads(w.heads_w0, A, a, seg - (j + 1) * v_seg) < L) m |= 1u << j;
+  } else {
+    const uint2 e = w.comp8_w0[a];
+#pragma unroll
+    for (uint32_t j = 0; j < kWin; j++)
+      if (j < K && min_dist8(e, (seg - (j + 1) * v_seg) & 0xFFFFu) < L) m |= 1u << j;
+  }
+  return m;
+}
+
+// The window's start (one lane per axon, lane A the inert entry).
+__global__ __launch_bounds__(256) void win_start_kernel(const AxcBurstHeads8* __restrict__ heads, uint32_t A,
+                                                        uint32_t hazard_lim, WinState w, DirtyMarks dm) {
+  const uint32_t a = blockIdx.x * 256 + threadIdx.x;
+  if (a == 0) {  // nobody reads ctl in this launch
+    dm.ctl[1] += 1;
+    w.cnt[0] += 1;
+  }
+  if (a < 2 * kShards) dm.shard_cnt[a] = 0;
+  if (a == A) w.comp8_w0[A] = make_uint2(0xFFFFFFFFu, 0xFFFFFFFFu);
+  if (a >= A) return;
+  const Heads8 h = load_heads(heads, a);
+  store_heads(w.heads_w0, a, h);
+  w.comp8_w0[a] = byte_heads(h);
+  // A live head at or above hazard_lim = 2^32 - (K+1) v_seg wraps within the
+  // window (its byte form would read as dead): 0 disables the test (v_seg 0).
+  const uint32_t hv[8] = {h.lo.x, h.lo.y, h.lo.z, h.lo.w, h.hi.x, h.hi.y, h.hi.z, h.hi.w};
+  bool hz = false;
+#pragma unroll
+  for (int k = 0; k < 8; k++) hz |= hazard_lim != 0 && hv[k] >= hazard_lim;
+  w.hazard[a] = hz ? 1 : 0;
+}
+
+// The segment of a reverse entry, and the first index in [r, b) whose
+// entry's segment is >= key (an axon's list is sorted by segment).
+__device__ __forceinline__ uint32_t rev_seg(const uint64_t e) { return (uint32_t)(e >> 40) & 0xFFu; }
+__device__ __forceinline__ uint32_t rev_lower(const uint64_t* __restrict__ rev, uint32_t r, uint32_t b, uint32_t key) {
+  while (r < b) {
+    const uint32_t mid = (r + b) >> 1;
+    if (rev_seg(rev[mid]) < key) r = mid + 1;
+    else b = mid;
+  }
+  return r;
+}
+
+// K2/K3 + K4 of tick j of the window (axon_tick_kernel), then the tick's
+// correction pieces. Between pushes an axon's heads only advance, and a
+// push of k shifts them by k and puts k new heads in front, so after m
+// pushes (at most 8) since the window started the current heads are m new
+// heads followed by the first 8 - m start heads advanced j+1 times (the
+// prediction P). The hit test is an OR over the heads, so a slot's
+// prediction and its actual test can differ only if one of the m new heads
+// or of the m start heads shifted out (advanced) lies in [s, s + L): only
+// slots with a segment in (x - Lmax, x] for such a value x are tested. New
+// heads are at most (j+1) v_seg: a prefix of the axon's segment-sorted
+// reverse list (its ends for segments up to 7 are stored with it). A value
+// >= 255 + Lmax hits no slot. Axons near the wrap (whose byte prediction
+// does not follow P) test every slot.
+__global__ __launch_bounds__(256) void win_axon_kernel(AxcBurstHeads8* __restrict__ heads, uint32_t A, uint32_t v_seg,
+                                                       uint32_t virt_off, uint32_t n_virt,
+                                                       uint32_t* __restrict__ push_cnt, int use_cnt,
+                                                       uint32_t* __restrict__ counters,
+                                                       const TickParams* __restrict__ prm, uint32_t t_off,
+                                                       uint32_t words, uint2* __restrict__ comp8,
+                                                       const uint32_t* __restrict__ axon_inv,
+                                                       const AxcVariantParameters* __restrict__ lut_g, WinState w,
+                                                       DirtyMarks dm, uint32_t j) {
+  const uint32_t a = blockIdx.x * blockDim.x + threadIdx.x, lane = threadIdx.x & 63u;
+  const uint32_t* __restrict__ bitmask = prm->bitmask ? prm->bitmask + (size_t)t_off * words : nullptr;
+  if (a < kNumCounters) counters[a] = 0;
+  if (a < kShards) dm.shard_cnt[((j + 1) & 1) * kShards + a] = 0;  // the next tick's (consumed last tick)
+  const uint32_t stamp = dm.ctl[1];
+  Heads8 cur{};
+  uint32_t m = 0;
+  bool hz = false;
+  if (a < A) {
+    const uint32_t fl = dm.flag[a];
+    hz = w.hazard[a] != 0;
+    const uint32_t mp = fl == stamp ? (uint32_t)dm.npush[a] : 0u;
+    const uint32_t k = axon_step(heads, a, v_seg, n_virt ? bitmask : nullptr, virt_off, n_virt, push_cnt,
+                                 use_cnt != 0, nullptr, comp8, axon_inv, &cur);
+    m = mp;
+    if (k) {
+      m = min(8u, mp + k);
+      dm.npush[a] = (uint8_t)m;
+      if (fl != stamp) dm.flag[a] = stamp;
+    }
+  }
+  uint32_t rs = 0, cnt = 0;
+  if (m > 0 || hz) {
+    uint32_t Lmax = 0;
+#pragma unroll
+    for (int k = 0; k < 16; k++) Lmax = max(Lmax, (uint32_t)lut_g[k].signal_propagation_length);
+    uint32_t lo = 256, hi = 0;  // the segments to test, [lo, hi]
+    if (hz) {
+      lo = 0;
+      hi = 255;
+    } else {
+      const Heads8 h0 = load_heads(w.heads_w0, a);
+      const uint32_t c[8] = {cur.lo.x, cur.lo.y, cur.lo.z, cur.lo.w, cur.hi.x, cur.hi.y, cur.hi.z, cur.hi.w};
+      const uint32_t z[8] = {h0.lo.x, h0.lo.y, h0.lo.z, h0.lo.w, h0.hi.x, h0.hi.y, h0.hi.z, h0.hi.w};
+      const uint32_t adv = (j + 1) * v_seg;
+#pragma unroll
+      for (uint32_t k = 0; k < 16; k++) {
+        // New head k, or start head k - 8 shifted out.
+        const bool take = k < 8 ? k < m : (k - 8 >= 8 - m && z[k - 8] != kSentinel);
+        const uint32_t x = k < 8 ? c[k] : z[k - 8] + adv;
+        if (take && x < 255u + Lmax) {
+          lo = min(lo, x >= Lmax ? x - Lmax + 1 : 0u);
+          hi = max(hi, min(x, 255u));
+        }
+      }
+    }
+    if (lo <= hi) {
+      const uint32_t r0 = w.rev_off[a], r1 = w.rev_off[a + 1];
+      rs = lo > 0 ? rev_lower(w.rev, r0, r1, lo) : r0;
+      uint32_t re = 0xFFFFFFFFu;
+      if (hi < kRevCuts) {
+        const uint4 cv = w.rev_cut[a];
+        const uint32_t wd = hi < 2 ? cv.x : hi < 4 ? cv.y : hi < 6 ? cv.z : cv.w;
+        const uint32_t off = (wd >> (16 * (hi & 1))) & 0xFFFFu;
+        if (off != 0xFFFFu) re = r0 + off;
+      }
+      if (re == 0xFFFFFFFFu) re = rev_lower(w.rev, rs, r1, hi + 1);
+      cnt = re > rs ? re - rs : 0u;
+    }
+  }
+  // The wave's pieces of up to 16 entries: one atomic on its part's counter
+  // reserves them (the wave's axons are one group of 64).
+  const uint32_t np = (cnt + 15) >> 4;
+  uint32_t x = np;  // inclusive prefix over the wave
+#pragma unroll
+  for (uint32_t off = 1; off < 64; off <<= 1) {
+    const uint32_t y = __shfl_up(x, off, 64);
+    if (lane >= off) x += y;
+  }
+  const uint32_t total = __shfl(x, 63, 64);
+  if (total) {  // wave-uniform
+    const uint32_t sh = (a >> 6) % kShards;
+    uint32_t base = 0;
+    if (lane == 63) base = atomicAdd(&dm.shard_cnt[(j & 1) * kShards + sh], total);
+    base = dm.shard_off[sh] + __shfl(base, 63, 64) + x - np;
+    for (uint32_t p = 0; p < np; p++) {
+      const uint32_t c = min(16u, cnt - 16 * p);
+      dm.pieces[base + p] = (uint64_t)(rs + 16 * p) | ((uint64_t)a << 32) | ((uint64_t)(c - 1) << 56);
+    }
+  }
+}
+
+// The own-axon pushes of the tick's spikers (push_spikers_kernel), counted
+// per axon for the window.
+__global__ __launch_bounds__(256) void win_push_kernel(AxcBurstHeads8* __restrict__ heads,
+                                                       const uint32_t* __restrict__ soma_to_axon, uint32_t A,
+                                                       ShardScratch sc, uint2* __restrict__ comp8, DirtyMarks dm) {
+  const uint32_t n = sc.counters[kCntSpikes];
+  const uint32_t stamp = dm.ctl[1];
+  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+    const uint32_t a = soma_to_axon[sc.spk_list[j]];
+    if (a == kNoAxon || a >= A) continue;
+    const uint32_t c = atomicExch(&sc.push_cnt[a], 0u);
+    if (c == 0) continue;
+    Heads8 h = load_heads(heads, a);
+    push_heads(h, c > 8 ? 8 : c);
+    store_heads(heads, a, h);
+    comp8[a] = byte_heads(h);
+    const uint32_t fl = dm.flag[a];
+    const uint32_t mp = fl == stamp ? (uint32_t)dm.npush[a] : 0u;
+    dm.npush[a] = (uint8_t)min(8u, mp + c);
+    if (fl != stamp) dm.flag[a] = stamp;
+  }
+}
+
+// The predicted hits and weight sums of one loaded chunk c of soma i at
+// ticks 0..K-1 (hit_chunk with K shifted segments). Entries: the block's LDS
+// copy of the window's byte snapshot (STAGED), else the snapshot itself.
+// Slots whose segment + length exceeds the byte form (and spilled slots,
+// segment 255 in the plan) are decided from their targets first (rare): in
+// the shifted-segment tests their weight and count lanes are zero, and
+// their exact hits are added after them.
+// one2: 0x00010001, passed in so that min(gap, 1) stays one packed min.
+// hits2[j / 2] counts the hits of ticks j (bits 0..7) and j + 1 (bits 8..15):
+// a lane's slots in one soma are at most 32.
+// The dot products use the clamped form (no accumulator tied to the
+// destination); the sums stay far below 2^31, so clamping changes nothing.
+template <bool STAGED, uint32_t K>
+__device__ __forceinline__ void win_chunk(const Chunk<STAGED, true>& now, uint32_t c, uint32_t N, uint32_t A,
+                                          const StagePlan& sp, const uint2* ent, const WinState& w, uint32_t i,
+                                          uint32_t prop, uint32_t v_seg, uint32_t one2, int32_t (&i_in)[K],
+                                          int32_t (&hits2)[(K + 1) / 2]) {
+  static_assert(K >= 1 && K <= 4, "four predicted ticks per slot in one word");
+  constexpr int U = 8;
+  const axc_us2 P = {(unsigned short)prop, (unsigned short)prop};
+  const axc_us2 kOne = __builtin_bit_cast(axc_us2, one2);
+  uint32_t mx;
+  if (STAGED) {
+    mx = max_byte8(now.k[4], now.k[5]);
+  } else {
+    mx = 0;
+#pragma unroll
+    for (int u = 0; u < U; u++) mx = max(mx, now.k[u] >> 24);
+  }
+  uint32_t slow_m = 0, slow_h = 0;  // bit u: slot u exact; bits 4u..4u+3: its hits
+  if (mx + prop > 255u) {
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      if (now.seg(u) + prop > 255u) {
+        uint32_t t;
+        if constexpr (STAGED)
+          t = *reinterpret_cast<const uint32_t*>(reinterpret_cast<const char*>(sp.tgt + (size_t)c * N * 8) +
+                                                 i * 32u + u * 4u);
+        else
+          t = now.k[u];
+        // The slot's own segment picks the rule (a spilled slot's may be
+        // small); the inert index never hits.
+        slow_m |= 1u << u;
+        slow_h |= win_pred_mask(w, A, snap_index(t, A), t >> 24, prop, v_seg, K) << (4 * u);
+      }
+    }
+  }
+  constexpr int H = 2;  // slots in pairs (halves of four spill at eight waves per SIMD)
+#pragma unroll
+  for (int h0 = 0; h0 < U; h0 += H) {
+    // The pair's entries spread to u16 lanes once: q[u][r] holds heads 2r,
+    // 2r+1 of slot h0+u; S[u] the slot's segment in both halves.
+    uint32_t q[H][4], S[H];
+#pragma unroll
+    for (int u = 0; u < H; u++) {
+      const uint2 e = STAGED ? ent[now.entry(h0 + u, A)] : w.comp8_w0[now.entry(h0 + u, A)];
+      q[u][0] = __builtin_amdgcn_perm(0u, e.x, 0x0c010c00u);
+      q[u][1] = __builtin_amdgcn_perm(0u, e.x, 0x0c030c02u);
+      q[u][2] = __builtin_amdgcn_perm(0u, e.y, 0x0c010c00u);
+      q[u][3] = __builtin_amdgcn_perm(0u, e.y, 0x0c030c02u);
+      const int v = h0 + u;
+      if (STAGED) S[u] = __builtin_amdgcn_perm(0u, now.k[4 + (v >> 2)], 0x0c000c00u | ((v & 3) * 0x00010001u));
+      else S[u] = __builtin_amdgcn_perm(0u, now.k[v], 0x0c030c03u);
+    }
+    // The pair's weights and count lanes of 1, zero for exact-path slots.
+    const uint32_t sm2 = (slow_m >> h0) & 3u;
+    const uint32_t keep = ((sm2 & 1u) ? 0u : 0x0000FFFFu) | ((sm2 & 2u) ? 0u : 0xFFFF0000u);
+    const uint32_t wz = now.w2[h0 >> 1] & keep, cm = 0x00010001u & keep, cm8 = 0x01000100u & keep;
+#pragma unroll
+    for (uint32_t j = 0; j < K; j++) {
+      const uint32_t sh = ((j + 1) * v_seg) & 0xFFFFu;
+      const axc_us2 D = {(unsigned short)sh, (unsigned short)sh};
+      uint32_t d2[2];
+#pragma unroll
+      for (int x = 0; x < 2; x++) {
+        const axc_us2 s = __builtin_bit_cast(axc_us2, S[x]) - D;
+        const axc_us2 m0 = __builtin_elementwise_min(__builtin_bit_cast(axc_us2, q[x][0]) - s,
+                                                     __builtin_bit_cast(axc_us2, q[x][1]) - s);
+        const axc_us2 m1 = __builtin_elementwise_min(__builtin_bit_cast(axc_us2, q[x][2]) - s,
+                                                     __builtin_bit_cast(axc_us2, q[x][3]) - s);
+        d2[x] = __builtin_bit_cast(uint32_t, __builtin_elementwise_min(m0, m1));
+      }
+      const uint32_t M = pair_min(d2[0], d2[1]);
+      // 1 in the u16 lanes whose distance is below the length.
+      const axc_us2 hv = __builtin_elementwise_min(__builtin_elementwise_sub_sat(P, __builtin_bit_cast(axc_us2, M)),
+                                                   kOne);
+      i_in[j] = __builtin_amdgcn_sdot2(__builtin_bit_cast(axc_s2, hv), __builtin_bit_cast(axc_s2, wz), i_in[j], true);
+      hits2[j / 2] = __builtin_amdgcn_sdot2(__builtin_bit_cast(axc_s2, hv), __builtin_bit_cast(axc_s2, (j & 1) ? cm8 : cm),
+                                            hits2[j / 2], true);
+    }
+  }
+  if (slow_m) {
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      if ((slow_m >> u) & 1u) {
+        const int32_t wt = now.weight(u);
+#pragma unroll
+        for (uint32_t j = 0; j < K; j++) {
+          if ((slow_h >> (4 * u + j)) & 1u) {
+            i_in[j] += wt;
+            hits2[j / 2] += (j & 1) ? 256 : 1;
+          }
+        }
+      }
+    }
+  }
+}
+
+// The window pass of soma block lb: per soma and tick j < K of the window
+// the predicted hits and their weight sum, added over the four lane groups
+// in LDS and stored to acc[j]. Somas refractory for the whole window
+// (timer >= K) are not scanned: no tick of the window reads their sums.
+template <uint32_t K>
+__global__ __launch_bounds__(kStageThreads) __attribute__((amdgpu_waves_per_eu(8, 8)))
+void win_pass_kernel(ShardVramPtrs vram, uint32_t N, uint32_t A, const AxcVariantParameters* __restrict__ lut_g,
+                     StagePlan sp, WinState w, uint32_t v_seg, uint32_t one2) {
+  extern __shared__ uint2 ent[];  // kStageCap + 1 entries
+  __shared__ AxcVariantParameters lut[16];
+  __shared__ unsigned long long vsh[K][kBlockSomas];
+  __shared__ uint32_t bcnt[2];  // the block's scanned somas and their filled slots
+  const uint32_t per_xcd = gridDim.x / kNumXcd;
+  const uint32_t lb = (blockIdx.x % kNumXcd) * per_xcd + blockIdx.x / kNumXcd;
+  const uint32_t tid = threadIdx.x;
+  if (tid < 64) reinterpret_cast<uint4*>(lut)[tid] = reinterpret_cast<const uint4*>(lut_g)[tid];
+  const uint32_t cnt = sp.cnt[lb];
+  const bool staged = cnt != kStageOverflow;
+  const uint32_t g = tid / kBlockSomas, li = tid % kBlockSomas;
+  const uint32_t i = lb * kBlockSomas + li;
+  uint32_t F = 0;
+  uint8_t flags = 0, timer = 0;
+  if (i < N) {
+    flags = vram.soma_flags[i];
+    timer = vram.timers[i];
+    F = sp.fill[i];
+  }
+  if (staged) stage_entries(ent, sp.set + (size_t)lb * kStageCap, cnt, w.comp8_w0, A, tid);
+  if (g == 0) {
+#pragma unroll
+    for (uint32_t j = 0; j < K; j++) vsh[j][li] = 0ull;
+  }
+  if (tid == 0) bcnt[0] = bcnt[1] = 0;
+  __syncthreads();
+  const bool scan = i < N && timer < K;
+  if (scan) {
+    const uint32_t prop = lut[(flags >> 4) & 0x0F].signal_propagation_length;
+    int32_t i_in[K], hits2[(K + 1) / 2];
+#pragma unroll
+    for (uint32_t j = 0; j < K; j++) i_in[j] = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < (K + 1) / 2; j++) hits2[j] = 0;
+    const uint32_t gu = __builtin_amdgcn_readfirstlane(g);
+    const uint32_t cg = (gu + kStageGroups - 1) % kStageGroups;
+#pragma unroll 1
+    for (uint32_t c = cg; c * 8 < F; c += kStageGroups) {
+      if (staged) {
+        Chunk<true, true> now;
+        now.load(vram, sp, N, c, i);
+        win_chunk<true, K>(now, c, N, A, sp, ent, w, i, prop, v_seg, one2, i_in, hits2);
+      } else {
+        Chunk<false, true> now;
+        now.load(vram, sp, N, c, i);
+        win_chunk<false, K>(now, c, N, A, sp, ent, w, i, prop, v_seg, one2, i_in, hits2);
+      }
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < K; j++) {
+      const int64_t V = win_word(i_in[j], (hits2[j / 2] >> (8 * (j & 1))) & 0xFF);
+      if (V != 0) atomicAdd(&vsh[j][li], (unsigned long long)V);
+    }
+  }
+  if (g == 0) {  // whole waves
+    uint32_t ns = scan ? 1u : 0u, nf = scan ? F : 0u;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      ns += __shfl_xor(ns, off, 64);
+      nf += __shfl_xor(nf, off, 64);
+    }
+    if ((tid & 63u) == 0 && ns) {
+      atomicAdd(&bcnt[0], ns);
+      atomicAdd(&bcnt[1], nf);
+    }
+  }
+  __syncthreads();
+  if (g == 0 && i < N) {
+#pragma unroll
+    for (uint32_t j = 0; j < K; j++) w.acc[(size_t)j * N + i] = (int64_t)vsh[j][li];
+  }
+  if (tid == 0 && bcnt[0]) {
+    atomicAdd((unsigned long long*)&w.cnt[1], (unsigned long long)bcnt[0]);
+    atomicAdd((unsigned long long*)&w.cnt[2], (unsigned long long)bcnt[1]);
+  }
+}
+
+// The corrections of tick j (the pieces its axon kernel made): a lane per
+// reverse entry, the exact prediction rule against the reference's test;
+// where they differ, +-(1 hit, the slot's current weight) into the soma's
+// sum of tick j. Workgroup b takes part b % kShards, every (gridDim.x /
+// kShards)-th run of 256 lanes of it. Counters per workgroup in w.cpart
+// (pieces, entries).
+__global__ __launch_bounds__(256) void win_correct_kernel(const AxcBurstHeads8* __restrict__ heads,
+                                                          const uint8_t* __restrict__ soma_flags, uint32_t N,
+                                                          uint32_t A, const AxcVariantParameters* __restrict__ lut_g,
+                                                          StagePlan sp, WinState w, DirtyMarks dm, uint32_t v_seg,
+                                                          uint32_t j) {
+  __shared__ uint32_t prop[16];
+  __shared__ uint32_t bsum[4][2];
+  const uint32_t tid = threadIdx.x;
+  if (tid < 16) prop[tid] = lut_g[tid].signal_propagation_length;
+  const uint32_t sh = blockIdx.x % kShards, runs = gridDim.x / kShards;
+  const uint32_t np = dm.shard_cnt[(j & 1) * kShards + sh];
+  const uint64_t* __restrict__ pieces = dm.pieces + dm.shard_off[sh];
+  __syncthreads();
+  uint32_t n_p = 0, n_e = 0;
+  for (uint32_t x = (blockIdx.x / kShards) * 256 + tid; (x >> 4) < np; x += runs * 256) {
+    const uint64_t pc = pieces[x >> 4];
+    const uint32_t l = x & 15u, c = (uint32_t)(pc >> 56) + 1;
+    n_p += l == 0 ? 1u : 0u;
+    if (l >= c) continue;
+    const uint32_t a = (uint32_t)(pc >> 32) & 0x00FFFFFFu;
+    const uint64_t en = w.rev[(uint32_t)pc + l];
+    const uint32_t i = (uint32_t)en, s = (uint32_t)(en >> 32) & 0xFFu, seg = rev_seg(en);
+    const uint32_t L = prop[soma_flags[i] >> 4];
+    const bool pred = (win_pred_mask(w, A, a, seg, L, v_seg, j + 1) >> j) & 1u;
+    const bool hit = any_hit(load_heads(heads, a), seg, L);
+    n_e += 1;
+    if (pred != hit) {
+      const int32_t wt = sp.w[chunk_at(N, s >> 3, i) + (s & 7)];
+      const int64_t d = hit ? win_word(wt, 1) : -win_word(wt, 1);
+      atomicAdd((unsigned long long*)&w.acc[(size_t)j * N + i], (unsigned long long)d);
+    }
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    n_p += __shfl_xor(n_p, off, 64);
+    n_e += __shfl_xor(n_e, off, 64);
+  }
+  if ((tid & 63u) == 0) {
+    bsum[tid >> 6][0] = n_p;
+    bsum[tid >> 6][1] = n_e;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    const uint32_t p = bsum[0][0] + bsum[1][0] + bsum[2][0] + bsum[3][0];
+    const uint32_t e = bsum[0][1] + bsum[1][1] + bsum[2][1] + bsum[3][1];
+    if (p | e) {
+      w.cpart[2 * blockIdx.x] += p;
+      w.cpart[2 * blockIdx.x + 1] += e;
+    }
+  }
+}
+
+// K1 + the soma part of K5 at tick j of the window, one lane per soma, from
+// acc[j]: the staged pass's finish (finish_block) without the scan; the
+// spikes as mask words (compact_spikes_kernel makes the list).
+__global__ __launch_bounds__(256) void win_soma_kernel(ShardVramPtrs vram, uint32_t N, uint32_t A,
+                                                       const AxcVariantParameters* __restrict__ lut_g,
+                                                       const TickParams* __restrict__ prm, uint32_t tick_arg,
+                                                       ShardScratch sc, StagePlan sp, WinState w, uint32_t j) {
+  __shared__ AxcVariantParameters lut[16];
+  __shared__ uint32_t red[4][8];
+  const uint32_t tid = threadIdx.x;
+  if (tid < 64) reinterpret_cast<uint4*>(lut)[tid] = reinterpret_cast<const uint4*>(lut_g)[tid];
+  __syncthreads();
+  const uint32_t tick = tick_of(prm, tick_arg);
+  const uint32_t i = blockIdx.x * 256 + tid;
+  bool fin = false, push_own = false;
+  uint32_t my_axon = 0;
+  uint32_t n_int = 0, n_scan = 0, n_tread = 0, n_hits = 0;
+  if (i < N) {
+    const uint8_t flags_in = vram.soma_flags[i];
+    const uint8_t timer = vram.timers[i];
+    const bool refr = timer > 0;
+    int32_t sum = 0, v_in = 0, thr_in = 0;
+    uint32_t hc = 0, ax_in = 0;
+    if (!refr) {
+      win_unpack(w.acc[(size_t)j * N + i], sum, hc);
+      v_in = vram.soma_voltage[i];
+      thr_in = vram.threshold_offset[i];
+      ax_in = vram.soma_to_axon[i];
+      const uint32_t F = sp.fill[i];
+      n_scan = F;
+      n_tread = F < kMaxDendrites ? F + 1 : kMaxDendrites;
+      n_hits = hc;
+      n_int = 1;
+    }
+    const uint32_t hb_i = sp.perm ? sp.perm[i] : i;
+    uint8_t flags = flags_in & 0xF1;  // K1
+    flags &= (uint8_t)~0x01;
+    soma_tick_with(vram, i, A, lut[(flags_in >> 4) & 0x0F], flags, timer, refr, tick, sum, thr_in, v_in, ax_in, fin,
+                   push_own, my_axon, hb_i);
+  }
+  if (push_own) atomicAdd(&sc.push_cnt[my_axon], 1u);
+  const uint32_t lane = tid & 63u;
+  const uint64_t bs = __ballot(fin);
+  if (lane == 0) sc.spk_mask[i >> 6] = bs;
+  uint32_t r[5] = {n_int, n_scan, n_tread, n_hits, (uint32_t)__popcll(bs)};
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) r[k] += __shfl_xor(r[k], off, 64);
+  }
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < 5; k++) red[tid >> 6][k] = r[k];
+  }
+  __syncthreads();
+  if (tid == 0) {  // no-return atomics: nothing waits for them
+    uint64_t* part = sc.stats_part + (size_t)blockIdx.x * kStatsPerBlock;
+    const uint32_t slot[5] = {kStIntegrated, kStScanned, kStTargets, kStHits, kStSpikes};
+#pragma unroll
+    for (int k = 0; k < 5; k++) {
+      const uint32_t v = red[0][k] + red[1][k] + red[2][k] + red[3][k];
+      if (v) atomicAdd((unsigned long long*)&part[slot[k]], (unsigned long long)v);
+    }
+    if (blockIdx.x == 0) atomicAdd((unsigned long long*)&part[kStTicks], 1ull);
+  }
+}
+
+// K6 on the working copy at tick j of a window of K ticks (gsop_chunks_kernel)
+// plus, for each slot whose weight changed, the change times its predicted
+// hits at the window's later ticks, into acc (summed over the spiker's 16
+// lanes first, one atomic per later tick). A spiker whose timer (set by this
+// tick's update) is T is refractory at ticks j+1 .. j+T, which ignore its
+// sums: only ticks after j + T take the changes.
+__global__ __launch_bounds__(256) void gsop_win_kernel(ShardVramPtrs vram, uint32_t N, uint32_t A,
+                                                       const AxcVariantParameters* __restrict__ lut_g,
+                                                       const TickParams* __restrict__ prm, int32_t dop_arg,
+                                                       ShardScratch sc, StagePlan sp, WinState w, uint32_t v_seg,
+                                                       uint32_t j, uint32_t K) {
+  __shared__ AxcVariantParameters lut[16];
+  __shared__ uint32_t red[4][2];
+  const uint32_t tid = threadIdx.x;
+  if (tid < 64) reinterpret_cast<uint4*>(lut)[tid] = reinterpret_cast<const uint4*>(lut_g)[tid];
+  __syncthreads();
+  const uint32_t n = sc.counters[kCntSpikes] * kGsopLanes;
+  const int32_t dop = dopamine_of(prm, dop_arg);
+  uint32_t cnt_slots = 0, cnt_tgt = 0;
+  // Whole waves take part in the shuffles below: every lane of a block runs
+  // the same iterations (the bound is the block's, not the lane's).
+  for (uint32_t q0 = blockIdx.x * 256; q0 < n; q0 += gridDim.x * 256) {
+    const uint32_t q = q0 + tid;
+    int32_t dv[kWin];
+#pragma unroll
+    for (uint32_t jj = 0; jj < kWin; jj++) dv[jj] = 0;
+    uint32_t i = 0;
+    if (q < n) {
+      i = sc.spk_list[q / kGsopLanes];
+      const uint32_t c = q % kGsopLanes;
+      const uint32_t F = sp.fill[i];
+      if (c * 8 < F) {
+        v4u* wq = reinterpret_cast<v4u*>(sp.w + chunk_at(N, c, i));
+        const v4u wb = *wq;
+        gsop_chunk(N, A, lut, dop, vram.soma_flags, sp, vram.axon_heads, i, c, F);
+        const uint32_t first = j + 1 + vram.timers[i];  // the first later tick that reads the sum
+        if (first < K) {
+          const v4u wa = *wq;
+          const uint32_t before[4] = {wb.x, wb.y, wb.z, wb.w}, after[4] = {wa.x, wa.y, wa.z, wa.w};
+          const uint32_t L = lut[(vram.soma_flags[i] >> 4) & 0x0F].signal_propagation_length;
+          const v4u* tq = reinterpret_cast<const v4u*>(sp.tgt + chunk_at(N, c, i));
+          const v4u t0 = tq[0], t1 = tq[1];
+          const uint32_t t[8] = {t0.x, t0.y, t0.z, t0.w, t1.x, t1.y, t1.z, t1.w};
+          const uint32_t nv = F - c * 8 < 8u ? F - c * 8 : 8u;
+#pragma unroll
+          for (int u = 0; u < 8; u++) {
+            const int32_t w0 = (int32_t)(int16_t)(uint16_t)(before[u >> 1] >> (16 * (u & 1)));
+            const int32_t w1 = (int32_t)(int16_t)(uint16_t)(after[u >> 1] >> (16 * (u & 1)));
+            if ((uint32_t)u >= nv || w0 == w1) continue;
+            const uint32_t m = win_pred_mask(w, A, snap_index(t[u], A), t[u] >> 24, L, v_seg, K);
+#pragma unroll
+            for (uint32_t jj = 0; jj < kWin; jj++)
+              if (jj >= first && ((m >> jj) & 1u)) dv[jj] += w1 - w0;
+          }
+        }
+      }
+      if (c == 0) {
+        cnt_slots += F;
+        cnt_tgt += F < kMaxDendrites ? F + 1 : kMaxDendrites;
+      }
+    }
+    // The spiker's 16 lanes are an aligned group of a wave.
+#pragma unroll
+    for (uint32_t jj = 0; jj < kWin; jj++) {
+#pragma unroll
+      for (int off = 8; off > 0; off >>= 1) dv[jj] += __shfl_xor(dv[jj], off, 16);
+      if (q < n && (q % kGsopLanes) == 0 && dv[jj] != 0)
+        atomicAdd((unsigned long long*)&w.acc[(size_t)jj * N + i], (unsigned long long)win_word(dv[jj], 0));
+    }
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    cnt_slots += __shfl_xor(cnt_slots, off, 64);
+    cnt_tgt += __shfl_xor(cnt_tgt, off, 64);
+  }
+  if ((tid & 63u) == 0) {
+    red[tid >> 6][0] = cnt_slots;
+    red[tid >> 6][1] = cnt_tgt;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    uint64_t* part = sc.stats_part2 + (size_t)blockIdx.x * kStatsPerBlock;
+    part[kStGsopSlots] += red[0][0] + red[1][0] + red[2][0] + red[3][0];
+    part[kStGsopTargets] += red[0][1] + red[1][1] + red[2][1] + red[3][1];
+  }
+}
+
+// ---- the reverse index (axon -> the slots it feeds), built with the plan ----
+// Filled slots per axon of the plan's ids.
+__global__ __launch_bounds__(256) void rev_count_kernel(uint32_t N, uint32_t A, StagePlan sp,
+                                                        uint32_t* __restrict__ cnt) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= N) return;
+  const uint32_t F = sp.fill[i];
+  for (uint32_t s = 0; s < F; s++) {
+    const uint32_t a = snap_index(sp.tgt[chunk_at(N, s >> 3, i) + (s & 7)], A);
+    if (a < A) atomicAdd(&cnt[a], 1u);
+  }
+}
+
+// Exclusive prefix of cnt[0, n) into off[0, n] by one workgroup, 1024 at a
+// time; cnt and off may be the same array.
+__global__ __launch_bounds__(1024) void rev_scan_kernel(const uint32_t* cnt, uint32_t n, uint32_t* off) {
+  __shared__ uint32_t part[1024];
+  uint32_t carry = 0;
+  for (uint32_t b0 = 0; b0 < n; b0 += 1024) {
+    const uint32_t k = b0 + threadIdx.x;
+    const uint32_t v = k < n ? cnt[k] : 0u;
+    part[threadIdx.x] = v;
+    __syncthreads();
+    for (uint32_t o = 1; o < 1024; o <<= 1) {
+      const uint32_t x = threadIdx.x >= o ? part[threadIdx.x - o] : 0u;
+      __syncthreads();
+      part[threadIdx.x] += x;
+      __syncthreads();
+    }
+    if (k < n) off[k] = carry + part[threadIdx.x] - v;
+    carry += part[1023];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) off[n] = carry;
+}
+
+// The entries, at cursor[a]++ (cursor: a copy of rev_off[0, A)), in no
+// particular order within an axon's list (rev_sort_kernel sorts them).
+__global__ __launch_bounds__(256) void rev_fill_kernel(uint32_t N, uint32_t A, StagePlan sp,
+                                                       uint32_t* __restrict__ cursor, uint64_t* __restrict__ rev) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= N) return;
+  const uint32_t F = sp.fill[i];
+  for (uint32_t s = 0; s < F; s++) {
+    const uint32_t t = sp.tgt[chunk_at(N, s >> 3, i) + (s & 7)];
+    const uint32_t a = snap_index(t, A);
+    if (a >= A) continue;
+    const uint32_t p = atomicAdd(&cursor[a], 1u);
+    rev[p] = (uint64_t)i | ((uint64_t)s << 32) | ((uint64_t)(t >> 24) << 40);
+  }
+}
+
+// Each axon's list sorted by segment (a wave per axon, a counting sort over
+// the 256 segments in LDS; the order within one segment is arbitrary), and
+// cut[a]: the ends (relative to the list's start) of its entries with a
+// segment <= s for s < kRevCuts, as u16.
+__global__ __launch_bounds__(256) void rev_sort_kernel(uint32_t A, const uint32_t* __restrict__ rev_off,
+                                                       const uint64_t* __restrict__ in, uint64_t* __restrict__ out,
+                                                       uint4* __restrict__ cut) {
+  __shared__ uint32_t hist[4][256];
+  const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
+  uint32_t* h = hist[wv];
+  for (uint32_t a = blockIdx.x * 4 + wv; a < A; a += gridDim.x * 4) {
+    const uint32_t r0 = rev_off[a], r1 = rev_off[a + 1];
+#pragma unroll
+    for (uint32_t k = 0; k < 4; k++) h[lane * 4 + k] = 0;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    for (uint32_t p = r0 + lane; p < r1; p += 64) atomicAdd(&h[(uint32_t)(in[p] >> 40) & 0xFFu], 1u);
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    // Exclusive prefix over the bins: lane l owns bins 4l .. 4l+3.
+    uint32_t c[4];
+#pragma unroll
+    for (uint32_t k = 0; k < 4; k++) c[k] = h[lane * 4 + k];
+    const uint32_t tot = c[0] + c[1] + c[2] + c[3];
+    uint32_t x = tot;
+#pragma unroll
+    for (uint32_t off = 1; off < 64; off <<= 1) {
+      const uint32_t y = __shfl_up(x, off, 64);
+      if (lane >= off) x += y;
+    }
+    uint32_t base = x - tot;
+#pragma unroll
+    for (uint32_t k = 0; k < 4; k++) {
+      h[lane * 4 + k] = base;
+      base += c[k];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    if (lane == 0) {  // the end of the entries with segment <= s, s < kRevCuts (0xFFFF: past u16)
+      uint32_t e[kRevCuts];
+#pragma unroll
+      for (uint32_t k = 0; k < kRevCuts; k++) e[k] = min(h[k + 1], 0xFFFFu);
+      cut[a] = make_uint4(e[0] | (e[1] << 16), e[2] | (e[3] << 16), e[4] | (e[5] << 16), e[6] | (e[7] << 16));
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    for (uint32_t p = r0 + lane; p < r1; p += 64) {
+      const uint64_t e = in[p];
+      out[r0 + atomicAdd(&h[(uint32_t)(e >> 40) & 0xFFu], 1u)] = e;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+  }
+}
+
+// The most correction pieces each part can take in a tick: per axon,
+// ceil(list length / 16) (a range of its list makes no more), summed over
+// the part's axons.
+__global__ __launch_bounds__(256) void shard_cap_kernel(uint32_t A, const uint32_t* __restrict__ rev_off,
+                                                        uint32_t* __restrict__ cap) {
+  const uint32_t a = blockIdx.x * 256 + threadIdx.x;
+  uint32_t c = a < A ? (rev_off[a + 1] - rev_off[a] + 15) >> 4 : 0u;
+  // The wave's axons are one group of 64: one atomic per wave.
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off, 64);
+  if ((threadIdx.x & 63u) == 0 && c) atomicAdd(&cap[(a >> 6) % kShards], c);
+}
+
+}  // namespace axc

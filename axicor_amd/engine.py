@@ -273,6 +273,15 @@ class ShardEngine:
         _check(day_lib().axc_shard_stats_read(C.byref(self.vram.ptrs), C.byref(s)), "axc_shard_stats_read")
         return {n: getattr(s, n) for n, _ in AxcShardStats._fields_}
 
+    WINDOW_STATS = ("windows", "pass_somas", "pass_slots", "correction_pieces", "corrected_entries")
+
+    def window_stats(self) -> dict:
+        """Work counters of the windowed batches since stats_reset()."""
+        self.synchronize()
+        out = (C.c_uint64 * len(self.WINDOW_STATS))()
+        _check(day_lib().axc_shard_window_stats(C.byref(self.vram.ptrs), out), "axc_shard_window_stats")
+        return dict(zip(self.WINDOW_STATS, (int(x) for x in out)))
+
     def destroy(self) -> None:
         if self.stream:
             day_lib().gpu_stream_destroy(self.stream)

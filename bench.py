@@ -284,7 +284,9 @@ def main() -> int:
                          "each after one more batch of --ticks ticks, timed phase by phase")
     ap.add_argument("--prune", type=int, default=15, help="prune threshold of the Nights")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--timing-ticks", type=int, default=10,
+    ap.add_argument("--window", type=int, default=1,
+                    help="ticks per window of the batch graphs' staged pass (axc_set_window_ticks; 1: per tick)")
+    ap.add_argument("--timing-ticks", type=int, default=12,
                     help="ticks per batch of the two batches whose kernels are timed one by one with HIP events")
     ap.add_argument("--batched-brains", type=int, default=128, help="0 skips the batched-brains leg")
     ap.add_argument("--batched-ticks", type=int, default=5000)
@@ -345,6 +347,8 @@ def main() -> int:
 
     # ---- GPU ----------------------------------------------------------------
     upload_constant_memory(brain.lut)
+    if lib.axc_set_window_ticks(args.window) != 0:
+        raise SystemExit(f"--window {args.window}: at most 4 ticks per window")
     vram = VramState(N, A)
     vram.upload_state(brain.state)
     vram.upload_axon_heads(brain.heads)
@@ -441,11 +445,26 @@ def main() -> int:
     fused = 4 * st_t["gsop_targets_fused"] + 4 * st_t["gsop_slots_fused"]
     per_kernel = {TIMED_KERNELS[k]: {"avg_us": kms[k] * 1e3 / kn[k], "launches": int(kn[k])}
                   for k in range(len(TIMED_KERNELS)) if kn[k]}
-    nt_us = per_kernel["neuron_tick"]["avg_us"]
-    nt_bytes = (ab_t["neuron_tick"] + fused) / per_kernel["neuron_tick"]["launches"]
-    achieved = nt_bytes / (nt_us * 1e-6) / 1e9
-    per_kernel["neuron_tick"]["algorithmic_bytes"] = nt_bytes
-    per_kernel["neuron_tick"]["achieved_GBs"] = achieved
+    wst = eng.window_stats()
+    if "window_pass" in per_kernel:
+        # Windows of ticks: the dominant kernel is the window pass. Its own
+        # compulsory traffic per launch: each scanned soma's filled slots in
+        # the reference's format (4 B target + 2 B weight), every soma's
+        # flags, timer and fill byte, one 8 B sum word per soma and tick of
+        # the window, and the window's byte snapshot (8 B per axon).
+        wp = per_kernel["window_pass"]
+        dom_bytes = (6 * wst["pass_slots"] + 3 * N * wst["windows"] + 8 * N * st_t["ticks"] +
+                     8 * (A + 1) * wst["windows"]) / wp["launches"]
+        dom_name, dom_us = "window_pass", wp["avg_us"]
+        dom_desc = (f"window_pass (the staged dendrite scan once per window of {args.window} ticks: "
+                    "predicted hits and weight sums per soma and tick)")
+    else:
+        dom_bytes = (ab_t["neuron_tick"] + fused) / per_kernel["neuron_tick"]["launches"]
+        dom_name, dom_us = "neuron_tick", per_kernel["neuron_tick"]["avg_us"]
+        dom_desc = "neuron_tick (K1 + K5: staged dendrite scan, GLIF soma update)"
+    achieved = dom_bytes / (dom_us * 1e-6) / 1e9
+    per_kernel[dom_name]["algorithmic_bytes"] = dom_bytes
+    per_kernel[dom_name]["achieved_GBs"] = achieved
     per_kernel["axon_tick"]["algorithmic_bytes"] = ab_t["axon_tick"] / per_kernel["axon_tick"]["launches"]
     per_kernel["axon_tick"]["achieved_GBs"] = (per_kernel["axon_tick"]["algorithmic_bytes"] /
                                                (per_kernel["axon_tick"]["avg_us"] * 1e-6) / 1e9)
@@ -456,7 +475,7 @@ def main() -> int:
     log("per-kernel: " + json.dumps(per_kernel))
 
     traffic = None
-    tf = os.path.join(REPO, "profiles", f"neuron_tick_{args.workload}_traffic.json")
+    tf = os.path.join(REPO, "profiles", f"{dom_name}_{args.workload}_traffic.json")
     if os.path.exists(tf):
         try:
             traffic = json.load(open(tf)).get("hbm_bytes_per_launch")
@@ -482,10 +501,10 @@ def main() -> int:
                          f"{spec.axon_len[0]}-{spec.axon_len[1]} segments, GSOP on, "
                          f"soma numbering {_numbering(spec.brick)}"),
             "padded_n": N, "total_axons": A, "ticks_per_step": T, "parallelism": "single GPU",
-            "engine_order": bool(args.engine_order),
+            "engine_order": bool(args.engine_order), "window_ticks": args.window,
         },
         "roofline": {
-            "bound": "hbm", "kernel": "neuron_tick (K1 + K5: staged dendrite scan, GLIF soma update)",
+            "bound": "hbm", "kernel": dom_desc,
             "achieved": achieved,
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
         },
@@ -499,6 +518,11 @@ def main() -> int:
         "us_per_tick": elapsed * 1e6 / ticks_timed,
         "per_kernel": per_kernel,
         "parity": parity,
+        "window": ({"ticks": args.window,
+                    "correction_pieces_per_tick": wst["correction_pieces"] / max(1, st_t["ticks"]),
+                    "corrected_entries_per_tick": wst["corrected_entries"] / max(1, st_t["ticks"]),
+                    "pass_slots_per_window": wst["pass_slots"] / max(1, wst["windows"])}
+                   if wst["windows"] else None),
     }
     if args.nights:
         def night_batch(ticks: int):

@@ -360,14 +360,34 @@ void axc_host_sort_and_prune(uint8_t *flags, uint32_t *targets, int16_t *weights
 void axc_staging_enable(int enable);
 void axc_shard_targets_changed(const ShardVramPtrs *vram);
 
+/* Windows of ticks (no counterpart in the reference; same results).
+ * axc_step_day_phase_batch with the staged scan streams the dendrite slots
+ * once per window of `ticks` ticks instead of once per tick: it predicts
+ * every slot's hits over the window from the heads at the window's start and
+ * corrects, tick by tick, the slots of the axons pushed since then (inputs,
+ * schedule, own spikes), through a reverse index (axon -> slots) kept with
+ * the plan. 1 (the default) is the per-tick pass; at most 4. A batch whose
+ * ticks x v_seg reach 2^15 runs per tick. Process-wide, like
+ * axc_staging_enable. Returns 0, or hipErrorInvalidValue for ticks > 4. */
+int32_t axc_set_window_ticks(uint32_t ticks);
+uint32_t axc_window_ticks(void);
+/* The shard's window work counters since axc_shard_stats_reset, in out[5]:
+ * windows, somas scanned by the window passes (those not refractory for the
+ * whole window), their filled slots, correction pieces (up to 16 reverse-index
+ * entries of one pushed axon each) and the entries they test, both summed
+ * over ticks. */
+int32_t axc_shard_window_stats(const ShardVramPtrs *vram, uint64_t *out);
+
 /* Per-kernel timing, for benchmarks: when enabled, HIP events are recorded
  * on the caller's stream around each launch of cu_step_day_phase, and
  * axc_step_day_phase_batch launches its kernels one by one instead of
  * replaying its graph (same kernels, same results) so that they can be timed.
  * axc_kernel_timing_read waits for the events and returns the summed
- * milliseconds and launch counts of 7 kernels in this order:
+ * milliseconds and launch counts of 11 kernels in this order:
  * axon_tick, neuron_tick, push_apply, gsop, readout, spike_count,
- * weights_copy (the batch's working copy of the weights, in and out). */
+ * weights_copy (the batch's working copy of the weights, in and out),
+ * window_start, window_pass, window_correct, window_soma (the windowed
+ * batches' own kernels, see axc_set_window_ticks). */
 void axc_kernel_timing_enable(int enable);
 int32_t axc_kernel_timing_read(double *ms_out, uint64_t *launches_out, int reset);
 
