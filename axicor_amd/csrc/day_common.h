@@ -119,17 +119,20 @@ struct StagePlan {
 #endif
 constexpr uint32_t kWin = AXC_WIN_TICKS;
 
-// Axons pushed since the window started: flag[a] == ctl[1] (the window's
-// stamp), npush[a] their pushes so far (at most 8: the start heads shifted
-// out). Each tick's corrections are pieces of up to 16 reverse entries,
-// start | axon << 32 | (count - 1) << 56, kept in kShards parts by axon
-// group ((a / 64) % kShards, so that no counter takes many atomics): part s
-// at pieces[shard_off[s] + k], k < shard_cnt[(tick & 1) * kShards + s]; its
-// capacity is the most pieces its axons can make.
+// Per axon and window (aux[a], valid while its low 32 bits equal the
+// window's stamp ctl[1]): bits 32-35 the pushes since the window started
+// (at most 8: the start heads shifted out), bit 36 a start head near the
+// u32 wrap, bits 37-49 and 50-62 the least and greatest window-start value
+// of the live heads shifted out so far (8191 and 0 for none; values >= 8191
+// are left out: they hit no slot). Each tick's corrections are pieces of up
+// to 16 reverse entries, start | axon << 32 | (count - 1) << 56, kept in
+// kShards parts by axon group ((a / 64) % kShards, so that no counter takes
+// many atomics): part s at pieces[shard_off[s] + k], k < shard_cnt[(tick &
+// 1) * kShards + s]; its capacity is the most pieces its axons can make.
 constexpr uint32_t kShards = 256;
+constexpr uint32_t kNoDrop = 8191;
 struct DirtyMarks {
-  uint32_t* flag;       // [A]
-  uint8_t* npush;       // [A]
+  uint64_t* aux;        // [A]
   uint32_t* ctl;        // [2]: (unused), stamp
   uint64_t* pieces;     // [shard_off[kShards]]
   uint32_t* shard_off;  // [kShards + 1]
@@ -138,9 +141,11 @@ struct DirtyMarks {
 
 struct WinState {
   int64_t* acc;               // [kWin][N] per soma and tick of the window: hits << 32 plus the hit weights' sum
-  uint2* comp8_w0;            // [A+1] byte snapshot of the window's start heads; entry A inert
-  AxcBurstHeads8* heads_w0;   // [A] the window's start heads
-  uint8_t* hazard;            // [A] a head wraps past 2^32 within the window
+  uint2* comp8_w0;            // [A+1] byte snapshot of the window's start heads (sp.comp8 or comp8_alt, by
+                              // window); entry A inert
+  uint2* comp8_alt;           // [A+1] the other byte snapshot buffer
+  AxcBurstHeads8* heads_w0;   // [A] the window's start heads, written for the axons the window pushes (or
+                              // that start near the wrap) at their first push
   uint32_t* rev_off;          // [A+1] the reverse index: axon a's slots at rev[rev_off[a], rev_off[a+1]),
                               // sorted by segment
   uint64_t* rev;              // soma | slot << 32 | segment << 40

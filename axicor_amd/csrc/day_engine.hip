@@ -45,6 +45,9 @@ namespace {
 constexpr uint32_t kThreads = 256;
 
 inline uint32_t blocks_for(uint64_t n, uint32_t t = kThreads) { return (uint32_t)((n + t - 1) / t); }
+// Workgroups of the grid-stride per-tick kernels: eight per CU, so that
+// short-lived workgroups do not wait on the launch rate.
+constexpr uint32_t kPersistBlocks = 2048;
 // Soma blocks of neuron_tick, padded to a multiple of the XCD count.
 inline uint32_t neuron_blocks(uint32_t N) { return (blocks_for(N) + kNumXcd - 1) / kNumXcd * kNumXcd; }
 // Blocks of the staged soma pass (kBlockSomas somas each), and that count
@@ -210,10 +213,10 @@ void free_order_arrays(Shard* s) {
 }
 
 void free_window(Shard* s) {
-  for (void* q : {(void*)s->win.acc, (void*)s->win.comp8_w0, (void*)s->win.heads_w0, (void*)s->win.hazard,
-                  (void*)s->win.rev_off, (void*)s->win.rev, (void*)s->win.rev_cut, (void*)s->win.cnt,
-                  (void*)s->win.cpart, (void*)s->dirty.flag, (void*)s->dirty.npush, (void*)s->dirty.ctl,
-                  (void*)s->dirty.pieces, (void*)s->dirty.shard_off, (void*)s->dirty.shard_cnt})
+  for (void* q : {(void*)s->win.acc, (void*)s->win.comp8_alt, (void*)s->win.heads_w0, (void*)s->win.rev_off,
+                  (void*)s->win.rev, (void*)s->win.rev_cut, (void*)s->win.cnt, (void*)s->win.cpart,
+                  (void*)s->dirty.aux, (void*)s->dirty.ctl, (void*)s->dirty.pieces, (void*)s->dirty.shard_off,
+                  (void*)s->dirty.shard_cnt})
     if (q) (void)hipFree(q);
   s->win = WinState{};
   s->dirty = DirtyMarks{};
@@ -661,13 +664,11 @@ hipError_t ensure_window(Shard* s, hipStream_t st) {
     WinState w{};
     DirtyMarks d{};
     e = hipMalloc((void**)&w.acc, (size_t)kWin * (N ? N : 1) * sizeof(int64_t));
-    if (e == hipSuccess) e = hipMalloc((void**)&w.comp8_w0, (A + 1) * sizeof(uint2));
+    if (e == hipSuccess) e = hipMalloc((void**)&w.comp8_alt, (A + 1) * sizeof(uint2));
     if (e == hipSuccess) e = hipMalloc((void**)&w.heads_w0, (A ? A : 1) * sizeof(AxcBurstHeads8));
-    if (e == hipSuccess) e = hipMalloc((void**)&w.hazard, A ? A : 1);
     if (e == hipSuccess) e = hipMalloc((void**)&w.rev_off, (A + 1) * sizeof(uint32_t));
     if (e == hipSuccess) e = hipMalloc((void**)&w.rev_cut, (A ? A : 1) * sizeof(uint4));
-    if (e == hipSuccess) e = hipMalloc((void**)&d.flag, (A ? A : 1) * sizeof(uint32_t));
-    if (e == hipSuccess) e = hipMalloc((void**)&d.npush, A ? A : 1);
+    if (e == hipSuccess) e = hipMalloc((void**)&d.aux, (A ? A : 1) * sizeof(uint64_t));
     if (e == hipSuccess) e = hipMalloc((void**)&d.ctl, 2 * sizeof(uint32_t));
     if (e == hipSuccess) e = hipMalloc((void**)&d.shard_off, (kShards + 1) * sizeof(uint32_t));
     if (e == hipSuccess) e = hipMalloc((void**)&d.shard_cnt, 2 * kShards * sizeof(uint32_t));
@@ -676,8 +677,10 @@ hipError_t ensure_window(Shard* s, hipStream_t st) {
     if (e == hipSuccess) e = hipMalloc((void**)&w.cpart, (size_t)kListBlocks * 2 * sizeof(uint64_t));
     if (e == hipSuccess) e = hipMemsetAsync(w.cnt, 0, kWinCounters * sizeof(uint64_t), st);
     if (e == hipSuccess) e = hipMemsetAsync(w.cpart, 0, (size_t)kListBlocks * 2 * sizeof(uint64_t), st);
-    if (e == hipSuccess) e = hipMemsetAsync(d.npush, 0, A ? A : 1, st);
-    if (e == hipSuccess) e = hipMemsetAsync(d.flag, 0, (A ? A : 1) * sizeof(uint32_t), st);
+    // Stamps start at 0 and the first window's is 1: no record is current.
+    if (e == hipSuccess) e = hipMemsetAsync(d.aux, 0, (A ? A : 1) * sizeof(uint64_t), st);
+    // The second snapshot's inert entry (the first's is the plan's).
+    if (e == hipSuccess) e = hipMemsetAsync(w.comp8_alt + A, 0xFF, sizeof(uint2), st);
     if (e == hipSuccess) e = hipMemsetAsync(d.ctl, 0, 2 * sizeof(uint32_t), st);
     s->win = w;
     s->dirty = d;
@@ -754,20 +757,30 @@ hipError_t enqueue_windows(const ShardVramPtrs& v, Shard* s, uint32_t v_seg, uin
   const bool k3 = has_spikes && max_spk > 0;
   const size_t lds = (kStageCap + 1) * sizeof(uint2);
   const uint32_t* axon_inv = s->sp.perm ? s->d_axon_inv : nullptr;
-  for (uint32_t t0 = 0; t0 < T; t0 += win) {
+  // The byte snapshot of the heads at the batch's start into the plan's
+  // buffer; the windows then take the two buffers in turn as their start
+  // snapshot, writing the current one into the other.
+  {
+    TimedScope ts(kKWinStart, st, timed);
+    win_snapshot_kernel<<<blocks_for((uint64_t)A + 1), kThreads, 0, st>>>(v.axon_heads, A, s->sp.comp8);
+  }
+  uint32_t wi = 0;
+  for (uint32_t t0 = 0; t0 < T; t0 += win, wi++) {
     const uint32_t K = std::min(win, T - t0);
     const uint32_t lim = v_seg ? (uint32_t)((1ull << 32) - (uint64_t)(K + 1) * v_seg) : 0u;
+    WinState w = s->win;
+    w.comp8_w0 = (wi & 1) ? s->win.comp8_alt : s->sp.comp8;
+    uint2* cur = (wi & 1) ? s->sp.comp8 : s->win.comp8_alt;
+    StagePlan spc = s->sp;  // GSOP's current snapshot
+    spc.comp8 = cur;
     {
-      TimedScope ts(kKWinStart, st, timed);
-      win_start_kernel<<<blocks_for((uint64_t)A + 1), kThreads, 0, st>>>(v.axon_heads, A, lim, s->win, s->dirty);
-    }
-    if (N > 0) {
       TimedScope ts(kKWinPass, st, timed);
       const uint32_t g = staged_grid(N);
-      if (K == 1) win_pass_kernel<1><<<g, kStageThreads, lds, st>>>(v, N, A, lut, s->sp, s->win, v_seg, 0x00010001u);
-      else if (K == 2) win_pass_kernel<2><<<g, kStageThreads, lds, st>>>(v, N, A, lut, s->sp, s->win, v_seg, 0x00010001u);
-      else if (K == 3) win_pass_kernel<3><<<g, kStageThreads, lds, st>>>(v, N, A, lut, s->sp, s->win, v_seg, 0x00010001u);
-      else win_pass_kernel<4><<<g, kStageThreads, lds, st>>>(v, N, A, lut, s->sp, s->win, v_seg, 0x00010001u);
+      const uint32_t one2 = 0x00010001u;
+      if (K == 1) win_pass_kernel<1><<<g, kStageThreads, lds, st>>>(v, N, A, lut, s->sp, w, s->dirty, v_seg, one2);
+      else if (K == 2) win_pass_kernel<2><<<g, kStageThreads, lds, st>>>(v, N, A, lut, s->sp, w, s->dirty, v_seg, one2);
+      else if (K == 3) win_pass_kernel<3><<<g, kStageThreads, lds, st>>>(v, N, A, lut, s->sp, w, s->dirty, v_seg, one2);
+      else win_pass_kernel<4><<<g, kStageThreads, lds, st>>>(v, N, A, lut, s->sp, w, s->dirty, v_seg, one2);
     }
     for (uint32_t j = 0; j < K; j++) {
       const uint32_t t = t0 + j;
@@ -778,29 +791,30 @@ hipError_t enqueue_windows(const ShardVramPtrs& v, Shard* s, uint32_t v_seg, uin
       }
       {
         TimedScope ts(kKAxon, st, timed);
-        win_axon_kernel<<<blocks_for(A > kNumCounters ? A : kNumCounters), kThreads, 0, st>>>(
+        win_axon_kernel<<<blocks_for(A > kShards ? A : kShards), kThreads, 0, st>>>(
             v.axon_heads, A, v_seg, virt_off, k2 ? n_virt : 0u, s->sc.push_cnt, k3 ? 1 : 0, s->sc.counters, prm, t,
-            words, s->sp.comp8, axon_inv, lut, s->win, s->dirty, j);
+            words, cur, axon_inv, lut, w, s->dirty, j, lim);
       }
-      if (N > 0) {
-        {
-          TimedScope ts(kKWinCorrect, st, timed);
-          win_correct_kernel<<<kListBlocks, kThreads, 0, st>>>(v.axon_heads, v.soma_flags, N, A, lut, s->sp, s->win,
-                                                               s->dirty, v_seg, j);
-        }
+      {
+        TimedScope ts(kKWinCorrect, st, timed);
+        win_correct_kernel<<<kListBlocks, kThreads, 0, st>>>(v.axon_heads, v.soma_flags, N, A, lut, s->sp, w, s->dirty,
+                                                             v_seg, j);
+      }
+      {
         TimedScope ts(kKWinSoma, st, timed);
-        win_soma_kernel<<<blocks_for(N), kThreads, 0, st>>>(v, N, A, lut, prm, t, s->sc, s->sp, s->win, j);
+        win_soma_kernel<<<std::min<uint32_t>(blocks_for(N), kPersistBlocks), kThreads, 0, st>>>(v, N, A, lut, prm, t,
+                                                                                             s->sc, s->sp, w, j);
       }
       {
         TimedScope ts(kKPush, st, timed);
         const uint32_t n_words = (N + 63) / 64;
-        if (N > 0) compact_spikes_kernel<<<blocks_for(n_words), kThreads, 0, st>>>(n_words, s->sc);
-        win_push_kernel<<<kListBlocks / 4, kThreads, 0, st>>>(v.axon_heads, v.soma_to_axon, A, s->sc, s->sp.comp8,
-                                                             s->dirty);
+        compact_spikes_kernel<<<blocks_for(n_words), kThreads, 0, st>>>(n_words, s->sc);
+        win_push_kernel<<<kListBlocks / 4, kThreads, 0, st>>>(v.axon_heads, v.soma_to_axon, A, s->sc, cur, w, s->dirty,
+                                                             v_seg, j);
       }
       {
         TimedScope ts(kKGsop, st, timed);
-        gsop_win_kernel<<<kListBlocks, kThreads, 0, st>>>(v, N, A, lut, prm, 0, s->sc, s->sp, s->win, v_seg, j, K);
+        gsop_win_kernel<<<kListBlocks, kThreads, 0, st>>>(v, N, A, lut, prm, 0, s->sc, spc, w, s->dirty, v_seg, j, K);
       }
       if (has_out && n_out > 0) {
         TimedScope ts(kKReadout, st, timed);
@@ -1472,7 +1486,7 @@ int32_t axc_step_day_phase_batch(const ShardVramPtrs* vram, uint32_t padded_n, u
   const bool fuse = !key.staged;
   const bool pw = key.staged;
   // Staged batches stream the slots once per window of key.win ticks.
-  key.win = key.staged ? window_for(v_seg) : 1u;
+  key.win = (key.staged && padded_n > 0) ? window_for(v_seg) : 1u;
   key.rev = nullptr;
   if (key.win > 1) {
     if ((e = ensure_window(s, st)) != hipSuccess) return (int32_t)e;

@@ -186,14 +186,15 @@ __device__ __forceinline__ GsopTerms gsop_terms(uint8_t flags, const AxcVariantP
 // ---------------------------------------------------------------------------
 // axon_inv (may be null): the caller's id of each of the engine's axons,
 // which sets the axon's bit in the input bitmask. Returns the pushes applied;
-// out (may be null) receives the new heads.
+// out and pre (may be null) receive the new heads and the heads before.
 __device__ __forceinline__ uint32_t axon_step(AxcBurstHeads8* __restrict__ heads, uint32_t a, uint32_t v_seg,
                                           const uint32_t* __restrict__ bitmask, uint32_t virt_off, uint32_t n_virt,
                                           uint32_t* __restrict__ push_cnt, bool use_cnt, uint4* __restrict__ comp,
                                           uint2* __restrict__ comp8 = nullptr,
                                           const uint32_t* __restrict__ axon_inv = nullptr,
-                                          Heads8* out = nullptr) {
+                                          Heads8* out = nullptr, Heads8* pre = nullptr) {
   Heads8 h = load_heads(heads, a);
+  if (pre) *pre = h;
   uint32_t k = 0;
   if (bitmask) {
     uint32_t i = (axon_inv ? axon_inv[a] : a) - virt_off;  // wraps for ids below virt_off

@@ -7,24 +7,25 @@
 // nothing pushes, the hits of their slots over the next K ticks follow from
 // the heads at the start of those ticks. The batch graphs therefore stream
 // the slots once per window of K <= kWin ticks:
-//   win_start_kernel    the window's start: u32 and byte copies of the heads,
-//                       a new stamp, and the axons with a head that wraps
-//                       past 2^32 within the window;
 //   win_pass_kernel     per soma and tick j of the window, the hits that the
 //                       start heads predict for tick j and the sum of their
 //                       weights (the staged scan, K tests per slot);
 // and at each tick j:
-//   win_axon_kernel     K2/K3 + K4 (axon_step); for every axon pushed since
-//                       the window started (or near the wrap), the range of
-//                       the slots it feeds (the reverse index, built with the
+//   win_axon_kernel     K2/K3 + K4 (axon_step), keeping per axon its pushes
+//                       since the window started (aux); for each such axon
+//                       (or one starting near the wrap), the range of the
+//                       slots it feeds (the reverse index, built with the
 //                       plan) whose prediction can be wrong, as pieces of 16;
 //   win_correct_kernel  for those slots, where the actual hit differs from
 //                       the predicted one, +-(1 hit, the slot's current
 //                       weight) into the soma's sum of tick j;
 //   win_soma_kernel     K1 + the soma update of K5 from the corrected sums;
-//   win_push_kernel     the spikers' own-axon pushes (counted per axon);
+//   win_push_kernel     the spikers' own-axon pushes (kept in aux too);
 //   gsop_win_kernel     K6 on the working copy, plus (new - old weight) for
 //                       every predicted hit at the window's later ticks.
+// The window's start heads are its byte snapshot (two buffers taken in turn:
+// a window's ticks write the other one) and, for the axons it pushes, their
+// u32 values saved at the first push.
 // So soma i's input at tick j is the sum over its slots of w_j * hit_j
 // exactly: predicted hits times the start weights, plus the GSOP changes on
 // predicted hits, plus the corrections (current weights) on the slots of
@@ -52,15 +53,19 @@ __device__ __forceinline__ void win_unpack(int64_t V, int32_t& sum, uint32_t& cn
 }
 
 // The predicted hits of a slot (axon a in the snapshot's ids, segment seg,
-// length L) at ticks 0..K-1 of the window, as bits.
-__device__ __forceinline__ uint32_t win_pred_mask(const WinState& w, uint32_t A, uint32_t a, uint32_t seg,
-                                                  uint32_t L, uint32_t v_seg, uint32_t K) {
+// length L) at ticks 0..K-1 of the window, as bits. The u32 rule reads the
+// start heads from hb, which holds them advanced `back` segments already
+// (the heads themselves at the window's start, or later for an axon the
+// window has not pushed).
+__device__ __forceinline__ uint32_t win_pred_mask(const WinState& w, const AxcBurstHeads8* __restrict__ hb,
+                                                  uint32_t back, uint32_t A, uint32_t a, uint32_t seg, uint32_t L,
+                                                  uint32_t v_seg, uint32_t K) {
   uint32_t m = 0;
   if (seg + L > 255u) {
     if (a >= A) return 0;
 #pragma unroll
     for (uint32_t j = 0; j < kWin; j++)
-      if (j < K && min_dist_heads(w.heads_w0, A, a, seg - (j + 1) * v_seg) < L) m |= 1u << j;
+      if (j < K && min_dist_heads(hb, A, a, seg + back - (j + 1) * v_seg) < L) m |= 1u << j;
   } else {
     const uint2 e = w.comp8_w0[a];
 #pragma unroll
@@ -70,27 +75,41 @@ __device__ __forceinline__ uint32_t win_pred_mask(const WinState& w, uint32_t A,
   return m;
 }
 
-// The window's start (one lane per axon, lane A the inert entry).
-__global__ __launch_bounds__(256) void win_start_kernel(const AxcBurstHeads8* __restrict__ heads, uint32_t A,
-                                                        uint32_t hazard_lim, WinState w, DirtyMarks dm) {
+// The byte snapshot of the heads (at a batch's start, for its first window).
+__global__ __launch_bounds__(256) void win_snapshot_kernel(const AxcBurstHeads8* __restrict__ heads, uint32_t A,
+                                                           uint2* __restrict__ comp8) {
   const uint32_t a = blockIdx.x * 256 + threadIdx.x;
-  if (a == 0) {  // nobody reads ctl in this launch
-    dm.ctl[1] += 1;
-    w.cnt[0] += 1;
-  }
-  if (a < 2 * kShards) dm.shard_cnt[a] = 0;
-  if (a == A) w.comp8_w0[A] = make_uint2(0xFFFFFFFFu, 0xFFFFFFFFu);
-  if (a >= A) return;
-  const Heads8 h = load_heads(heads, a);
-  store_heads(w.heads_w0, a, h);
-  w.comp8_w0[a] = byte_heads(h);
-  // A live head at or above hazard_lim = 2^32 - (K+1) v_seg wraps within the
-  // window (its byte form would read as dead): 0 disables the test (v_seg 0).
+  if (a < A) comp8[a] = byte_heads(load_heads(heads, a));
+  if (a == A) comp8[A] = make_uint2(0xFFFFFFFFu, 0xFFFFFFFFu);
+}
+
+__device__ __forceinline__ uint64_t aux_pack(uint32_t stamp, uint32_t m, bool hz, uint32_t dmin, uint32_t dmax) {
+  return (uint64_t)stamp | ((uint64_t)m << 32) | ((uint64_t)(hz ? 1u : 0u) << 36) | ((uint64_t)dmin << 37) |
+         ((uint64_t)dmax << 50);
+}
+// Heads shifted out by a push of k: the last min(k, 8) of h; their values at
+// the window's start (h advanced `back` past it) go into [dmin, dmax].
+__device__ __forceinline__ void note_dropped(const Heads8& h, uint32_t k, uint32_t back, uint32_t& dmin,
+                                             uint32_t& dmax) {
   const uint32_t hv[8] = {h.lo.x, h.lo.y, h.lo.z, h.lo.w, h.hi.x, h.hi.y, h.hi.z, h.hi.w};
-  bool hz = false;
+  const uint32_t kk = k > 8 ? 8u : k;
 #pragma unroll
-  for (int k = 0; k < 8; k++) hz |= hazard_lim != 0 && hv[k] >= hazard_lim;
-  w.hazard[a] = hz ? 1 : 0;
+  for (uint32_t x = 0; x < 8; x++) {
+    const uint32_t v0 = hv[x] - back;
+    if (x >= 8 - kk && hv[x] != kSentinel && v0 < kNoDrop) {
+      dmin = min(dmin, v0);
+      dmax = max(dmax, v0);
+    }
+  }
+}
+// The window's start heads from heads advanced `back` past them (live heads;
+// a sentinel stays one).
+__device__ __forceinline__ Heads8 start_heads(const Heads8& h, uint32_t back) {
+  auto f = [back](uint32_t x) { return x != kSentinel ? x - back : x; };
+  Heads8 r;
+  r.lo = make_uint4(f(h.lo.x), f(h.lo.y), f(h.lo.z), f(h.lo.w));
+  r.hi = make_uint4(f(h.hi.x), f(h.hi.y), f(h.hi.z), f(h.hi.w));
+  return r;
 }
 
 // The segment of a reverse entry, and the first index in [r, b) whose
@@ -112,12 +131,14 @@ __device__ __forceinline__ uint32_t rev_lower(const uint64_t* __restrict__ rev, 
 // heads followed by the first 8 - m start heads advanced j+1 times (the
 // prediction P). The hit test is an OR over the heads, so a slot's
 // prediction and its actual test can differ only if one of the m new heads
-// or of the m start heads shifted out (advanced) lies in [s, s + L): only
+// or of the start heads shifted out (advanced) lies in [s, s + L): only
 // slots with a segment in (x - Lmax, x] for such a value x are tested. New
 // heads are at most (j+1) v_seg: a prefix of the axon's segment-sorted
 // reverse list (its ends for segments up to 7 are stored with it). A value
-// >= 255 + Lmax hits no slot. Axons near the wrap (whose byte prediction
-// does not follow P) test every slot.
+// >= 255 + Lmax hits no slot. Axons starting with a head at or above
+// hazard_lim = 2^32 - (K+1) v_seg (found at the window's first tick; 0: no
+// test) wrap within the window, where their byte prediction does not follow
+// P: every slot of theirs is tested. comp8: this window's current snapshot.
 __global__ __launch_bounds__(256) void win_axon_kernel(AxcBurstHeads8* __restrict__ heads, uint32_t A, uint32_t v_seg,
                                                        uint32_t virt_off, uint32_t n_virt,
                                                        uint32_t* __restrict__ push_cnt, int use_cnt,
@@ -126,26 +147,42 @@ __global__ __launch_bounds__(256) void win_axon_kernel(AxcBurstHeads8* __restric
                                                        uint32_t words, uint2* __restrict__ comp8,
                                                        const uint32_t* __restrict__ axon_inv,
                                                        const AxcVariantParameters* __restrict__ lut_g, WinState w,
-                                                       DirtyMarks dm, uint32_t j) {
+                                                       DirtyMarks dm, uint32_t j, uint32_t hazard_lim) {
   const uint32_t a = blockIdx.x * blockDim.x + threadIdx.x, lane = threadIdx.x & 63u;
   const uint32_t* __restrict__ bitmask = prm->bitmask ? prm->bitmask + (size_t)t_off * words : nullptr;
   if (a < kNumCounters) counters[a] = 0;
   if (a < kShards) dm.shard_cnt[((j + 1) & 1) * kShards + a] = 0;  // the next tick's (consumed last tick)
   const uint32_t stamp = dm.ctl[1];
   Heads8 cur{};
-  uint32_t m = 0;
+  uint32_t m = 0, dmin = kNoDrop, dmax = 0;
   bool hz = false;
   if (a < A) {
-    const uint32_t fl = dm.flag[a];
-    hz = w.hazard[a] != 0;
-    const uint32_t mp = fl == stamp ? (uint32_t)dm.npush[a] : 0u;
+    const uint64_t ax = dm.aux[a];
+    const bool mine = (uint32_t)ax == stamp;
+    if (mine) {
+      m = (uint32_t)(ax >> 32) & 0xFu;
+      hz = (ax >> 36) & 1u;
+      dmin = (uint32_t)(ax >> 37) & 0x1FFFu;
+      dmax = (uint32_t)(ax >> 50) & 0x1FFFu;
+    }
+    Heads8 pre;
     const uint32_t k = axon_step(heads, a, v_seg, n_virt ? bitmask : nullptr, virt_off, n_virt, push_cnt,
-                                 use_cnt != 0, nullptr, comp8, axon_inv, &cur);
-    m = mp;
-    if (k) {
-      m = min(8u, mp + k);
-      dm.npush[a] = (uint8_t)m;
-      if (fl != stamp) dm.flag[a] = stamp;
+                                 use_cnt != 0, nullptr, comp8, axon_inv, &cur, &pre);
+    bool hz_new = false;
+    if (j == 0 && hazard_lim != 0) {
+      const uint32_t hv[8] = {pre.lo.x, pre.lo.y, pre.lo.z, pre.lo.w, pre.hi.x, pre.hi.y, pre.hi.z, pre.hi.w};
+#pragma unroll
+      for (int x = 0; x < 8; x++) hz_new |= hv[x] != kSentinel && hv[x] >= hazard_lim;
+    }
+    if (k || hz_new) {
+      // The heads before this tick are the start heads advanced j times.
+      if (!mine) store_heads(w.heads_w0, a, start_heads(pre, j * v_seg));
+      if (k) {
+        note_dropped(pre, k, j * v_seg, dmin, dmax);
+        m = min(8u, m + k);
+      }
+      hz |= hz_new;
+      dm.aux[a] = aux_pack(stamp, m, hz, dmin, dmax);
     }
   }
   uint32_t rs = 0, cnt = 0;
@@ -154,23 +191,23 @@ __global__ __launch_bounds__(256) void win_axon_kernel(AxcBurstHeads8* __restric
 #pragma unroll
     for (int k = 0; k < 16; k++) Lmax = max(Lmax, (uint32_t)lut_g[k].signal_propagation_length);
     uint32_t lo = 256, hi = 0;  // the segments to test, [lo, hi]
+    auto take = [&](uint32_t xl, uint32_t xh) {  // values in [xl, xh]
+      if (xl < 255u + Lmax) {
+        lo = min(lo, xl >= Lmax ? xl - Lmax + 1 : 0u);
+        hi = max(hi, min(xh, 255u));
+      }
+    };
     if (hz) {
       lo = 0;
       hi = 255;
     } else {
-      const Heads8 h0 = load_heads(w.heads_w0, a);
       const uint32_t c[8] = {cur.lo.x, cur.lo.y, cur.lo.z, cur.lo.w, cur.hi.x, cur.hi.y, cur.hi.z, cur.hi.w};
-      const uint32_t z[8] = {h0.lo.x, h0.lo.y, h0.lo.z, h0.lo.w, h0.hi.x, h0.hi.y, h0.hi.z, h0.hi.w};
-      const uint32_t adv = (j + 1) * v_seg;
 #pragma unroll
-      for (uint32_t k = 0; k < 16; k++) {
-        // New head k, or start head k - 8 shifted out.
-        const bool take = k < 8 ? k < m : (k - 8 >= 8 - m && z[k - 8] != kSentinel);
-        const uint32_t x = k < 8 ? c[k] : z[k - 8] + adv;
-        if (take && x < 255u + Lmax) {
-          lo = min(lo, x >= Lmax ? x - Lmax + 1 : 0u);
-          hi = max(hi, min(x, 255u));
-        }
+      for (uint32_t k = 0; k < 8; k++)
+        if (k < m) take(c[k], c[k]);
+      if (dmin <= dmax) {
+        const uint32_t adv = (j + 1) * v_seg;
+        take(dmin + adv, dmax + adv);
       }
     }
     if (lo <= hi) {
@@ -209,26 +246,37 @@ __global__ __launch_bounds__(256) void win_axon_kernel(AxcBurstHeads8* __restric
   }
 }
 
-// The own-axon pushes of the tick's spikers (push_spikers_kernel), counted
-// per axon for the window.
+// The own-axon pushes of the tick's spikers (push_spikers_kernel), kept in
+// aux for the window (comp8: the window's current snapshot).
 __global__ __launch_bounds__(256) void win_push_kernel(AxcBurstHeads8* __restrict__ heads,
                                                        const uint32_t* __restrict__ soma_to_axon, uint32_t A,
-                                                       ShardScratch sc, uint2* __restrict__ comp8, DirtyMarks dm) {
+                                                       ShardScratch sc, uint2* __restrict__ comp8, WinState w,
+                                                       DirtyMarks dm, uint32_t v_seg, uint32_t j) {
   const uint32_t n = sc.counters[kCntSpikes];
   const uint32_t stamp = dm.ctl[1];
-  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
-    const uint32_t a = soma_to_axon[sc.spk_list[j]];
+  for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+    const uint32_t a = soma_to_axon[sc.spk_list[q]];
     if (a == kNoAxon || a >= A) continue;
     const uint32_t c = atomicExch(&sc.push_cnt[a], 0u);
     if (c == 0) continue;
+    const uint64_t ax = dm.aux[a];
+    const bool mine = (uint32_t)ax == stamp;
+    uint32_t m = 0, dmin = kNoDrop, dmax = 0;
+    bool hz = false;
+    if (mine) {
+      m = (uint32_t)(ax >> 32) & 0xFu;
+      hz = (ax >> 36) & 1u;
+      dmin = (uint32_t)(ax >> 37) & 0x1FFFu;
+      dmax = (uint32_t)(ax >> 50) & 0x1FFFu;
+    }
     Heads8 h = load_heads(heads, a);
+    // These heads are the start heads advanced j+1 times.
+    if (!mine) store_heads(w.heads_w0, a, start_heads(h, (j + 1) * v_seg));
+    note_dropped(h, c, (j + 1) * v_seg, dmin, dmax);
     push_heads(h, c > 8 ? 8 : c);
     store_heads(heads, a, h);
     comp8[a] = byte_heads(h);
-    const uint32_t fl = dm.flag[a];
-    const uint32_t mp = fl == stamp ? (uint32_t)dm.npush[a] : 0u;
-    dm.npush[a] = (uint8_t)min(8u, mp + c);
-    if (fl != stamp) dm.flag[a] = stamp;
+    dm.aux[a] = aux_pack(stamp, min(8u, m + c), hz, dmin, dmax);
   }
 }
 
@@ -246,8 +294,9 @@ __global__ __launch_bounds__(256) void win_push_kernel(AxcBurstHeads8* __restric
 // destination); the sums stay far below 2^31, so clamping changes nothing.
 template <bool STAGED, uint32_t K>
 __device__ __forceinline__ void win_chunk(const Chunk<STAGED, true>& now, uint32_t c, uint32_t N, uint32_t A,
-                                          const StagePlan& sp, const uint2* ent, const WinState& w, uint32_t i,
-                                          uint32_t prop, uint32_t v_seg, uint32_t one2, int32_t (&i_in)[K],
+                                          const StagePlan& sp, const uint2* ent, const WinState& w,
+                                          const AxcBurstHeads8* __restrict__ heads, uint32_t i, uint32_t prop,
+                                          uint32_t v_seg, uint32_t one2, int32_t (&i_in)[K],
                                           int32_t (&hits2)[(K + 1) / 2]) {
   static_assert(K >= 1 && K <= 4, "four predicted ticks per slot in one word");
   constexpr int U = 8;
@@ -275,7 +324,7 @@ __device__ __forceinline__ void win_chunk(const Chunk<STAGED, true>& now, uint32
         // The slot's own segment picks the rule (a spilled slot's may be
         // small); the inert index never hits.
         slow_m |= 1u << u;
-        slow_h |= win_pred_mask(w, A, snap_index(t, A), t >> 24, prop, v_seg, K) << (4 * u);
+        slow_h |= win_pred_mask(w, heads, 0, A, snap_index(t, A), t >> 24, prop, v_seg, K) << (4 * u);
       }
     }
   }
@@ -344,10 +393,20 @@ __device__ __forceinline__ void win_chunk(const Chunk<STAGED, true>& now, uint32
 // the predicted hits and their weight sum, added over the four lane groups
 // in LDS and stored to acc[j]. Somas refractory for the whole window
 // (timer >= K) are not scanned: no tick of the window reads their sums.
+// Workgroup 0 also starts the window's bookkeeping: a new stamp (every aux
+// record turns stale), the piece counters cleared (nothing reads them during
+// the pass).
 template <uint32_t K>
 __global__ __launch_bounds__(kStageThreads) __attribute__((amdgpu_waves_per_eu(8, 8)))
 void win_pass_kernel(ShardVramPtrs vram, uint32_t N, uint32_t A, const AxcVariantParameters* __restrict__ lut_g,
-                     StagePlan sp, WinState w, uint32_t v_seg, uint32_t one2) {
+                     StagePlan sp, WinState w, DirtyMarks dm, uint32_t v_seg, uint32_t one2) {
+  if (blockIdx.x == 0) {
+    if (threadIdx.x == 0) {
+      dm.ctl[1] += 1;
+      w.cnt[0] += 1;
+    }
+    if (threadIdx.x < 2 * kShards) dm.shard_cnt[threadIdx.x] = 0;
+  }
   extern __shared__ uint2 ent[];  // kStageCap + 1 entries
   __shared__ AxcVariantParameters lut[16];
   __shared__ unsigned long long vsh[K][kBlockSomas];
@@ -389,11 +448,11 @@ void win_pass_kernel(ShardVramPtrs vram, uint32_t N, uint32_t A, const AxcVarian
       if (staged) {
         Chunk<true, true> now;
         now.load(vram, sp, N, c, i);
-        win_chunk<true, K>(now, c, N, A, sp, ent, w, i, prop, v_seg, one2, i_in, hits2);
+        win_chunk<true, K>(now, c, N, A, sp, ent, w, vram.axon_heads, i, prop, v_seg, one2, i_in, hits2);
       } else {
         Chunk<false, true> now;
         now.load(vram, sp, N, c, i);
-        win_chunk<false, K>(now, c, N, A, sp, ent, w, i, prop, v_seg, one2, i_in, hits2);
+        win_chunk<false, K>(now, c, N, A, sp, ent, w, vram.axon_heads, i, prop, v_seg, one2, i_in, hits2);
       }
     }
 #pragma unroll
@@ -454,7 +513,7 @@ __global__ __launch_bounds__(256) void win_correct_kernel(const AxcBurstHeads8* 
     const uint64_t en = w.rev[(uint32_t)pc + l];
     const uint32_t i = (uint32_t)en, s = (uint32_t)(en >> 32) & 0xFFu, seg = rev_seg(en);
     const uint32_t L = prop[soma_flags[i] >> 4];
-    const bool pred = (win_pred_mask(w, A, a, seg, L, v_seg, j + 1) >> j) & 1u;
+    const bool pred = (win_pred_mask(w, w.heads_w0, 0, A, a, seg, L, v_seg, j + 1) >> j) & 1u;
     const bool hit = any_hit(load_heads(heads, a), seg, L);
     n_e += 1;
     if (pred != hit) {
@@ -492,42 +551,44 @@ __global__ __launch_bounds__(256) void win_soma_kernel(ShardVramPtrs vram, uint3
                                                        ShardScratch sc, StagePlan sp, WinState w, uint32_t j) {
   __shared__ AxcVariantParameters lut[16];
   __shared__ uint32_t red[4][8];
-  const uint32_t tid = threadIdx.x;
+  const uint32_t tid = threadIdx.x, lane = tid & 63u;
   if (tid < 64) reinterpret_cast<uint4*>(lut)[tid] = reinterpret_cast<const uint4*>(lut_g)[tid];
   __syncthreads();
   const uint32_t tick = tick_of(prm, tick_arg);
-  const uint32_t i = blockIdx.x * 256 + tid;
-  bool fin = false, push_own = false;
-  uint32_t my_axon = 0;
-  uint32_t n_int = 0, n_scan = 0, n_tread = 0, n_hits = 0;
-  if (i < N) {
-    const uint8_t flags_in = vram.soma_flags[i];
-    const uint8_t timer = vram.timers[i];
-    const bool refr = timer > 0;
-    int32_t sum = 0, v_in = 0, thr_in = 0;
-    uint32_t hc = 0, ax_in = 0;
-    if (!refr) {
-      win_unpack(w.acc[(size_t)j * N + i], sum, hc);
-      v_in = vram.soma_voltage[i];
-      thr_in = vram.threshold_offset[i];
-      ax_in = vram.soma_to_axon[i];
-      const uint32_t F = sp.fill[i];
-      n_scan = F;
-      n_tread = F < kMaxDendrites ? F + 1 : kMaxDendrites;
-      n_hits = hc;
-      n_int = 1;
+  uint32_t r[5] = {0, 0, 0, 0, 0};  // integrated, scanned, targets, hits, spikes
+  // Grid-stride over blocks of 256 somas (whole waves each round).
+  for (uint32_t i0 = blockIdx.x * 256; i0 < N; i0 += gridDim.x * 256) {
+    const uint32_t i = i0 + tid;
+    bool fin = false, push_own = false;
+    uint32_t my_axon = 0;
+    if (i < N) {
+      const uint8_t flags_in = vram.soma_flags[i];
+      const uint8_t timer = vram.timers[i];
+      const bool refr = timer > 0;
+      int32_t sum = 0, v_in = 0, thr_in = 0;
+      uint32_t hc = 0, ax_in = 0;
+      if (!refr) {
+        win_unpack(w.acc[(size_t)j * N + i], sum, hc);
+        v_in = vram.soma_voltage[i];
+        thr_in = vram.threshold_offset[i];
+        ax_in = vram.soma_to_axon[i];
+        const uint32_t F = sp.fill[i];
+        r[0] += 1;
+        r[1] += F;
+        r[2] += F < kMaxDendrites ? F + 1 : kMaxDendrites;
+        r[3] += hc;
+      }
+      const uint32_t hb_i = sp.perm ? sp.perm[i] : i;
+      uint8_t flags = flags_in & 0xF1;  // K1
+      flags &= (uint8_t)~0x01;
+      soma_tick_with(vram, i, A, lut[(flags_in >> 4) & 0x0F], flags, timer, refr, tick, sum, thr_in, v_in, ax_in,
+                     fin, push_own, my_axon, hb_i);
     }
-    const uint32_t hb_i = sp.perm ? sp.perm[i] : i;
-    uint8_t flags = flags_in & 0xF1;  // K1
-    flags &= (uint8_t)~0x01;
-    soma_tick_with(vram, i, A, lut[(flags_in >> 4) & 0x0F], flags, timer, refr, tick, sum, thr_in, v_in, ax_in, fin,
-                   push_own, my_axon, hb_i);
+    if (push_own) atomicAdd(&sc.push_cnt[my_axon], 1u);
+    const uint64_t bs = __ballot(fin);
+    if (lane == 0) sc.spk_mask[i >> 6] = bs;
+    r[4] += (uint32_t)__popcll(bs);
   }
-  if (push_own) atomicAdd(&sc.push_cnt[my_axon], 1u);
-  const uint32_t lane = tid & 63u;
-  const uint64_t bs = __ballot(fin);
-  if (lane == 0) sc.spk_mask[i >> 6] = bs;
-  uint32_t r[5] = {n_int, n_scan, n_tread, n_hits, (uint32_t)__popcll(bs)};
 #pragma unroll
   for (int k = 0; k < 4; k++) {
 #pragma unroll
@@ -559,8 +620,8 @@ __global__ __launch_bounds__(256) void win_soma_kernel(ShardVramPtrs vram, uint3
 __global__ __launch_bounds__(256) void gsop_win_kernel(ShardVramPtrs vram, uint32_t N, uint32_t A,
                                                        const AxcVariantParameters* __restrict__ lut_g,
                                                        const TickParams* __restrict__ prm, int32_t dop_arg,
-                                                       ShardScratch sc, StagePlan sp, WinState w, uint32_t v_seg,
-                                                       uint32_t j, uint32_t K) {
+                                                       ShardScratch sc, StagePlan sp, WinState w, DirtyMarks dm,
+                                                       uint32_t v_seg, uint32_t j, uint32_t K) {
   __shared__ AxcVariantParameters lut[16];
   __shared__ uint32_t red[4][2];
   const uint32_t tid = threadIdx.x;
@@ -599,7 +660,16 @@ __global__ __launch_bounds__(256) void gsop_win_kernel(ShardVramPtrs vram, uint3
             const int32_t w0 = (int32_t)(int16_t)(uint16_t)(before[u >> 1] >> (16 * (u & 1)));
             const int32_t w1 = (int32_t)(int16_t)(uint16_t)(after[u >> 1] >> (16 * (u & 1)));
             if ((uint32_t)u >= nv || w0 == w1) continue;
-            const uint32_t m = win_pred_mask(w, A, snap_index(t[u], A), t[u] >> 24, L, v_seg, K);
+            // The u32 rule (segment + length > 255) needs the start heads: saved
+            // for an axon the window pushed, else the heads advanced j+1 times.
+            const uint32_t a = snap_index(t[u], A);
+            const AxcBurstHeads8* hb = vram.axon_heads;
+            uint32_t back = (j + 1) * v_seg;
+            if ((t[u] >> 24) + L > 255u && a < A && (uint32_t)dm.aux[a] == dm.ctl[1]) {
+              hb = w.heads_w0;
+              back = 0;
+            }
+            const uint32_t m = win_pred_mask(w, hb, back, A, a, t[u] >> 24, L, v_seg, K);
 #pragma unroll
             for (uint32_t jj = 0; jj < kWin; jj++)
               if (jj >= first && ((m >> jj) & 1u)) dv[jj] += w1 - w0;
