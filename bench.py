@@ -204,25 +204,36 @@ def bench_batched(n_brains: int, ticks: int, input_ticks: int, batch_ticks: int,
         lib.gpu_stream_destroy(st)
         return {"brains": n_brains, "ticks_per_brain": ticks, "neurons": neurons, "synapses": synapses,
                 "batched_brains_per_s": n_brains / t_batched, "batched_seconds": t_batched, "tuning": tune}
-    # Serial: the same brains from their initial state, one after another.
-    for (b, _, _), m in zip(members, mems):
-        m["vram"].upload_state(b.state)
-        m["vram"].upload_axon_heads(b.heads)
-    w0 = time.perf_counter()
-    for (b, dop, _), m in zip(members, mems):
-        upload_constant_memory(b.lut)
-        v = m["vram"]
-        for t in range(0, ticks, batch_ticks):
-            T = min(batch_ticks, ticks - t)
-            row = t % input_ticks
-            err = lib.axc_step_day_phase_batch(C.byref(v.ptrs), v.padded_n, v.total_axons, 1, t, T,
-                                               m["d_inputs"].ptr + row * b.input_words * 4, b.input_words,
-                                               b.virtual_offset, b.n_inputs, None, 0, None, None, None, 0, dop, st)
-            if err:
-                raise RuntimeError(f"serial step failed {err}")
-        lib.gpu_stream_synchronize(st)
-    t_serial = time.perf_counter() - w0
-    log(f"serial: {ticks} ticks x {n_brains} brains in {t_serial:.2f}s")
+    # Serial: the same brains from their initial state, one after another,
+    # with the engine's per-tick staged pass and with windows of 4 ticks; the
+    # faster of the two is the baseline (small brains favour the per-tick
+    # pass).
+    saved_window = lib.axc_window_ticks()
+    serial_s = {}
+    for win in (1, 4):
+        lib.axc_set_window_ticks(win)
+        for (b, _, _), m in zip(members, mems):
+            m["vram"].upload_state(b.state)
+            m["vram"].upload_axon_heads(b.heads)
+        w0 = time.perf_counter()
+        for (b, dop, _), m in zip(members, mems):
+            upload_constant_memory(b.lut)
+            v = m["vram"]
+            for t in range(0, ticks, batch_ticks):
+                T = min(batch_ticks, ticks - t)
+                row = t % input_ticks
+                err = lib.axc_step_day_phase_batch(C.byref(v.ptrs), v.padded_n, v.total_axons, 1, t, T,
+                                                   m["d_inputs"].ptr + row * b.input_words * 4, b.input_words,
+                                                   b.virtual_offset, b.n_inputs, None, 0, None, None, None, 0, dop,
+                                                   st)
+                if err:
+                    raise RuntimeError(f"serial step failed {err}")
+            lib.gpu_stream_synchronize(st)
+        serial_s[win] = time.perf_counter() - w0
+        log(f"serial (windows of {win}): {ticks} ticks x {n_brains} brains in {serial_s[win]:.2f}s")
+    lib.axc_set_window_ticks(saved_window)
+    best_window = min(serial_s, key=serial_s.get)
+    t_serial = serial_s[best_window]
     identical = all(np.array_equal(rb[0], m["vram"].download_state()) and
                     np.array_equal(rb[1], m["vram"].download_axons()) for rb, m in zip(res_b, mems))
     log(f"batched vs serial: {'bit-identical' if identical else 'MISMATCH'} for all {n_brains} brains")
@@ -255,7 +266,9 @@ def bench_batched(n_brains: int, ticks: int, input_ticks: int, batch_ticks: int,
         "batched_brains_per_s": n_brains / t_batched, "serial_brains_per_s": n_brains / t_serial,
         "speedup": t_serial / t_batched, "bit_identical_all_brains": identical, "tuning": tune,
         "batched_seconds": t_batched, "serial_seconds": t_serial,
-        "serial": "each brain alone through axc_step_day_phase_batch (HIP-graph batches), one after another",
+        "serial": (f"each brain alone through axc_step_day_phase_batch (HIP-graph batches), one after another; "
+                   f"the faster of windows of 1 and 4 ticks ({best_window})"),
+        "serial_seconds_by_window": {str(k): v for k, v in serial_s.items()},
         "serial_per_tick_brains_per_s": n_brains / t_per_tick, "speedup_vs_per_tick": t_per_tick / t_batched,
         "serial_per_tick": f"each brain alone, one cu_step_day_phase call per tick (the reference's launch "
                            f"structure), timed over its first {pt_ticks} ticks and scaled to {ticks}",
@@ -284,7 +297,7 @@ def main() -> int:
                          "each after one more batch of --ticks ticks, timed phase by phase")
     ap.add_argument("--prune", type=int, default=15, help="prune threshold of the Nights")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--window", type=int, default=1,
+    ap.add_argument("--window", type=int, default=4,
                     help="ticks per window of the batch graphs' staged pass (axc_set_window_ticks; 1: per tick)")
     ap.add_argument("--timing-ticks", type=int, default=12,
                     help="ticks per batch of the two batches whose kernels are timed one by one with HIP events")
@@ -293,8 +306,9 @@ def main() -> int:
     ap.add_argument("--batched-only", action="store_true",
                     help="run only the batched-brains leg (for profiling); prints its JSON object")
     ap.add_argument("--batched-no-serial", action="store_true", help="skip the serial runs of the batched leg")
-    ap.add_argument("--copy-calibration-mb", type=int, default=0,
-                    help="also run one device-to-device copy of this size (reference for HBM byte counters)")
+    ap.add_argument("--copy-calibration-mb", default="",
+                    help="comma-separated sizes: one device-to-device copy of each (reference points for the HBM "
+                         "byte counters)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -392,14 +406,14 @@ def main() -> int:
     log(f"warmup done at tick {tick}")
 
     hip = C.CDLL("libamdhip64.so")
-    if args.copy_calibration_mb:
-        nb = args.copy_calibration_mb << 20
+    for mb in [int(x) for x in str(args.copy_calibration_mb).split(",") if x.strip()]:
+        nb = mb << 20
         src, dst = DeviceBuffer(nb), DeviceBuffer(nb)
         hip.hipMemcpy(C.c_void_p(dst.ptr), C.c_void_p(src.ptr), C.c_size_t(nb), 3)  # hipMemcpyDeviceToDevice
         hip.hipDeviceSynchronize()
         src.free()
         dst.free()
-        log(f"copy calibration: {args.copy_calibration_mb} MiB device-to-device")
+        log(f"copy calibration: {mb} MiB device-to-device")
 
     eng.stats_reset()
     start = C.c_void_p()
