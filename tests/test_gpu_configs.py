@@ -2,10 +2,11 @@
 oracle (and, for the batched brains, against each brain run alone):
 
 - the 1.5M-neuron brain (100 slots, 150M synapses): every soma's spike bit
-  every tick for 500 ticks through the batch graph, the whole state and the
-  heads at tick 500, one Night (GPU sort & prune at 15, host sprouting, the
+  every tick for 500 ticks through the batch graph (windows of 4 ticks to
+  tick 250, the per-tick pass after), the whole state and the heads at
+  ticks 250 and 500, one Night (GPU sort & prune at 15, host sprouting, the
   upload, shard_thread.rs:269-378) checked against the oracle's Night, then
-  50 more ticks;
+  50 more ticks in windows of 4;
 - 128 batched brains of 8k-250k neurons with 32, 64 or 128 slots, each with
   its own parameter table, inputs and dopamine, for 100 ticks: every brain's
   state and heads equal to the same brain stepped alone, and three sampled
@@ -42,10 +43,17 @@ def test_c2_500_ticks_then_a_night_with_sprouting(gpu):
     p.d_mapped = DeviceBuffer.from_array(p.mapped)
     types = NT.sprout_types_from_lut(b.lut)
     geo_gpu, geo_cpu = NT.geometry_of(b), NT.geometry_of(b)
-    while p.tick < 500:
-        out_ref, _ = p.step(50, dopamine=(40 if (p.tick // 50) % 2 else -25), use_graph=True)
-        np.testing.assert_array_equal(p.outputs(50), out_ref, err_msg=f"C2 raster, ticks to {p.tick}")
-        print(f"[c2] ticks to {p.tick}: raster equal, {int(out_ref.sum())} spikes", flush=True)
+    lib = day_lib()
+    try:
+        while p.tick < 500:
+            lib.axc_set_window_ticks(4 if p.tick < 250 else 1)
+            out_ref, _ = p.step(50, dopamine=(40 if (p.tick // 50) % 2 else -25), use_graph=True)
+            np.testing.assert_array_equal(p.outputs(50), out_ref, err_msg=f"C2 raster, ticks to {p.tick}")
+            print(f"[c2] ticks to {p.tick}: raster equal, {int(out_ref.sum())} spikes", flush=True)
+            if p.tick == 250:
+                p.check("C2 at tick 250 (windows of 4)")
+    finally:
+        lib.axc_set_window_ticks(1)
     p.check("C2 before the Night at tick 500")
     res = NT.execute_night_phase(p.vram, 15, geo_gpu, types, epoch=0, master_seed=b.spec.seed)
     v = StateView(p.ref_state, N)
@@ -60,7 +68,11 @@ def test_c2_500_ticks_then_a_night_with_sprouting(gpu):
         np.testing.assert_array_equal(getattr(geo_gpu, f), getattr(geo_cpu, f), err_msg=f)
     p.check("C2 after the Night at tick 500")
     print(f"[c2] Night at 500: {ref.new_synapses} new synapses, state equal", flush=True)
-    out_ref, _ = p.step(50, dopamine=10, use_graph=True)
+    lib.axc_set_window_ticks(4)
+    try:
+        out_ref, _ = p.step(50, dopamine=10, use_graph=True)
+    finally:
+        lib.axc_set_window_ticks(1)
     np.testing.assert_array_equal(p.outputs(50), out_ref, err_msg="C2 raster after the Night")
     p.check("C2 at tick 550")
 
