@@ -261,6 +261,12 @@ __device__ __forceinline__ uint32_t pair_min(uint32_t ru, uint32_t rv) {
   const axc_us2 hi = __builtin_bit_cast(axc_us2, __builtin_amdgcn_perm(rv, ru, 0x07060302u));
   return __builtin_bit_cast(uint32_t, __builtin_elementwise_min(lo, hi));
 }
+// [max of ru, max of rv]: pair_min with the larger lane.
+__device__ __forceinline__ uint32_t pair_max(uint32_t ru, uint32_t rv) {
+  const axc_us2 lo = __builtin_bit_cast(axc_us2, __builtin_amdgcn_perm(rv, ru, 0x05040100u));
+  const axc_us2 hi = __builtin_bit_cast(axc_us2, __builtin_amdgcn_perm(rv, ru, 0x07060302u));
+  return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(lo, hi));
+}
 // Largest of the eight bytes of (x, y).
 __device__ __forceinline__ uint32_t max_byte8(uint32_t x, uint32_t y) {
   const axc_us2 a = __builtin_bit_cast(axc_us2, x & 0x00FF00FFu), b = __builtin_bit_cast(axc_us2, (x >> 8) & 0x00FF00FFu);

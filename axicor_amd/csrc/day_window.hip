@@ -292,12 +292,12 @@ __global__ __launch_bounds__(256) void win_push_kernel(AxcBurstHeads8* __restric
 // a lane's slots in one soma are at most 32.
 // The dot products use the clamped form (no accumulator tied to the
 // destination); the sums stay far below 2^31, so clamping changes nothing.
-template <bool STAGED, uint32_t K>
+template <bool STAGED, uint32_t K, bool ONCE>
 __device__ __forceinline__ void win_chunk(const Chunk<STAGED, true>& now, uint32_t c, uint32_t N, uint32_t A,
                                           const StagePlan& sp, const uint2* ent, const WinState& w,
                                           const AxcBurstHeads8* __restrict__ heads, uint32_t i, uint32_t prop,
-                                          uint32_t v_seg, uint32_t one2, int32_t (&i_in)[K],
-                                          int32_t (&hits2)[(K + 1) / 2]) {
+                                          uint32_t v_seg, uint32_t one2, const uint32_t (&plen)[K],
+                                          int32_t (&i_in)[K], int32_t (&hits2)[(K + 1) / 2]) {
   static_assert(K >= 1 && K <= 4, "four predicted ticks per slot in one word");
   constexpr int U = 8;
   const axc_us2 P = {(unsigned short)prop, (unsigned short)prop};
@@ -349,6 +349,46 @@ __device__ __forceinline__ void win_chunk(const Chunk<STAGED, true>& now, uint32
     const uint32_t sm2 = (slow_m >> h0) & 3u;
     const uint32_t keep = ((sm2 & 1u) ? 0u : 0x0000FFFFu) | ((sm2 & 2u) ? 0u : 0xFFFF0000u);
     const uint32_t wz = now.w2[h0 >> 1] & keep, cm = 0x00010001u & keep, cm8 = 0x01000100u & keep;
+    if constexpr (ONCE) {
+      // One pass over the heads for all K ticks. D = head - segment + v_seg
+      // is the tick-0 distance; tick j adds j v_seg. As u16, D's from heads
+      // behind the segment (negative) lie above 2^16 - 255 and the others
+      // below 2^16 - 255 - K v_seg, so the least u16 D is the nearest head
+      // at or past the segment (hit at tick j if D + j v_seg < L) and the
+      // greatest one the nearest head behind it (hit at tick j if
+      // D + j v_seg wraps, that is D > 0xFFFF - j v_seg, as long as
+      // j v_seg <= L: the caller takes this path only when (K-1) v_seg <= L).
+      const uint32_t v1 = v_seg & 0xFFFFu;
+      const axc_us2 V = {(unsigned short)v1, (unsigned short)v1};
+      uint32_t lo2[2], hi2[2];
+#pragma unroll
+      for (int x = 0; x < 2; x++) {
+        const axc_us2 s = __builtin_bit_cast(axc_us2, S[x]) - V;
+        const axc_us2 d0 = __builtin_bit_cast(axc_us2, q[x][0]) - s, d1 = __builtin_bit_cast(axc_us2, q[x][1]) - s;
+        const axc_us2 d2 = __builtin_bit_cast(axc_us2, q[x][2]) - s, d3 = __builtin_bit_cast(axc_us2, q[x][3]) - s;
+        lo2[x] = __builtin_bit_cast(uint32_t, __builtin_elementwise_min(__builtin_elementwise_min(d0, d1),
+                                                                        __builtin_elementwise_min(d2, d3)));
+        hi2[x] = __builtin_bit_cast(uint32_t, __builtin_elementwise_max(__builtin_elementwise_max(d0, d1),
+                                                                        __builtin_elementwise_max(d2, d3)));
+      }
+      const axc_us2 MN = __builtin_bit_cast(axc_us2, pair_min(lo2[0], lo2[1]));
+      const axc_us2 MX = __builtin_bit_cast(axc_us2, pair_max(hi2[0], hi2[1]));
+#pragma unroll
+      for (uint32_t j = 0; j < K; j++) {
+        // plen[j]: max(L - j v_seg, 0) in both lanes.
+        axc_us2 t = __builtin_elementwise_sub_sat(__builtin_bit_cast(axc_us2, plen[j]), MN);
+        if (j > 0) {
+          const uint32_t qj = (0xFFFFu - j * v1) & 0xFFFFu;
+          const axc_us2 Q = {(unsigned short)qj, (unsigned short)qj};
+          t = __builtin_elementwise_max(t, __builtin_elementwise_sub_sat(MX, Q));
+        }
+        const axc_us2 hv = __builtin_elementwise_min(t, kOne);
+        i_in[j] = __builtin_amdgcn_sdot2(__builtin_bit_cast(axc_s2, hv), __builtin_bit_cast(axc_s2, wz), i_in[j], true);
+        hits2[j / 2] = __builtin_amdgcn_sdot2(__builtin_bit_cast(axc_s2, hv), __builtin_bit_cast(axc_s2, (j & 1) ? cm8 : cm),
+                                              hits2[j / 2], true);
+      }
+      continue;
+    }
 #pragma unroll
     for (uint32_t j = 0; j < K; j++) {
       const uint32_t sh = ((j + 1) * v_seg) & 0xFFFFu;
@@ -443,16 +483,38 @@ void win_pass_kernel(ShardVramPtrs vram, uint32_t N, uint32_t A, const AxcVarian
     for (uint32_t j = 0; j < (K + 1) / 2; j++) hits2[j] = 0;
     const uint32_t gu = __builtin_amdgcn_readfirstlane(g);
     const uint32_t cg = (gu + kStageGroups - 1) % kStageGroups;
+    uint32_t plen[K];
+#pragma unroll
+    for (uint32_t j = 0; j < K; j++) {
+      const uint32_t pj = prop > j * v_seg ? min(prop - j * v_seg, 0xFFFFu) : 0u;
+      plen[j] = pj * 0x00010001u;
+    }
+    // The one-pass test of all K ticks holds while (K-1) v_seg <= L.
+    if (prop >= (K - 1) * v_seg) {
 #pragma unroll 1
-    for (uint32_t c = cg; c * 8 < F; c += kStageGroups) {
-      if (staged) {
-        Chunk<true, true> now;
-        now.load(vram, sp, N, c, i);
-        win_chunk<true, K>(now, c, N, A, sp, ent, w, vram.axon_heads, i, prop, v_seg, one2, i_in, hits2);
-      } else {
-        Chunk<false, true> now;
-        now.load(vram, sp, N, c, i);
-        win_chunk<false, K>(now, c, N, A, sp, ent, w, vram.axon_heads, i, prop, v_seg, one2, i_in, hits2);
+      for (uint32_t c = cg; c * 8 < F; c += kStageGroups) {
+        if (staged) {
+          Chunk<true, true> now;
+          now.load(vram, sp, N, c, i);
+          win_chunk<true, K, true>(now, c, N, A, sp, ent, w, vram.axon_heads, i, prop, v_seg, one2, plen, i_in, hits2);
+        } else {
+          Chunk<false, true> now;
+          now.load(vram, sp, N, c, i);
+          win_chunk<false, K, true>(now, c, N, A, sp, ent, w, vram.axon_heads, i, prop, v_seg, one2, plen, i_in, hits2);
+        }
+      }
+    } else {
+#pragma unroll 1
+      for (uint32_t c = cg; c * 8 < F; c += kStageGroups) {
+        if (staged) {
+          Chunk<true, true> now;
+          now.load(vram, sp, N, c, i);
+          win_chunk<true, K, false>(now, c, N, A, sp, ent, w, vram.axon_heads, i, prop, v_seg, one2, plen, i_in, hits2);
+        } else {
+          Chunk<false, true> now;
+          now.load(vram, sp, N, c, i);
+          win_chunk<false, K, false>(now, c, N, A, sp, ent, w, vram.axon_heads, i, prop, v_seg, one2, plen, i_in, hits2);
+        }
       }
     }
 #pragma unroll
