@@ -151,7 +151,7 @@ SIGNATURES = {
 }
 
 TIMED_KERNELS = ["axon_tick", "neuron_tick", "push_apply", "gsop", "readout", "spike_count", "weights_copy",
-                 "window_start", "window_pass", "window_correct", "window_soma"]
+                 "window_start", "window_pass", "window_correct", "window_soma", "window_end"]
 
 SURROGATE_SIGNATURES = {
     "axc_mlp_create": (_I32, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint64, C.POINTER(C.c_void_p)]),

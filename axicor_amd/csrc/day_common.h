@@ -125,18 +125,20 @@ constexpr uint32_t kWin = AXC_WIN_TICKS;
 // u32 wrap, bits 37-49 and 50-62 the least and greatest window-start value
 // of the live heads shifted out so far (8191 and 0 for none; values >= 8191
 // are left out: they hit no slot). Each tick's corrections are pieces of up
-// to 16 reverse entries, start | axon << 32 | (count - 1) << 56, kept in
-// kShards parts by axon group ((a / 64) % kShards, so that no counter takes
-// many atomics): part s at pieces[shard_off[s] + k], k < shard_cnt[(tick &
-// 1) * kShards + s]; its capacity is the most pieces its axons can make.
-constexpr uint32_t kShards = 256;
+// to 16 reverse entries, start | axon << 32 | (count - 1) << 56, at
+// pieces[k], k < piece_cnt[tick & 1]; the pool holds the most pieces all
+// axons can make in a tick (an axon's range makes at most ceil(its list
+// length / 16)).
 constexpr uint32_t kNoDrop = 8191;
+// The axons whose record is current in a window (pushed since it started,
+// or marked near the wrap) are listed once each in act, by the parity of
+// the window's stamp: act[(stamp & 1) * A + k], k < ctl[2 + (stamp & 1)].
 struct DirtyMarks {
   uint64_t* aux;        // [A]
-  uint32_t* ctl;        // [2]: (unused), stamp
-  uint64_t* pieces;     // [shard_off[kShards]]
-  uint32_t* shard_off;  // [kShards + 1]
-  uint32_t* shard_cnt;  // [2][kShards] pieces of even and odd ticks
+  uint32_t* ctl;        // [4]: (unused), stamp, active counts of even and odd stamps
+  uint64_t* pieces;     // the pool
+  uint32_t* piece_cnt;  // [2] pieces of even and odd ticks
+  uint32_t* act;        // [2][A]
 };
 
 struct WinState {

@@ -172,7 +172,7 @@ std::unordered_map<const void*, Shard*> g_shards;
 // caller's stream around each launch, summed by axc_kernel_timing_read.
 enum TimedKernel {
   kKAxon = 0, kKNeuron, kKPush, kKGsop, kKReadout, kKSpikes, kKWeights,
-  kKWinStart, kKWinPass, kKWinCorrect, kKWinSoma, kNumTimed
+  kKWinStart, kKWinPass, kKWinCorrect, kKWinSoma, kKWinEnd, kNumTimed
 };
 struct TimingState {
   bool enabled = false;
@@ -215,8 +215,8 @@ void free_order_arrays(Shard* s) {
 void free_window(Shard* s) {
   for (void* q : {(void*)s->win.acc, (void*)s->win.comp8_alt, (void*)s->win.heads_w0, (void*)s->win.rev_off,
                   (void*)s->win.rev, (void*)s->win.rev_cut, (void*)s->win.cnt, (void*)s->win.cpart,
-                  (void*)s->dirty.aux, (void*)s->dirty.ctl, (void*)s->dirty.pieces, (void*)s->dirty.shard_off,
-                  (void*)s->dirty.shard_cnt})
+                  (void*)s->dirty.aux, (void*)s->dirty.ctl, (void*)s->dirty.pieces, (void*)s->dirty.piece_cnt,
+                  (void*)s->dirty.act})
     if (q) (void)hipFree(q);
   s->win = WinState{};
   s->dirty = DirtyMarks{};
@@ -669,10 +669,10 @@ hipError_t ensure_window(Shard* s, hipStream_t st) {
     if (e == hipSuccess) e = hipMalloc((void**)&w.rev_off, (A + 1) * sizeof(uint32_t));
     if (e == hipSuccess) e = hipMalloc((void**)&w.rev_cut, (A ? A : 1) * sizeof(uint4));
     if (e == hipSuccess) e = hipMalloc((void**)&d.aux, (A ? A : 1) * sizeof(uint64_t));
-    if (e == hipSuccess) e = hipMalloc((void**)&d.ctl, 2 * sizeof(uint32_t));
-    if (e == hipSuccess) e = hipMalloc((void**)&d.shard_off, (kShards + 1) * sizeof(uint32_t));
-    if (e == hipSuccess) e = hipMalloc((void**)&d.shard_cnt, 2 * kShards * sizeof(uint32_t));
-    if (e == hipSuccess) e = hipMemsetAsync(d.shard_cnt, 0, 2 * kShards * sizeof(uint32_t), st);
+    if (e == hipSuccess) e = hipMalloc((void**)&d.ctl, 4 * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc((void**)&d.act, 2 * (A ? A : 1) * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc((void**)&d.piece_cnt, 2 * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMemsetAsync(d.piece_cnt, 0, 2 * sizeof(uint32_t), st);
     if (e == hipSuccess) e = hipMalloc((void**)&w.cnt, kWinCounters * sizeof(uint64_t));
     if (e == hipSuccess) e = hipMalloc((void**)&w.cpart, (size_t)kListBlocks * 2 * sizeof(uint64_t));
     if (e == hipSuccess) e = hipMemsetAsync(w.cnt, 0, kWinCounters * sizeof(uint64_t), st);
@@ -681,7 +681,7 @@ hipError_t ensure_window(Shard* s, hipStream_t st) {
     if (e == hipSuccess) e = hipMemsetAsync(d.aux, 0, (A ? A : 1) * sizeof(uint64_t), st);
     // The second snapshot's inert entry (the first's is the plan's).
     if (e == hipSuccess) e = hipMemsetAsync(w.comp8_alt + A, 0xFF, sizeof(uint2), st);
-    if (e == hipSuccess) e = hipMemsetAsync(d.ctl, 0, 2 * sizeof(uint32_t), st);
+    if (e == hipSuccess) e = hipMemsetAsync(d.ctl, 0, 4 * sizeof(uint32_t), st);
     s->win = w;
     s->dirty = d;
     if (e != hipSuccess) {
@@ -729,12 +729,7 @@ hipError_t ensure_window(Shard* s, hipStream_t st) {
                                                                                            s->win.rev_off, tmp,
                                                                                            s->win.rev,
                                                                                            s->win.rev_cut);
-  // The correction parts' capacities and offsets (cursor as scratch).
-  if (e == hipSuccess) e = hipMemsetAsync(cursor, 0, kShards * sizeof(uint32_t), st);
-  if (e == hipSuccess && A)
-    shard_cap_kernel<<<blocks_for(A), kThreads, 0, st>>>((uint32_t)A, s->win.rev_off, cursor);
-  if (e == hipSuccess) rev_scan_kernel<<<1, 1024, 0, st>>>(cursor, kShards, s->dirty.shard_off);
-  e = hipGetLastError();
+  if (e == hipSuccess) e = hipGetLastError();
   const hipError_t e2 = hipStreamSynchronize(st);
   (void)hipFree(tmp);
   (void)hipFree(cursor);
@@ -760,14 +755,21 @@ hipError_t enqueue_windows(const ShardVramPtrs& v, Shard* s, uint32_t v_seg, uin
   // The byte snapshot of the heads at the batch's start into the plan's
   // buffer; the windows then take the two buffers in turn as their start
   // snapshot, writing the current one into the other.
+  // The wrap limit of the window starting at t0 (2^32 - (K+1) v_seg; 0: no
+  // window there).
+  auto wrap_lim = [&](uint32_t t0) -> uint32_t {
+    if (t0 >= T || v_seg == 0) return 0u;
+    const uint32_t K = std::min(win, T - t0);
+    return (uint32_t)((1ull << 32) - (uint64_t)(K + 1) * v_seg);
+  };
   {
     TimedScope ts(kKWinStart, st, timed);
-    win_snapshot_kernel<<<blocks_for((uint64_t)A + 1), kThreads, 0, st>>>(v.axon_heads, A, s->sp.comp8);
+    win_snapshot_kernel<<<blocks_for((uint64_t)A + 1), kThreads, 0, st>>>(v.axon_heads, A, s->sp.comp8, wrap_lim(0),
+                                                                         s->win, s->dirty);
   }
   uint32_t wi = 0;
   for (uint32_t t0 = 0; t0 < T; t0 += win, wi++) {
     const uint32_t K = std::min(win, T - t0);
-    const uint32_t lim = v_seg ? (uint32_t)((1ull << 32) - (uint64_t)(K + 1) * v_seg) : 0u;
     WinState w = s->win;
     w.comp8_w0 = (wi & 1) ? s->win.comp8_alt : s->sp.comp8;
     uint2* cur = (wi & 1) ? s->sp.comp8 : s->win.comp8_alt;
@@ -791,9 +793,23 @@ hipError_t enqueue_windows(const ShardVramPtrs& v, Shard* s, uint32_t v_seg, uin
       }
       {
         TimedScope ts(kKAxon, st, timed);
-        win_axon_kernel<<<blocks_for(A > kShards ? A : kShards), kThreads, 0, st>>>(
-            v.axon_heads, A, v_seg, virt_off, k2 ? n_virt : 0u, s->sc.push_cnt, k3 ? 1 : 0, s->sc.counters, prm, t,
-            words, cur, axon_inv, lut, w, s->dirty, j, lim);
+        const uint32_t nv = k2 ? n_virt : 0u;
+        const int uc = k3 ? 1 : 0;
+        if (axon_inv) {
+          win_axon_kernel<true><<<blocks_for(A ? A : 1, 256 * kAxonsPerLane), kThreads, 0, st>>>(
+              A, v_seg, virt_off, nv, s->sc.push_cnt, uc, s->sc.counters, prm, t, words, cur, axon_inv, w, s->dirty,
+              j);
+          win_active_kernel<true><<<kListBlocks / 4, kThreads, 0, st>>>(v.axon_heads, A, v_seg, virt_off, nv,
+                                                                       s->sc.push_cnt, uc, prm, t, words, cur,
+                                                                       axon_inv, lut, w, s->dirty, j);
+        } else {
+          win_axon_kernel<false><<<blocks_for(A ? A : 1, 256 * kAxonsPerLane), kThreads, 0, st>>>(
+              A, v_seg, virt_off, nv, s->sc.push_cnt, uc, s->sc.counters, prm, t, words, cur, nullptr, w, s->dirty,
+              j);
+          win_active_kernel<false><<<kListBlocks / 4, kThreads, 0, st>>>(v.axon_heads, A, v_seg, virt_off, nv,
+                                                                        s->sc.push_cnt, uc, prm, t, words, cur,
+                                                                        nullptr, lut, w, s->dirty, j);
+        }
       }
       {
         TimedScope ts(kKWinCorrect, st, timed);
@@ -821,6 +837,10 @@ hipError_t enqueue_windows(const ShardVramPtrs& v, Shard* s, uint32_t v_seg, uin
         readout_kernel<<<blocks_for(n_out), kThreads, 0, st>>>(v.soma_flags, N, mapped, nullptr, n_out, prm, t,
                                                                s->sp.perm ? s->d_perm_inv : nullptr);
       }
+    }
+    {
+      TimedScope ts(kKWinEnd, st, timed);
+      win_end_kernel<<<blocks_for(A), kThreads, 0, st>>>(v.axon_heads, A, v_seg, K, wrap_lim(t0 + K), w, s->dirty);
     }
   }
   return hipGetLastError();

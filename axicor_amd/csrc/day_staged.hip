@@ -631,10 +631,36 @@ __global__ __launch_bounds__(256) void input_key_kernel(const uint32_t* __restri
 // working copy: 8 slots from one 16-byte weight load and two 16-byte target
 // loads; the heads from the post-push byte snapshot (comp8), or the compact
 // one (comp) where segment + length > 255. F: the soma's fill (> 8c).
+// Heads inside a window of ticks (day_window.hip): an axon whose record in
+// aux is not current (stamp) was not pushed since the window started, and
+// its u32 heads are still the window's start heads; its current ones are
+// those advanced `steps` ticks. aux null: the heads are current.
+struct LazyHeads {
+  const uint64_t* aux;
+  uint32_t stamp, steps, v_seg;
+};
+__device__ __forceinline__ uint32_t lazy_adv(uint32_t x, uint32_t v, uint32_t n) {
+#pragma unroll
+  for (uint32_t k = 0; k < 8; k++)
+    if (k < n && x != kSentinel) x += v;
+  return x;
+}
+// min_dist_heads on the current heads of axon a.
+__device__ __forceinline__ uint32_t min_dist_lazy(const AxcBurstHeads8* __restrict__ heads, uint32_t A, uint32_t a,
+                                                  uint32_t seg, const LazyHeads& lz) {
+  if (a >= A) return 0xFFFFFFFFu;
+  const Heads8 h = load_heads(heads, a);
+  const uint32_t n = (lz.aux && (uint32_t)lz.aux[a] != lz.stamp) ? lz.steps : 0u;
+  const uint32_t hv[8] = {h.lo.x, h.lo.y, h.lo.z, h.lo.w, h.hi.x, h.hi.y, h.hi.z, h.hi.w};
+  uint32_t m = 0xFFFFFFFFu;
+#pragma unroll
+  for (int x = 0; x < 8; x++) m = min(m, lazy_adv(hv[x], lz.v_seg, n) - seg);
+  return m;
+}
 __device__ __forceinline__ void gsop_chunk(uint32_t N, uint32_t A, const AxcVariantParameters* lut, int32_t dop,
                                            const uint8_t* __restrict__ flags_all, const StagePlan& sp,
                                            const AxcBurstHeads8* __restrict__ heads, uint32_t i, uint32_t c,
-                                           uint32_t F) {
+                                           uint32_t F, const LazyHeads lz = LazyHeads{nullptr, 0, 0, 0}) {
   const uint8_t flags = flags_all[i];
   const AxcVariantParameters& p = lut[(flags >> 4) & 0x0F];
   const GsopTerms g = gsop_terms(flags, p, dop);
@@ -647,8 +673,10 @@ __device__ __forceinline__ void gsop_chunk(uint32_t N, uint32_t A, const AxcVari
   uint32_t w2[4] = {wv.x, wv.y, wv.z, wv.w};
   uint32_t d[8];
 #pragma unroll
-  for (int u = 0; u < 8; u++)
-    d[u] = gsop_dist(heads, A, sp.comp8, (uint32_t)u < nv ? snap_index(t[u], A) : A, t[u] >> 24, g.len);
+  for (int u = 0; u < 8; u++) {
+    const uint32_t a = (uint32_t)u < nv ? snap_index(t[u], A) : A, seg = t[u] >> 24;
+    d[u] = seg + g.len > 255u ? min_dist_lazy(heads, A, a, seg, lz) : min_dist8(sp.comp8[a], seg);
+  }
 #pragma unroll
   for (int u = 0; u < 8; u++) {
     if ((uint32_t)u >= nv) continue;

@@ -26,6 +26,12 @@
 // The window's start heads are its byte snapshot (two buffers taken in turn:
 // a window's ticks write the other one) and, for the axons it pushes, their
 // u32 values saved at the first push.
+// Inside a window only the axons it has pushed (or that start near the u32
+// wrap) keep exact u32 heads tick by tick; the heads of every other axon stay
+// at the window's start, and their current byte form is the start snapshot
+// advanced (j+1) v_seg. win_end_kernel advances those K ticks at the
+// window's end, so between windows (and between API calls) all heads are the
+// reference's.
 // So soma i's input at tick j is the sum over its slots of w_j * hit_j
 // exactly: predicted hits times the start weights, plus the GSOP changes on
 // predicted hits, plus the corrections (current weights) on the slots of
@@ -52,6 +58,43 @@ __device__ __forceinline__ void win_unpack(int64_t V, int32_t& sum, uint32_t& cn
   sum = (int32_t)(V - (c << 32));
 }
 
+// Heads advanced n <= kWin ticks by K4 (physics.cu:137-154): a live head
+// moves v_seg per tick and stays where it meets the sentinel.
+__device__ __forceinline__ uint32_t adv_head(uint32_t x, uint32_t v, uint32_t n) {
+#pragma unroll
+  for (uint32_t k = 0; k < kWin; k++)
+    if (k < n && x != kSentinel) x += v;
+  return x;
+}
+__device__ __forceinline__ Heads8 advance_heads(const Heads8& h, uint32_t v, uint32_t n) {
+  Heads8 r;
+  r.lo = make_uint4(adv_head(h.lo.x, v, n), adv_head(h.lo.y, v, n), adv_head(h.lo.z, v, n), adv_head(h.lo.w, v, n));
+  r.hi = make_uint4(adv_head(h.hi.x, v, n), adv_head(h.hi.y, v, n), adv_head(h.hi.z, v, n), adv_head(h.hi.w, v, n));
+  return r;
+}
+// The byte form of heads advanced d segments, from their byte form: a byte
+// below 255 is the head itself, far below the sentinel and the wrap, so it
+// becomes min(b + d, 255); 255 stays.
+__device__ __forceinline__ uint2 advance_bytes(const uint2 e, uint32_t d) {
+  if (d >= 255u) return make_uint2(0xFFFFFFFFu, 0xFFFFFFFFu);
+  const axc_us2 D = {(unsigned short)d, (unsigned short)d}, M = {255, 255};
+  const uint32_t p[4] = {__builtin_amdgcn_perm(0u, e.x, 0x0c010c00u), __builtin_amdgcn_perm(0u, e.x, 0x0c030c02u),
+                         __builtin_amdgcn_perm(0u, e.y, 0x0c010c00u), __builtin_amdgcn_perm(0u, e.y, 0x0c030c02u)};
+  uint32_t q[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++)
+    q[k] = __builtin_bit_cast(uint32_t, __builtin_elementwise_min(__builtin_bit_cast(axc_us2, p[k]) + D, M));
+  return make_uint2(__builtin_amdgcn_perm(q[1], q[0], 0x06040200u), __builtin_amdgcn_perm(q[3], q[2], 0x06040200u));
+}
+// Some live head at or above lim: the axon wraps past 2^32 within the window.
+__device__ __forceinline__ bool near_wrap(const Heads8& h, uint32_t lim) {
+  const uint32_t hv[8] = {h.lo.x, h.lo.y, h.lo.z, h.lo.w, h.hi.x, h.hi.y, h.hi.z, h.hi.w};
+  bool r = false;
+#pragma unroll
+  for (int x = 0; x < 8; x++) r |= hv[x] != kSentinel && hv[x] >= lim;
+  return r;
+}
+
 // The predicted hits of a slot (axon a in the snapshot's ids, segment seg,
 // length L) at ticks 0..K-1 of the window, as bits. The u32 rule reads the
 // start heads from hb, which holds them advanced `back` segments already
@@ -75,17 +118,70 @@ __device__ __forceinline__ uint32_t win_pred_mask(const WinState& w, const AxcBu
   return m;
 }
 
-// The byte snapshot of the heads (at a batch's start, for its first window).
-__global__ __launch_bounds__(256) void win_snapshot_kernel(const AxcBurstHeads8* __restrict__ heads, uint32_t A,
-                                                           uint2* __restrict__ comp8) {
-  const uint32_t a = blockIdx.x * 256 + threadIdx.x;
-  if (a < A) comp8[a] = byte_heads(load_heads(heads, a));
-  if (a == A) comp8[A] = make_uint2(0xFFFFFFFFu, 0xFFFFFFFFu);
-}
-
 __device__ __forceinline__ uint64_t aux_pack(uint32_t stamp, uint32_t m, bool hz, uint32_t dmin, uint32_t dmax) {
   return (uint64_t)stamp | ((uint64_t)m << 32) | ((uint64_t)(hz ? 1u : 0u) << 36) | ((uint64_t)dmin << 37) |
          ((uint64_t)dmax << 50);
+}
+// An axon whose start heads come near the u32 wrap in the next window (the
+// window of stamp ctl[1] + 1, whose window pass takes that stamp): its start
+// heads saved and its record current for that window, with no pushes and the
+// wrap bit, so that its heads stay exact and every slot of it is corrected.
+// Lists axon a in the active list of the window with stamp st (once per
+// axon and window: where its record turns current). A lane at a time.
+__device__ __forceinline__ void act_append(const DirtyMarks& dm, uint32_t A, uint32_t st, uint32_t a) {
+  const uint32_t p = st & 1u;
+  dm.act[(size_t)p * A + atomicAdd(&dm.ctl[2 + p], 1u)] = a;
+}
+// The same for the lanes of a whole wave that want it: one atomic per wave.
+__device__ __forceinline__ void act_append_wave(const DirtyMarks& dm, uint32_t A, uint32_t st, bool want,
+                                                uint32_t a) {
+  const uint64_t bal = __ballot(want);
+  if (bal == 0) return;
+  const uint32_t lane = threadIdx.x & 63u, lead = (uint32_t)__builtin_ctzll(bal), p = st & 1u;
+  uint32_t base = 0;
+  if (lane == lead) base = atomicAdd(&dm.ctl[2 + p], (uint32_t)__popcll(bal));
+  base = __shfl(base, (int)lead, 64);
+  if (want) dm.act[(size_t)p * A + base + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull))] = a;
+}
+__device__ __forceinline__ void mark_near_wrap(const Heads8& h, uint32_t a, uint32_t A, uint32_t lim,
+                                               const WinState& w, const DirtyMarks& dm) {
+  if (lim != 0 && near_wrap(h, lim)) {
+    store_heads(w.heads_w0, a, h);
+    dm.aux[a] = aux_pack(dm.ctl[1] + 1, 0, true, kNoDrop, 0);
+    act_append(dm, A, dm.ctl[1] + 1, a);
+  }
+}
+
+// The byte snapshot of the heads at a batch's start, for its first window
+// (lim: that window's wrap limit, 2^32 - (K+1) v_seg; 0: none).
+__global__ __launch_bounds__(256) void win_snapshot_kernel(const AxcBurstHeads8* __restrict__ heads, uint32_t A,
+                                                           uint2* __restrict__ comp8, uint32_t lim, WinState w,
+                                                           DirtyMarks dm) {
+  const uint32_t a = blockIdx.x * 256 + threadIdx.x;
+  if (a < A) {
+    const Heads8 h = load_heads(heads, a);
+    comp8[a] = byte_heads(h);
+    mark_near_wrap(h, a, A, lim, w, dm);
+  }
+  if (a == A) comp8[A] = make_uint2(0xFFFFFFFFu, 0xFFFFFFFFu);
+}
+
+// The end of a window of K ticks: the heads of the axons it did not push
+// advanced the window's K ticks (their u32 heads stayed at its start), so all
+// heads are the reference's again; and the next window's near-wrap axons
+// marked (lim_next: its limit; 0: no next window in this batch).
+__global__ __launch_bounds__(256) void win_end_kernel(AxcBurstHeads8* __restrict__ heads, uint32_t A, uint32_t v_seg,
+                                                      uint32_t K, uint32_t lim_next, WinState w, DirtyMarks dm) {
+  const uint32_t a = blockIdx.x * 256 + threadIdx.x;
+  if (a >= A) return;
+  const bool mine = (uint32_t)dm.aux[a] == dm.ctl[1];
+  if (mine && lim_next == 0) return;
+  Heads8 h = load_heads(heads, a);
+  if (!mine) {
+    h = advance_heads(h, v_seg, K);
+    store_heads(heads, a, h);
+  }
+  mark_near_wrap(h, a, A, lim_next, w, dm);
 }
 // Heads shifted out by a push of k: the last min(k, 8) of h; their values at
 // the window's start (h advanced `back` past it) go into [dmin, dmax].
@@ -102,16 +198,6 @@ __device__ __forceinline__ void note_dropped(const Heads8& h, uint32_t k, uint32
     }
   }
 }
-// The window's start heads from heads advanced `back` past them (live heads;
-// a sentinel stays one).
-__device__ __forceinline__ Heads8 start_heads(const Heads8& h, uint32_t back) {
-  auto f = [back](uint32_t x) { return x != kSentinel ? x - back : x; };
-  Heads8 r;
-  r.lo = make_uint4(f(h.lo.x), f(h.lo.y), f(h.lo.z), f(h.lo.w));
-  r.hi = make_uint4(f(h.hi.x), f(h.hi.y), f(h.hi.z), f(h.hi.w));
-  return r;
-}
-
 // The segment of a reverse entry, and the first index in [r, b) whose
 // entry's segment is >= key (an axon's list is sorted by segment).
 __device__ __forceinline__ uint32_t rev_seg(const uint64_t e) { return (uint32_t)(e >> 40) & 0xFFu; }
@@ -124,125 +210,206 @@ __device__ __forceinline__ uint32_t rev_lower(const uint64_t* __restrict__ rev, 
   return r;
 }
 
-// K2/K3 + K4 of tick j of the window (axon_tick_kernel), then the tick's
-// correction pieces. Between pushes an axon's heads only advance, and a
-// push of k shifts them by k and puts k new heads in front, so after m
-// pushes (at most 8) since the window started the current heads are m new
-// heads followed by the first 8 - m start heads advanced j+1 times (the
-// prediction P). The hit test is an OR over the heads, so a slot's
-// prediction and its actual test can differ only if one of the m new heads
-// or of the start heads shifted out (advanced) lies in [s, s + L): only
-// slots with a segment in (x - Lmax, x] for such a value x are tested. New
-// heads are at most (j+1) v_seg: a prefix of the axon's segment-sorted
-// reverse list (its ends for segments up to 7 are stored with it). A value
-// >= 255 + Lmax hits no slot. Axons starting with a head at or above
-// hazard_lim = 2^32 - (K+1) v_seg (found at the window's first tick; 0: no
-// test) wrap within the window, where their byte prediction does not follow
-// P: every slot of theirs is tested. comp8: this window's current snapshot.
-__global__ __launch_bounds__(256) void win_axon_kernel(AxcBurstHeads8* __restrict__ heads, uint32_t A, uint32_t v_seg,
-                                                       uint32_t virt_off, uint32_t n_virt,
-                                                       uint32_t* __restrict__ push_cnt, int use_cnt,
+// K2/K3 + K4 of tick j of the window (axon_tick_kernel), in two kernels.
+// An axon that nothing has pushed since the window started (and that does
+// not start near the wrap) keeps its start heads in `heads`; win_axon_kernel
+// writes its current byte form from the start snapshot, and lists it as
+// active where this tick pushes it. win_active_kernel then steps the listed
+// axons, whose heads are exact (at its first push an axon's heads are its
+// start heads, saved for the window, advanced j ticks), and makes the tick's
+// correction pieces.
+// Between pushes an axon's heads only advance, and a push of k shifts them
+// by k and puts k new heads in front, so after m pushes (at most 8) since
+// the window started the current heads are m new heads followed by the first
+// 8 - m start heads advanced j+1 times (the prediction P). The hit test is an
+// OR over the heads, so a slot's prediction and its actual test can differ
+// only if one of the m new heads or of the start heads shifted out
+// (advanced) lies in [s, s + L): only slots with a segment in (x - Lmax, x]
+// for such a value x are tested. New heads are at most (j+1) v_seg: a prefix
+// of the axon's segment-sorted reverse list (its ends for segments up to 7
+// are stored with it). A value >= 255 + Lmax hits no slot. Axons marked near
+// the wrap (mark_near_wrap) have every slot tested. comp8: this window's
+// current snapshot.
+
+// This tick's pushes of axon a: its K2 input bit and its K3 count. PERM:
+// the caller's id of the axon comes from axon_inv (a separate instantiation,
+// so that without it the bitmask load does not wait for the loads before it).
+template <bool PERM>
+__device__ __forceinline__ uint32_t tick_pushes(uint32_t a, const uint32_t* __restrict__ bitmask, uint32_t virt_off,
+                                                uint32_t n_virt, const uint32_t* __restrict__ push_cnt, int use_cnt,
+                                                const uint32_t* __restrict__ axon_inv) {
+  uint32_t k = 0;
+  if (bitmask) {
+    const uint32_t i = (PERM ? axon_inv[a] : a) - virt_off;  // wraps for ids below virt_off
+    if (i < n_virt) k += (*(const __attribute__((address_space(1))) uint32_t*)(bitmask + (i >> 5)) >> (i & 31)) & 1u;
+  }
+  if (use_cnt) k += push_cnt[a];
+  return k;
+}
+
+constexpr uint32_t kAxonsPerLane = 4;  // win_axon_kernel: a workgroup takes 1,024 axons
+template <bool PERM>
+__global__ __launch_bounds__(256) void win_axon_kernel(uint32_t A, uint32_t v_seg, uint32_t virt_off, uint32_t n_virt,
+                                                       const uint32_t* __restrict__ push_cnt, int use_cnt,
                                                        uint32_t* __restrict__ counters,
                                                        const TickParams* __restrict__ prm, uint32_t t_off,
                                                        uint32_t words, uint2* __restrict__ comp8,
-                                                       const uint32_t* __restrict__ axon_inv,
-                                                       const AxcVariantParameters* __restrict__ lut_g, WinState w,
-                                                       DirtyMarks dm, uint32_t j, uint32_t hazard_lim) {
-  const uint32_t a = blockIdx.x * blockDim.x + threadIdx.x, lane = threadIdx.x & 63u;
-  const uint32_t* __restrict__ bitmask = prm->bitmask ? prm->bitmask + (size_t)t_off * words : nullptr;
-  if (a < kNumCounters) counters[a] = 0;
-  if (a < kShards) dm.shard_cnt[((j + 1) & 1) * kShards + a] = 0;  // the next tick's (consumed last tick)
+                                                       const uint32_t* __restrict__ axon_inv, WinState w,
+                                                       DirtyMarks dm, uint32_t j) {
+  const uint32_t tid = threadIdx.x;
+  const uint32_t* __restrict__ bitmask = (prm->bitmask && n_virt) ? prm->bitmask + (size_t)t_off * words : nullptr;
+  const uint32_t a0 = blockIdx.x * (256 * kAxonsPerLane) + tid;
+  // All of the lane's loads are issued before any of them is used, so that
+  // they make one round trip (a load under a branch followed by its use
+  // would wait for every load before it). An input bit outside the bitmask
+  // reads word 0 and is dropped.
+  uint32_t rec[kAxonsPerLane], bw[kAxonsPerLane], pc[kAxonsPerLane], bi[kAxonsPerLane];
+  uint2 e[kAxonsPerLane];
+#pragma unroll
+  for (uint32_t r = 0; r < kAxonsPerLane; r++) {
+    const uint32_t a = min(a0 + 256 * r, A ? A - 1 : 0u);
+    rec[r] = reinterpret_cast<const uint32_t*>(dm.aux)[2 * (size_t)a];
+    e[r] = w.comp8_w0[a];
+    bi[r] = (PERM ? axon_inv[a] : a) - virt_off;  // wraps for ids below virt_off
+    // (a global load: as a generic one it would also count as an LDS
+    // access, and the kernel's next wait on those would take it along)
+    bw[r] = bitmask ? *(const __attribute__((address_space(1))) uint32_t*)(bitmask +
+                                                                           (bi[r] < n_virt ? bi[r] >> 5 : 0u))
+                    : 0u;
+    pc[r] = use_cnt ? push_cnt[a] : 0u;
+  }
   const uint32_t stamp = dm.ctl[1];
-  Heads8 cur{};
-  uint32_t m = 0, dmin = kNoDrop, dmax = 0;
-  bool hz = false;
-  if (a < A) {
-    const uint64_t ax = dm.aux[a];
-    const bool mine = (uint32_t)ax == stamp;
-    if (mine) {
-      m = (uint32_t)(ax >> 32) & 0xFu;
-      hz = (ax >> 36) & 1u;
-      dmin = (uint32_t)(ax >> 37) & 0x1FFFu;
-      dmax = (uint32_t)(ax >> 50) & 0x1FFFu;
-    }
-    Heads8 pre;
-    const uint32_t k = axon_step(heads, a, v_seg, n_virt ? bitmask : nullptr, virt_off, n_virt, push_cnt,
-                                 use_cnt != 0, nullptr, comp8, axon_inv, &cur, &pre);
-    bool hz_new = false;
-    if (j == 0 && hazard_lim != 0) {
-      const uint32_t hv[8] = {pre.lo.x, pre.lo.y, pre.lo.z, pre.lo.w, pre.hi.x, pre.hi.y, pre.hi.z, pre.hi.w};
 #pragma unroll
-      for (int x = 0; x < 8; x++) hz_new |= hv[x] != kSentinel && hv[x] >= hazard_lim;
-    }
-    if (k || hz_new) {
-      // The heads before this tick are the start heads advanced j times.
-      if (!mine) store_heads(w.heads_w0, a, start_heads(pre, j * v_seg));
-      if (k) {
-        note_dropped(pre, k, j * v_seg, dmin, dmax);
-        m = min(8u, m + k);
-      }
-      hz |= hz_new;
-      dm.aux[a] = aux_pack(stamp, m, hz, dmin, dmax);
-    }
+  for (uint32_t r = 0; r < kAxonsPerLane; r++) {
+    const uint32_t a = a0 + 256 * r;
+    const uint32_t k = ((bi[r] < n_virt) ? (bw[r] >> (bi[r] & 31)) & 1u : 0u) + pc[r];
+    const bool idle = a < A && rec[r] != stamp;  // not pushed since the window started
+    if (idle && k == 0) comp8[a] = advance_bytes(e[r], (j + 1) * v_seg);
+    // Pushed now for the first time: win_active_kernel steps it from now on.
+    act_append_wave(dm, A, stamp, idle && k != 0, a);
   }
-  uint32_t rs = 0, cnt = 0;
-  if (m > 0 || hz) {
-    uint32_t Lmax = 0;
+  if (blockIdx.x == 0) {
+    if (tid < kNumCounters) counters[tid] = 0;
+    if (tid == 0) dm.piece_cnt[(j + 1) & 1] = 0;  // the next tick's (consumed last tick)
+  }
+}
+
+template <bool PERM>
+__global__ __launch_bounds__(256) void win_active_kernel(AxcBurstHeads8* __restrict__ heads, uint32_t A,
+                                                         uint32_t v_seg, uint32_t virt_off, uint32_t n_virt,
+                                                         uint32_t* __restrict__ push_cnt, int use_cnt,
+                                                         const TickParams* __restrict__ prm, uint32_t t_off,
+                                                         uint32_t words, uint2* __restrict__ comp8,
+                                                         const uint32_t* __restrict__ axon_inv,
+                                                         const AxcVariantParameters* __restrict__ lut_g, WinState w,
+                                                         DirtyMarks dm, uint32_t j) {
+  __shared__ uint32_t wsum[4], wbase;
+  const uint32_t tid = threadIdx.x, lane = tid & 63u, wv = tid >> 6;
+  const uint32_t* __restrict__ bitmask = (prm->bitmask && n_virt) ? prm->bitmask + (size_t)t_off * words : nullptr;
+  const uint32_t stamp = dm.ctl[1];
+  const uint32_t n_act = dm.ctl[2 + (stamp & 1u)];
+  const uint32_t* __restrict__ act = dm.act + (size_t)(stamp & 1u) * A;
+  uint32_t Lmax = 0;
 #pragma unroll
-    for (int k = 0; k < 16; k++) Lmax = max(Lmax, (uint32_t)lut_g[k].signal_propagation_length);
-    uint32_t lo = 256, hi = 0;  // the segments to test, [lo, hi]
-    auto take = [&](uint32_t xl, uint32_t xh) {  // values in [xl, xh]
-      if (xl < 255u + Lmax) {
-        lo = min(lo, xl >= Lmax ? xl - Lmax + 1 : 0u);
-        hi = max(hi, min(xh, 255u));
-      }
-    };
-    if (hz) {
-      lo = 0;
-      hi = 255;
-    } else {
-      const uint32_t c[8] = {cur.lo.x, cur.lo.y, cur.lo.z, cur.lo.w, cur.hi.x, cur.hi.y, cur.hi.z, cur.hi.w};
-#pragma unroll
-      for (uint32_t k = 0; k < 8; k++)
-        if (k < m) take(c[k], c[k]);
-      if (dmin <= dmax) {
-        const uint32_t adv = (j + 1) * v_seg;
-        take(dmin + adv, dmax + adv);
-      }
-    }
-    if (lo <= hi) {
+  for (int k = 0; k < 16; k++) Lmax = max(Lmax, (uint32_t)lut_g[k].signal_propagation_length);
+  // Whole workgroups run every round: the round's pieces are reserved in the
+  // pool with one atomic per workgroup.
+  for (uint32_t q0 = blockIdx.x * 256; q0 < n_act; q0 += gridDim.x * 256) {
+    const uint32_t q = q0 + tid;
+    uint32_t a = 0, rs = 0, cnt = 0;
+    if (q < n_act) {
+      a = act[q];
+      // Everything indexed by the axon is loaded up front (one round trip).
+      const uint64_t ax = dm.aux[a];
+      Heads8 pre = load_heads(heads, a);
       const uint32_t r0 = w.rev_off[a], r1 = w.rev_off[a + 1];
-      rs = lo > 0 ? rev_lower(w.rev, r0, r1, lo) : r0;
-      uint32_t re = 0xFFFFFFFFu;
-      if (hi < kRevCuts) {
-        const uint4 cv = w.rev_cut[a];
-        const uint32_t wd = hi < 2 ? cv.x : hi < 4 ? cv.y : hi < 6 ? cv.z : cv.w;
-        const uint32_t off = (wd >> (16 * (hi & 1))) & 0xFFFFu;
-        if (off != 0xFFFFu) re = r0 + off;
+      const uint4 cv = w.rev_cut[a];
+      const uint32_t k = tick_pushes<PERM>(a, bitmask, virt_off, n_virt, push_cnt, use_cnt, axon_inv);
+      const bool mine = (uint32_t)ax == stamp;
+      uint32_t m = 0, dmin = kNoDrop, dmax = 0;
+      bool hz = false;
+      if (mine) {
+        m = (uint32_t)(ax >> 32) & 0xFu;
+        hz = (ax >> 36) & 1u;
+        dmin = (uint32_t)(ax >> 37) & 0x1FFFu;
+        dmax = (uint32_t)(ax >> 50) & 0x1FFFu;
       }
-      if (re == 0xFFFFFFFFu) re = rev_lower(w.rev, rs, r1, hi + 1);
-      cnt = re > rs ? re - rs : 0u;
-    }
-  }
-  // The wave's pieces of up to 16 entries: one atomic on its part's counter
-  // reserves them (the wave's axons are one group of 64).
-  const uint32_t np = (cnt + 15) >> 4;
-  uint32_t x = np;  // inclusive prefix over the wave
+      if (use_cnt) push_cnt[a] = 0;
+      if (!mine) {  // listed by this tick's win_axon_kernel: pre holds the start heads
+        store_heads(w.heads_w0, a, pre);
+        pre = advance_heads(pre, v_seg, j);
+      }
+      Heads8 h = pre;
+      push_heads(h, k > 8 ? 8 : k);
+      h = advance_heads(h, v_seg, 1);
+      store_heads(heads, a, h);
+      comp8[a] = byte_heads(h);
+      if (k || !mine) {
+        if (k) {
+          note_dropped(pre, k, j * v_seg, dmin, dmax);
+          m = min(8u, m + k);
+        }
+        dm.aux[a] = aux_pack(stamp, m, hz, dmin, dmax);
+      }
+      uint32_t lo = 256, hi = 0;  // the segments to test, [lo, hi]
+      auto take = [&](uint32_t xl, uint32_t xh) {  // values in [xl, xh]
+        if (xl < 255u + Lmax) {
+          lo = min(lo, xl >= Lmax ? xl - Lmax + 1 : 0u);
+          hi = max(hi, min(xh, 255u));
+        }
+      };
+      if (hz) {
+        lo = 0;
+        hi = 255;
+      } else {
+        const uint32_t c[8] = {h.lo.x, h.lo.y, h.lo.z, h.lo.w, h.hi.x, h.hi.y, h.hi.z, h.hi.w};
 #pragma unroll
-  for (uint32_t off = 1; off < 64; off <<= 1) {
-    const uint32_t y = __shfl_up(x, off, 64);
-    if (lane >= off) x += y;
-  }
-  const uint32_t total = __shfl(x, 63, 64);
-  if (total) {  // wave-uniform
-    const uint32_t sh = (a >> 6) % kShards;
-    uint32_t base = 0;
-    if (lane == 63) base = atomicAdd(&dm.shard_cnt[(j & 1) * kShards + sh], total);
-    base = dm.shard_off[sh] + __shfl(base, 63, 64) + x - np;
+        for (uint32_t x = 0; x < 8; x++)
+          if (x < m) take(c[x], c[x]);
+        if (dmin <= dmax) {
+          const uint32_t adv = (j + 1) * v_seg;
+          take(dmin + adv, dmax + adv);
+        }
+      }
+      if (lo <= hi) {
+        rs = lo > 0 ? rev_lower(w.rev, r0, r1, lo) : r0;
+        uint32_t re = 0xFFFFFFFFu;
+        if (hi < kRevCuts) {
+          const uint32_t wd = hi < 2 ? cv.x : hi < 4 ? cv.y : hi < 6 ? cv.z : cv.w;
+          const uint32_t off = (wd >> (16 * (hi & 1))) & 0xFFFFu;
+          if (off != 0xFFFFu) re = r0 + off;
+        }
+        if (re == 0xFFFFFFFFu) re = rev_lower(w.rev, rs, r1, hi + 1);
+        cnt = re > rs ? re - rs : 0u;
+      }
+    }
+    // The axon's pieces of up to 16 entries: an inclusive prefix over the
+    // wave, the waves' totals added in LDS, one atomic for the workgroup.
+    const uint32_t np = (cnt + 15) >> 4;
+    uint32_t x = np;
+#pragma unroll
+    for (uint32_t off = 1; off < 64; off <<= 1) {
+      const uint32_t y = __shfl_up(x, off, 64);
+      if (lane >= off) x += y;
+    }
+    if (lane == 63) wsum[wv] = x;
+    __syncthreads();
+    if (tid == 0) {
+      const uint32_t t0 = wsum[0], t1 = wsum[1], t2 = wsum[2], t3 = wsum[3];
+      const uint32_t tot = t0 + t1 + t2 + t3;
+      wbase = tot ? atomicAdd(&dm.piece_cnt[j & 1], tot) : 0u;
+      wsum[0] = 0;
+      wsum[1] = t0;
+      wsum[2] = t0 + t1;
+      wsum[3] = t0 + t1 + t2;
+    }
+    __syncthreads();
+    const uint32_t base = wbase + wsum[wv] + x - np;
     for (uint32_t p = 0; p < np; p++) {
       const uint32_t c = min(16u, cnt - 16 * p);
       dm.pieces[base + p] = (uint64_t)(rs + 16 * p) | ((uint64_t)a << 32) | ((uint64_t)(c - 1) << 56);
     }
+    __syncthreads();  // wsum and wbase are reused by the next round
   }
 }
 
@@ -254,13 +421,26 @@ __global__ __launch_bounds__(256) void win_push_kernel(AxcBurstHeads8* __restric
                                                        DirtyMarks dm, uint32_t v_seg, uint32_t j) {
   const uint32_t n = sc.counters[kCntSpikes];
   const uint32_t stamp = dm.ctl[1];
-  for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
-    const uint32_t a = soma_to_axon[sc.spk_list[q]];
-    if (a == kNoAxon || a >= A) continue;
-    const uint32_t c = atomicExch(&sc.push_cnt[a], 0u);
-    if (c == 0) continue;
-    const uint64_t ax = dm.aux[a];
+  // Whole waves run every round (the appends to the active list take one
+  // atomic per wave).
+  for (uint32_t q0 = blockIdx.x * blockDim.x; q0 < n; q0 += gridDim.x * blockDim.x) {
+    const uint32_t q = q0 + threadIdx.x;
+    uint32_t a = kNoAxon, c = 0;
+    uint64_t ax = 0;
+    Heads8 h{};
+    if (q < n) {
+      a = soma_to_axon[sc.spk_list[q]];
+      if (a != kNoAxon && a < A) {
+        // Only the lane that takes the count writes this axon's record and
+        // heads, so they can be loaded beside the exchange.
+        ax = dm.aux[a];
+        h = load_heads(heads, a);
+        c = atomicExch(&sc.push_cnt[a], 0u);
+      }
+    }
     const bool mine = (uint32_t)ax == stamp;
+    act_append_wave(dm, A, stamp, c != 0 && !mine, a);
+    if (c == 0) continue;
     uint32_t m = 0, dmin = kNoDrop, dmax = 0;
     bool hz = false;
     if (mine) {
@@ -269,9 +449,12 @@ __global__ __launch_bounds__(256) void win_push_kernel(AxcBurstHeads8* __restric
       dmin = (uint32_t)(ax >> 37) & 0x1FFFu;
       dmax = (uint32_t)(ax >> 50) & 0x1FFFu;
     }
-    Heads8 h = load_heads(heads, a);
-    // These heads are the start heads advanced j+1 times.
-    if (!mine) store_heads(w.heads_w0, a, start_heads(h, (j + 1) * v_seg));
+    // Unpushed until now: h holds the start heads; the current ones are
+    // those advanced j+1 ticks.
+    if (!mine) {
+      store_heads(w.heads_w0, a, h);
+      h = advance_heads(h, v_seg, j + 1);
+    }
     note_dropped(h, c, (j + 1) * v_seg, dmin, dmax);
     push_heads(h, c > 8 ? 8 : c);
     store_heads(heads, a, h);
@@ -329,25 +512,21 @@ __device__ __forceinline__ void win_chunk(const Chunk<STAGED, true>& now, uint32
     }
   }
   constexpr int H = 2;  // slots in pairs (halves of four spill at eight waves per SIMD)
+  // The pairs' keep masks: zero lanes for exact-path slots (their weight
+  // and count are added after the shifted-segment tests).
+  uint32_t keepw[U / H];
+#pragma unroll
+  for (int p = 0; p < U / H; p++) keepw[p] = 0xFFFFFFFFu;
+  if (slow_m) {
+#pragma unroll
+    for (int p = 0; p < U / H; p++) {
+      const uint32_t sm2 = (slow_m >> (H * p)) & 3u;
+      keepw[p] = ((sm2 & 1u) ? 0u : 0x0000FFFFu) | ((sm2 & 2u) ? 0u : 0xFFFF0000u);
+    }
+  }
 #pragma unroll
   for (int h0 = 0; h0 < U; h0 += H) {
-    // The pair's entries spread to u16 lanes once: q[u][r] holds heads 2r,
-    // 2r+1 of slot h0+u; S[u] the slot's segment in both halves.
-    uint32_t q[H][4], S[H];
-#pragma unroll
-    for (int u = 0; u < H; u++) {
-      const uint2 e = STAGED ? ent[now.entry(h0 + u, A)] : w.comp8_w0[now.entry(h0 + u, A)];
-      q[u][0] = __builtin_amdgcn_perm(0u, e.x, 0x0c010c00u);
-      q[u][1] = __builtin_amdgcn_perm(0u, e.x, 0x0c030c02u);
-      q[u][2] = __builtin_amdgcn_perm(0u, e.y, 0x0c010c00u);
-      q[u][3] = __builtin_amdgcn_perm(0u, e.y, 0x0c030c02u);
-      const int v = h0 + u;
-      if (STAGED) S[u] = __builtin_amdgcn_perm(0u, now.k[4 + (v >> 2)], 0x0c000c00u | ((v & 3) * 0x00010001u));
-      else S[u] = __builtin_amdgcn_perm(0u, now.k[v], 0x0c030c03u);
-    }
-    // The pair's weights and count lanes of 1, zero for exact-path slots.
-    const uint32_t sm2 = (slow_m >> h0) & 3u;
-    const uint32_t keep = ((sm2 & 1u) ? 0u : 0x0000FFFFu) | ((sm2 & 2u) ? 0u : 0xFFFF0000u);
+    const uint32_t keep = keepw[h0 / H];
     const uint32_t wz = now.w2[h0 >> 1] & keep, cm = 0x00010001u & keep, cm8 = 0x01000100u & keep;
     if constexpr (ONCE) {
       // One pass over the heads for all K ticks. D = head - segment + v_seg
@@ -358,21 +537,25 @@ __device__ __forceinline__ void win_chunk(const Chunk<STAGED, true>& now, uint32
       // greatest one the nearest head behind it (hit at tick j if
       // D + j v_seg wraps, that is D > 0xFFFF - j v_seg, as long as
       // j v_seg <= L: the caller takes this path only when (K-1) v_seg <= L).
+      // The pair interleaved: R[k] holds head k of both slots (low lane: slot
+      // h0), S2 their segments, so min and max need no cross-lane step.
+      const uint2 e0 = STAGED ? ent[now.entry(h0, A)] : w.comp8_w0[now.entry(h0, A)];
+      const uint2 e1 = STAGED ? ent[now.entry(h0 + 1, A)] : w.comp8_w0[now.entry(h0 + 1, A)];
+      uint32_t S2;
+      if (STAGED) S2 = __builtin_amdgcn_perm(0u, now.k[4 + (h0 >> 2)], 0x0c000c00u | (h0 & 3) | ((h0 & 3) + 1) << 16);
+      else S2 = __builtin_amdgcn_perm(now.k[h0 + 1], now.k[h0], 0x0c070c03u);
       const uint32_t v1 = v_seg & 0xFFFFu;
       const axc_us2 V = {(unsigned short)v1, (unsigned short)v1};
-      uint32_t lo2[2], hi2[2];
+      const axc_us2 sv = __builtin_bit_cast(axc_us2, S2) - V;
+      axc_us2 MN, MX;
 #pragma unroll
-      for (int x = 0; x < 2; x++) {
-        const axc_us2 s = __builtin_bit_cast(axc_us2, S[x]) - V;
-        const axc_us2 d0 = __builtin_bit_cast(axc_us2, q[x][0]) - s, d1 = __builtin_bit_cast(axc_us2, q[x][1]) - s;
-        const axc_us2 d2 = __builtin_bit_cast(axc_us2, q[x][2]) - s, d3 = __builtin_bit_cast(axc_us2, q[x][3]) - s;
-        lo2[x] = __builtin_bit_cast(uint32_t, __builtin_elementwise_min(__builtin_elementwise_min(d0, d1),
-                                                                        __builtin_elementwise_min(d2, d3)));
-        hi2[x] = __builtin_bit_cast(uint32_t, __builtin_elementwise_max(__builtin_elementwise_max(d0, d1),
-                                                                        __builtin_elementwise_max(d2, d3)));
+      for (int k = 0; k < 8; k++) {
+        const uint32_t sel = 0x0c000c00u | (uint32_t)(k & 3) | ((uint32_t)(4 + (k & 3)) << 16);
+        const uint32_t r = k < 4 ? __builtin_amdgcn_perm(e1.x, e0.x, sel) : __builtin_amdgcn_perm(e1.y, e0.y, sel);
+        const axc_us2 d = __builtin_bit_cast(axc_us2, r) - sv;
+        MN = k ? __builtin_elementwise_min(MN, d) : d;
+        MX = k ? __builtin_elementwise_max(MX, d) : d;
       }
-      const axc_us2 MN = __builtin_bit_cast(axc_us2, pair_min(lo2[0], lo2[1]));
-      const axc_us2 MX = __builtin_bit_cast(axc_us2, pair_max(hi2[0], hi2[1]));
 #pragma unroll
       for (uint32_t j = 0; j < K; j++) {
         // plen[j]: max(L - j v_seg, 0) in both lanes.
@@ -388,6 +571,21 @@ __device__ __forceinline__ void win_chunk(const Chunk<STAGED, true>& now, uint32
                                               hits2[j / 2], true);
       }
       continue;
+    }
+    // The per-tick tests (short propagation lengths). The pair's entries
+    // spread to u16 lanes once: q[u][r] holds heads 2r, 2r+1 of slot h0+u;
+    // S[u] the slot's segment in both halves.
+    uint32_t q[H][4], S[H];
+#pragma unroll
+    for (int u = 0; u < H; u++) {
+      const uint2 e = STAGED ? ent[now.entry(h0 + u, A)] : w.comp8_w0[now.entry(h0 + u, A)];
+      q[u][0] = __builtin_amdgcn_perm(0u, e.x, 0x0c010c00u);
+      q[u][1] = __builtin_amdgcn_perm(0u, e.x, 0x0c030c02u);
+      q[u][2] = __builtin_amdgcn_perm(0u, e.y, 0x0c010c00u);
+      q[u][3] = __builtin_amdgcn_perm(0u, e.y, 0x0c030c02u);
+      const int v = h0 + u;
+      if (STAGED) S[u] = __builtin_amdgcn_perm(0u, now.k[4 + (v >> 2)], 0x0c000c00u | ((v & 3) * 0x00010001u));
+      else S[u] = __builtin_amdgcn_perm(0u, now.k[v], 0x0c030c03u);
     }
 #pragma unroll
     for (uint32_t j = 0; j < K; j++) {
@@ -442,10 +640,12 @@ void win_pass_kernel(ShardVramPtrs vram, uint32_t N, uint32_t A, const AxcVarian
                      StagePlan sp, WinState w, DirtyMarks dm, uint32_t v_seg, uint32_t one2) {
   if (blockIdx.x == 0) {
     if (threadIdx.x == 0) {
-      dm.ctl[1] += 1;
+      const uint32_t st = dm.ctl[1] + 1;
+      dm.ctl[1] = st;
+      dm.ctl[2 + ((st + 1) & 1u)] = 0;  // the next window's list (its marks come at this window's end)
       w.cnt[0] += 1;
     }
-    if (threadIdx.x < 2 * kShards) dm.shard_cnt[threadIdx.x] = 0;
+    if (threadIdx.x < 2) dm.piece_cnt[threadIdx.x] = 0;
   }
   extern __shared__ uint2 ent[];  // kStageCap + 1 entries
   __shared__ AxcVariantParameters lut[16];
@@ -549,9 +749,7 @@ void win_pass_kernel(ShardVramPtrs vram, uint32_t N, uint32_t A, const AxcVarian
 // The corrections of tick j (the pieces its axon kernel made): a lane per
 // reverse entry, the exact prediction rule against the reference's test;
 // where they differ, +-(1 hit, the slot's current weight) into the soma's
-// sum of tick j. Workgroup b takes part b % kShards, every (gridDim.x /
-// kShards)-th run of 256 lanes of it. Counters per workgroup in w.cpart
-// (pieces, entries).
+// sum of tick j. Counters per workgroup in w.cpart (pieces, entries).
 __global__ __launch_bounds__(256) void win_correct_kernel(const AxcBurstHeads8* __restrict__ heads,
                                                           const uint8_t* __restrict__ soma_flags, uint32_t N,
                                                           uint32_t A, const AxcVariantParameters* __restrict__ lut_g,
@@ -561,12 +759,11 @@ __global__ __launch_bounds__(256) void win_correct_kernel(const AxcBurstHeads8* 
   __shared__ uint32_t bsum[4][2];
   const uint32_t tid = threadIdx.x;
   if (tid < 16) prop[tid] = lut_g[tid].signal_propagation_length;
-  const uint32_t sh = blockIdx.x % kShards, runs = gridDim.x / kShards;
-  const uint32_t np = dm.shard_cnt[(j & 1) * kShards + sh];
-  const uint64_t* __restrict__ pieces = dm.pieces + dm.shard_off[sh];
+  const uint32_t np = dm.piece_cnt[j & 1];
+  const uint64_t* __restrict__ pieces = dm.pieces;
   __syncthreads();
   uint32_t n_p = 0, n_e = 0;
-  for (uint32_t x = (blockIdx.x / kShards) * 256 + tid; (x >> 4) < np; x += runs * 256) {
+  for (uint32_t x = blockIdx.x * 256 + tid; (x >> 4) < np; x += gridDim.x * 256) {
     const uint64_t pc = pieces[x >> 4];
     const uint32_t l = x & 15u, c = (uint32_t)(pc >> 56) + 1;
     n_p += l == 0 ? 1u : 0u;
@@ -696,60 +893,64 @@ __global__ __launch_bounds__(256) void gsop_win_kernel(ShardVramPtrs vram, uint3
   // the same iterations (the bound is the block's, not the lane's).
   for (uint32_t q0 = blockIdx.x * 256; q0 < n; q0 += gridDim.x * 256) {
     const uint32_t q = q0 + tid;
+    uint32_t i = 0, c = 0, F = 0, first = K;
+    if (q < n) {
+      i = sc.spk_list[q / kGsopLanes];
+      c = q % kGsopLanes;
+      F = sp.fill[i];
+      if (c * 8 < F) first = j + 1 + vram.timers[i];  // the first later tick that reads the sum
+    }
+    // Spikers refractory for the rest of the window (the usual case) take
+    // the plain update; the others also keep their weight changes.
+    const bool later = first < K;
     int32_t dv[kWin];
 #pragma unroll
     for (uint32_t jj = 0; jj < kWin; jj++) dv[jj] = 0;
-    uint32_t i = 0;
-    if (q < n) {
-      i = sc.spk_list[q / kGsopLanes];
-      const uint32_t c = q % kGsopLanes;
-      const uint32_t F = sp.fill[i];
-      if (c * 8 < F) {
-        v4u* wq = reinterpret_cast<v4u*>(sp.w + chunk_at(N, c, i));
-        const v4u wb = *wq;
-        gsop_chunk(N, A, lut, dop, vram.soma_flags, sp, vram.axon_heads, i, c, F);
-        const uint32_t first = j + 1 + vram.timers[i];  // the first later tick that reads the sum
-        if (first < K) {
-          const v4u wa = *wq;
-          const uint32_t before[4] = {wb.x, wb.y, wb.z, wb.w}, after[4] = {wa.x, wa.y, wa.z, wa.w};
-          const uint32_t L = lut[(vram.soma_flags[i] >> 4) & 0x0F].signal_propagation_length;
-          const v4u* tq = reinterpret_cast<const v4u*>(sp.tgt + chunk_at(N, c, i));
-          const v4u t0 = tq[0], t1 = tq[1];
-          const uint32_t t[8] = {t0.x, t0.y, t0.z, t0.w, t1.x, t1.y, t1.z, t1.w};
-          const uint32_t nv = F - c * 8 < 8u ? F - c * 8 : 8u;
+    if (q < n && c * 8 < F) {
+      v4u* wq = reinterpret_cast<v4u*>(sp.w + chunk_at(N, c, i));
+      v4u wb{};
+      if (later) wb = *wq;
+      gsop_chunk(N, A, lut, dop, vram.soma_flags, sp, vram.axon_heads, i, c, F,
+                 LazyHeads{dm.aux, dm.ctl[1], j + 1, v_seg});
+      if (later) {
+        const v4u wa = *wq;
+        const uint32_t before[4] = {wb.x, wb.y, wb.z, wb.w}, after[4] = {wa.x, wa.y, wa.z, wa.w};
+        const uint32_t L = lut[(vram.soma_flags[i] >> 4) & 0x0F].signal_propagation_length;
+        const v4u* tq = reinterpret_cast<const v4u*>(sp.tgt + chunk_at(N, c, i));
+        const v4u t0 = tq[0], t1 = tq[1];
+        const uint32_t t[8] = {t0.x, t0.y, t0.z, t0.w, t1.x, t1.y, t1.z, t1.w};
+        const uint32_t nv = F - c * 8 < 8u ? F - c * 8 : 8u;
 #pragma unroll
-          for (int u = 0; u < 8; u++) {
-            const int32_t w0 = (int32_t)(int16_t)(uint16_t)(before[u >> 1] >> (16 * (u & 1)));
-            const int32_t w1 = (int32_t)(int16_t)(uint16_t)(after[u >> 1] >> (16 * (u & 1)));
-            if ((uint32_t)u >= nv || w0 == w1) continue;
-            // The u32 rule (segment + length > 255) needs the start heads: saved
-            // for an axon the window pushed, else the heads advanced j+1 times.
-            const uint32_t a = snap_index(t[u], A);
-            const AxcBurstHeads8* hb = vram.axon_heads;
-            uint32_t back = (j + 1) * v_seg;
-            if ((t[u] >> 24) + L > 255u && a < A && (uint32_t)dm.aux[a] == dm.ctl[1]) {
-              hb = w.heads_w0;
-              back = 0;
-            }
-            const uint32_t m = win_pred_mask(w, hb, back, A, a, t[u] >> 24, L, v_seg, K);
+        for (int u = 0; u < 8; u++) {
+          const int32_t w0 = (int32_t)(int16_t)(uint16_t)(before[u >> 1] >> (16 * (u & 1)));
+          const int32_t w1 = (int32_t)(int16_t)(uint16_t)(after[u >> 1] >> (16 * (u & 1)));
+          if ((uint32_t)u >= nv || w0 == w1) continue;
+          // The u32 rule (segment + length > 255) needs the start heads: saved
+          // for an axon the window pushed, else still in the heads.
+          const uint32_t a = snap_index(t[u], A);
+          const AxcBurstHeads8* hb = vram.axon_heads;
+          if ((t[u] >> 24) + L > 255u && a < A && (uint32_t)dm.aux[a] == dm.ctl[1]) hb = w.heads_w0;
+          const uint32_t m = win_pred_mask(w, hb, 0, A, a, t[u] >> 24, L, v_seg, K);
 #pragma unroll
-            for (uint32_t jj = 0; jj < kWin; jj++)
-              if (jj >= first && ((m >> jj) & 1u)) dv[jj] += w1 - w0;
-          }
+          for (uint32_t jj = 0; jj < kWin; jj++)
+            if (jj >= first && ((m >> jj) & 1u)) dv[jj] += w1 - w0;
         }
       }
-      if (c == 0) {
-        cnt_slots += F;
-        cnt_tgt += F < kMaxDendrites ? F + 1 : kMaxDendrites;
-      }
     }
-    // The spiker's 16 lanes are an aligned group of a wave.
+    if (q < n && c == 0) {
+      cnt_slots += F;
+      cnt_tgt += F < kMaxDendrites ? F + 1 : kMaxDendrites;
+    }
+    // The spiker's 16 lanes are an aligned group of a wave (whole waves run
+    // every iteration, so the test is alike over the wave).
+    if (__any(later)) {
 #pragma unroll
-    for (uint32_t jj = 0; jj < kWin; jj++) {
+      for (uint32_t jj = 0; jj < kWin; jj++) {
 #pragma unroll
-      for (int off = 8; off > 0; off >>= 1) dv[jj] += __shfl_xor(dv[jj], off, 16);
-      if (q < n && (q % kGsopLanes) == 0 && dv[jj] != 0)
-        atomicAdd((unsigned long long*)&w.acc[(size_t)jj * N + i], (unsigned long long)win_word(dv[jj], 0));
+        for (int off = 8; off > 0; off >>= 1) dv[jj] += __shfl_xor(dv[jj], off, 16);
+        if (q < n && (q % kGsopLanes) == 0 && dv[jj] != 0)
+          atomicAdd((unsigned long long*)&w.acc[(size_t)jj * N + i], (unsigned long long)win_word(dv[jj], 0));
+      }
     }
   }
 #pragma unroll
@@ -874,19 +1075,6 @@ __global__ __launch_bounds__(256) void rev_sort_kernel(uint32_t A, const uint32_
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
   }
-}
-
-// The most correction pieces each part can take in a tick: per axon,
-// ceil(list length / 16) (a range of its list makes no more), summed over
-// the part's axons.
-__global__ __launch_bounds__(256) void shard_cap_kernel(uint32_t A, const uint32_t* __restrict__ rev_off,
-                                                        uint32_t* __restrict__ cap) {
-  const uint32_t a = blockIdx.x * 256 + threadIdx.x;
-  uint32_t c = a < A ? (rev_off[a + 1] - rev_off[a] + 15) >> 4 : 0u;
-  // The wave's axons are one group of 64: one atomic per wave.
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off, 64);
-  if ((threadIdx.x & 63u) == 0 && c) atomicAdd(&cap[(a >> 6) % kShards], c);
 }
 
 }  // namespace axc

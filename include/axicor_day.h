@@ -383,11 +383,11 @@ int32_t axc_shard_window_stats(const ShardVramPtrs *vram, uint64_t *out);
  * axc_step_day_phase_batch launches its kernels one by one instead of
  * replaying its graph (same kernels, same results) so that they can be timed.
  * axc_kernel_timing_read waits for the events and returns the summed
- * milliseconds and launch counts of 11 kernels in this order:
+ * milliseconds and launch counts of 12 kernels in this order:
  * axon_tick, neuron_tick, push_apply, gsop, readout, spike_count,
  * weights_copy (the batch's working copy of the weights, in and out),
- * window_start, window_pass, window_correct, window_soma (the windowed
- * batches' own kernels, see axc_set_window_ticks). */
+ * window_start, window_pass, window_correct, window_soma, window_end (the
+ * windowed batches' own kernels, see axc_set_window_ticks). */
 void axc_kernel_timing_enable(int enable);
 int32_t axc_kernel_timing_read(double *ms_out, uint64_t *launches_out, int reset);
 

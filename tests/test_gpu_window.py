@@ -171,6 +171,32 @@ def test_window_long_propagation(window):
         p.check(f"window {window}: long propagation batch {k}")
 
 
+def test_window_short_refractory_and_short_propagation(window):
+    """Refractory periods of 0-2 ticks, so a spiker's weight changes reach
+    the window's later ticks (GSOP adds them to the predicted hits there),
+    and propagation lengths of 1-3 segments below (K-1) v_seg, where the
+    window pass tests each tick on its own, beside types that take the
+    one-pass test, mixed in the same waves."""
+    lut = B.lut_from([
+        B.template_variant(False, refractory_period=0, signal_propagation_length=2, threshold=12000),
+        B.template_variant(False, refractory_period=1, signal_propagation_length=10, threshold=12000),
+        B.template_variant(True, refractory_period=2, signal_propagation_length=1, threshold=12000),
+        B.template_variant(False, refractory_period=0, signal_propagation_length=3, threshold=12000,
+                           adaptive_mode=1, adaptive_leak_gain=64, adaptive_leak_max=3000),
+    ])
+    spec = B.spec_small(4000, 64, 300, seed=31, axon_len=(1, 40), lut=lut,
+                        type_ppm=[300_000, 300_000, 200_000, 200_000])
+    for v_seg in (1, 2):
+        p = Pair(B.generate(spec), n_out=32)
+        fired = 0
+        for k, dop in enumerate([0, 120, -90, 255]):
+            out_ref, _ = p.step(13, dopamine=dop, v_seg=v_seg, use_graph=True)
+            np.testing.assert_array_equal(p.outputs(13), out_ref)
+            fired += int(out_ref.sum())
+            p.check(f"window {window}: v_seg {v_seg}, short refractory batch {k}")
+        assert fired > 0
+
+
 def test_window_large_v_seg(window):
     """v_seg 9,000: windows of 2 ticks stay exact in the byte form; 3 and 4
     ticks reach 2^15 and the batch runs per tick (same results)."""
@@ -234,5 +260,6 @@ def test_window_timed_path(window4):
     assert lib.axc_kernel_timing_read(ms, n, 1) == 0
     got = dict(zip(TIMED_KERNELS, n))
     assert got["window_pass"] == 3 * 3       # windows of 4, 4, 2 per batch
+    assert got["window_end"] == 3 * 3
     assert got["window_soma"] == 30 and got["window_correct"] == 30 and got["axon_tick"] == 30
     assert got["neuron_tick"] == 0
