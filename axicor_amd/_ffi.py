@@ -66,7 +66,8 @@ class AxcBrainDesc(C.Structure):
 
 
 class AxcBatchTuning(C.Structure):
-    _fields_ = [("axon_blocks", C.c_uint32), ("soma_blocks", C.c_uint32), ("list_blocks", C.c_uint32)]
+    _fields_ = [("axon_blocks", C.c_uint32), ("soma_blocks", C.c_uint32), ("list_blocks", C.c_uint32),
+                ("stage_order", C.c_uint32)]
 
 
 class AxcSynapseStats(C.Structure):

@@ -61,10 +61,11 @@ class BatchEngine:
         err = day_lib().axc_batch_autotune(self.h, ticks, stream, C.byref(t))
         if err:
             raise GpuError(f"axc_batch_autotune failed ({err})")
-        return {"axon_blocks": t.axon_blocks, "soma_blocks": t.soma_blocks, "list_blocks": t.list_blocks}
+        return {"axon_blocks": t.axon_blocks, "soma_blocks": t.soma_blocks, "list_blocks": t.list_blocks,
+                "stage_order": t.stage_order}
 
-    def set_tuning(self, axon_blocks: int, soma_blocks: int, list_blocks: int) -> None:
-        t = AxcBatchTuning(axon_blocks, soma_blocks, list_blocks)
+    def set_tuning(self, axon_blocks: int, soma_blocks: int, list_blocks: int, stage_order: int = 0) -> None:
+        t = AxcBatchTuning(axon_blocks, soma_blocks, list_blocks, stage_order)
         err = day_lib().axc_batch_set_tuning(self.h, C.byref(t))
         if err:
             raise GpuError(f"axc_batch_set_tuning failed ({err})")

@@ -149,6 +149,10 @@ __global__ void batch_set_tick_kernel(BatchTick* __restrict__ bt, uint32_t tick_
 // a multiple of 8). GSOP counts are taken here, for batch_gsop_chunks_kernel.
 // (A persistent grid walking the items measured slower: the values the
 // compiler keeps across items spill.)
+// `items` is the dispatch order (the autotuner's choice): XCD x takes
+// entries [x * per_xcd, (x + 1) * per_xcd) in order; padding entries carry
+// brain kNoBrain.
+constexpr uint32_t kNoBrain = 0xFFFFFFFFu;
 __global__ __launch_bounds__(kStageThreads) __attribute__((amdgpu_waves_per_eu(8, 8)))
 void batch_staged_kernel(const BrainDev* __restrict__ brains, const WorkItem* __restrict__ items, uint32_t n_items,
                          const BatchTick* __restrict__ bt, uint32_t t_off, uint32_t snap) {
@@ -157,11 +161,35 @@ void batch_staged_kernel(const BrainDev* __restrict__ brains, const WorkItem* __
   const uint32_t it = (blockIdx.x % kNumXcd) * per_xcd + blockIdx.x / kNumXcd;
   if (it >= n_items) return;  // the whole workgroup
   const WorkItem w = items[it];
+  if (w.brain == kNoBrain) return;
   const BrainDev& b = brains[w.brain];
   if (threadIdx.x < 64)
     reinterpret_cast<uint4*>(lut)[threadIdx.x] = reinterpret_cast<const uint4*>(b.lut)[threadIdx.x];
   staged_block<true>(b.vram, b.N, b.A, lut, bt->tick_base + t_off, b.sc, b.sp,
                      w.start / kBlockSomas, b.stats, w.start == 0, b.type_spikes, true);
+}
+
+// The staged pass's work per item, for the autotuner's balanced orders: the
+// chunks of 8 slots its somas scan plus its staged entries / 32 (the
+// gathers of one entry cost about a 32nd of a chunk's loads and tests).
+__global__ __launch_bounds__(kBlockSomas) void batch_stage_cost_kernel(const BrainDev* __restrict__ brains,
+                                                                       const WorkItem* __restrict__ items,
+                                                                       uint32_t* __restrict__ cost) {
+  __shared__ uint32_t part[kBlockSomas / 64];
+  const WorkItem w = items[blockIdx.x];
+  const BrainDev& b = brains[w.brain];
+  const uint32_t i = w.start + threadIdx.x;
+  uint32_t c = i < b.N ? (b.sp.fill[i] + 7u) / 8u : 0u;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off, 64);
+  if ((threadIdx.x & 63u) == 0) part[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t = 0;
+    for (uint32_t k = 0; k < kBlockSomas / 64; k++) t += part[k];
+    const uint32_t n = b.sp.cnt[w.start / kBlockSomas];
+    cost[blockIdx.x] = t + (n == kStageOverflow ? kStageCap : n) / 32u;
+  }
 }
 
 // Exclusive prefix of the brains' list counts of this tick (one workgroup):

@@ -1800,18 +1800,81 @@ struct Batch {
   WorkItem* d_soma_items = nullptr;
   WorkItem* d_stage_items = nullptr;  // (brain, first soma) of each kBlockSomas block, for the staged pass
   uint32_t n_axon_items = 0, n_soma_items = 0, n_stage_items = 0;
+  // The staged pass's dispatch order (AxcBatchTuning::stage_order): the
+  // stage items themselves for 0, else a balanced list of 8 equal shares
+  // padded with kNoBrain entries (capacity 8 * n_stage_items).
+  WorkItem* d_stage_order = nullptr;
+  uint32_t n_stage_order = 0, order_of = 0xFFFFFFFFu;
   BatchTick* d_tick = nullptr;
-  AxcBatchTuning tune{0, 0, 0};
+  AxcBatchTuning tune{0, 0, 0, 0};
   hipGraphExec_t gexec = nullptr;
   hipGraph_t graph = nullptr;
   uint32_t graph_T = 0;
-  AxcBatchTuning graph_tune{0, 0, 0};
+  AxcBatchTuning graph_tune{0, 0, 0, 0};
   // Staged soma pass: each brain's plan lives in its workspace (shards[k]);
   // built_version[k] is the targets version it was built from.
   bool staged = false;
   uint32_t* d_prefix = nullptr;  // [2][n+1] spike- and owner-count prefixes of the current tick
   std::vector<uint64_t> built_version;
 };
+
+// The staged pass's dispatch order for stage_order k (see AxcBatchTuning):
+// k = 0 is the stage items in brain order; otherwise every brain's blocks
+// are cut into runs of k, and the runs, heaviest first, go to the XCD share
+// with the least work so far (longest-processing-time), each share keeping
+// its runs in that order; shares are padded to equal length.
+hipError_t build_stage_order(Batch* b, uint32_t k, hipStream_t st) {
+  if (b->order_of == k) return hipSuccess;
+  if (k == 0) {
+    b->n_stage_order = b->n_stage_items;
+    hipError_t e = hipMemcpyAsync(b->d_stage_order, b->d_stage_items, (size_t)b->n_stage_items * sizeof(WorkItem),
+                                  hipMemcpyDeviceToDevice, st);
+    if (e == hipSuccess) b->order_of = 0;
+    return e;
+  }
+  const uint32_t n = b->n_stage_items;
+  uint32_t* d_cost = nullptr;
+  hipError_t e = hipMalloc((void**)&d_cost, (size_t)n * 4);
+  if (e != hipSuccess) return e;
+  batch_stage_cost_kernel<<<n, kBlockSomas, 0, st>>>(b->d_brains, b->d_stage_items, d_cost);
+  std::vector<uint32_t> cost(n);
+  std::vector<WorkItem> items(n);
+  e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(cost.data(), d_cost, (size_t)n * 4, hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(items.data(), b->d_stage_items, (size_t)n * sizeof(WorkItem), hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  (void)hipFree(d_cost);
+  if (e != hipSuccess) return e;
+  // Runs: [first item, end) within one brain, at most k items.
+  struct Run { uint32_t first, end; uint64_t w; };
+  std::vector<Run> runs;
+  for (uint32_t q = 0; q < n;) {
+    uint32_t r = q;
+    uint64_t w = 0;
+    while (r < n && r - q < k && items[r].brain == items[q].brain) w += cost[r++];
+    runs.push_back(Run{q, r, w});
+    q = r;
+  }
+  std::stable_sort(runs.begin(), runs.end(), [](const Run& x, const Run& y) { return x.w > y.w; });
+  std::vector<std::vector<WorkItem>> share(kNumXcd);
+  std::vector<uint64_t> load(kNumXcd, 0);
+  for (const Run& r : runs) {
+    const uint32_t x = (uint32_t)(std::min_element(load.begin(), load.end()) - load.begin());
+    load[x] += r.w;
+    for (uint32_t q = r.first; q < r.end; q++) share[x].push_back(items[q]);
+  }
+  size_t per = 0;
+  for (const auto& v : share) per = std::max(per, v.size());
+  std::vector<WorkItem> order(per * kNumXcd, WorkItem{kNoBrain, 0});
+  for (uint32_t x = 0; x < kNumXcd; x++) std::copy(share[x].begin(), share[x].end(), order.begin() + x * per);
+  e = hipMemcpy(b->d_stage_order, order.data(), order.size() * sizeof(WorkItem), hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    b->n_stage_order = (uint32_t)order.size();
+    b->order_of = k;
+  }
+  return e;
+}
 
 hipError_t batch_capture(Batch* b, uint32_t T) {
   if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
@@ -1833,8 +1896,8 @@ hipError_t batch_capture(Batch* b, uint32_t T) {
       for (uint32_t t = 0; t < T; t++) {
         batch_axon_kernel<<<b->tune.axon_blocks, kThreads, 0, cap>>>(b->d_brains, b->d_axon_items, b->n_axon_items,
                                                                      b->d_tick, t, t);
-        batch_staged_kernel<<<(b->n_stage_items + kNumXcd - 1) / kNumXcd * kNumXcd, kStageThreads, lds, cap>>>(
-            b->d_brains, b->d_stage_items, b->n_stage_items, b->d_tick, t, t);
+        batch_staged_kernel<<<(b->n_stage_order + kNumXcd - 1) / kNumXcd * kNumXcd, kStageThreads, lds, cap>>>(
+            b->d_brains, b->d_stage_order, b->n_stage_order, b->d_tick, t, t);
         batch_compact_spikes_kernel<<<blocks_for((uint64_t)b->n_stage_items * kItemWords), kThreads, 0, cap>>>(
             b->d_brains, b->d_stage_items, b->n_stage_items);
         batch_spike_prefix_kernel<<<1, 1024, 0, cap>>>(b->d_brains, n, b->d_prefix, kCntSpikes);
@@ -1878,8 +1941,10 @@ hipError_t batch_capture(Batch* b, uint32_t T) {
 
 hipError_t batch_run(Batch* b, uint32_t tick_base, uint32_t T, hipStream_t st) {
   if (!(b->gexec && b->graph_T == T && b->graph_tune.axon_blocks == b->tune.axon_blocks &&
-        b->graph_tune.soma_blocks == b->tune.soma_blocks && b->graph_tune.list_blocks == b->tune.list_blocks)) {
-    hipError_t e = batch_capture(b, T);
+        b->graph_tune.soma_blocks == b->tune.soma_blocks && b->graph_tune.list_blocks == b->tune.list_blocks &&
+        b->graph_tune.stage_order == b->tune.stage_order)) {
+    hipError_t e = b->staged ? build_stage_order(b, b->tune.stage_order, st) : hipSuccess;
+    if (e == hipSuccess) e = batch_capture(b, T);
     if (e != hipSuccess) return e;
   }
   batch_set_tick_kernel<<<1, 1, 0, st>>>(b->d_tick, tick_base);
@@ -1899,6 +1964,7 @@ void batch_free(Batch* b) {
   (void)hipFree(b->d_axon_items);
   (void)hipFree(b->d_soma_items);
   (void)hipFree(b->d_stage_items);
+  (void)hipFree(b->d_stage_order);
   (void)hipFree(b->d_prefix);
   (void)hipFree(b->d_tick);
   delete b;
@@ -1961,6 +2027,7 @@ int32_t axc_batch_create(const AxcBrainDesc* brains, uint32_t n, void** out) {
     if (e == hipSuccess) e = hipMalloc((void**)&b->d_stage_items, sg.size() * sizeof(WorkItem));
     if (e == hipSuccess)
       e = hipMemcpy(b->d_stage_items, sg.data(), sg.size() * sizeof(WorkItem), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMalloc((void**)&b->d_stage_order, kNumXcd * (sg.size() + 1) * sizeof(WorkItem));
   }
   if (e == hipSuccess) {
     b->host_dev.resize(n);
@@ -2024,7 +2091,7 @@ int32_t axc_batch_step(void* h, uint32_t tick_base, uint32_t ticks, void* stream
 
 int32_t axc_batch_set_tuning(void* h, const AxcBatchTuning* t) {
   if (!h || !t || t->axon_blocks == 0 || t->list_blocks == 0 || t->soma_blocks == 0 ||
-      t->soma_blocks % kNumXcd != 0)
+      t->soma_blocks % kNumXcd != 0 || t->stage_order > (1u << 20))
     return (int32_t)hipErrorInvalidValue;
   ((Batch*)h)->tune = *t;
   return 0;
@@ -2060,14 +2127,25 @@ int32_t axc_batch_autotune(void* h, uint32_t ticks, void* stream, AxcBatchTuning
   hipEvent_t e0, e1;
   (void)hipEventCreate(&e0);
   (void)hipEventCreate(&e1);
-  const uint32_t soma_mult[] = {2, 4, 8, 16};
-  const uint32_t axon_mult[] = {4, 16};
-  for (uint32_t sm : soma_mult) {
-    for (uint32_t am : axon_mult) {
+  // Candidates: on the staged path the soma pass's dispatch order (brain
+  // order, or balanced runs of 4, 32 or 256 blocks), the list grid (pushes,
+  // GSOP) and the axon grid; on the direct path the soma, axon and list
+  // grids.
+  std::vector<AxcBatchTuning> cand;
+  if (b->staged) {
+    for (uint32_t so : {0u, 4u, 32u, 256u})
+      for (uint32_t lm : {1u, 2u, 4u})
+        for (uint32_t am : {4u, 16u}) cand.push_back(AxcBatchTuning{(uint32_t)cus * am, base.soma_blocks, (uint32_t)cus * lm, so});
+  } else {
+    for (uint32_t sm : {2u, 4u, 8u, 16u})
+      for (uint32_t lm : {1u, 2u, 4u})
+        for (uint32_t am : {4u, 16u})
+          cand.push_back(AxcBatchTuning{(uint32_t)cus * am, (uint32_t)cus * sm, (uint32_t)cus * lm, 0u});
+  }
+  for (const AxcBatchTuning& c : cand) {
+    {
       if (e != hipSuccess) break;
-      b->tune.soma_blocks = (uint32_t)cus * sm;
-      b->tune.axon_blocks = (uint32_t)cus * am;
-      b->tune.list_blocks = base.list_blocks;
+      b->tune = c;
       e = batch_run(b, 0, ticks, st);  // capture + warm
       if (e == hipSuccess) (void)hipEventRecord(e0, st);
       if (e == hipSuccess) e = batch_run(b, ticks, ticks, st);

@@ -324,8 +324,8 @@ def main() -> int:
     lib.gpu_set_device(0)
 
     if args.batched_only:
-        print(json.dumps({"batched": bench_batched(args.batched_brains, args.batched_ticks, 1000, 50, 20261203,
-                                                   serial=not args.batched_no_serial)}), flush=True)
+        print(json.dumps({"batched": bench_batched(args.batched_brains, args.batched_ticks, args.batched_ticks, 50,
+                                                   20261203, serial=not args.batched_no_serial)}), flush=True)
         return 0
 
     spec = {"c2": B.spec_c2, "c1": B.spec_c1, "c2-linear": lambda: B.spec_c2(brick=0),
@@ -547,7 +547,7 @@ def main() -> int:
         out["nights"] = bench_nights(lib, brain, vram, eng, night_batch, args.nights, args.prune)
     if args.batched_brains:
         del vram, eng, d_in
-        out["batched"] = bench_batched(args.batched_brains, args.batched_ticks, 1000, 50, 20261203,
+        out["batched"] = bench_batched(args.batched_brains, args.batched_ticks, args.batched_ticks, 50, 20261203,
                                        serial=not args.batched_no_serial)
     if cpu is not None:
         out["cpu_baseline"] = {

@@ -410,16 +410,21 @@ typedef struct AxcBrainDesc {
 } AxcBrainDesc;
 
 /* Launch shape: workgroups of the axon, soma and list kernels (the soma count
- * must be a multiple of 8, one share per XCD). */
+ * must be a multiple of 8, one share per XCD), and the staged soma pass's
+ * dispatch order: 0 deals the brains' 256-soma blocks to the 8 XCDs in
+ * contiguous eighths of the brain order; k > 0 cuts every brain into runs
+ * of k blocks and deals the runs, heaviest first, to the XCD with the least
+ * work so far (work = the blocks' dendrite chunks and staged entries). */
 typedef struct AxcBatchTuning {
-    uint32_t axon_blocks, soma_blocks, list_blocks;
+    uint32_t axon_blocks, soma_blocks, list_blocks, stage_order;
 } AxcBatchTuning;
 
 int32_t axc_batch_create(const AxcBrainDesc *brains, uint32_t n_brains, void **out_batch);
 /* Ticks tick_base .. tick_base+ticks-1 for every brain, asynchronously. */
 int32_t axc_batch_step(void *batch, uint32_t tick_base, uint32_t ticks, void *stream);
-/* Times a few launch shapes on `ticks` ticks from a snapshot of every brain,
- * restores the snapshot, keeps the fastest shape (also returned). */
+/* Times launch shapes on `ticks` ticks from a snapshot of every brain (the
+ * staged pass's dispatch order, the list and the axon grids), restores the
+ * snapshot, keeps the fastest shape (also returned). */
 int32_t axc_batch_autotune(void *batch, uint32_t ticks, void *stream, AxcBatchTuning *best);
 int32_t axc_batch_set_tuning(void *batch, const AxcBatchTuning *tuning);
 int32_t axc_batch_get_tuning(void *batch, AxcBatchTuning *tuning);

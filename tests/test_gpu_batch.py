@@ -94,6 +94,29 @@ def test_batch_graph_replays_and_autotune_restore(gpu):
     be.destroy()
 
 
+@pytest.mark.parametrize("stage_order", [0, 1, 3, 64])
+def test_batch_stage_orders_equal_the_oracle(gpu, stage_order):
+    """Every dispatch order of the staged pass (brain order, balanced runs of
+    1, 3 or 64 blocks, the XCD shares padded) gives every brain the oracle's
+    state: the order only moves work between XCDs."""
+    members = make_members(21, 6, 36)
+    mem = [load(b, dop, inp) for b, dop, inp in members]
+    be = BatchEngine(mem)
+    t = be.autotune(2)
+    assert t["stage_order"] in (0, 4, 32, 256)
+    be.set_tuning(t["axon_blocks"], t["soma_blocks"], t["list_blocks"], stage_order)
+    for k in range(3):
+        be.step(k * 12, 12)
+    day_lib().gpu_device_synchronize()
+    for (b, dop, inp), m in zip(members, mem):
+        st, hd = b.state.copy(), b.heads.copy()
+        O.step_batch(st, b.padded_n, hd, b.total_axons, b.lut, 1, 0, 36, inp, b.virtual_offset, b.n_inputs,
+                     dopamine=dop)
+        assert np.array_equal(m["vram"].download_state(), st), f"order {stage_order}: state != oracle"
+        assert np.array_equal(m["vram"].download_axons(), hd), f"order {stage_order}: heads != oracle"
+    be.destroy()
+
+
 def test_batch_synapse_stats_match_the_downloaded_state(gpu):
     """axc_batch_synapse_stats against the same sums over the state blob."""
     from axicor_amd.layout import compute_state_offsets
