@@ -157,6 +157,7 @@ TIMED_KERNELS = ["axon_tick", "neuron_tick", "push_apply", "gsop", "readout", "s
 SURROGATE_SIGNATURES = {
     "axc_mlp_create": (_I32, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint64, C.POINTER(C.c_void_p)]),
     "axc_mlp_forward": (_I32, [_P, _P, C.c_int64, _P, _P]),
+    "axc_mlp_set_fused_forward": (_I32, [_P, C.c_int]),
     "axc_mlp_train_step": (_I32, [_P, _P, _P, C.c_int, C.c_float, C.POINTER(C.c_float), _P]),
     "axc_mlp_gradients": (_I32, [_P, _P, _P, C.c_int, _P, C.POINTER(C.c_float), _P]),
     "axc_mlp_layout": (_I32, [_P, C.POINTER(C.c_int32), _P, _P, C.POINTER(C.c_int64)]),

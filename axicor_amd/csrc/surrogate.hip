@@ -26,6 +26,7 @@
 #include <math.h>
 #include <cmath>
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 namespace axcmlp {
@@ -40,16 +41,22 @@ constexpr int kWgSplits = 16;     // batch splits of the weight gradient (split 
 
 enum Epi { EPI_BIAS_GELU = 0, EPI_BIAS = 1, EPI_GELU_GRAD = 2, EPI_STORE = 3, EPI_BIAS_GELU_NOZ = 4 };
 
-__device__ __forceinline__ float gelu_f(float z) {
-  // GELU, tanh form: 0.5 z (1 + tanh(sqrt(2/pi) (z + 0.044715 z^3)))
-  const float c = 0.7978845608028654f;
-  return 0.5f * z * (1.0f + tanhf(c * (z + 0.044715f * z * z * z)));
+// GELU, tanh form: 0.5 z (1 + tanh(u)), u = sqrt(2/pi) (z + 0.044715 z^3).
+// Since 0.5 (1 + tanh(u)) = 1 / (1 + e^(-2u)), it is z s with
+// s = 1 / (1 + 2^(-2u log2(e))): one hardware exp2 and one reciprocal (each
+// about 1 ulp; |error| of s below 1e-6). For u -> +inf s -> 1, for
+// u -> -inf s -> 0 with relative accuracy.
+__device__ __forceinline__ float gelu_sig(float z) {
+  const float k = -2.0f * 0.7978845608028654f * 1.4426950408889634f;  // -2 sqrt(2/pi) log2(e)
+  const float inner = __builtin_fmaf(0.044715f * z * z, z, z);         // z + 0.044715 z^3
+  return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(k * inner));
 }
+__device__ __forceinline__ float gelu_f(float z) { return z * gelu_sig(z); }
+// d/dz: s + z s (1 - s) 2 sqrt(2/pi) (1 + 3 * 0.044715 z^2).
 __device__ __forceinline__ float gelu_grad_f(float z) {
-  const float c = 0.7978845608028654f;
-  const float t = tanhf(c * (z + 0.044715f * z * z * z));
-  const float du = c * (1.0f + 3.0f * 0.044715f * z * z);
-  return 0.5f * (1.0f + t) + 0.5f * z * (1.0f - t * t) * du;
+  const float s = gelu_sig(z);
+  const float du = 2.0f * 0.7978845608028654f * (1.0f + 3.0f * 0.044715f * z * z);
+  return s + z * s * (1.0f - s) * du;
 }
 
 // Stages a 64 (rows r of the tile) x 32 (k) block into T[r][k] from a matrix
@@ -359,6 +366,139 @@ __global__ __launch_bounds__(256) void to_bf16_kernel(const float* __restrict__ 
   yt[(size_t)(i % cols) * rows + i / cols] = b;
 }
 
+// ---- fused inference ---------------------------------------------------------
+// The whole network for a block of 128 rows in one workgroup of 16 waves, for
+// the surrogate's shape: inputs padded to 32, hidden layers of 512, outputs
+// padded to 16 (16 waves). The block's activations stay in LDS (128 x 512 bf16) from
+// layer to layer; the weights (the transposed bf16 copies, contiguous in K)
+// are read from L2. Each product runs the same MFMA 16x16x32 steps in the
+// same K order as gemm_kernel / gemm_big_kernel and the same epilogue
+// arithmetic (bias, tanh GELU, bf16 rounding), so the outputs equal the
+// layer-by-layer forward pass bit for bit.
+constexpr int kFusedRows = 64, kFusedIn = 32, kFusedHidden = 512, kFusedOut = 16, kFusedMaxLayers = 8;
+constexpr int kFusedPitch = kFusedHidden + 8;  // bf16 per LDS row (1040 B): the fragment reads spread over banks
+constexpr size_t kFusedLds = (size_t)kFusedRows * kFusedPitch * sizeof(__bf16);
+// Weights in blocks of 8 along K: layer l's weight (k, n) at
+// WB[((k / 8) * out_l + n) * 8 + k % 8], so that the 16 columns and 4 k
+// blocks of one MFMA B fragment are four runs of 256 contiguous bytes.
+struct FusedNet {
+  const __bf16* WB[kFusedMaxLayers];
+  const float* b[kFusedMaxLayers];
+  int n_layers;                       // hidden layers + 1
+};
+
+// WB (above) from the transposed copy WT[n][k] of a layer of `in` x `out`.
+__global__ __launch_bounds__(256) void block_weights_kernel(const __bf16* __restrict__ WT, int in, int out,
+                                                            __bf16* __restrict__ WB) {
+  const int t = blockIdx.x * 256 + threadIdx.x;  // one 16-byte piece: (k / 8, n)
+  if (t >= (in / 8) * out) return;
+  const int kb = t / out, nn = t % out;
+  *reinterpret_cast<uint4*>(WB + (size_t)t * 8) = *reinterpret_cast<const uint4*>(WT + (size_t)nn * in + kb * 8);
+}
+
+// ROWS rows per workgroup, 8 waves per 64 rows (each wave a 64 x 64 tile).
+template <int ROWS>
+__global__ __launch_bounds__(ROWS * 8) void mlp_fused_kernel(int64_t n, int d_in, int d_out,
+                                                            const float* __restrict__ X, float* __restrict__ Y,
+                                                            FusedNet net) {
+  extern __shared__ __attribute__((aligned(16))) __bf16 act[];  // [ROWS][kFusedPitch]
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int64_t r0 = (int64_t)blockIdx.x * ROWS;
+  // The inputs, as cast_rows_kernel makes them: bf16, zero past d_in and n.
+  if (tid < ROWS * kFusedIn / 8) {
+    const int row = tid >> 2, c = (tid & 3) * 8;
+    __attribute__((aligned(16))) __bf16 v[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      const int col = c + j;
+      v[j] = (r0 + row < n && col < d_in) ? (__bf16)X[(r0 + row) * d_in + col] : (__bf16)0.0f;
+    }
+    *reinterpret_cast<uint4*>(&act[row * kFusedPitch + c]) = *reinterpret_cast<const uint4*>(v);
+  }
+  __syncthreads();
+  // Hidden layers: wave w computes rows 64 (w / 8).., columns 64 (w % 8)..
+  // as 4 x 4 MFMA tiles.
+  const int rg = w >> 3, cg = w & 7;
+  for (int l = 0; l + 1 < net.n_layers; l++) {
+    const int K = l == 0 ? kFusedIn : kFusedHidden;
+    const __bf16* __restrict__ WB = net.WB[l];
+    f32x4 acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+      for (int j = 0; j < 4; j++) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    // B fragments of the next step loaded while this one is multiplied.
+    bf16x8 b[4], bn[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+      b[j] = *reinterpret_cast<const bf16x8*>(WB + ((size_t)(lane >> 4) * kFusedHidden + cg * 64 + j * 16 +
+                                                    (lane & 15)) * 8);
+    for (int k0 = 0; k0 < K; k0 += BK) {
+      const bool more = k0 + BK < K;
+      if (more) {
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+          bn[j] = *reinterpret_cast<const bf16x8*>(WB + ((size_t)((k0 + BK) / 8 + (lane >> 4)) * kFusedHidden +
+                                                         cg * 64 + j * 16 + (lane & 15)) * 8);
+      }
+      bf16x8 a[4];
+#pragma unroll
+      for (int i = 0; i < 4; i++)
+        a[i] = *reinterpret_cast<const bf16x8*>(&act[(rg * 64 + i * 16 + (lane & 15)) * kFusedPitch + k0 +
+                                                     8 * (lane >> 4)]);
+      // The transposed product (weights as the A operand, activation rows as
+      // the B operand): accumulator r of lane l is output feature
+      // 4 (l / 16) + r of the tile for activation row l % 16, so a lane's
+      // four results are adjacent in the row.
+#pragma unroll
+      for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b[j], a[i], acc[i][j], 0, 0, 0);
+      if (more) {
+#pragma unroll
+        for (int j = 0; j < 4; j++) b[j] = bn[j];
+      }
+    }
+    __syncthreads();  // every wave is done reading this layer's inputs
+    const float* __restrict__ bias = net.b[l];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int col = cg * 64 + j * 16 + 4 * (lane >> 4);
+      const float4 bc = *reinterpret_cast<const float4*>(bias + col);
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const int row = rg * 64 + i * 16 + (lane & 15);
+        __attribute__((aligned(8))) __bf16 v[4] = {(__bf16)gelu_f(acc[i][j][0] + bc.x), (__bf16)gelu_f(acc[i][j][1] + bc.y),
+                                                   (__bf16)gelu_f(acc[i][j][2] + bc.z), (__bf16)gelu_f(acc[i][j][3] + bc.w)};
+        *reinterpret_cast<uint2*>(&act[row * kFusedPitch + col]) = *reinterpret_cast<const uint2*>(v);
+      }
+    }
+    __syncthreads();
+  }
+  // Output layer: waves 0-7 compute rows 16 w.. as one MFMA tile of 16
+  // columns each (K in order, as the layer-by-layer pass).
+  if (w < ROWS / 16) {
+    const int l = net.n_layers - 1;
+    const __bf16* __restrict__ WB = net.WB[l];
+    f32x4 acc = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int k0 = 0; k0 < kFusedHidden; k0 += BK) {
+      const bf16x8 a =
+          *reinterpret_cast<const bf16x8*>(&act[(w * 16 + (lane & 15)) * kFusedPitch + k0 + 8 * (lane >> 4)]);
+      const bf16x8 b = *reinterpret_cast<const bf16x8*>(WB + ((size_t)(k0 / 8 + (lane >> 4)) * kFusedOut +
+                                                              (lane & 15)) * 8);
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b, a, acc, 0, 0, 0);  // transposed, as above
+    }
+    const int64_t row = r0 + w * 16 + (lane & 15);
+    if (row < n) {
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int col = 4 * (lane >> 4) + r;
+        if (col < d_out) Y[row * d_out + col] = acc[r] + net.b[l][col];
+      }
+    }
+  }
+}
+
 }  // namespace axcmlp
 
 using namespace axcmlp;
@@ -370,6 +510,7 @@ struct Layer {
   float *W, *b;            // fp32 master parameters, W[in][out] row-major
   __bf16* W16;             // bf16 copy read by the GEMMs
   __bf16* W16T;            // its transpose [out][in]: the forward product's operand, contiguous in K
+  __bf16* W16B;            // blocked copy for the fused forward (FusedNet::WB), rebuilt by each fused call
   float *gW, *gb, *mW, *vW, *mb, *vb;
   uint8_t *maskW, *maskb;  // 1 on real entries, 0 on padding
 };
@@ -388,7 +529,18 @@ struct Mlp {
   float* wpart = nullptr;     // [kWgSplits][max in x out]: split-K partials of the weight gradient
   int step = 0;
   int max_out = 0;
+  bool fused = true;  // axc_mlp_forward takes mlp_fused_kernel where the shape allows
 };
+
+// The shape mlp_fused_kernel serves: inputs padded to 32, hidden layers of
+// 512 (at most kFusedMaxLayers - 1 of them), outputs padded to 16.
+bool fused_shape(const Mlp* m) {
+  const int nl = (int)m->L.size();
+  if (nl < 2 || nl > kFusedMaxLayers || m->L[0].in != kFusedIn || m->L[nl - 1].out != kFusedOut) return false;
+  for (int l = 0; l + 1 < nl; l++)
+    if (m->L[l].out != kFusedHidden || m->L[l + 1].in != kFusedHidden) return false;
+  return true;
+}
 
 inline int pad(int x, int m) { return (x + m - 1) / m * m; }
 inline unsigned grid1(size_t n) { return (unsigned)((n + 255) / 256); }
@@ -479,7 +631,8 @@ uint64_t splitmix(uint64_t& s) {
 
 void release(Mlp* m) {
   for (Layer& y : m->L)
-    for (void* p : {(void*)y.W, (void*)y.b, (void*)y.W16, (void*)y.W16T, (void*)y.gW, (void*)y.gb, (void*)y.mW,
+    for (void* p : {(void*)y.W, (void*)y.b, (void*)y.W16, (void*)y.W16T, (void*)y.W16B, (void*)y.gW, (void*)y.gb,
+                    (void*)y.mW,
                     (void*)y.vW, (void*)y.mb, (void*)y.vb, (void*)y.maskW, (void*)y.maskb})
       if (p) (void)hipFree(p);
   for (auto p : m->H) if (p) (void)hipFree(p);
@@ -536,6 +689,7 @@ int32_t axc_mlp_create(int d_in, int hidden, int depth, int d_out, int max_batch
       if (e == hipSuccess) e = hipMalloc((void**)p, (size_t)y.out * 4);
     if (e == hipSuccess) e = hipMalloc((void**)&y.W16, nw * 2);
     if (e == hipSuccess) e = hipMalloc((void**)&y.W16T, nw * 2);
+    if (e == hipSuccess) e = hipMalloc((void**)&y.W16B, nw * 2);
     if (e == hipSuccess) e = hipMalloc((void**)&y.maskW, nw);
     if (e == hipSuccess) e = hipMalloc((void**)&y.maskb, y.out);
     if (e == hipSuccess) e = hipMemcpy(y.W, hw.data(), nw * 4, hipMemcpyHostToDevice);
@@ -590,6 +744,32 @@ int32_t axc_mlp_forward(void* h, const float* d_x, int64_t n, float* d_y, void* 
   if (!m || n < 0 || (n > 0 && (!d_x || !d_y))) return (int32_t)hipErrorInvalidValue;
   hipStream_t st = (hipStream_t)stream;
   const int nl = (int)m->L.size();
+  if (n == 0) return 0;
+  if (m->fused && fused_shape(m)) {
+    // 130 KB of dynamic LDS per workgroup: above the default limit.
+    static std::once_flag attr_once;
+    std::call_once(attr_once, [] {
+      (void)hipFuncSetAttribute((const void*)mlp_fused_kernel<kFusedRows>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)kFusedLds);
+    });
+    FusedNet net{};
+    net.n_layers = nl;
+    for (int l = 0; l < nl; l++) {
+      const Layer& y = m->L[l];
+      block_weights_kernel<<<grid1((size_t)(y.in / 8) * y.out), 256, 0, st>>>(y.W16T, y.in, y.out, y.W16B);
+      net.WB[l] = y.W16B;
+      net.b[l] = y.b;
+    }
+    constexpr int R = kFusedRows;
+    const int64_t blocks = (n + R - 1) / R;
+    for (int64_t b0 = 0; b0 < blocks; b0 += 0x7FFFFFFF) {  // grid x limit (never reached in practice)
+      const int64_t nb = std::min<int64_t>(blocks - b0, 0x7FFFFFFF);
+      const int64_t rows = std::min<int64_t>(n - b0 * R, nb * R);
+      mlp_fused_kernel<R><<<(unsigned)nb, R * 8, (size_t)R * kFusedPitch * sizeof(__bf16), st>>>(
+          rows, m->d_in, m->d_out, d_x + b0 * R * m->d_in, d_y + b0 * R * m->d_out, net);
+    }
+    return (int32_t)hipGetLastError();
+  }
   for (int64_t r0 = 0; r0 < n; r0 += m->max_batch) {
     const int b = (int)std::min<int64_t>(m->max_batch, n - r0);
     load_input(m, d_x + r0 * m->d_in, b, st);
@@ -734,6 +914,13 @@ int32_t axc_mlp_gemm(int M, int N, int K, int ta, int tb, const void* d_a, int l
   else if (tb) gemm<false, true, EPI_STORE>(st, M, N, K, A, lda, B, ldb, nullptr, nullptr, 0, d_c, nullptr, ldc);
   else gemm<false, false, EPI_STORE>(st, M, N, K, A, lda, B, ldb, nullptr, nullptr, 0, d_c, nullptr, ldc);
   return (int32_t)hipGetLastError();
+}
+
+int32_t axc_mlp_set_fused_forward(void* h, int enable) {
+  Mlp* m = (Mlp*)h;
+  if (!m) return (int32_t)hipErrorInvalidValue;
+  m->fused = enable != 0;
+  return fused_shape(m) ? 1 : 0;
 }
 
 void axc_mlp_destroy(void* h) {

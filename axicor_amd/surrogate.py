@@ -61,6 +61,14 @@ class Mlp:
     def forward_device(self, d_x: int, n: int, d_y: int, stream=None) -> None:
         _check(self.lib.axc_mlp_forward(self.h, d_x, n, d_y, stream), "axc_mlp_forward")
 
+    def set_fused(self, enable: bool) -> bool:
+        """Forward passes in one fused kernel (default) or layer by layer;
+        returns whether this network's shape has the fused kernel."""
+        r = self.lib.axc_mlp_set_fused_forward(self.h, 1 if enable else 0)
+        if r < 0 or r > 1:
+            raise RuntimeError(f"axc_mlp_set_fused_forward failed ({r})")
+        return r == 1
+
     def predict(self, X: np.ndarray) -> np.ndarray:
         X = np.ascontiguousarray(X, np.float32)
         n = X.shape[0]

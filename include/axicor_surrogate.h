@@ -72,6 +72,14 @@ int32_t axc_mlp_reset_optimizer(void* mlp);
 int32_t axc_mlp_gemm(int M, int N, int K, int ta, int tb, const void* d_a, int lda, const void* d_b, int ldb,
                      float* d_c, int ldc, void* stream);
 
+/* Whether axc_mlp_forward runs the whole network in one kernel (128 rows
+ * per workgroup, activations kept in LDS; on by default) or layer by layer.
+ * Both give the same outputs bit for bit. The fused kernel serves networks
+ * with d_in <= 32, hidden 481..512, d_out <= 16 and at most 7 hidden layers;
+ * others always run layer by layer. Returns 1 if the network has that
+ * shape, 0 if not (hipErrorInvalidValue for a null handle). */
+int32_t axc_mlp_set_fused_forward(void* mlp, int enable);
+
 /* Waits for the device and frees every buffer of the network. */
 void axc_mlp_destroy(void* mlp);
 

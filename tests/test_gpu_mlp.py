@@ -102,6 +102,46 @@ def test_forward_matches_oracle(gpu):
     net.destroy()
 
 
+def test_fused_forward_equals_layer_by_layer(gpu):
+    """The fused kernel (128 rows per workgroup, activations in LDS through
+    all layers) runs the same MFMA steps in the same order and the same
+    epilogues as the layer-by-layer pass: the outputs are equal bit for bit,
+    for ragged row counts and for more rows than the layer-by-layer buffers
+    hold (chunked there, one launch fused)."""
+    import time
+    from axicor_amd._ffi import day_lib
+    from axicor_amd.engine import DeviceBuffer
+    net, lay = _net(seed=7, max_batch=4096)
+    rng = np.random.default_rng(8)
+    net.set_params(lay.random_params(rng).astype(np.float32))
+    assert net.set_fused(True)
+    for n in (1, 127, 128, 129, 300, 4096, 10001):
+        X = rng.standard_normal((n, 16)).astype(np.float32)
+        net.set_fused(True)
+        Yf = net.predict(X)
+        net.set_fused(False)
+        Yl = net.predict(X)
+        assert np.array_equal(Yf, Yl), f"n={n}: max |diff| {np.abs(Yf - Yl).max()}"
+    # Throughput on a million candidates, both ways.
+    n = 1 << 20
+    dx = DeviceBuffer.from_array(rng.standard_normal((n, 16)).astype(np.float32))
+    dy = DeviceBuffer(n * 8 * 4)
+    for fused in (True, False):
+        net.set_fused(fused)
+        net.forward_device(dx.ptr, n, dy.ptr)
+        day_lib().gpu_device_synchronize()
+        t0 = time.perf_counter()
+        for _ in range(3):
+            net.forward_device(dx.ptr, n, dy.ptr)
+        day_lib().gpu_device_synchronize()
+        print(f"forward of {n} rows, {'fused' if fused else 'layer by layer'}: "
+              f"{(time.perf_counter() - t0) / 3 * 1e3:.2f} ms")
+    dx.free()
+    dy.free()
+    net.set_fused(True)
+    net.destroy()
+
+
 def test_forward_with_big_tiles_matches_oracle(gpu):
     """A batch of 40,000 rows: every layer's product has at least 313 tiles
     of 128x128 and takes the double-buffered kernel."""
