@@ -22,12 +22,27 @@
 // that later power indices read; the candidate scan of each empty slot is
 // split over threads with a merge that keeps the sequential choice (step 5).
 #include <cstdint>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <vector>
 #include <omp.h>
 #include <algorithm>
 
 namespace {
+
+// Wall time of each phase on stderr when AXC_NIGHT_PROFILE is set.
+struct PhaseClock {
+  bool on = std::getenv("AXC_NIGHT_PROFILE") != nullptr;
+  double t = omp_get_wtime();
+  void mark(const char* what) {
+    if (!on) return;
+    const double now = omp_get_wtime();
+    std::fprintf(stderr, "[sprout] %-28s %8.1f ms\n", what, (now - t) * 1e3);
+    t = now;
+  }
+};
 
 constexpr uint32_t kMaxD = 128;
 
@@ -120,14 +135,19 @@ int axc_sprout_pass(uint32_t* targets, int16_t* weights, const uint8_t* flags, c
   const size_t N = padded_n, A = total_axons;
   const size_t ghost_start = N, ghost_end = N + total_ghosts;
   if (ghost_end > A) return -1;
+  PhaseClock clk;
 
   // 1. Liveness and the axon -> soma map (every writer stores the same 1).
+  // Liveness is read only for ghost axons (step 6): a shard without ghosts
+  // skips the scan of its targets.
   std::vector<uint8_t> active(A, 0);
+  if (ghost_end > ghost_start) {
 #pragma omp parallel for schedule(static)
-  for (int64_t k = 0; k < (int64_t)((size_t)kMaxD * N); k++) {
-    if (targets[k] != 0) {
-      uint32_t a = unpack_axon(targets[k]);
-      if (a < A) __atomic_store_n(&active[a], (uint8_t)1, __ATOMIC_RELAXED);
+    for (int64_t k = 0; k < (int64_t)((size_t)kMaxD * N); k++) {
+      if (targets[k] != 0) {
+        uint32_t a = unpack_axon(targets[k]);
+        if (a < A) __atomic_store_n(&active[a], (uint8_t)1, __ATOMIC_RELAXED);
+      }
     }
   }
   std::vector<uint32_t> axon_to_soma(A, 0xFFFFFFFFu);
@@ -136,6 +156,7 @@ int axc_sprout_pass(uint32_t* targets, int16_t* weights, const uint8_t* flags, c
     if (a != 0xFFFFFFFFu && a < A) axon_to_soma[a] = (uint32_t)s;
   }
 
+  clk.mark("liveness, axon owners");
   // 2. Absorb incoming handovers into free ghost slots.
   uint32_t n_acks = 0;
   size_t next_free = ghost_start;
@@ -155,6 +176,7 @@ int axc_sprout_pass(uint32_t* targets, int16_t* weights, const uint8_t* flags, c
     next_free++;
   }
 
+  clk.mark("handovers");
   // 3. Nudge living local axons and every ghost axon.
   uint32_t n_hand = 0;
   auto nudge = [&](size_t a) {
@@ -202,21 +224,26 @@ int axc_sprout_pass(uint32_t* targets, int16_t* weights, const uint8_t* flags, c
   }
   for (size_t a = ghost_start; a < ghost_end; a++) nudge(a);
 
+  clk.mark("nudges");
   // Power index of every soma (sprouting.rs:32-40), explore noise of every
   // axon (random_f32 of seed + axon + epoch, sprouting.rs:280-282), and the
   // somas that sprout candidates (placed, bursting, an empty last slot) in
   // index order.
-  auto power_index = [&](uint32_t soma) -> float {
-    uint32_t pw = 0;
-    for (uint32_t s = 0; s < kMaxD; s++) {
-      int32_t w = weights[(size_t)s * N + soma];
-      pw += (uint32_t)(w < 0 ? -w : w);
-    }
-    return (float)pw / ((float)kMaxD * 32767.0f);
-  };
+  // The power index is (float)sum|w| / (128 * 32767.0f); the integer sums
+  // are kept so that a sprouting soma's index follows from its new weights.
+  auto power_of = [](uint32_t sum_abs) -> float { return (float)sum_abs / ((float)kMaxD * 32767.0f); };
+  std::vector<uint32_t> sum_abs(N);
   std::vector<float> power(N);
 #pragma omp parallel for schedule(static)
-  for (int64_t s = 0; s < (int64_t)N; s++) power[s] = power_index((uint32_t)s);
+  for (int64_t s = 0; s < (int64_t)N; s++) {
+    uint32_t pw = 0;
+    for (uint32_t k = 0; k < kMaxD; k++) {
+      const int32_t w = weights[(size_t)k * N + s];
+      pw += (uint32_t)(w < 0 ? -w : w);
+    }
+    sum_abs[s] = pw;
+    power[s] = power_of(pw);
+  }
   std::vector<uint32_t> sprouters;
   for (size_t i = 0; i < N; i++)
     if (soma_positions[i] != 0 && (flags[i] & 0x0E) != 0 && targets[(size_t)(kMaxD - 1) * N + i] == 0)
@@ -224,14 +251,17 @@ int axc_sprout_pass(uint32_t* targets, int16_t* weights, const uint8_t* flags, c
   std::vector<uint8_t> may_change(N, 0);
   for (uint32_t i : sprouters) may_change[i] = 1;
 
+  clk.mark("power index, sprouters");
   // 4. Segment grid, cells of 2 voxels, axons in reverse order, dense CSR
   // with the insertion order of the reference's per-cell vectors. Built in
   // parallel: thread t takes the t-th contiguous range of axons in reverse
   // order and places its segments after those of threads t' < t in every
-  // cell, which is the sequential insertion order. Only the first segment of
-  // an axon (and type) in a cell is kept: the axon's other segments there
-  // come after it in that cell and score the same (the score depends on the
-  // axon and its type only), so the scan's strict `>` never picks them.
+  // cell, which is the sequential insertion order. A segment is left out
+  // when its axon's previous segment or the one before (same type) lies in
+  // the same cell: such a segment comes after that one in the cell and
+  // scores the same (the score depends on the axon and its type only), so
+  // the scan's strict `>` never picks it. (Later segments of an axon that
+  // re-enters a cell stay in; they are never picked either.)
   const uint32_t cell = 2;
   uint32_t gx = 1, gy = 1, gz = 1;
 #pragma omp parallel for schedule(static) reduction(max : gx, gy, gz)
@@ -244,6 +274,7 @@ int axc_sprout_pass(uint32_t* targets, int16_t* weights, const uint8_t* flags, c
       gz = std::max(gz, pz(p) / cell + 1);
     }
   }
+  clk.mark("grid extent");
   const size_t ncell = (size_t)gx * gy * gz;
   auto cid = [&](uint32_t p) { return ((size_t)(pz(p) / cell) * gy + py(p) / cell) * gx + px(p) / cell; };
   const int nth = omp_get_max_threads();
@@ -251,35 +282,23 @@ int axc_sprout_pass(uint32_t* targets, int16_t* weights, const uint8_t* flags, c
   auto lo_of = [&](int t) { return (size_t)((A * (size_t)t) / (size_t)nth); };
   std::vector<uint32_t> cnt((size_t)nth * ncell, 0);
   std::vector<uint32_t> start(ncell + 1, 0);
-  std::vector<SegRef> refs;
+  // Every entry is written by the fill below: left uninitialized, so that
+  // the filling threads touch its pages first (hundreds of MB at 1.5M
+  // neurons).
+  std::unique_ptr<SegRef[]> refs;
   {
-    // Calls f(segment, cell) for the first segment of axon ai of each (cell,
-    // type), in segment order; `seen` is the caller's table (stamped by axon).
-    constexpr uint32_t kSeen = 1024;
-    struct Seen {
-      uint32_t key[kSeen];
-      uint32_t stamp[kSeen];
-    };
-    auto firsts = [&](size_t ai, Seen& sn, auto&& f) {
-      const uint32_t st = (uint32_t)ai + 1;
+    // Calls f(segment, position, cell) for the segments of axon ai that
+    // enter a (cell, type) other than those of its two previous segments.
+    auto firsts = [&](size_t ai, auto&& f) {
+      uint64_t k1 = ~0ull, k2 = ~0ull;
       for (uint32_t s = 0; s < lengths[ai]; s++) {
         const uint32_t p = paths[ai * 256 + s];
         if (!p) continue;
         const size_t c = cid(p);
-        const uint32_t key = (uint32_t)c * 16u + ptype(p);
-        uint32_t h = (key * 2654435761u) >> 22;
-        bool dup = false;
-        while (sn.stamp[h] == st) {
-          if (sn.key[h] == key) {
-            dup = true;
-            break;
-          }
-          h = (h + 1) & (kSeen - 1);
-        }
-        if (dup) continue;
-        sn.stamp[h] = st;
-        sn.key[h] = key;
-        f(s, p, c);
+        const uint64_t key = (uint64_t)c * 16u + ptype(p);
+        if (key != k1 && key != k2) f(s, p, c);
+        k2 = k1;
+        k1 = key;
       }
     };
     // Per (thread, cell) counts.
@@ -287,12 +306,9 @@ int axc_sprout_pass(uint32_t* targets, int16_t* weights, const uint8_t* flags, c
     {
       const int t = omp_get_thread_num();
       uint32_t* my = cnt.data() + (size_t)t * ncell;
-      Seen* sn = new Seen;
-      std::fill(sn->stamp, sn->stamp + kSeen, 0u);
-      for (size_t k = lo_of(t); k < lo_of(t + 1); k++)
-        firsts(A - 1 - k, *sn, [&](uint32_t, uint32_t, size_t c) { my[c]++; });
-      delete sn;
+      for (size_t k = lo_of(t); k < lo_of(t + 1); k++) firsts(A - 1 - k, [&](uint32_t, uint32_t, size_t c) { my[c]++; });
     }
+    clk.mark("grid: counts");
     // Exclusive offsets: cell-major, thread order within a cell.
     size_t run = 0;
     for (size_t c = 0; c < ncell; c++) {
@@ -304,27 +320,69 @@ int axc_sprout_pass(uint32_t* targets, int16_t* weights, const uint8_t* flags, c
       }
     }
     start[ncell] = (uint32_t)run;
-    refs.resize(run);
+    refs.reset(new SegRef[run ? run : 1]);
+    clk.mark("grid: offsets");
 #pragma omp parallel num_threads(nth)
     {
       const int t = omp_get_thread_num();
       uint32_t* posn = cnt.data() + (size_t)t * ncell;
-      Seen* sn = new Seen;
-      std::fill(sn->stamp, sn->stamp + kSeen, 0u);
       for (size_t k = lo_of(t); k < lo_of(t + 1); k++) {
         const size_t ai = A - 1 - k;
         const uint32_t owner = axon_to_soma[ai];
         const float nz = types ? random_f32(master_seed + (uint64_t)ai + epoch) : 0.0f;
         const float pw = owner == 0xFFFFFFFFu ? 1.0f : power[owner];
         const uint8_t ms = owner == 0xFFFFFFFFu ? 0 : may_change[owner];
-        firsts(ai, *sn, [&](uint32_t s, uint32_t p, size_t c) {
+        firsts(ai, [&](uint32_t s, uint32_t p, size_t c) {
           refs[posn[c]++] = SegRef{(uint32_t)ai, owner, nz, pw, (uint16_t)s, (uint8_t)ptype(p), ms};
         });
       }
-      delete sn;
     }
   }
 
+  clk.mark("grid: fill");
+  // Per cell, the range of what a score reads of its references: explore
+  // noise, presynaptic types, and the owners' power index (separately for
+  // owners that sprout in this pass, whose index may still rise by their
+  // sprouts' weights). Phase A uses them to pass over cells that cannot
+  // change its result (below).
+  struct CellBound {
+    float nmin, nmax, pmin, pmax, qmin, qmax;
+    uint16_t types;
+    uint8_t any_p, any_q;
+  };
+  std::vector<CellBound> cb(ncell);
+#pragma omp parallel for schedule(dynamic, 256)
+  for (int64_t c = 0; c < (int64_t)ncell; c++) {
+    CellBound b{1e30f, -1e30f, 1e30f, -1e30f, 1e30f, -1e30f, 0, 0, 0};
+    for (uint32_t r = start[c]; r < start[c + 1]; r++) {
+      const SegRef& sr = refs[r];
+      b.nmin = std::min(b.nmin, sr.noise);
+      b.nmax = std::max(b.nmax, sr.noise);
+      b.types |= (uint16_t)(1u << (sr.type_idx & 15));
+      if (sr.owner_sprouts) {
+        b.qmin = std::min(b.qmin, sr.pw);
+        b.qmax = std::max(b.qmax, sr.pw);
+        b.any_q = 1;
+      } else {
+        b.pmin = std::min(b.pmin, sr.pw);
+        b.pmax = std::max(b.pmax, sr.pw);
+        b.any_p = 1;
+      }
+    }
+    cb[c] = b;
+  }
+  // The most a sprouting owner's power index can rise in phase B: its
+  // max_sprouts new synapses of at most the largest initial weight, with
+  // room for the f32 rounding of the quotient.
+  uint32_t w0max = 74;
+  if (types) {
+    w0max = 0;
+    for (uint32_t k = 0; k < n_types; k++) w0max = std::max<uint32_t>(w0max, types[k].initial_synapse_weight);
+  }
+  const float power_rise =
+      ((float)max_sprouts * (float)std::min<uint32_t>(w0max, 32768u) + 4.0f) / ((float)kMaxD * 32767.0f) + 1e-6f;
+
+  clk.mark("segment grid");
   // 5. Synaptogenesis. The reference visits the somas in index order, and
   // for each empty slot scans the candidate segments of the 5x5x5 cells
   // around the soma (z, y, x ascending, each cell's segments in grid order)
@@ -404,6 +462,36 @@ int axc_sprout_pass(uint32_t* targets, int16_t* weights, const uint8_t* flags, c
       std::vector<Cand>& lst = listed[(size_t)q];
       uint32_t pos = 0;
       for (size_t c : cells) {
+        // Once the kept list is full, a cell whose every score is at most the
+        // last kept one cannot change the result: its candidates come later
+        // in the scan (equal scores lose), its kept candidates cannot enter,
+        // and a listed one (rescored in phase B) could only be picked by
+        // beating every unpicked kept entry. The bound follows the score's
+        // own f32 steps with each term at its extreme over the cell (each
+        // step is monotone in its operands).
+        if (nt == K) {
+          const CellBound& b = cb[c];
+          float bound = 1.0f;
+          if (cfg) {
+            const float n_ext = cfg->w_explore >= 0.0f ? b.nmax : b.nmin;
+            const bool has_mine = (b.types >> my_type) & 1u, has_other = (b.types & ~(1u << my_type)) != 0;
+            const float same_ext = cfg->w_type >= 0.0f ? (has_mine ? 1.0f : 0.0f) : (has_other ? 0.0f : 1.0f);
+            float p_ext;
+            if (cfg->w_power >= 0.0f)
+              p_ext = std::max(b.any_p ? b.pmax : -1e30f, b.any_q ? b.qmax + power_rise : -1e30f);
+            else
+              p_ext = std::min(b.any_p ? b.pmin : 1e30f, b.any_q ? b.qmin : 1e30f);
+            float acc = cfg->w_distance * 1.0f;
+            acc = acc + cfg->w_explore * n_ext;
+            acc = acc + cfg->w_type * same_ext;
+            acc = acc + cfg->w_power * p_ext;
+            bound = acc;
+          }
+          if (bound <= top[K - 1].score) {
+            pos += start[c + 1] - start[c];
+            continue;
+          }
+        }
         for (uint32_t r = start[c]; r < start[c + 1]; r++, pos++) {
           const SegRef& sr = refs[r];
           const uint32_t a = sr.axon_id;
@@ -451,6 +539,7 @@ int axc_sprout_pass(uint32_t* targets, int16_t* weights, const uint8_t* flags, c
     }
   }
 
+  clk.mark("phase A (parallel scan)");
   uint32_t new_syn = 0;
   std::vector<uint32_t> picked;
   for (size_t q = 0; q < n_spr; q++) {
@@ -493,16 +582,20 @@ int axc_sprout_pass(uint32_t* targets, int16_t* weights, const uint8_t* flags, c
           w0 = (int16_t)types[br.type_idx].initial_synapse_weight;
         }
       }
+      const int32_t w_old = weights[col];
       weights[col] = inhib ? (int16_t)-w0 : w0;
+      const int32_t w_new = weights[col];
+      sum_abs[i] += (uint32_t)(w_new < 0 ? -w_new : w_new) - (uint32_t)(w_old < 0 ? -w_old : w_old);
       picked.push_back(br.axon_id);
       active[br.axon_id] = 1;
       new_syn++;
       sprouts++;
       if (sprouts >= max_sprouts) break;
     }
-    if (sprouts) power[i] = power_index(i);  // this soma's weights changed
+    if (sprouts) power[i] = power_of(sum_abs[i]);  // this soma's weights changed
   }
 
+  clk.mark("phase B (index order)");
   // 6. Orphan ghost axons.
   uint32_t n_prunes = 0;
   for (size_t g = ghost_start; g < ghost_end; g++) {
@@ -514,6 +607,7 @@ int axc_sprout_pass(uint32_t* targets, int16_t* weights, const uint8_t* flags, c
       }
     }
   }
+  clk.mark("orphan ghosts");
   if (out) *out = AxcSproutResult{new_syn, n_hand, n_acks, n_prunes};
   return 0;
 }
