@@ -28,6 +28,8 @@
 #include <memory>
 #include <vector>
 #include <omp.h>
+#include <sys/mman.h>
+#include <new>
 #include <algorithm>
 
 namespace {
@@ -68,19 +70,38 @@ inline uint32_t unpack_axon(uint32_t t) {  // layout.rs:268-271 (saturating -1)
   return a ? a - 1 : 0;
 }
 
-// One candidate segment of the grid, with what scoring it needs of its
-// axon inline: the owner soma (or ~0), the axon's explore noise and its
-// owner's power index at the start of the pass, and whether that owner
-// sprouts in this pass (so its power index may change).
+// One candidate segment of the grid (8 B): its axon (24 bits) and segment
+// index (8 bits); its type, a flag for an owner soma that sprouts in this
+// pass (whose power index may still change), a flag for an axon without an
+// owner (power index 1.0), and the owner's sum of |weights| at the start of
+// the pass (at most 128 * 32768 = 2^22), from which its power index follows.
 struct SegRef {
-  uint32_t axon_id;
-  uint32_t owner;
-  float noise;
-  float pw;
-  uint16_t seg_idx;
-  uint8_t type_idx;
-  uint8_t owner_sprouts;
+  static constexpr uint32_t kOwnerSprouts = 0x10, kNoOwner = 0x20;
+  uint32_t axon_seg;
+  uint32_t meta;  // type | flags | sum_abs << 8
+  uint32_t axon() const { return axon_seg & 0x00FFFFFFu; }
+  uint32_t seg() const { return axon_seg >> 24; }
+  uint32_t type() const { return meta & 15u; }
+  bool owner_sprouts() const { return (meta & kOwnerSprouts) != 0; }
+  bool no_owner() const { return (meta & kNoOwner) != 0; }
+  uint32_t sum_abs() const { return meta >> 8; }
 };
+static_assert(sizeof(SegRef) == 8, "grid references are 8 B");
+
+// Large host arrays on transparent huge pages where the kernel grants them
+// (fewer first-touch faults for the grid's hundreds of MB).
+struct FreeDeleter {
+  void operator()(void* p) const { std::free(p); }
+};
+template <class T>
+std::unique_ptr<T[], FreeDeleter> huge_array(size_t n) {
+  constexpr size_t kHuge = size_t(2) << 20;
+  const size_t bytes = ((n ? n : 1) * sizeof(T) + kHuge - 1) / kHuge * kHuge;
+  void* p = std::aligned_alloc(kHuge, bytes);
+  if (!p) throw std::bad_alloc();
+  madvise(p, bytes, MADV_HUGEPAGE);
+  return std::unique_ptr<T[], FreeDeleter>(static_cast<T*>(p));
+}
 
 }  // namespace
 
@@ -135,6 +156,7 @@ int axc_sprout_pass(uint32_t* targets, int16_t* weights, const uint8_t* flags, c
   const size_t N = padded_n, A = total_axons;
   const size_t ghost_start = N, ghost_end = N + total_ghosts;
   if (ghost_end > A) return -1;
+  if (A >= (1u << 24)) return -1;  // a target holds axon + 1 in 24 bits (layout.rs:268-271)
   PhaseClock clk;
 
   // 1. Liveness and the axon -> soma map (every writer stores the same 1).
@@ -262,89 +284,124 @@ int axc_sprout_pass(uint32_t* targets, int16_t* weights, const uint8_t* flags, c
   // scores the same (the score depends on the axon and its type only), so
   // the scan's strict `>` never picks it. (Later segments of an axon that
   // re-enters a cell stay in; they are never picked either.)
+  // Each thread walks its axons' paths once: it counts per cell and appends
+  // (cell, reference) to a list of its own in walk order; the lists are then
+  // scattered into the grid.
   const uint32_t cell = 2;
-  uint32_t gx = 1, gy = 1, gz = 1;
-#pragma omp parallel for schedule(static) reduction(max : gx, gy, gz)
-  for (int64_t a = 0; a < (int64_t)A; a++) {
-    for (uint32_t s = 0; s < lengths[a]; s++) {
-      uint32_t p = paths[a * 256 + s];
-      if (!p) continue;
-      gx = std::max(gx, px(p) / cell + 1);
-      gy = std::max(gy, py(p) / cell + 1);
-      gz = std::max(gz, pz(p) / cell + 1);
-    }
-  }
-  clk.mark("grid extent");
-  const size_t ncell = (size_t)gx * gy * gz;
-  auto cid = [&](uint32_t p) { return ((size_t)(pz(p) / cell) * gy + py(p) / cell) * gx + px(p) / cell; };
+  // What a score reads of an axon: its explore noise (random_f32 of seed +
+  // axon + epoch, sprouting.rs:280-282) and its owner's power index at the
+  // start of the pass (1.0 without an owner, sprouting.rs:285-290).
+  auto noise_of = [&](uint32_t a) { return types ? random_f32(master_seed + (uint64_t)a + epoch) : 0.0f; };
+  auto pw_of = [&](SegRef r) { return r.no_owner() ? 1.0f : power_of(r.sum_abs()); };
   const int nth = omp_get_max_threads();
   // Axon range of thread t in reverse order: axons [A - hi(t), A - lo(t)).
   auto lo_of = [&](int t) { return (size_t)((A * (size_t)t) / (size_t)nth); };
+  std::vector<std::unique_ptr<uint32_t[], FreeDeleter>> t_cell(nth);
+  std::vector<std::unique_ptr<SegRef[], FreeDeleter>> t_ref(nth);
+  std::vector<size_t> t_len(nth, 0);
+  // Every grid entry is written by the scatter below: left uninitialized,
+  // so that the scattering threads touch its pages first (hundreds of MB at
+  // 1.5M neurons).
+  std::unique_ptr<SegRef[], FreeDeleter> refs;
+  // The walk stores each kept segment's cell as z/2 << 18 | y/2 << 9 | x/2
+  // (the grid's extent is known only after it) and the greatest cell
+  // coordinates; the count pass turns the keys into dense cell ids.
+  std::vector<uint32_t> t_max(3 * (size_t)nth, 0);
+#pragma omp parallel num_threads(nth)
+  {
+    const int t = omp_get_thread_num();
+    size_t cap = 1;  // the branch-free walk writes one entry past the last kept one
+    for (size_t k = lo_of(t); k < lo_of(t + 1); k++) cap += lengths[A - 1 - k];
+    t_cell[t] = huge_array<uint32_t>(cap);
+    t_ref[t] = huge_array<SegRef>(cap);
+    uint32_t* oc = t_cell[t].get();
+    SegRef* orf = t_ref[t].get();
+    size_t n = 0;
+    uint32_t mx = 0, my = 0, mz = 0;
+    for (size_t k = lo_of(t); k < lo_of(t + 1); k++) {
+      const uint32_t ai = (uint32_t)(A - 1 - k);
+      const uint32_t owner = axon_to_soma[ai];
+      const uint32_t meta = owner == 0xFFFFFFFFu
+                                ? SegRef::kNoOwner
+                                : (sum_abs[owner] << 8) | (may_change[owner] ? SegRef::kOwnerSprouts : 0u);
+      const uint32_t* path = paths + (size_t)ai * 256;
+      uint32_t k1 = ~0u, k2 = ~0u;
+      for (uint32_t s = 0, len = lengths[ai]; s < len; s++) {
+        const uint32_t p = path[s];
+        const uint32_t x = px(p) / cell, y = py(p) / cell, z = pz(p) / cell;
+        const uint32_t fk = (z << 18) | (y << 9) | x;
+        const uint32_t key = (fk << 4) | ptype(p);
+        const bool live = p != 0;
+        oc[n] = fk;
+        orf[n] = SegRef{ai | (s << 24), meta | ptype(p)};
+        n += (size_t)(live & (key != k1) & (key != k2));
+        mx = live ? std::max(mx, x) : mx;
+        my = live ? std::max(my, y) : my;
+        mz = live ? std::max(mz, z) : mz;
+        k2 = live ? k1 : k2;
+        k1 = live ? key : k1;
+      }
+    }
+    t_len[t] = n;
+    t_max[3 * (size_t)t] = mx;
+    t_max[3 * (size_t)t + 1] = my;
+    t_max[3 * (size_t)t + 2] = mz;
+  }
+  clk.mark("grid: walk");
+  uint32_t gx = 1, gy = 1, gz = 1;
+  for (int t = 0; t < nth; t++)
+    if (t_len[t]) {
+      gx = std::max(gx, t_max[3 * (size_t)t] + 1);
+      gy = std::max(gy, t_max[3 * (size_t)t + 1] + 1);
+      gz = std::max(gz, t_max[3 * (size_t)t + 2] + 1);
+    }
+  const size_t ncell = (size_t)gx * gy * gz;
   std::vector<uint32_t> cnt((size_t)nth * ncell, 0);
   std::vector<uint32_t> start(ncell + 1, 0);
-  // Every entry is written by the fill below: left uninitialized, so that
-  // the filling threads touch its pages first (hundreds of MB at 1.5M
-  // neurons).
-  std::unique_ptr<SegRef[]> refs;
+#pragma omp parallel num_threads(nth)
   {
-    // Calls f(segment, position, cell) for the segments of axon ai that
-    // enter a (cell, type) other than those of its two previous segments.
-    auto firsts = [&](size_t ai, auto&& f) {
-      uint64_t k1 = ~0ull, k2 = ~0ull;
-      for (uint32_t s = 0; s < lengths[ai]; s++) {
-        const uint32_t p = paths[ai * 256 + s];
-        if (!p) continue;
-        const size_t c = cid(p);
-        const uint64_t key = (uint64_t)c * 16u + ptype(p);
-        if (key != k1 && key != k2) f(s, p, c);
-        k2 = k1;
-        k1 = key;
-      }
-    };
-    // Per (thread, cell) counts.
-#pragma omp parallel num_threads(nth)
-    {
-      const int t = omp_get_thread_num();
-      uint32_t* my = cnt.data() + (size_t)t * ncell;
-      for (size_t k = lo_of(t); k < lo_of(t + 1); k++) firsts(A - 1 - k, [&](uint32_t, uint32_t, size_t c) { my[c]++; });
-    }
-    clk.mark("grid: counts");
-    // Exclusive offsets: cell-major, thread order within a cell.
-    size_t run = 0;
-    for (size_t c = 0; c < ncell; c++) {
-      start[c] = (uint32_t)run;
-      for (int t = 0; t < nth; t++) {
-        const uint32_t k = cnt[(size_t)t * ncell + c];
-        cnt[(size_t)t * ncell + c] = (uint32_t)run;
-        run += k;
-      }
-    }
-    start[ncell] = (uint32_t)run;
-    refs.reset(new SegRef[run ? run : 1]);
-    clk.mark("grid: offsets");
-#pragma omp parallel num_threads(nth)
-    {
-      const int t = omp_get_thread_num();
-      uint32_t* posn = cnt.data() + (size_t)t * ncell;
-      for (size_t k = lo_of(t); k < lo_of(t + 1); k++) {
-        const size_t ai = A - 1 - k;
-        const uint32_t owner = axon_to_soma[ai];
-        const float nz = types ? random_f32(master_seed + (uint64_t)ai + epoch) : 0.0f;
-        const float pw = owner == 0xFFFFFFFFu ? 1.0f : power[owner];
-        const uint8_t ms = owner == 0xFFFFFFFFu ? 0 : may_change[owner];
-        firsts(ai, [&](uint32_t s, uint32_t p, size_t c) {
-          refs[posn[c]++] = SegRef{(uint32_t)ai, owner, nz, pw, (uint16_t)s, (uint8_t)ptype(p), ms};
-        });
-      }
+    const int t = omp_get_thread_num();
+    uint32_t* mine = cnt.data() + (size_t)t * ncell;
+    uint32_t* oc = t_cell[t].get();
+    for (size_t k = 0; k < t_len[t]; k++) {
+      const uint32_t fk = oc[k];
+      const uint32_t c = (uint32_t)(((size_t)(fk >> 18) * gy + ((fk >> 9) & 511u)) * gx + (fk & 511u));
+      oc[k] = c;
+      mine[c]++;
     }
   }
-
-  clk.mark("grid: fill");
-  // Per cell, the range of what a score reads of its references: explore
-  // noise, presynaptic types, and the owners' power index (separately for
-  // owners that sprout in this pass, whose index may still rise by their
-  // sprouts' weights). Phase A uses them to pass over cells that cannot
-  // change its result (below).
+  clk.mark("grid: counts");
+  // Exclusive offsets: cell-major, thread order within a cell.
+  size_t run = 0;
+  for (size_t c = 0; c < ncell; c++) {
+    start[c] = (uint32_t)run;
+    for (int t = 0; t < nth; t++) {
+      const uint32_t k = cnt[(size_t)t * ncell + c];
+      cnt[(size_t)t * ncell + c] = (uint32_t)run;
+      run += k;
+    }
+  }
+  start[ncell] = (uint32_t)run;
+  refs = huge_array<SegRef>(run);
+  clk.mark("grid: offsets");
+#pragma omp parallel num_threads(nth)
+  {
+    const int t = omp_get_thread_num();
+    uint32_t* posn = cnt.data() + (size_t)t * ncell;
+    const uint32_t* oc = t_cell[t].get();
+    const SegRef* orf = t_ref[t].get();
+    for (size_t k = 0; k < t_len[t]; k++) refs[posn[oc[k]]++] = orf[k];
+    t_cell[t].reset();
+    t_ref[t].reset();
+  }
+  clk.mark("grid: scatter");
+  // Per cell, bounds on what a score reads of its references: explore
+  // noise (random_f32 is nondecreasing in seed + axon + epoch, so the
+  // cell's least and greatest axon bound it unless that sum wraps between
+  // them), power index (separately for owners that sprout in this pass,
+  // whose index may still rise by their sprouts' weights; power_of is
+  // nondecreasing in the sum) and the presynaptic types. Phase A uses them
+  // to pass over cells that cannot change its result (below).
   struct CellBound {
     float nmin, nmax, pmin, pmax, qmin, qmax;
     uint16_t types;
@@ -353,21 +410,43 @@ int axc_sprout_pass(uint32_t* targets, int16_t* weights, const uint8_t* flags, c
   std::vector<CellBound> cb(ncell);
 #pragma omp parallel for schedule(dynamic, 256)
   for (int64_t c = 0; c < (int64_t)ncell; c++) {
-    CellBound b{1e30f, -1e30f, 1e30f, -1e30f, 1e30f, -1e30f, 0, 0, 0};
+    uint32_t a_lo = ~0u, a_hi = 0, p_lo = ~0u, p_hi = 0, q_lo = ~0u, q_hi = 0, tmask = 0, meta_or = 0;
     for (uint32_t r = start[c]; r < start[c + 1]; r++) {
-      const SegRef& sr = refs[r];
-      b.nmin = std::min(b.nmin, sr.noise);
-      b.nmax = std::max(b.nmax, sr.noise);
-      b.types |= (uint16_t)(1u << (sr.type_idx & 15));
-      if (sr.owner_sprouts) {
-        b.qmin = std::min(b.qmin, sr.pw);
-        b.qmax = std::max(b.qmax, sr.pw);
-        b.any_q = 1;
-      } else {
-        b.pmin = std::min(b.pmin, sr.pw);
-        b.pmax = std::max(b.pmax, sr.pw);
-        b.any_p = 1;
+      const SegRef sr = refs[r];
+      a_lo = std::min(a_lo, sr.axon());
+      a_hi = std::max(a_hi, sr.axon());
+      tmask |= 1u << sr.type();
+      meta_or |= sr.meta;
+      if (sr.owner_sprouts()) {
+        q_lo = std::min(q_lo, sr.sum_abs());
+        q_hi = std::max(q_hi, sr.sum_abs());
+      } else if (!sr.no_owner()) {
+        p_lo = std::min(p_lo, sr.sum_abs());
+        p_hi = std::max(p_hi, sr.sum_abs());
       }
+    }
+    CellBound b{0.0f, 1.0f, 1e30f, -1e30f, 1e30f, -1e30f, (uint16_t)tmask, 0, 0};
+    if (start[c + 1] > start[c]) {
+      const uint64_t x_lo = master_seed + (uint64_t)a_lo + epoch, x_hi = master_seed + (uint64_t)a_hi + epoch;
+      if (x_lo <= x_hi) {
+        b.nmin = noise_of(a_lo);
+        b.nmax = noise_of(a_hi);
+      }
+    }
+    if (p_lo <= p_hi) {
+      b.pmin = power_of(p_lo);
+      b.pmax = power_of(p_hi);
+      b.any_p = 1;
+    }
+    if (meta_or & SegRef::kNoOwner) {
+      b.pmin = std::min(b.pmin, 1.0f);
+      b.pmax = std::max(b.pmax, 1.0f);
+      b.any_p = 1;
+    }
+    if (q_lo <= q_hi) {
+      b.qmin = power_of(q_lo);
+      b.qmax = power_of(q_hi);
+      b.any_q = 1;
     }
     cb[c] = b;
   }
@@ -404,9 +483,10 @@ int axc_sprout_pass(uint32_t* targets, int16_t* weights, const uint8_t* flags, c
   // The soma's present axons are found in a small hash set.
   const uint32_t K = max_sprouts > 0 ? max_sprouts : 1u;
   struct Cand {
-    float score;   // full score (kept candidates) or the part before the power term (listed ones)
-    uint32_t pos;  // scan position: earlier wins a tie
+    float score;     // full score (kept candidates) or the part before the power term (listed ones)
+    uint32_t pos;    // scan position: earlier wins a tie
     SegRef ref;
+    uint32_t owner;  // the owner soma (listed candidates)
   };
   auto better = [](float s1, uint32_t p1, float s2, uint32_t p2) { return s1 > s2 || (s1 == s2 && p1 < p2); };
   const size_t n_spr = sprouters.size();
@@ -493,21 +573,24 @@ int axc_sprout_pass(uint32_t* targets, int16_t* weights, const uint8_t* flags, c
           }
         }
         for (uint32_t r = start[c]; r < start[c + 1]; r++, pos++) {
-          const SegRef& sr = refs[r];
-          const uint32_t a = sr.axon_id;
+          const SegRef sr = refs[r];
+          const uint32_t a = sr.axon();
           float score = 1.0f;
           if (cfg) {
             // Left to right, products first, no fused multiply-add (the
             // library is built with -ffp-contract=off).
-            const float same = (my_type == sr.type_idx) ? 1.0f : 0.0f;
+            const float same = (my_type == sr.type()) ? 1.0f : 0.0f;
             float acc = cfg->w_distance * 1.0f;
-            acc = acc + cfg->w_explore * sr.noise;
+            acc = acc + cfg->w_explore * noise_of(a);
             acc = acc + cfg->w_type * same;
-            if (sr.owner_sprouts && sr.owner < i) {  // rescored in phase B
-              if (a != my_axon && !present(a)) lst.push_back(Cand{acc, pos, sr});
-              continue;
+            if (sr.owner_sprouts()) {
+              const uint32_t owner = axon_to_soma[a];
+              if (owner < i) {  // rescored in phase B
+                if (a != my_axon && !present(a)) lst.push_back(Cand{acc, pos, sr, owner});
+                continue;
+              }
             }
-            acc = acc + cfg->w_power * sr.pw;
+            acc = acc + cfg->w_power * pw_of(sr);
             score = acc;
           }
           // The scan starts from a best score of -1: lower (or NaN) never wins;
@@ -519,7 +602,7 @@ int axc_sprout_pass(uint32_t* targets, int16_t* weights, const uint8_t* flags, c
           // The axon's best reference so far replaces its entry; a new axon
           // enters if the list has room or it beats the last entry.
           uint32_t k = 0;
-          while (k < nt && top[k].ref.axon_id != a) k++;
+          while (k < nt && top[k].ref.axon() != a) k++;
           if (k < nt) {
             if (!better(score, pos, top[k].score, top[k].pos)) continue;
           } else if (nt < K) {
@@ -528,7 +611,7 @@ int axc_sprout_pass(uint32_t* targets, int16_t* weights, const uint8_t* flags, c
             if (!better(score, pos, top[K - 1].score, top[K - 1].pos)) continue;
             k = K - 1;
           }
-          top[k] = Cand{score, pos, sr};
+          top[k] = Cand{score, pos, sr, 0};
           while (k > 0 && better(top[k].score, top[k].pos, top[k - 1].score, top[k - 1].pos)) {
             std::swap(top[k], top[k - 1]);
             k--;
@@ -557,14 +640,14 @@ int axc_sprout_pass(uint32_t* targets, int16_t* weights, const uint8_t* flags, c
       const Cand* best = nullptr;
       float best_score = -1.0f;
       for (uint32_t k = 0; k < n_kept[q]; k++) {
-        if (is_picked(top[k].ref.axon_id)) continue;
+        if (is_picked(top[k].ref.axon())) continue;
         best = &top[k];  // the kept list is ranked: its first unpicked entry is its best
         best_score = top[k].score;
         break;
       }
       for (const Cand& c : lst) {
-        if (is_picked(c.ref.axon_id)) continue;
-        const float sc = c.score + cfg->w_power * power[c.ref.owner];
+        if (is_picked(c.ref.axon())) continue;
+        const float sc = c.score + cfg->w_power * power[c.owner];
         if (!(sc > -1.0f)) continue;
         if (!best || better(sc, c.pos, best_score, best->pos)) {
           best = &c;
@@ -573,21 +656,22 @@ int axc_sprout_pass(uint32_t* targets, int16_t* weights, const uint8_t* flags, c
       }
       if (!best) break;
       const SegRef br = best->ref;
-      targets[col] = ((uint32_t)br.seg_idx << 24) | ((br.axon_id + 1u) & 0x00FFFFFFu);
+      const uint32_t b_axon = br.axon(), b_type = br.type();
+      targets[col] = (br.seg() << 24) | ((b_axon + 1u) & 0x00FFFFFFu);
       bool inhib = false;
       int16_t w0 = 74;
       if (types) {
-        if (br.type_idx < n_types) {
-          inhib = types[br.type_idx].is_inhibitory != 0;
-          w0 = (int16_t)types[br.type_idx].initial_synapse_weight;
+        if (b_type < n_types) {
+          inhib = types[b_type].is_inhibitory != 0;
+          w0 = (int16_t)types[b_type].initial_synapse_weight;
         }
       }
       const int32_t w_old = weights[col];
       weights[col] = inhib ? (int16_t)-w0 : w0;
       const int32_t w_new = weights[col];
       sum_abs[i] += (uint32_t)(w_new < 0 ? -w_new : w_new) - (uint32_t)(w_old < 0 ? -w_old : w_old);
-      picked.push_back(br.axon_id);
-      active[br.axon_id] = 1;
+      picked.push_back(b_axon);
+      active[b_axon] = 1;
       new_syn++;
       sprouts++;
       if (sprouts >= max_sprouts) break;

@@ -13,7 +13,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from ._ffi import day_lib, gen_lib, ptr
-from .engine import GpuError, VramState
+from .engine import GpuError, PinnedBuffer, VramState
 from .layout import MAX_DENDRITES
 
 HANDOVER_DTYPE = np.dtype([("origin_zone_hash", "<u4"), ("local_axon_id", "<u4"), ("entry_x", "<u2"),
@@ -95,29 +95,59 @@ def sprout_pass(targets: np.ndarray, weights: np.ndarray, flags: np.ndarray, som
     return SproutResult(r.new_synapses, hand[:r.handovers_out].copy(), acks[:r.acks].copy(), prunes[:r.prunes].copy())
 
 
+class NightStaging:
+    """Page-locked host copies of one shard's flags, soma-to-axon map,
+    weights and targets, for the Night's downloads and upload: allocated once
+    per shard and reused every Night (pinning 1.2 GB at 1.5M neurons costs
+    more than a Night's copies, which page-locked memory runs at the link's
+    full rate)."""
+
+    def __init__(self, padded_n: int):
+        n = int(padded_n)
+        self.padded_n = n
+        self._bufs = [PinnedBuffer(n, np.uint8), PinnedBuffer(n, np.uint32),
+                      PinnedBuffer(MAX_DENDRITES * n, np.int16), PinnedBuffer(MAX_DENDRITES * n, np.uint32)]
+        self.flags = self._bufs[0].array
+        self.soma_to_axon = self._bufs[1].array
+        self.weights = self._bufs[2].array.reshape(MAX_DENDRITES, n)
+        self.targets = self._bufs[3].array.reshape(MAX_DENDRITES, n)
+        # The host maps page-locked pages on their first CPU access: touch
+        # them here, not in the first Night's sprouting pass.
+        for b in self._bufs:
+            b.array.fill(0)
+
+    @staticmethod
+    def of(vram: VramState) -> "NightStaging":
+        """The shard's staging buffers, made on its first Night."""
+        st = getattr(vram, "_night_staging", None)
+        if st is None or st.padded_n != vram.padded_n:
+            st = NightStaging(vram.padded_n)
+            vram._night_staging = st
+        return st
+
+
 def execute_night_phase(vram: VramState, prune_threshold: int, geo: Geometry | None, types: np.ndarray | None,
                         epoch: int, master_seed: int, max_sprouts: int = 4) -> SproutResult | None:
     """shard_thread.rs:269-378 on one shard; returns the sprouting result (None
-    when no geometry is given: sort & prune only, as when no daemon is up)."""
+    when no geometry is given: sort & prune only, as when no daemon is up).
+    The host copies live in the shard's NightStaging."""
     lib = day_lib()
     N = vram.padded_n
-    flags = np.empty(N, np.uint8)
+    st = NightStaging.of(vram)
     lib.gpu_device_synchronize()
-    if not lib.gpu_memcpy_device_to_host(ptr(flags), vram.ptrs.soma_flags, N):
+    if not lib.gpu_memcpy_device_to_host(ptr(st.flags), vram.ptrs.soma_flags, N):
         raise GpuError("flags download failed")
     lib.launch_sort_and_prune(C.byref(vram.ptrs), N, prune_threshold)
     lib.gpu_device_synchronize()
     if geo is None:
         return None
-    weights = np.empty((MAX_DENDRITES, N), np.int16)
-    targets = np.empty((MAX_DENDRITES, N), np.uint32)
-    s2a = np.empty(N, np.uint32)
-    for dst, src in ((weights, vram.ptrs.dendrite_weights), (targets, vram.ptrs.dendrite_targets),
-                     (s2a, vram.ptrs.soma_to_axon)):
+    for dst, src in ((st.weights, vram.ptrs.dendrite_weights), (st.targets, vram.ptrs.dendrite_targets),
+                     (st.soma_to_axon, vram.ptrs.soma_to_axon)):
         if not lib.gpu_memcpy_device_to_host(ptr(dst), src, dst.nbytes):
             raise GpuError("night download failed")
-    res = sprout_pass(targets, weights, flags, s2a, N, geo, types, epoch, master_seed, max_sprouts=max_sprouts)
-    for src, dst in ((targets, vram.ptrs.dendrite_targets), (weights, vram.ptrs.dendrite_weights)):
+    res = sprout_pass(st.targets, st.weights, st.flags, st.soma_to_axon, N, geo, types, epoch, master_seed,
+                      max_sprouts=max_sprouts)
+    for src, dst in ((st.targets, vram.ptrs.dendrite_targets), (st.weights, vram.ptrs.dendrite_weights)):
         if not lib.gpu_memcpy_host_to_device(dst, ptr(src), src.nbytes):
             raise GpuError("night upload failed")
     return res

@@ -105,16 +105,15 @@ def bench_nights(lib, brain, vram, eng, batch, nights: int, prune: int) -> dict:
     phase, averaged."""
     import ctypes as C
     from axicor_amd import night as NT
-    from axicor_amd.layout import MAX_DENDRITES
     from axicor_amd._ffi import ptr
     N = brain.padded_n
     types = NT.sprout_types_from_lut(brain.lut)
     geo = NT.geometry_of(brain)
     ph = {"prune_s": 0.0, "download_s": 0.0, "sprout_s": 0.0, "upload_s": 0.0}
     new_syn = 0
-    flags = np.empty(N, np.uint8)
-    w = np.empty((MAX_DENDRITES, N), np.int16)
-    t = np.empty((MAX_DENDRITES, N), np.uint32)
+    # Page-locked host copies, allocated once per shard before its first Night.
+    st = NT.NightStaging.of(vram)
+    flags, w, t, s2a = st.flags, st.weights, st.targets, st.soma_to_axon
     for k in range(nights):
         batch(50)
         eng.synchronize()
@@ -125,9 +124,9 @@ def bench_nights(lib, brain, vram, eng, batch, nights: int, prune: int) -> dict:
         t1 = time.perf_counter()
         lib.gpu_memcpy_device_to_host(ptr(w), vram.ptrs.dendrite_weights, w.nbytes)
         lib.gpu_memcpy_device_to_host(ptr(t), vram.ptrs.dendrite_targets, t.nbytes)
+        lib.gpu_memcpy_device_to_host(ptr(s2a), vram.ptrs.soma_to_axon, s2a.nbytes)
         t2 = time.perf_counter()
-        res = NT.sprout_pass(t, w, flags, np.ascontiguousarray(brain.view().soma_to_axon), N, geo, types, k,
-                             brain.spec.seed)
+        res = NT.sprout_pass(t, w, flags, s2a, N, geo, types, k, brain.spec.seed)
         t3 = time.perf_counter()
         lib.gpu_memcpy_host_to_device(vram.ptrs.dendrite_targets, ptr(t), t.nbytes)
         lib.gpu_memcpy_host_to_device(vram.ptrs.dendrite_weights, ptr(w), w.nbytes)
