@@ -291,7 +291,7 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--cpu-max-ticks", type=int, default=20)
     ap.add_argument("--cpu-single-ticks", type=int, default=3, help="ticks of the one-thread CPU oracle run")
-    ap.add_argument("--nights", type=int, default=0,
+    ap.add_argument("--nights", type=int, default=2,
                     help="after the timed steps, this many Nights (GPU sort & prune, host sprouting), "
                          "each after one more batch of --ticks ticks, timed phase by phase")
     ap.add_argument("--prune", type=int, default=15, help="prune threshold of the Nights")
