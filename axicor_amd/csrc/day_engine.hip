@@ -403,9 +403,11 @@ bool ensure_stage(Shard* s, const ShardVramPtrs& v, hipStream_t st, hipError_t* 
     std::call_once(attr_once, [] {
       const int bytes = (int)((kStageCap + 1) * sizeof(uint2));
       for (const void* f : {(const void*)neuron_staged_kernel<false>, (const void*)neuron_staged_kernel<true>,
-                            (const void*)batch_staged_kernel, (const void*)win_pass_kernel<1>,
-                            (const void*)win_pass_kernel<2>, (const void*)win_pass_kernel<3>,
-                            (const void*)win_pass_kernel<4>})
+                            (const void*)batch_staged_kernel, (const void*)win_pass_kernel<1, false>,
+                            (const void*)win_pass_kernel<2, false>, (const void*)win_pass_kernel<3, false>,
+                            (const void*)win_pass_kernel<4, false>, (const void*)win_pass_kernel<1, true>,
+                            (const void*)win_pass_kernel<2, true>, (const void*)win_pass_kernel<3, true>,
+                            (const void*)win_pass_kernel<4, true>})
         (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
     });
     StagePlan p{};
@@ -779,10 +781,18 @@ hipError_t enqueue_windows(const ShardVramPtrs& v, Shard* s, uint32_t v_seg, uin
       TimedScope ts(kKWinPass, st, timed);
       const uint32_t g = staged_grid(N);
       const uint32_t one2 = 0x00010001u;
-      if (K == 1) win_pass_kernel<1><<<g, kStageThreads, lds, st>>>(v, N, A, lut, s->sp, w, s->dirty, v_seg, one2);
-      else if (K == 2) win_pass_kernel<2><<<g, kStageThreads, lds, st>>>(v, N, A, lut, s->sp, w, s->dirty, v_seg, one2);
-      else if (K == 3) win_pass_kernel<3><<<g, kStageThreads, lds, st>>>(v, N, A, lut, s->sp, w, s->dirty, v_seg, one2);
-      else win_pass_kernel<4><<<g, kStageThreads, lds, st>>>(v, N, A, lut, s->sp, w, s->dirty, v_seg, one2);
+      auto pass = [&](auto kern) { kern<<<g, kStageThreads, lds, st>>>(v, N, A, lut, s->sp, w, s->dirty, v_seg, one2); };
+      if (N <= kBufMaxSomas) {
+        if (K == 1) pass(win_pass_kernel<1, true>);
+        else if (K == 2) pass(win_pass_kernel<2, true>);
+        else if (K == 3) pass(win_pass_kernel<3, true>);
+        else pass(win_pass_kernel<4, true>);
+      } else {
+        if (K == 1) pass(win_pass_kernel<1, false>);
+        else if (K == 2) pass(win_pass_kernel<2, false>);
+        else if (K == 3) pass(win_pass_kernel<3, false>);
+        else pass(win_pass_kernel<4, false>);
+      }
     }
     for (uint32_t j = 0; j < K; j++) {
       const uint32_t t = t0 + j;

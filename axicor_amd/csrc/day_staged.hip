@@ -191,6 +191,9 @@ __device__ __forceinline__ uint4 snap_at(const uint4* __restrict__ snap, uint32_
 // the 8 weights, two per register: PW, from the plan's working copy (one
 // 16-byte load); otherwise from the 8 reference rows.
 typedef uint32_t v4u __attribute__((ext_vector_type(4)));
+// The most somas whose [16][N][8] plan arrays (16 B per chunk) stay within
+// the 32-bit offsets of Chunk::load_buf.
+constexpr uint32_t kBufMaxSomas = 1u << 24;
 typedef uint32_t v2u __attribute__((ext_vector_type(2)));
 
 __device__ __forceinline__ size_t chunk_at(uint32_t N, uint32_t c, uint32_t i) { return ((size_t)c * N + i) * 8; }
@@ -228,6 +231,24 @@ struct Chunk {
                 ((uint32_t)__builtin_nontemporal_load(reinterpret_cast<const uint16_t*>(r1 + i * 2u)) << 16);
       }
     }
+  }
+  // load() of the staged form with the working copy (STAGED && PW) through
+  // buffer loads: the chunk's row offset in a scalar register (c is the same
+  // in every lane of the wave) and one 32-bit offset per lane, so that the
+  // loop keeps no 64-bit address per lane and array. Every [16][N][8] array
+  // must lie within 2^32 bytes (kBufMaxSomas).
+  __device__ __forceinline__ void load_buf(const StagePlan& sp, uint32_t N, uint32_t c, uint32_t i) {
+    static_assert(STAGED && PW, "the staged form with the working copy");
+    constexpr int kFlags = 0x00020000, kNt = 2;  // raw buffer (gfx9 word 3); nontemporal
+    const __amdgpu_buffer_rsrc_t rc = __builtin_amdgcn_make_buffer_rsrc(sp.code, (short)0, -1, kFlags);
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(sp.seg, (short)0, -1, kFlags);
+    const __amdgpu_buffer_rsrc_t rw = __builtin_amdgcn_make_buffer_rsrc(sp.w, (short)0, -1, kFlags);
+    const uint32_t row8 = __builtin_amdgcn_readfirstlane(c * N * 8u);
+    const v4u e = __builtin_bit_cast(v4u, __builtin_amdgcn_raw_buffer_load_b128(rc, i * 16u, row8 * 2u, kNt));
+    const v2u g = __builtin_bit_cast(v2u, __builtin_amdgcn_raw_buffer_load_b64(rs, i * 8u, row8, kNt));
+    const v4u wv = __builtin_bit_cast(v4u, __builtin_amdgcn_raw_buffer_load_b128(rw, i * 16u, row8 * 2u, kNt));
+    k[0] = e.x; k[1] = e.y; k[2] = e.z; k[3] = e.w; k[4] = g.x; k[5] = g.y;
+    w2[0] = wv.x; w2[1] = wv.y; w2[2] = wv.z; w2[3] = wv.w;
   }
   __device__ __forceinline__ uint32_t seg(int u) const {
     return STAGED ? ((k[4 + (u >> 2)] >> (8 * (u & 3))) & 0xFFu) : (k[u] >> 24);

@@ -634,7 +634,9 @@ __device__ __forceinline__ void win_chunk(const Chunk<STAGED, true>& now, uint32
 // Workgroup 0 also starts the window's bookkeeping: a new stamp (every aux
 // record turns stale), the piece counters cleared (nothing reads them during
 // the pass).
-template <uint32_t K>
+// BUF: the staged chunks through buffer loads (Chunk::load_buf; shards of at
+// most kBufMaxSomas somas).
+template <uint32_t K, bool BUF>
 __global__ __launch_bounds__(kStageThreads) __attribute__((amdgpu_waves_per_eu(8, 8)))
 void win_pass_kernel(ShardVramPtrs vram, uint32_t N, uint32_t A, const AxcVariantParameters* __restrict__ lut_g,
                      StagePlan sp, WinState w, DirtyMarks dm, uint32_t v_seg, uint32_t one2) {
@@ -695,7 +697,8 @@ void win_pass_kernel(ShardVramPtrs vram, uint32_t N, uint32_t A, const AxcVarian
       for (uint32_t c = cg; c * 8 < F; c += kStageGroups) {
         if (staged) {
           Chunk<true, true> now;
-          now.load(vram, sp, N, c, i);
+          if constexpr (BUF) now.load_buf(sp, N, c, i);
+          else now.load(vram, sp, N, c, i);
           win_chunk<true, K, true>(now, c, N, A, sp, ent, w, vram.axon_heads, i, prop, v_seg, one2, plen, i_in, hits2);
         } else {
           Chunk<false, true> now;
@@ -708,7 +711,8 @@ void win_pass_kernel(ShardVramPtrs vram, uint32_t N, uint32_t A, const AxcVarian
       for (uint32_t c = cg; c * 8 < F; c += kStageGroups) {
         if (staged) {
           Chunk<true, true> now;
-          now.load(vram, sp, N, c, i);
+          if constexpr (BUF) now.load_buf(sp, N, c, i);
+          else now.load(vram, sp, N, c, i);
           win_chunk<true, K, false>(now, c, N, A, sp, ent, w, vram.axon_heads, i, prop, v_seg, one2, plen, i_in, hits2);
         } else {
           Chunk<false, true> now;
