@@ -32,14 +32,15 @@ def with_ghosts(g, n_ghosts, at):
                        g.soma_positions.copy(), n_ghosts, None, g.max_x, g.max_y)
 
 
-def run_both(b, types, epoch=7, ghosts=0, incoming=None, ghost_origins=None, max_sprouts=4, geo=None):
+def run_both(b, types, epoch=7, ghosts=0, incoming=None, ghost_origins=None, max_sprouts=4, geo=None, seed=None):
+    seed = b.spec.seed if seed is None else seed
     v = b.view()
     t1, w1 = v.targets.copy(), v.weights.copy()
     g1 = geo if geo is not None else NT.geometry_of(b)
     g1 = NT.Geometry(g1.lengths.copy(), g1.paths.copy(), g1.tips.copy(), g1.dirs.copy(), g1.soma_positions.copy(),
                      ghosts, ghost_origins, g1.max_x, g1.max_y)
     res = NT.sprout_pass(t1, w1, v.flags.copy(), v.soma_to_axon.copy(), b.padded_n, g1, types, epoch,
-                         b.spec.seed, zone_hash=0xABCD, max_sprouts=max_sprouts, incoming=incoming)
+                         seed, zone_hash=0xABCD, max_sprouts=max_sprouts, incoming=incoming)
     t2, w2 = v.targets.copy(), v.weights.copy()
     g2 = geo if geo is not None else NT.geometry_of(b)
     g2 = NT.Geometry(g2.lengths.copy(), g2.paths.copy(), g2.tips.copy(), g2.dirs.copy(), g2.soma_positions.copy(),
@@ -47,7 +48,7 @@ def run_both(b, types, epoch=7, ghosts=0, incoming=None, ghost_origins=None, max
     tl = None if types is None else [dict(zip(types.dtype.names, x)) for x in types.tolist()]
     inc = [] if incoming is None else [dict(zip(incoming.dtype.names, x)) for x in incoming.tolist()]
     new, hand, acks, prunes = SO.sprout_pass(t2, w2, v.flags.copy(), v.soma_to_axon.copy(), b.padded_n, g2.lengths,
-                                             g2.paths, g2.tips, g2.dirs, g2.soma_positions, tl, epoch, b.spec.seed,
+                                             g2.paths, g2.tips, g2.dirs, g2.soma_positions, tl, epoch, seed,
                                              g2.max_x, g2.max_y, ghosts, ghost_origins, inc, 0xABCD, max_sprouts)
     assert res.new_synapses == new
     np.testing.assert_array_equal(t1, t2)
@@ -199,3 +200,34 @@ def test_sprout_matches_oracle_over_seeds_and_settings():
         for k, st in enumerate(settings):
             types = NT.sprout_types_from_lut(b.lut, **st)
             run_both(b, types, epoch=seed * 7 + k, max_sprouts=(0, 1, 3, 7)[(seed + k) % 4])
+
+
+def test_sprout_matches_oracle_when_the_explore_noise_varies():
+    """The explore noise random_f32(seed + axon + epoch) is the top 23 bits of
+    the sum, so over one shard it takes at most two values, or jumps from
+    nearly 1 to 0 where the sum wraps past 2^64. Random scenes with such
+    seeds and large explore weights: equal to the oracle's sequential scan.
+    (test_sprout_kats.py places a scene where the cell bounds decide.)"""
+    b = small_brain(77, n=240, slots=24)
+    A = b.total_axons
+    seeds = [2**64 - A // 2,            # the sum wraps in the middle of the axons
+             5 * 2**41 - A // 3,        # two noise values, 2^-23 apart
+             2**64 - 3]                 # wraps at once: nearly every axon has noise 0
+    for k, seed in enumerate(seeds):
+        for st in (dict(w_distance=0.5, w_power=0.4, w_explore=2.0e5, w_type=0.0),
+                   dict(w_distance=0.1, w_power=0.9, w_explore=-2.0e6, w_type=0.3)):
+            types = NT.sprout_types_from_lut(b.lut, **st)
+            res, _, _ = run_both(b, types, epoch=k, seed=seed, max_sprouts=(1, 3, 4)[k])
+            assert res.new_synapses > 0
+
+
+def test_sprout_rejects_more_axons_than_a_target_holds():
+    """A target holds axon + 1 in 24 bits (layout.rs:268-271): a pass over
+    2^24 axons or more is refused before it reads the geometry."""
+    import pytest
+    A = 1 << 24
+    geo = NT.Geometry(np.zeros(A, np.uint8), np.zeros((1, 256), np.uint32), np.zeros(A, np.uint32),
+                      np.zeros(A, np.uint32), np.zeros(64, np.uint32), 0, None, 1024, 1024)
+    with pytest.raises(ValueError):
+        NT.sprout_pass(np.zeros((128, 64), np.uint32), np.zeros((128, 64), np.int16), np.zeros(64, np.uint8),
+                       np.zeros(64, np.uint32), 64, geo, None, 0, 1)

@@ -323,6 +323,77 @@ def test_explore_noise_orders_the_picks():
     assert [int(t[k, 0]) for k in (127, 126, 125, 124)] == [pack_target(a, 0) for a in (22, 23, 24, 25)]
 
 
+@pytest.mark.parametrize("case", ["two_values", "wrap", "negative_weight"])
+def test_a_later_cell_with_mixed_noise_is_scanned(case):
+    """One sprout (max_sprouts 1), w_explore only. Cell (3, 3, 3), the first
+    of the soma's radius in scan order, holds one axon; cell (7, 7, 7), the
+    last, holds axons 23 and 30, whose noise differs. The winner sits in the
+    last cell, so a pass that skips cells by a bound on their scores must
+    bound that cell by its best axon, not by its least or greatest axon id:
+    - two_values: seed + a + epoch crosses 5 * 2^41 at a = 24: axons 20 and 23
+      score 4 / 2^23, axon 30 scores 5 / 2^23: axon 30;
+    - wrap: seed + a + epoch passes 2^64 at a = 25: axon 23 scores
+      (2^23 - 1) / 2^23, axons 30 and 40 score 0: axon 23 (the cell's
+      greatest axon has the least noise);
+    - negative_weight: w_explore = -1 with the seed of two_values and axon 35
+      (5 / 2^23) in the first cell: axon 23 (least noise) wins."""
+    epoch = 5
+    s = Scene()
+    s.soma(0, (10, 10, 10))
+    if case == "wrap":
+        seed = (1 << 64) - 25 - epoch
+        first, w_e, want = 40, 1.0, 23
+    else:
+        seed = (5 << 41) - 24 - epoch
+        first, w_e, want = (20, 1.0, 30) if case == "two_values" else (35, -1.0, 23)
+    s.segments(first, [(6, 6, 6)])
+    s.segments(23, [(14, 14, 14)])
+    s.segments(30, [(14, 14, 14)])
+    noise = {a: float(f32_closed_form(seed + a + epoch)) for a in (first, 23, 30)}
+    best = max(noise, key=lambda a: w_e * noise[a])
+    assert best == want and sorted(w_e * v for v in noise.values())[-2] < w_e * noise[want]
+    n, t, *_ = s.run(types_table(w_explore=w_e), epoch=epoch, seed=seed, max_sprouts=1)
+    assert n == 1 and int(t[127, 0]) == pack_target(want, 0)
+
+
+@pytest.mark.parametrize("w_type,first_type,last_type", [(-1.0, 0, 1), (1.0, 1, 0)])
+def test_type_affinity_decides_across_cells(w_type, first_type, last_type):
+    """w_distance 0.5 and w_type, one sprout; soma 5 is of type 0. Its first
+    cell holds axon 30, its last cell axon 31. Score = 0.5 + w_type * [same
+    type] (sprouting.rs:292-296): with a negative weight axon 30 of the
+    soma's own type scores 0.5 - 1 and axon 31 of type 1 scores 0.5; with a
+    positive weight axon 30 of type 1 scores 0.5 and axon 31 of type 0 1.5.
+    Either way the last cell's axon wins (strictly greater)."""
+    s = Scene()
+    s.soma(5, (10, 10, 10), t=0)
+    s.segments(30, [(6, 6, 6)], t=first_type)
+    s.segments(31, [(14, 14, 14)], t=last_type)
+    n, t, *_ = s.run(types_table(w_distance=0.5, w_type=w_type), max_sprouts=1)
+    assert n == 1 and int(t[127, 5]) == pack_target(31, 0)
+
+
+def test_an_owner_that_sprouts_earlier_wins_with_its_new_power():
+    """w_power only, one sprout per soma. Soma 40 (not sprouting) owns axon 40,
+    the only axon in soma 5's first cell, with sum |w| = 1000; soma 2 owns
+    axon 2, the only axon in soma 5's last cell, with sum |w| = 950. Soma 2
+    bursts and sprouts first (index order) onto axon 50 near it, +74, so its
+    power index is 1024 / (128 * 32767) when soma 5 scores axon 2
+    (sprouting.rs:285-290, 331-333): soma 5 picks axon 2, which at the start
+    of the pass scored below axon 40."""
+    s = Scene()
+    s.soma(2, (40, 40, 40))
+    s.soma(5, (10, 10, 10))
+    s.segments(50, [(40, 40, 41)])     # soma 2's candidate
+    s.segments(40, [(6, 6, 6)])        # soma 5's first cell
+    s.segments(2, [(14, 14, 14)])      # soma 5's last cell
+    s.targets[0, 2], s.weights[0, 2] = pack_target(60, 0), 950
+    s.targets[0, 40], s.weights[0, 40] = pack_target(61, 0), 1000
+    n, t, w, *_ = s.run(types_table(w_power=1.0), max_sprouts=1)
+    assert n == 2
+    assert int(t[127, 2]) == pack_target(50, 0) and int(w[127, 2]) == 74
+    assert int(t[127, 5]) == pack_target(2, 0)
+
+
 # --------------------------------------------------------------------------
 # nudge_axon and handovers
 # --------------------------------------------------------------------------
