@@ -14,9 +14,9 @@ oracle (and, for the batched brains, against each brain run alone):
 - surrogate-data trials (20k neurons x 64 slots, 2,000 ticks, the search's
   own Simulator on the batched engine): the final state of each trial equal
   to the oracle's;
-- the 12M-neuron scale point (128 slots, 1.5B synapses) for 100 ticks: 64
-  mapped somas' spike bits every tick and the whole state after each batch
-  of 50.
+- the 12M-neuron scale point (128 slots, 1.5B synapses) for 100 ticks per
+  tick and 50 in windows of 4: 64 mapped somas' spike bits every tick and the
+  whole state after each batch of 50.
 """
 import ctypes as C
 
@@ -157,7 +157,7 @@ def test_c4_surrogate_data_trials_equal_oracle(gpu):
     sim.free()
 
 
-def test_c5_full_brain_100_ticks(gpu):
+def test_c5_full_brain_150_ticks(gpu):
     b = B.generate(B.spec_c5())
     p = Pair(b, n_out=64, max_batch=50)
     for k in range(2):
@@ -165,3 +165,14 @@ def test_c5_full_brain_100_ticks(gpu):
         np.testing.assert_array_equal(p.outputs(50), out_ref, err_msg=f"C5 outputs, batch {k}")
         p.check(f"C5 after {p.tick} ticks")
         print(f"[c5] ticks to {p.tick}: state equal", flush=True)
+    # 50 more ticks in windows of 4 (the window pass at 12M somas, with the
+    # blocks past the LDS capacity spilling slots to the exact path).
+    lib = day_lib()
+    lib.axc_set_window_ticks(4)
+    try:
+        out_ref, _ = p.step(50, dopamine=10, use_graph=True)
+    finally:
+        lib.axc_set_window_ticks(1)
+    np.testing.assert_array_equal(p.outputs(50), out_ref, err_msg="C5 outputs, windows of 4")
+    p.check(f"C5 after {p.tick} ticks, windows of 4")
+    print(f"[c5] ticks to {p.tick} in windows of 4: state equal", flush=True)
