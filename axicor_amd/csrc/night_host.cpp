@@ -307,9 +307,10 @@ int axc_sprout_pass(uint32_t* targets, int16_t* weights, const uint8_t* flags, c
   // (the grid's extent is known only after it) and the greatest cell
   // coordinates; the count pass turns the keys into dense cell ids.
   std::vector<uint32_t> t_max(3 * (size_t)nth, 0);
+  // Range t is taken by thread t % (threads granted): every range is built
+  // whatever number of threads the runtime grants.
 #pragma omp parallel num_threads(nth)
-  {
-    const int t = omp_get_thread_num();
+  for (int t = omp_get_thread_num(); t < nth; t += omp_get_num_threads()) {
     size_t cap = 1;  // the branch-free walk writes one entry past the last kept one
     for (size_t k = lo_of(t); k < lo_of(t + 1); k++) cap += lengths[A - 1 - k];
     t_cell[t] = huge_array<uint32_t>(cap);
@@ -359,8 +360,7 @@ int axc_sprout_pass(uint32_t* targets, int16_t* weights, const uint8_t* flags, c
   std::vector<uint32_t> cnt((size_t)nth * ncell, 0);
   std::vector<uint32_t> start(ncell + 1, 0);
 #pragma omp parallel num_threads(nth)
-  {
-    const int t = omp_get_thread_num();
+  for (int t = omp_get_thread_num(); t < nth; t += omp_get_num_threads()) {
     uint32_t* mine = cnt.data() + (size_t)t * ncell;
     uint32_t* oc = t_cell[t].get();
     for (size_t k = 0; k < t_len[t]; k++) {
@@ -385,8 +385,7 @@ int axc_sprout_pass(uint32_t* targets, int16_t* weights, const uint8_t* flags, c
   refs = huge_array<SegRef>(run);
   clk.mark("grid: offsets");
 #pragma omp parallel num_threads(nth)
-  {
-    const int t = omp_get_thread_num();
+  for (int t = omp_get_thread_num(); t < nth; t += omp_get_num_threads()) {
     uint32_t* posn = cnt.data() + (size_t)t * ncell;
     const uint32_t* oc = t_cell[t].get();
     const SegRef* orf = t_ref[t].get();

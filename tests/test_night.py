@@ -170,19 +170,23 @@ def test_sprout_same_result_for_any_thread_count():
     """The segment grid is built and each slot's candidate scan is split over
     OpenMP threads; the merge keeps the sequential choice, so 1, 3 and 8
     threads give the same targets and weights (6,000 somas, about 30% of
-    them bursting, with empty tails)."""
+    them bursting, with empty tails), and with 8 asked for under a limit of 2
+    (OMP_THREAD_LIMIT)."""
     import os
     import subprocess
     import sys
     repo = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
     outs = []
-    for n in (1, 3, 8):
+    for n, limit in ((1, None), (3, None), (8, None), (8, 2)):
         env = dict(os.environ, OMP_NUM_THREADS=str(n))
+        env.pop("OMP_THREAD_LIMIT", None)
+        if limit:
+            env["OMP_THREAD_LIMIT"] = str(limit)
         r = subprocess.run([sys.executable, "-c", _SPROUT_CHILD, repo], env=env, capture_output=True, text=True,
                            timeout=300, check=True)
         outs.append(r.stdout.strip())
     assert int(outs[0].split()[0]) > 0
-    assert outs[0] == outs[1] == outs[2], outs
+    assert len(set(outs)) == 1, outs
 
 
 def test_sprout_matches_oracle_over_seeds_and_settings():
