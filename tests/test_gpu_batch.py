@@ -139,3 +139,35 @@ def test_batch_synapse_stats_match_the_downloaded_state(gpu):
         aw = np.abs(w[t != 0])
         want = [aw.size, aw.sum(), (aw * aw).sum(), np.count_nonzero(aw < prune[k]), thr.sum()]
         assert got[k].tolist() == want, (k, got[k].tolist(), want)
+
+
+def test_batch_after_a_brain_is_pruned_mid_run(gpu):
+    """Sort & prune at 200 on one brain between two batch steps: the batch
+    rebuilds that brain's staging plan before the next step (its fills
+    shrink, its slots move) and reloads its working copy of the weights.
+    Every brain equals the oracle run through the same sequence."""
+    import ctypes as C
+    T = 24
+    members = make_members(33, 5, 2 * T)
+    mem = [load(b, dop, inp) for b, dop, inp in members]
+    be = BatchEngine(mem)
+    be.autotune(2)
+    lib = day_lib()
+    be.step(0, T)
+    lib.gpu_device_synchronize()
+    v0 = mem[0]["vram"]
+    lib.launch_sort_and_prune(C.byref(v0.ptrs), v0.padded_n, 200)
+    lib.gpu_device_synchronize()
+    be.step(T, T)
+    lib.gpu_device_synchronize()
+    for k, ((b, dop, inp), m) in enumerate(zip(members, mem)):
+        st, hd = b.state.copy(), b.heads.copy()
+        O.step_batch(st, b.padded_n, hd, b.total_axons, b.lut, 1, 0, T, inp, b.virtual_offset, b.n_inputs,
+                     dopamine=dop)
+        if k == 0:
+            O.sort_and_prune_blob(st, b.padded_n, 200)
+        O.step_batch(st, b.padded_n, hd, b.total_axons, b.lut, 1, T, T, inp[T:], b.virtual_offset, b.n_inputs,
+                     dopamine=dop)
+        assert np.array_equal(m["vram"].download_state(), st), f"brain {k}: state != oracle"
+        assert np.array_equal(m["vram"].download_axons(), hd), f"brain {k}: heads != oracle"
+    be.destroy()
