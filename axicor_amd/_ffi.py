@@ -144,6 +144,7 @@ SIGNATURES = {
     "axc_window_ticks": (_U32, []),
     "axc_shard_window_stats": (_I32, [_VP, C.POINTER(C.c_uint64)]),
     "axc_shard_targets_changed": (None, [C.POINTER(ShardVramPtrs)]),
+    "axc_shard_write_slots": (C.c_int32, [C.POINTER(ShardVramPtrs), _U32, _P, _P, _P, _U32, _P]),
     "axc_shard_graph_captures": (_I32, [_VP, C.POINTER(C.c_uint64)]),
     "axc_shard_set_soma_positions": (_I32, [_VP, _P, _U32]),
     "axc_dma_h2d_schedule": (_I32, [_P, _P, _U32, _P, _U32, _P]),
@@ -235,7 +236,7 @@ def gen_lib() -> C.CDLL:
         lib.axc_gen_paths.restype = C.c_int
         lib.axc_sprout_pass.argtypes = [_P, _P, _P, _P, _P, _U32, _U32, _P, _P, _P, _U32, _U32, _U32, _U32, _U32,
                                         _P, _U32, C.c_uint64, _P, _P, _P, C.c_uint64, _U32, C.c_uint16, _P, _P,
-                                        _U32, _P]
+                                        _U32, _P, _P, _U32]
         lib.axc_sprout_pass.restype = C.c_int
         _gen = lib
     return _gen

@@ -454,6 +454,21 @@ bool ensure_stage(Shard* s, const ShardVramPtrs& v, hipStream_t st, hipError_t* 
 // A host write of [dst, dst+size) may have changed some shard's targets.
 // For an asynchronous write (`async`, already enqueued on `st`) an event is
 // recorded after it, which the next plan rebuild waits for.
+// Slots given by column (slot * padded_n + soma) written in the reference
+// rows (axc_shard_write_slots); columns past the rows are skipped.
+__global__ __launch_bounds__(256) void write_slots_kernel(uint32_t* __restrict__ targets, int16_t* __restrict__ weights,
+                                                          uint64_t n_cols, const uint32_t* __restrict__ cols,
+                                                          const uint32_t* __restrict__ t, const int16_t* __restrict__ w,
+                                                          uint32_t n) {
+  for (uint32_t k = blockIdx.x * 256 + threadIdx.x; k < n; k += gridDim.x * 256) {
+    const uint32_t c = cols[k];
+    if (c < n_cols) {
+      targets[c] = t[k];
+      weights[c] = w[k];
+    }
+  }
+}
+
 void note_device_write(const void* dst, size_t size, bool async = false, hipStream_t st = nullptr) {
   if (!dst || size == 0) return;
   const char* a = (const char*)dst;
@@ -1749,6 +1764,24 @@ int32_t axc_shard_graph_captures(const ShardVramPtrs* vram, uint64_t* out) {
 
 void axc_shard_targets_changed(const ShardVramPtrs* vram) {
   if (vram) mark_stage_dirty(vram->soma_voltage);
+}
+
+int32_t axc_shard_write_slots(const ShardVramPtrs* vram, uint32_t padded_n, const uint32_t* d_cols,
+                              const uint32_t* d_targets, const int16_t* d_weights, uint32_t n, void* stream) {
+  if (!vram || !vram->dendrite_targets || !vram->dendrite_weights || (n && (!d_cols || !d_targets || !d_weights)))
+    return (int32_t)hipErrorInvalidValue;
+  hipStream_t st = (hipStream_t)stream;
+  if (n) {
+    write_slots_kernel<<<std::min<uint32_t>(blocks_for(n), 1024u), kThreads, 0, st>>>(
+        vram->dendrite_targets, vram->dendrite_weights, (uint64_t)kMaxDendrites * padded_n, d_cols, d_targets,
+        d_weights, n);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return (int32_t)e;
+  }
+  // As an asynchronous copy into the targets: the plan is rebuilt at the next
+  // step, after this stream's write.
+  note_device_write(vram->dendrite_targets, (size_t)kMaxDendrites * padded_n * 4, true, st);
+  return 0;
 }
 
 void axc_kernel_timing_enable(int enable) {

@@ -146,13 +146,17 @@ struct AxcSproutResult {
 // paths [total_axons][256]; soma_positions [padded_n]. types (n_types
 // entries, may be null: all scores 1.0 and weight +74 as in the reference).
 // acks (capacity n_in) and prunes (capacity max_prunes) receive the outputs.
+// changed_cols (may be null; capacity max_changed) receives the column
+// (slot * padded_n + soma) of every new synapse in the order made, so that a
+// caller can upload only those slots (axc_shard_write_slots).
 int axc_sprout_pass(uint32_t* targets, int16_t* weights, const uint8_t* flags, const uint32_t* ghost_origins,
                     AxcHandoverEvent* handovers, uint32_t n_in, uint32_t max_handovers, uint32_t* tips,
                     uint32_t* dirs, const uint32_t* soma_to_axon, uint32_t padded_n, uint32_t total_ghosts,
                     uint32_t total_axons, uint32_t max_x, uint32_t max_y, const AxcSproutType* types,
                     uint32_t n_types, uint64_t epoch, uint8_t* lengths, uint32_t* paths,
                     const uint32_t* soma_positions, uint64_t master_seed, uint32_t zone_hash, uint16_t max_sprouts,
-                    AxcHandoverAck* acks, AxcHandoverPrune* prunes, uint32_t max_prunes, AxcSproutResult* out) {
+                    AxcHandoverAck* acks, AxcHandoverPrune* prunes, uint32_t max_prunes, AxcSproutResult* out,
+                    uint32_t* changed_cols, uint32_t max_changed) {
   const size_t N = padded_n, A = total_axons;
   const size_t ghost_start = N, ghost_end = N + total_ghosts;
   if (ghost_end > A) return -1;
@@ -669,6 +673,7 @@ int axc_sprout_pass(uint32_t* targets, int16_t* weights, const uint8_t* flags, c
       weights[col] = inhib ? (int16_t)-w0 : w0;
       const int32_t w_new = weights[col];
       sum_abs[i] += (uint32_t)(w_new < 0 ? -w_new : w_new) - (uint32_t)(w_old < 0 ? -w_old : w_old);
+      if (changed_cols && new_syn < max_changed) changed_cols[new_syn] = (uint32_t)col;
       picked.push_back(b_axon);
       active[b_axon] = 1;
       new_syn++;

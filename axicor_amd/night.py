@@ -4,7 +4,8 @@
 genesis-baker/src/bake/sprouting.rs:119-378). `execute_night_phase` is the
 sequence of genesis-node/src/node/shard_thread.rs:269-378: download the flags
 before pruning, launch_sort_and_prune, download weights and targets, sprout on
-the host, upload targets and weights (timers, voltages and heads stay)."""
+the host, upload the new synapses' slots of targets and weights (timers,
+voltages and heads stay)."""
 from __future__ import annotations
 
 import ctypes as C
@@ -13,7 +14,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from ._ffi import day_lib, gen_lib, ptr
-from .engine import GpuError, PinnedBuffer, VramState
+from .engine import DeviceBuffer, GpuError, PinnedBuffer, VramState
 from .layout import MAX_DENDRITES
 
 HANDOVER_DTYPE = np.dtype([("origin_zone_hash", "<u4"), ("local_axon_id", "<u4"), ("entry_x", "<u2"),
@@ -59,6 +60,7 @@ class SproutResult:
     handovers: np.ndarray = field(default_factory=lambda: np.zeros(0, HANDOVER_DTYPE))
     acks: np.ndarray = field(default_factory=lambda: np.zeros(0, ACK_DTYPE))
     prunes: np.ndarray = field(default_factory=lambda: np.zeros(0, PRUNE_DTYPE))
+    changed_cols: np.ndarray | None = None  # columns (slot * padded_n + soma) of the new synapses, if asked for
 
 
 class _Res(C.Structure):
@@ -69,9 +71,11 @@ class _Res(C.Structure):
 def sprout_pass(targets: np.ndarray, weights: np.ndarray, flags: np.ndarray, soma_to_axon: np.ndarray,
                 padded_n: int, geo: Geometry, types: np.ndarray | None, epoch: int, master_seed: int,
                 zone_hash: int = 0, max_sprouts: int = 4,
-                incoming: np.ndarray | None = None) -> SproutResult:
+                incoming: np.ndarray | None = None, changed: np.ndarray | None = None) -> SproutResult:
     """run_sprouting_pass in place on host arrays (targets/weights columnar
-    [128, padded_n], geometry arrays updated)."""
+    [128, padded_n], geometry arrays updated). With `changed` (uint32, room
+    for every new synapse: padded_n * max(1, max_sprouts) always suffices),
+    the result's changed_cols lists the columns the pass filled."""
     for a in (targets, weights, flags, soma_to_axon, geo.lengths, geo.paths, geo.tips, geo.dirs, geo.soma_positions):
         assert a.flags["C_CONTIGUOUS"]
     A = geo.tips.shape[0]
@@ -89,10 +93,17 @@ def sprout_pass(targets: np.ndarray, weights: np.ndarray, flags: np.ndarray, som
                                    padded_n, geo.total_ghosts, A, geo.max_x, geo.max_y, ptr(ty),
                                    0 if ty is None else len(ty), epoch & 0xFFFFFFFFFFFFFFFF, ptr(geo.lengths),
                                    ptr(geo.paths), ptr(geo.soma_positions), master_seed & 0xFFFFFFFFFFFFFFFF,
-                                   zone_hash, max_sprouts, ptr(acks), ptr(prunes), MAX_PRUNES_PER_NIGHT, C.byref(r))
+                                   zone_hash, max_sprouts, ptr(acks), ptr(prunes), MAX_PRUNES_PER_NIGHT, C.byref(r),
+                                   ptr(changed), 0 if changed is None else len(changed))
     if rc != 0:
         raise ValueError("axc_sprout_pass rejected its arguments")
-    return SproutResult(r.new_synapses, hand[:r.handovers_out].copy(), acks[:r.acks].copy(), prunes[:r.prunes].copy())
+    cols = None
+    if changed is not None:
+        if r.new_synapses > len(changed):
+            raise ValueError(f"`changed` holds {len(changed)} columns, the pass made {r.new_synapses} synapses")
+        cols = changed[:r.new_synapses]
+    return SproutResult(r.new_synapses, hand[:r.handovers_out].copy(), acks[:r.acks].copy(), prunes[:r.prunes].copy(),
+                        cols)
 
 
 class NightStaging:
@@ -116,6 +127,33 @@ class NightStaging:
         for b in self._bufs:
             b.array.fill(0)
 
+    def changed_buffer(self, max_sprouts: int) -> np.ndarray:
+        """Room for the columns of every synapse a pass can make
+        (padded_n * max(1, max_sprouts)), kept for the next Nights."""
+        need = self.padded_n * max(1, int(max_sprouts))
+        if getattr(self, "_changed", None) is None or len(self._changed) < need:
+            self._changed = np.empty(need, np.uint32)
+        return self._changed[:need]
+
+    def upload_slots(self, vram: VramState, cols: np.ndarray) -> None:
+        """The host values of the listed slots (targets and weights) written
+        to the shard's reference rows (axc_shard_write_slots): the upload of a
+        Night whose sprouting changed only those slots."""
+        n = len(cols)
+        if n == 0:
+            return
+        if getattr(self, "_d_slots", None) is None or self._d_slots[0] < n:
+            cap = max(n, 1 << 16)
+            self._d_slots = (cap, DeviceBuffer(4 * cap), DeviceBuffer(4 * cap), DeviceBuffer(2 * cap))
+        _, d_cols, d_t, d_w = self._d_slots
+        d_cols.upload(np.ascontiguousarray(cols, np.uint32))
+        d_t.upload(self.targets.reshape(-1)[cols])
+        d_w.upload(self.weights.reshape(-1)[cols])
+        err = day_lib().axc_shard_write_slots(C.byref(vram.ptrs), self.padded_n, d_cols.ptr, d_t.ptr, d_w.ptr, n,
+                                              None)
+        if err:
+            raise GpuError(f"axc_shard_write_slots failed ({err})")
+
     @staticmethod
     def of(vram: VramState) -> "NightStaging":
         """The shard's staging buffers, made on its first Night."""
@@ -130,7 +168,8 @@ def execute_night_phase(vram: VramState, prune_threshold: int, geo: Geometry | N
                         epoch: int, master_seed: int, max_sprouts: int = 4) -> SproutResult | None:
     """shard_thread.rs:269-378 on one shard; returns the sprouting result (None
     when no geometry is given: sort & prune only, as when no daemon is up).
-    The host copies live in the shard's NightStaging."""
+    The host copies live in the shard's NightStaging; the upload writes only
+    the slots the sprouting filled."""
     lib = day_lib()
     N = vram.padded_n
     st = NightStaging.of(vram)
@@ -146,10 +185,11 @@ def execute_night_phase(vram: VramState, prune_threshold: int, geo: Geometry | N
         if not lib.gpu_memcpy_device_to_host(ptr(dst), src, dst.nbytes):
             raise GpuError("night download failed")
     res = sprout_pass(st.targets, st.weights, st.flags, st.soma_to_axon, N, geo, types, epoch, master_seed,
-                      max_sprouts=max_sprouts)
-    for src, dst in ((st.targets, vram.ptrs.dendrite_targets), (st.weights, vram.ptrs.dendrite_weights)):
-        if not lib.gpu_memcpy_host_to_device(dst, ptr(src), src.nbytes):
-            raise GpuError("night upload failed")
+                      max_sprouts=max_sprouts, changed=st.changed_buffer(max_sprouts))
+    # The device already holds the pruned rows; only the new synapses' slots
+    # differ (the reference uploads both whole arrays, shard_thread.rs:339-350).
+    st.upload_slots(vram, res.changed_cols)
+    lib.gpu_device_synchronize()
     return res
 
 

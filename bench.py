@@ -101,7 +101,8 @@ def cpu_single_thread(brain, inputs_host: np.ndarray, ticks: int) -> dict:
 def bench_nights(lib, brain, vram, eng, batch, nights: int, prune: int) -> dict:
     """Nights of shard_thread.rs:269-378 on the workload brain, each after one
     more Day batch: flags download, GPU sort & prune, weights and targets
-    download, host sprouting (csrc/night_host.cpp), upload. Wall-clock per
+    download, host sprouting (csrc/night_host.cpp), upload of the slots the
+    sprouting filled (night.execute_night_phase's sequence). Wall-clock per
     phase, averaged."""
     import ctypes as C
     from axicor_amd import night as NT
@@ -126,10 +127,9 @@ def bench_nights(lib, brain, vram, eng, batch, nights: int, prune: int) -> dict:
         lib.gpu_memcpy_device_to_host(ptr(t), vram.ptrs.dendrite_targets, t.nbytes)
         lib.gpu_memcpy_device_to_host(ptr(s2a), vram.ptrs.soma_to_axon, s2a.nbytes)
         t2 = time.perf_counter()
-        res = NT.sprout_pass(t, w, flags, s2a, N, geo, types, k, brain.spec.seed)
+        res = NT.sprout_pass(t, w, flags, s2a, N, geo, types, k, brain.spec.seed, changed=st.changed_buffer(4))
         t3 = time.perf_counter()
-        lib.gpu_memcpy_host_to_device(vram.ptrs.dendrite_targets, ptr(t), t.nbytes)
-        lib.gpu_memcpy_host_to_device(vram.ptrs.dendrite_weights, ptr(w), w.nbytes)
+        st.upload_slots(vram, res.changed_cols)  # only the new synapses' slots differ from the device
         lib.gpu_device_synchronize()
         t4 = time.perf_counter()
         for key, dt in (("prune_s", t1 - t0), ("download_s", t2 - t1), ("sprout_s", t3 - t2), ("upload_s", t4 - t3)):

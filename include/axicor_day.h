@@ -359,6 +359,15 @@ void axc_host_sort_and_prune(uint8_t *flags, uint32_t *targets, int16_t *weights
  * switches the whole process to the direct scan (same results). */
 void axc_staging_enable(int enable);
 void axc_shard_targets_changed(const ShardVramPtrs *vram);
+/* Writes n slots of the shard's reference rows on `stream`:
+ * dendrite_targets[cols[k]] = targets[k] and dendrite_weights[cols[k]] =
+ * weights[k], cols[k] = slot * padded_n + soma (device arrays; columns past
+ * 128 * padded_n are skipped), and rebuilds the staging plan at the next step
+ * after that write. No counterpart in the reference, whose Night uploads the
+ * whole targets and weights (shard_thread.rs:339-350): for a host Night that
+ * changed a few slots (the new synapses of axc_sprout_pass). */
+int32_t axc_shard_write_slots(const ShardVramPtrs *vram, uint32_t padded_n, const uint32_t *cols,
+                              const uint32_t *targets, const int16_t *weights, uint32_t n, void *stream);
 
 /* Windows of ticks (no counterpart in the reference; same results).
  * axc_step_day_phase_batch with the staged scan streams the dendrite slots

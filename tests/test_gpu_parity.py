@@ -311,3 +311,49 @@ def test_hot_checkpoint_resume(gpu, tmp_path):
         assert np.array_equal(p.vram.download_state(), q.vram.download_state())
         assert np.array_equal(p.vram.download_axons(), q.vram.download_axons())
         q.check(f"resumed batch {k}")
+
+
+@ORDERS
+def test_write_slots_then_step(gpu, engine_order):
+    """axc_shard_write_slots puts new synapses into empty slots (targets and
+    weights) after a batch, as a host Night's upload does; a column past the
+    rows is skipped. The next batches (graph and windows of 4) must read the
+    new targets (the staging plan is rebuilt) and equal the oracle given the
+    same slots."""
+    from axicor_amd._ffi import day_lib
+    from axicor_amd.engine import DeviceBuffer
+    from axicor_amd.layout import StateView
+    b = B.generate(B.spec_c1())
+    N = b.padded_n
+    p = Pair(b, max_batch=50, positions=random_positions(N, 5) if engine_order else None)
+    p.step(30, use_graph=True)
+    p.check("before the writes")
+    v = StateView(p.ref_state, N)
+    rng = np.random.default_rng(3)
+    t_all = v.targets.reshape(-1)
+    empty = np.flatnonzero(t_all == 0)
+    cols = np.sort(rng.choice(empty, size=min(4000, len(empty)), replace=False)).astype(np.uint32)
+    axons = rng.integers(0, b.total_axons, size=len(cols))
+    segs = rng.integers(0, 8, size=len(cols))
+    tgt = ((segs << 24) | (axons + 1)).astype(np.uint32)
+    wts = rng.integers(-400, 400, size=len(cols)).astype(np.int16)
+    t_all[cols] = tgt
+    v.weights.reshape(-1)[cols] = wts
+    junk = np.uint32(128 * N + 17)  # past the rows: skipped
+    d_cols = DeviceBuffer.from_array(np.append(cols, junk))
+    d_t = DeviceBuffer.from_array(np.append(tgt, np.uint32(5)))
+    d_w = DeviceBuffer.from_array(np.append(wts, np.int16(9)))
+    lib = day_lib()
+    err = lib.axc_shard_write_slots(C.byref(p.vram.ptrs), N, d_cols.ptr, d_t.ptr, d_w.ptr, len(cols) + 1,
+                                    p.eng.stream)
+    assert err == 0
+    p.eng.synchronize()
+    p.check("after the writes")
+    p.step(30, use_graph=True)
+    p.check("a batch after the writes")
+    lib.axc_set_window_ticks(4)
+    try:
+        p.step(30, use_graph=True)
+    finally:
+        lib.axc_set_window_ticks(1)
+    p.check("windows of 4 after the writes")

@@ -39,8 +39,16 @@ def run_both(b, types, epoch=7, ghosts=0, incoming=None, ghost_origins=None, max
     g1 = geo if geo is not None else NT.geometry_of(b)
     g1 = NT.Geometry(g1.lengths.copy(), g1.paths.copy(), g1.tips.copy(), g1.dirs.copy(), g1.soma_positions.copy(),
                      ghosts, ghost_origins, g1.max_x, g1.max_y)
+    t0 = t1.copy()
+    changed = np.empty(b.padded_n * max(1, max_sprouts), np.uint32)
     res = NT.sprout_pass(t1, w1, v.flags.copy(), v.soma_to_axon.copy(), b.padded_n, g1, types, epoch,
-                         seed, zone_hash=0xABCD, max_sprouts=max_sprouts, incoming=incoming)
+                         seed, zone_hash=0xABCD, max_sprouts=max_sprouts, incoming=incoming, changed=changed)
+    # The listed columns are exactly the slots whose targets changed (each an
+    # empty slot that got a synapse), each once.
+    cols = res.changed_cols
+    assert len(cols) == res.new_synapses and len(np.unique(cols)) == len(cols)
+    np.testing.assert_array_equal(np.sort(cols), np.flatnonzero((t1 != t0).reshape(-1)))
+    assert (t0.reshape(-1)[cols] == 0).all() and (t1.reshape(-1)[cols] != 0).all()
     t2, w2 = v.targets.copy(), v.weights.copy()
     g2 = geo if geo is not None else NT.geometry_of(b)
     g2 = NT.Geometry(g2.lengths.copy(), g2.paths.copy(), g2.tips.copy(), g2.dirs.copy(), g2.soma_positions.copy(),
