@@ -3,13 +3,19 @@
 // ShardEngine per shard, genesis-compute/src/compute/shard.rs:18-125).
 //
 // Each brain keeps the reference layout (its own cu_allocate_shard arrays)
-// and its own 16-entry parameter table. Per tick the batch launches three
-// fixed-grid kernels for all brains together — axon phase, soma phase (with
-// the previous tick's GSOP fused in), push phase — that walk flat work lists
-// of (brain, 256-item chunk) entries; a batch of ticks is captured in one HIP
-// graph and ends with a GSOP launch. Each brain runs exactly the device code
-// of the single-shard path (axon_step, neuron_block, push_apply_one,
-// gsop_soma), so every brain is bit-identical to running it alone.
+// and its own 16-entry parameter table. A batch of ticks is captured in one
+// HIP graph. With staging (the default) the graph loads every brain's working
+// copy of the weights, then per tick launches, for all brains together, the
+// axon phase over a flat list of (brain, 256-axon) items, the staged soma
+// phase over (brain, kBlockSomas-soma) items in the autotuner's dispatch
+// order, the spike-list compaction, a prefix of the brains' spike counts, the
+// own-axon pushes and the chunked GSOP over the flattened spike lists; it
+// ends by storing the working copies back. Without staging, per tick: axon
+// phase, direct soma phase (the previous tick's GSOP fused in), owner-list
+// pushes, and one GSOP launch at the end. Each brain runs exactly the device
+// code of the single-shard path (axon_step, staged_block, push_spiker_one,
+// gsop_chunk; neuron_block, push_apply_one, gsop_soma), so every brain is
+// bit-identical to running it alone.
 //
 // Included by day_engine.hip (one translation unit).
 
