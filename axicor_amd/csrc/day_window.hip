@@ -825,23 +825,29 @@ __global__ __launch_bounds__(256) void win_soma_kernel(ShardVramPtrs vram, uint3
     bool fin = false, push_own = false;
     uint32_t my_axon = 0;
     if (i < N) {
+      // Every load is issued before the refractory test, so that a soma
+      // costs one round trip (the loads under the test would wait for the
+      // timer's); a refractory soma's values go unused.
       const uint8_t flags_in = vram.soma_flags[i];
       const uint8_t timer = vram.timers[i];
+      const uint64_t acc = w.acc[(size_t)j * N + i];
+      const int32_t v_ld = vram.soma_voltage[i], thr_ld = vram.threshold_offset[i];
+      const uint32_t ax_ld = vram.soma_to_axon[i];
+      const uint32_t F = sp.fill[i];
+      const uint32_t hb_i = sp.perm ? sp.perm[i] : i;
       const bool refr = timer > 0;
       int32_t sum = 0, v_in = 0, thr_in = 0;
       uint32_t hc = 0, ax_in = 0;
       if (!refr) {
-        win_unpack(w.acc[(size_t)j * N + i], sum, hc);
-        v_in = vram.soma_voltage[i];
-        thr_in = vram.threshold_offset[i];
-        ax_in = vram.soma_to_axon[i];
-        const uint32_t F = sp.fill[i];
+        win_unpack(acc, sum, hc);
+        v_in = v_ld;
+        thr_in = thr_ld;
+        ax_in = ax_ld;
         r[0] += 1;
         r[1] += F;
         r[2] += F < kMaxDendrites ? F + 1 : kMaxDendrites;
         r[3] += hc;
       }
-      const uint32_t hb_i = sp.perm ? sp.perm[i] : i;
       uint8_t flags = flags_in & 0xF1;  // K1
       flags &= (uint8_t)~0x01;
       soma_tick_with(vram, i, A, lut[(flags_in >> 4) & 0x0F], flags, timer, refr, tick, sum, thr_in, v_in, ax_in,
