@@ -1,8 +1,8 @@
-"""Long runs on the GPU against the CPU oracle, skipped unless AXC_LONG=1
-(many minutes of oracle time, so not part of the default suite, which runs
-each configuration for fewer ticks in tests/test_gpu_configs.py):
+"""Long runs of the full 1.5M-neuron brain on the GPU:
 
-- the full 1.5M-neuron brain for 2,000 ticks through the batch graph with
+- (skipped unless AXC_LONG=1: two minutes of oracle time; the default suite
+  runs this brain against the oracle for 500 ticks and a Night in
+  tests/test_gpu_configs.py) the brain for 2,000 ticks through the batch graph with
   a Night every 500 ticks as shard_thread.rs:269-378 runs it (the flags
   taken before the GPU sort & prune at 15, the host sprouting, the upload):
   every soma's spike bit every tick, and the whole state and the heads
@@ -12,6 +12,7 @@ each configuration for fewer ticks in tests/test_gpu_configs.py):
   and with the per-tick pass, the two runs' states and heads equal before
   and after every Night and at the end (the oracle covers the first 2,000
   ticks above; 10,000 ticks of it would take over a quarter of an hour).
+  Part of the default suite (about 40 s).
 
 Run: AXC_LONG=1 python -u -m pytest tests/test_gpu_long.py -s --timeout 1100"""
 import os
@@ -27,10 +28,10 @@ from axicor_amd.layout import StateView
 from helpers import Pair
 from oracle import oracle as O
 
-pytestmark = [pytest.mark.gpu,
-              pytest.mark.skipif(os.environ.get("AXC_LONG") != "1", reason="long run: set AXC_LONG=1")]
+pytestmark = pytest.mark.gpu
 
 
+@pytest.mark.skipif(os.environ.get("AXC_LONG") != "1", reason="long run: set AXC_LONG=1")
 def test_c2_2000_ticks_with_a_night_every_500(gpu):
     b = B.generate(B.spec_c2(), with_paths=True)
     N = b.padded_n
