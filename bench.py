@@ -308,6 +308,9 @@ def main() -> int:
     ap.add_argument("--copy-calibration-mb", default="",
                     help="comma-separated sizes: one device-to-device copy of each (reference points for the HBM "
                          "byte counters)")
+    ap.add_argument("--gather-calibration-mb", default="",
+                    help="comma-separated sizes: one gather of every 64-byte row of a buffer of each size in random "
+                         "order (torch index_select; scattered full-line reads for the HBM byte counters)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -413,6 +416,20 @@ def main() -> int:
         src.free()
         dst.free()
         log(f"copy calibration: {mb} MiB device-to-device")
+    for mb in [int(x) for x in str(args.gather_calibration_mb).split(",") if x.strip()]:
+        # Reads every 64-byte row once, in a random order (plus 8 B of index
+        # per row), and writes the rows out in that order.
+        import torch
+        rows = (mb << 20) // 64
+        g = torch.Generator(device="cuda").manual_seed(7)
+        src = torch.empty((rows, 64), dtype=torch.uint8, device="cuda").random_(0, 256, generator=g)
+        idx = torch.randperm(rows, device="cuda", generator=g)
+        torch.cuda.synchronize()
+        out = src.index_select(0, idx)
+        torch.cuda.synchronize()
+        log(f"gather calibration: {mb} MiB of 64-byte rows in random order ({rows} rows; reads {mb} MiB + "
+            f"{rows * 8} B of indices, writes {mb} MiB; checksum {int(out[:, 0].sum())})")
+        del src, idx, out
 
     eng.stats_reset()
     start = C.c_void_p()
