@@ -477,7 +477,10 @@ void note_device_write(const void* dst, size_t size, bool async = false, hipStre
     Shard* s = kv.second;
     const char* t0 = (const char*)s->state_base + (size_t)s->N * 14;
     const char* t1 = t0 + (size_t)s->N * 4 * kMaxDendrites;
-    if (a < t1 && a + size > t0) {
+    // The flags column too: the reverse index keeps each soma's type.
+    const char* f0 = (const char*)s->state_base + (size_t)s->N * 4;
+    const char* f1 = f0 + (size_t)s->N;
+    if ((a < t1 && a + size > t0) || (a < f1 && a + size > f0)) {
       if (async) {
         if (!s->upload_ev) (void)hipEventCreateWithFlags(&s->upload_ev, hipEventDisableTiming);
         if (s->upload_ev && hipEventRecord(s->upload_ev, st) == hipSuccess) s->upload_pending = true;
@@ -674,7 +677,7 @@ uint32_t window_for(uint32_t v_seg) {
 // reverse index of the current plan when it is missing or stale. Outside
 // graph capture: it synchronizes `st` once to size the index (a larger
 // index moves, and the batch graph holding its address is recorded again).
-hipError_t ensure_window(Shard* s, hipStream_t st) {
+hipError_t ensure_window(Shard* s, const uint8_t* flags, hipStream_t st) {
   const size_t N = s->N, A = s->A;
   hipError_t e = hipSuccess;
   if (!s->win.acc) {
@@ -741,7 +744,7 @@ hipError_t ensure_window(Shard* s, hipStream_t st) {
     (void)hipFree(cursor);
     return e;
   }
-  if (N) rev_fill_kernel<<<blocks_for(N), kThreads, 0, st>>>((uint32_t)N, (uint32_t)A, s->sp, cursor, tmp);
+  if (N) rev_fill_kernel<<<blocks_for(N), kThreads, 0, st>>>((uint32_t)N, (uint32_t)A, s->sp, flags, cursor, tmp);
   if (A) rev_sort_kernel<<<std::min<uint32_t>(blocks_for(A, 4), 8192u), kThreads, 0, st>>>((uint32_t)A,
                                                                                            s->win.rev_off, tmp,
                                                                                            s->win.rev,
@@ -1534,7 +1537,7 @@ int32_t axc_step_day_phase_batch(const ShardVramPtrs* vram, uint32_t padded_n, u
   key.win = (key.staged && padded_n > 0) ? window_for(v_seg) : 1u;
   key.rev = nullptr;
   if (key.win > 1) {
-    if ((e = ensure_window(s, st)) != hipSuccess) return (int32_t)e;
+    if ((e = ensure_window(s, vram->soma_flags, st)) != hipSuccess) return (int32_t)e;
     key.rev = s->win.rev;
   }
   std::vector<uint32_t> counts(T, 0);

@@ -775,7 +775,7 @@ __global__ __launch_bounds__(256) void win_correct_kernel(const AxcBurstHeads8* 
     const uint32_t a = (uint32_t)(pc >> 32) & 0x00FFFFFFu;
     const uint64_t en = w.rev[(uint32_t)pc + l];
     const uint32_t i = (uint32_t)en, s = (uint32_t)(en >> 32) & 0xFFu, seg = rev_seg(en);
-    const uint32_t L = prop[soma_flags[i] >> 4];
+    const uint32_t L = prop[(uint32_t)(en >> 48) & 0x0Fu];  // the soma's type, kept in the entry
     const bool pred = (win_pred_mask(w, w.heads_w0, 0, A, a, seg, L, v_seg, j + 1) >> j) & 1u;
     const bool hit = any_hit(load_heads(heads, a), seg, L);
     n_e += 1;
@@ -1018,17 +1018,23 @@ __global__ __launch_bounds__(1024) void rev_scan_kernel(const uint32_t* cnt, uin
 
 // The entries, at cursor[a]++ (cursor: a copy of rev_off[0, A)), in no
 // particular order within an axon's list (rev_sort_kernel sorts them).
+// An entry also holds the soma's variant type (bits 48-51, from the caller's
+// flags through the plan's order), which picks the propagation length of
+// the slot's test: the type changes only with an upload of the state or of
+// the flags, which rebuilds the plan and this index.
 __global__ __launch_bounds__(256) void rev_fill_kernel(uint32_t N, uint32_t A, StagePlan sp,
+                                                       const uint8_t* __restrict__ flags,
                                                        uint32_t* __restrict__ cursor, uint64_t* __restrict__ rev) {
   const uint32_t i = blockIdx.x * 256 + threadIdx.x;
   if (i >= N) return;
   const uint32_t F = sp.fill[i];
+  const uint64_t type = (uint64_t)((flags[sp.perm ? sp.perm[i] : i] >> 4) & 0x0Fu) << 48;
   for (uint32_t s = 0; s < F; s++) {
     const uint32_t t = sp.tgt[chunk_at(N, s >> 3, i) + (s & 7)];
     const uint32_t a = snap_index(t, A);
     if (a >= A) continue;
     const uint32_t p = atomicAdd(&cursor[a], 1u);
-    rev[p] = (uint64_t)i | ((uint64_t)s << 32) | ((uint64_t)(t >> 24) << 40);
+    rev[p] = (uint64_t)i | ((uint64_t)s << 32) | ((uint64_t)(t >> 24) << 40) | type;
   }
 }
 

@@ -353,9 +353,11 @@ void axc_host_sort_and_prune(uint8_t *flags, uint32_t *targets, int16_t *weights
  * axons (whose head snapshots the soma pass stages in LDS) and each slot's
  * block-local index and segment. The plan is rebuilt at the next step after
  * the targets change through cu_upload_state_blob, launch_sort_and_prune or
- * gpu_memcpy_host_to_device(_async) into the targets array. A caller that
- * writes dendrite_targets any other way (its own copies or kernels) calls
- * axc_shard_targets_changed before the next step. axc_staging_enable(0)
+ * gpu_memcpy_host_to_device(_async) into the targets array or the flags
+ * array (the windows keep each soma's variant type, soma_flags >> 4). A
+ * caller that writes dendrite_targets or the variant types any other way
+ * (its own copies or kernels) calls axc_shard_targets_changed before the
+ * next step. axc_staging_enable(0)
  * switches the whole process to the direct scan (same results). */
 void axc_staging_enable(int enable);
 void axc_shard_targets_changed(const ShardVramPtrs *vram);
